@@ -1,0 +1,384 @@
+"""Python binding of the gfx950 C ABI (include/airbox_hip.h) via ctypes.
+
+The host-side mirror of the reference's backend API for this path
+(src/engine.js: prepare -> setupSim/GpuSim -> steps/probe/read -> evaluate).
+It loads ``libairbox_hip.so`` built in-tree; when the library or a gfx950 GPU
+is missing every call raises: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libairbox_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "airbox_hip.h")
+
+F32, F32_F64, F64 = 0, 1, 2
+PRECISIONS = {"f32": F32, "f32_f64": F32_F64, "f64": F64}
+
+
+class AbxError(RuntimeError):
+    """A failed library call; the message is the library's (the reference's
+    own text where the reference raises the same error)."""
+
+
+class PrepParams(C.Structure):
+    _fields_ = [("dx", C.c_double), ("sealAxis", C.c_int), ("sealSign", C.c_int), ("seal", C.c_double),
+                ("inletAxis", C.c_int), ("inletSign", C.c_int), ("inletYaw", C.c_double), ("cy", C.c_double),
+                ("sy", C.c_double), ("has_click", C.c_int), ("click", C.c_double * 3), ("pleat", C.c_double),
+                ("has_outlet", C.c_int), ("out_center", C.c_double * 3), ("out_diameter", C.c_double),
+                ("out_height", C.c_double), ("max_cells", C.c_double)]
+
+
+class GeomInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("N", C.c_int64), ("nf", C.c_int64),
+                ("top", C.c_int), ("jm", C.c_int), ("kw0", C.c_int), ("h", C.c_double),
+                ("origin", C.c_double * 3), ("mn", C.c_double * 3), ("mx", C.c_double * 3),
+                ("nHoles", C.c_int), ("nOpen", C.c_int), ("nOut", C.c_int), ("Af", C.c_int), ("leak", C.c_int),
+                ("pleatOk", C.c_int), ("has_outlet", C.c_int),
+                ("volume_l", C.c_double), ("inlet_cm2", C.c_double), ("filter_cm2", C.c_double),
+                ("outlet_cm2", C.c_double), ("outlet_cm2_real", C.c_double),
+                ("nOpenPts", C.c_int64), ("nFilterPts", C.c_int64), ("nOutletPts", C.c_int64),
+                ("dAx", C.c_int), ("iAx", C.c_int), ("oAx", C.c_int), ("sd", C.c_int), ("si", C.c_int),
+                ("seal", C.c_double), ("yaw", C.c_double), ("cy", C.c_double), ("sy", C.c_double),
+                ("pleat", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("fluid", C.c_void_p), ("opening", C.c_void_p),
+                ("outlet", C.c_void_p), ("dc", C.c_void_p), ("top", C.c_int), ("jm", C.c_int),
+                ("pleat", C.c_double)]
+
+
+class RunParams(C.Structure):
+    _fields_ = [("K", C.c_double), ("aniso", C.c_double), ("u", C.c_double), ("tau", C.c_double)]
+
+
+class SimInfo(C.Structure):
+    _fields_ = [("NF", C.c_int64), ("Ain", C.c_int), ("Af", C.c_int), ("Lp", C.c_int), ("nM", C.c_int),
+                ("nT", C.c_int), ("u0", C.c_double), ("tau0", C.c_double), ("sig", C.c_double),
+                ("aniso", C.c_double), ("cs2s", C.c_double), ("precision", C.c_int), ("B", C.c_int),
+                ("device_bytes", C.c_int64)]
+
+
+G_SOLID, G_FLUID, G_OPENING, G_OUTLET, G_XC, G_DC, G_ZC, G_OPENPTS, G_FILTERPTS, G_OUTLETPTS, G_FINE = range(11)
+
+_lib = None
+
+
+def lib_path() -> str:
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Load libairbox_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AbxError(f"{LIB_PATH} is missing: build it with `make -C airboxcalc_amd/csrc` "
+                       "or __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, d, f32p = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p
+    sig = {
+        "abx_last_error": (C.c_char_p, []),
+        "abx_version": (C.c_char_p, []),
+        "abx_device_open": (i32, [i32, C.POINTER(vp), C.c_char_p, C.c_size_t]),
+        "abx_device_close": (None, [vp]),
+        "abx_parse_stl": (i32, [C.c_char_p, i64, vp, i64, C.POINTER(i64)]),
+        "abx_voxelize": (i32, [vp, vp, i64, vp, i32, i32, i32, d, vp]),
+        "abx_prepare": (i32, [vp, vp, i64, C.POINTER(PrepParams), C.POINTER(vp)]),
+        "abx_geom_info_get": (i32, [vp, C.POINTER(GeomInfo)]),
+        "abx_geom_read": (i32, [vp, i32, vp, i64]),
+        "abx_geom_free": (None, [vp]),
+        "abx_geom_grid": (i32, [vp, C.POINTER(Grid)]),
+        "abx_sim_create": (i32, [vp, C.POINTER(Grid), C.POINTER(RunParams), i32, i32, vp, C.POINTER(vp)]),
+        "abx_sim_info_get": (i32, [vp, C.POINTER(SimInfo)]),
+        "abx_sim_steps": (i32, [vp, i32, i32]),
+        "abx_sim_probe": (i32, [vp, i32, vp]),
+        "abx_sim_read_acc": (i32, [vp, i32, d, vp]),
+        "abx_sim_read_acc_f64": (i32, [vp, i32, d, vp]),
+        "abx_sim_read_f": (i32, [vp, i32, vp]),
+        "abx_sim_write_f": (i32, [vp, i32, vp]),
+        "abx_sim_init_eq": (i32, [vp, vp]),
+        "abx_sim_clear_acc": (i32, [vp]),
+        "abx_sim_set_ns": (i32, [vp, vp]),
+        "abx_sim_sync": (i32, [vp]),
+        "abx_sim_set_launch": (i32, [vp, i32]),
+        "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
+        "abx_sim_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_functions() -> list:
+    """Names of the functions include/airbox_hip.h declares."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(abx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise AbxError(load().abx_last_error().decode("utf-8"))
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Device:
+    """A gfx950 GPU (replaces gpuDevice, engine.js:479-491)."""
+
+    def __init__(self, ordinal: int = 0):
+        L = load()
+        self.h = C.c_void_p()
+        name = C.create_string_buffer(256)
+        _check(L.abx_device_open(ordinal, C.byref(self.h), name, 256))
+        self.name = name.value.decode()
+
+    def close(self):
+        if self.h:
+            load().abx_device_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_DEV = {}
+
+
+def device(ordinal: int = 0) -> Device:
+    if ordinal not in _DEV:
+        _DEV[ordinal] = Device(ordinal)
+    return _DEV[ordinal]
+
+
+def parse_stl(data: bytes) -> np.ndarray:
+    """parseSTL (engine.js:15-28)."""
+    L = load()
+    n = C.c_int64()
+    _check(L.abx_parse_stl(data, len(data), None, 0, C.byref(n)))
+    out = np.empty(n.value, np.float32)
+    _check(L.abx_parse_stl(data, len(data), _p(out), out.size, C.byref(n)))
+    return out
+
+
+def voxelize(dev: Device, T: np.ndarray, origin: Sequence[float], nx: int, ny: int, nz: int, h: float) -> np.ndarray:
+    """voxelize (engine.js:45-93) on the GPU."""
+    T = np.ascontiguousarray(T, np.float32).ravel()
+    o = np.ascontiguousarray(origin, np.float64)
+    vox = np.empty(nx * ny * nz, np.uint8)
+    _check(load().abx_voxelize(dev.h, _p(T), T.size // 9, _p(o), nx, ny, nz, h, _p(vox)))
+    return vox
+
+
+def prep_params(p: dict, max_cells: float = 4.5e6, cy=None, sy=None) -> PrepParams:
+    P = PrepParams()
+    P.dx = float(p["dx"])
+    P.sealAxis, P.sealSign, P.seal = int(p["sealAxis"]), int(p["sealSign"]), float(p["seal"])
+    P.inletAxis, P.inletSign = int(p["inletAxis"]), int(p["inletSign"])
+    P.inletYaw = float(p.get("inletYaw") or 0.0)
+    P.cy = math.nan if cy is None else cy
+    P.sy = math.nan if sy is None else sy
+    if p.get("click") is not None:
+        P.has_click = 1
+        for a in range(3):
+            P.click[a] = float(p["click"][a])
+    P.pleat = float(p["pleat"])
+    o = p.get("outlet")
+    if o and o.get("enabled"):
+        P.has_outlet = 1
+        for a in range(3):
+            P.out_center[a] = float(o["center"][a])
+        P.out_diameter, P.out_height = float(o["diameter"]), float(o["height"])
+    P.max_cells = max_cells
+    return P
+
+
+class Geometry:
+    """Prepared geometry (prepare, engine.js:110-234)."""
+
+    def __init__(self, dev: Device, tris: np.ndarray, p: dict, max_cells: float = 4.5e6, cy=None, sy=None):
+        L = load()
+        tris = np.ascontiguousarray(tris, np.float32).ravel()
+        P = prep_params(p, max_cells, cy, sy)
+        self.h = C.c_void_p()
+        _check(L.abx_prepare(dev.h, _p(tris), tris.size, C.byref(P), C.byref(self.h)))
+        I = GeomInfo()
+        _check(L.abx_geom_info_get(self.h, C.byref(I)))
+        self.info = I
+        for k, _ in GeomInfo._fields_:
+            v = getattr(I, k)
+            setattr(self, k, list(v) if isinstance(v, C.Array) else v)
+        self._cache = {}
+
+    def _read(self, what: int, n: int, dtype) -> np.ndarray:
+        out = np.empty(n, dtype)
+        _check(load().abx_geom_read(self.h, what, _p(out), n))
+        return out
+
+    def array(self, name: str) -> np.ndarray:
+        if name in self._cache:
+            return self._cache[name]
+        N, I = self.N, self.info
+        spec = {"solid": (G_SOLID, N, np.uint8), "fluid": (G_FLUID, N, np.uint8),
+                "opening": (G_OPENING, I.nx * I.ny, np.uint8), "outlet": (G_OUTLET, N, np.uint8),
+                "xc": (G_XC, I.nx, np.float32), "dc": (G_DC, I.ny, np.float32), "zc": (G_ZC, I.nz, np.float32),
+                "openPts": (G_OPENPTS, 3 * I.nOpenPts, np.float64),
+                "filterPts": (G_FILTERPTS, 3 * I.nFilterPts, np.float64),
+                "outletPts": (G_OUTLETPTS, 3 * I.nOutletPts, np.float64),
+                "fine": (G_FINE, 27 * N, np.uint8)}[name]
+        a = self._read(*spec)
+        if name != "fine":
+            self._cache[name] = a
+        return a
+
+    def grid(self) -> "GridArrays":
+        g = Grid()
+        _check(load().abx_geom_grid(self.h, C.byref(g)))
+        return GridArrays.from_struct(g, self)
+
+    def free(self):
+        if self.h:
+            load().abx_geom_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GridArrays:
+    """The lattice a simulation runs on (abx_grid) from plain arrays."""
+
+    def __init__(self, nx, ny, nz, fluid, opening, outlet, dc, top, jm, pleat, owner=None):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.fluid = np.ascontiguousarray(fluid, np.uint8)
+        self.opening = np.ascontiguousarray(opening, np.uint8)
+        self.outlet = np.ascontiguousarray(outlet, np.uint8)
+        self.dc = np.ascontiguousarray(dc, np.float32)
+        self.top, self.jm, self.pleat = int(top), int(jm), float(pleat)
+        self.owner = owner
+
+    @classmethod
+    def from_struct(cls, g: Grid, owner):
+        N = g.nx * g.ny * g.nz
+        arr = lambda p, n, t: np.ctypeslib.as_array(C.cast(p, C.POINTER(t)), shape=(n,))
+        return cls(g.nx, g.ny, g.nz, arr(g.fluid, N, C.c_uint8), arr(g.opening, g.nx * g.ny, C.c_uint8),
+                   arr(g.outlet, N, C.c_uint8), arr(g.dc, g.ny, C.c_float), g.top, g.jm, g.pleat, owner)
+
+    def struct(self) -> Grid:
+        g = Grid()
+        g.nx, g.ny, g.nz = self.nx, self.ny, self.nz
+        g.fluid, g.opening, g.outlet, g.dc = (self.fluid.ctypes.data, self.opening.ctypes.data,
+                                              self.outlet.ctypes.data, self.dc.ctypes.data)
+        g.top, g.jm, g.pleat = self.top, self.jm, self.pleat
+        return g
+
+
+class Sim:
+    """setupSim + GpuSim (engine.js:239-264, 493-594) on the GPU."""
+
+    def __init__(self, dev: Device, grid: GridArrays, K=3.0, aniso=10.0, u=0.06, tau=0.51, precision="f32",
+                 B: int = 1, ns: Optional[np.ndarray] = None):
+        L = load()
+        self.prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.B = B
+        rp = RunParams(K, aniso, u, tau)
+        gs = grid.struct()
+        self.grid = grid
+        self.h = C.c_void_p()
+        nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
+        _check(L.abx_sim_create(dev.h, C.byref(gs), C.byref(rp), self.prec, B, _p(nsa), C.byref(self.h)))
+        I = SimInfo()
+        _check(L.abx_sim_info_get(self.h, C.byref(I)))
+        self.info = I
+        self.NF = I.NF
+        self.sdtype = np.float64 if self.prec == F64 else np.float32
+
+    def steps(self, n: int, doacc: bool = False) -> None:
+        _check(load().abx_sim_steps(self.h, n, 1 if doacc else 0))
+
+    def time_steps(self, n: int, doacc: bool = False) -> float:
+        ms = C.c_double()
+        _check(load().abx_sim_time_steps(self.h, n, 1 if doacc else 0, C.byref(ms)))
+        return ms.value
+
+    def probe(self, step: int = 0):
+        out = np.zeros(3)
+        _check(load().abx_sim_probe(self.h, step, _p(out)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def read_acc(self, divisor: float = 1.0, b: int = 0, f64: bool = False) -> np.ndarray:
+        if f64:
+            out = np.empty(4 * self.NF, np.float64)
+            _check(load().abx_sim_read_acc_f64(self.h, b, divisor, _p(out)))
+        else:
+            out = np.empty(4 * self.NF, np.float32)
+            _check(load().abx_sim_read_acc(self.h, b, divisor, _p(out)))
+        return out
+
+    def read_f(self, b: int = 0) -> np.ndarray:
+        out = np.empty(19 * self.NF, self.sdtype)
+        _check(load().abx_sim_read_f(self.h, b, _p(out)))
+        return out
+
+    def write_f(self, f: np.ndarray, b: int = -1) -> None:
+        f = np.ascontiguousarray(f, self.sdtype)
+        assert f.size == 19 * self.NF
+        _check(load().abx_sim_write_f(self.h, b, _p(f)))
+
+    def init_eq(self, rho_u: np.ndarray) -> None:
+        r = np.ascontiguousarray(rho_u, np.float32)
+        assert r.size == 4 * self.NF
+        _check(load().abx_sim_init_eq(self.h, _p(r)))
+
+    def clear_acc(self) -> None:
+        _check(load().abx_sim_clear_acc(self.h))
+
+    def set_ns(self, ns: Optional[np.ndarray]) -> None:
+        nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
+        _check(load().abx_sim_set_ns(self.h, _p(nsa)))
+
+    def set_launch(self, xcd_map: int) -> None:
+        _check(load().abx_sim_set_launch(self.h, xcd_map))
+
+    def sync(self) -> None:
+        _check(load().abx_sim_sync(self.h))
+
+    def destroy(self):
+        if self.h:
+            load().abx_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def gpu_available() -> bool:
+    """True when libairbox_hip.so loads and a gfx950 GPU opens."""
+    try:
+        device(0)
+        return True
+    except Exception:
+        return False

@@ -1,0 +1,813 @@
+// Geometry preparation: the reference's prepare(p) (engine.js:110-234) with
+// its voxelizer (engine.js:45-93) and 27-subcell vote (engine.js:127-135)
+// on the GPU, and the order-dependent steps (opening choice by BFS order,
+// engine.js:141-171) on the host.
+//
+// Voxelizer.  The reference loops triangle by triangle over the columns of
+// its bounding box, tests the column centre with f64 barycentrics, stores the
+// hit depth as float32, then sorts each column and fills between pairs.  On
+// the GPU the (triangle, column) pairs are enumerated flat (a prefix sum of
+// per-triangle column counts, a binary search per pair), counted and
+// scattered into per-column hit lists with atomics, and every column is then
+// rank-sorted and filled by one wavefront.  The order in which hits land in a
+// column does not matter: the rank sort orders equal values by slot and the
+// fill only reads values.  Arithmetic is IEEE f64 without contraction
+// (-ffp-contract=off), so every hit, every bound and every filled voxel is the
+// reference's bit for bit.
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "abx_common.hpp"
+
+namespace abx {
+
+constexpr double JX = 1.37e-4, JY = 2.71e-4;  // ray jitter (engine.js:47)
+
+struct TriCols {
+  int i0, j0, ni, nj;
+};
+
+__device__ __forceinline__ double dmin3(double a, double b, double c) {
+  // Math.min(a, b, c) for finite arguments
+  double m = a < b ? a : b;
+  return m < c ? m : c;
+}
+__device__ __forceinline__ double dmax3(double a, double b, double c) {
+  double m = a > b ? a : b;
+  return m > c ? m : c;
+}
+
+struct TriPts {
+  double ax, ay, az, bx, by, bz, cx, cy, cz, den;
+};
+
+__device__ __forceinline__ TriPts tri_pts(const float* T, int64_t t, const double* o) {
+  const float* p = T + 9 * t;
+  TriPts r;
+  r.ax = (double)p[0] - o[0] - JX;
+  r.ay = (double)p[1] - o[1] - JY;
+  r.az = (double)p[2] - o[2];
+  r.bx = (double)p[3] - o[0] - JX;
+  r.by = (double)p[4] - o[1] - JY;
+  r.bz = (double)p[5] - o[2];
+  r.cx = (double)p[6] - o[0] - JX;
+  r.cy = (double)p[7] - o[1] - JY;
+  r.cz = (double)p[8] - o[2];
+  r.den = (r.by - r.cy) * (r.ax - r.cx) + (r.cx - r.bx) * (r.ay - r.cy);
+  return r;
+}
+
+struct VoxArgs {
+  const float* T;
+  int64_t ntri;
+  double o[3];
+  int nx, ny, nz;
+  double h;
+};
+
+// Column range of each triangle's bounding box (engine.js:56-59).
+__global__ void k_tri_cols(VoxArgs a, TriCols* tc, int64_t* npairs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ntri) return;
+  const TriPts p = tri_pts(a.T, t, a.o);
+  TriCols r = {0, 0, 0, 0};
+  if (!(fabs(p.den) < 1e-12)) {
+    const double i0 = fmax(0.0, floor(dmin3(p.ax, p.bx, p.cx) / a.h - 0.5));
+    const double i1 = fmin((double)(a.nx - 1), ceil(dmax3(p.ax, p.bx, p.cx) / a.h - 0.5));
+    const double j0 = fmax(0.0, floor(dmin3(p.ay, p.by, p.cy) / a.h - 0.5));
+    const double j1 = fmin((double)(a.ny - 1), ceil(dmax3(p.ay, p.by, p.cy) / a.h - 0.5));
+    if (i0 <= i1 && j0 <= j1) {
+      r.i0 = (int)i0;
+      r.j0 = (int)j0;
+      r.ni = (int)(i1 - i0) + 1;
+      r.nj = (int)(j1 - j0) + 1;
+    }
+  }
+  tc[t] = r;
+  npairs[t] = (int64_t)r.ni * r.nj;
+}
+
+// One (triangle, column) pair per thread: barycentric test at the column
+// centre (engine.js:60-73).  Pass 0 counts hits per column, pass 1 stores
+// the float32 hit depth into the column's slot range.
+template <int PASS>
+__global__ void k_tri_pairs(VoxArgs a, const TriCols* tc, const int64_t* pstart, int64_t total, int* cnt, int* pos,
+                            float* hits) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= total) return;
+  // last triangle whose first pair is <= n
+  int64_t lo = 0, hi = a.ntri - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (pstart[mid] <= n)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int64_t t = lo;
+  const TriCols c = tc[t];
+  const int64_t r = n - pstart[t];
+  const int i = c.i0 + (int)(r / c.nj), j = c.j0 + (int)(r % c.nj);
+  const TriPts p = tri_pts(a.T, t, a.o);
+  const double px = ((double)i + 0.5) * a.h, py = ((double)j + 0.5) * a.h;
+  const double l1 = ((p.by - p.cy) * (px - p.cx) + (p.cx - p.bx) * (py - p.cy)) / p.den;
+  if (l1 < 0) return;
+  const double l2 = ((p.cy - p.ay) * (px - p.cx) + (p.ax - p.cx) * (py - p.cy)) / p.den;
+  if (l2 < 0) return;
+  const double l3 = 1 - l1 - l2;
+  if (l3 < 0) return;
+  const int col = i * a.ny + j;
+  if (PASS == 0) {
+    atomicAdd(&cnt[col], 1);
+  } else {
+    const int slot = atomicAdd(&pos[col], 1);
+    hits[slot] = (float)(l1 * p.az + l2 * p.bz + l3 * p.cz);
+  }
+}
+
+// One wavefront per column: rank-sort the column's hits, then set the voxels
+// between consecutive pairs (engine.js:82-91).  Every byte of the column is
+// written, so vox needs no clearing.
+__global__ __launch_bounds__(256) void k_fill_cols(const int* off, int ncol, const float* hits, float* sorted,
+                                                   int nz, double h, uint8_t* vox) {
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const bool live = col < ncol;
+  const int a = live ? off[col] : 0, b = live ? off[col + 1] : 0, n = b - a;
+  if (n >= 2) {
+    for (int e = lane; e < n; e += 64) {
+      const float x = hits[a + e];
+      int rank = 0;
+      for (int m = 0; m < n; ++m) {
+        const float y = hits[a + m];
+        rank += (y < x) || (y == x && m < e);
+      }
+      sorted[a + rank] = x;
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (!live) return;
+  uint8_t* v = vox + (int64_t)col * nz;
+  for (int k = lane; k < nz; k += 64) {
+    uint8_t inside = 0;
+    for (int m = 0; m + 1 < n; m += 2) {
+      const double k0 = fmax(0.0, ceil((double)sorted[a + m] / h - 0.5));
+      const double k1 = fmin((double)(nz - 1), ceil((double)sorted[a + m + 1] / h - 0.5) - 1);
+      if ((double)k >= k0 && (double)k <= k1) inside = 1;
+    }
+    v[k] = inside;
+  }
+}
+
+// solid = (sum of the 3x3x3 fine voxels) >= 7 (engine.js:127-135).
+__global__ void k_vote(const uint8_t* fine, int nx, int ny, int nz, uint8_t* solid) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t N = (int64_t)nx * ny * nz;
+  if (c >= N) return;
+  const int k = (int)(c % nz);
+  const int64_t ij = c / nz;
+  const int j = (int)(ij % ny), i = (int)(ij / ny);
+  const int64_t nyf = 3 * (int64_t)ny, nzf = 3 * (int64_t)nz;
+  int s = 0;
+  for (int da = 0; da < 3; ++da)
+    for (int db = 0; db < 3; ++db) {
+      const int64_t base = ((int64_t)(3 * i + da) * nyf + (3 * j + db)) * nzf + 3 * k;
+      s += fine[base] + fine[base + 1] + fine[base + 2];
+    }
+  solid[c] = s >= 7 ? 1 : 0;
+}
+
+// empty[i*nyf + j] = !fine[i][j][kf] (engine.js:142-144).
+__global__ void k_slice(const uint8_t* fine, int nxf, int nyf, int nzf, int kf, uint8_t* empty) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)nxf * nyf) return;
+  const bool in = kf >= 0 && kf < nzf;
+  empty[c] = (in && fine[c * nzf + kf]) ? 0 : 1;
+}
+
+inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// RAII device buffer for the preparation stages.
+struct DBuf {
+  void* p = nullptr;
+  ~DBuf() {
+    if (p) hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+      set_errorf("hipMalloc of %zu bytes failed in geometry preparation", bytes);
+      p = nullptr;
+      return -1;
+    }
+    return 0;
+  }
+  template <typename T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// GPU voxelization of T into the device byte grid vox (nx*ny*nz).
+int voxelize_device(const float* T_dev, int64_t ntri, const double origin[3], int nx, int ny, int nz, double h,
+                    uint8_t* vox_dev, hipStream_t st) {
+  const int64_t ncol = (int64_t)nx * ny;
+  if (ncol >= INT32_MAX || (int64_t)nz * ncol > ((int64_t)1 << 40)) {
+    set_error("voxelize: fine grid too large");
+    return -1;
+  }
+  VoxArgs a;
+  a.T = T_dev;
+  a.ntri = ntri;
+  a.o[0] = origin[0];
+  a.o[1] = origin[1];
+  a.o[2] = origin[2];
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.h = h;
+  DBuf tc, np, ps, cnt, pos, hits, sorted;
+  if (tc.alloc(sizeof(TriCols) * (size_t)std::max<int64_t>(ntri, 1)) ||
+      np.alloc(8 * (size_t)std::max<int64_t>(ntri, 1)) || ps.alloc(8 * (size_t)std::max<int64_t>(ntri, 1)) ||
+      cnt.alloc(4 * (size_t)(ncol + 1)) || pos.alloc(4 * (size_t)(ncol + 1)))
+    return -1;
+  std::vector<int64_t> pairs((size_t)std::max<int64_t>(ntri, 1), 0);
+  int64_t total = 0;
+  if (ntri > 0) {
+    hipLaunchKernelGGL(k_tri_cols, dim3(nb(ntri, 256)), dim3(256), 0, st, a, tc.as<TriCols>(), np.as<int64_t>());
+    ABX_HIP(hipGetLastError());
+    ABX_HIP(hipMemcpyAsync(pairs.data(), np.p, 8 * (size_t)ntri, hipMemcpyDeviceToHost, st));
+    ABX_HIP(hipStreamSynchronize(st));
+    for (int64_t t = 0; t < ntri; ++t) {
+      const int64_t c = pairs[(size_t)t];
+      pairs[(size_t)t] = total;
+      total += c;
+    }
+    ABX_HIP(hipMemcpyAsync(ps.p, pairs.data(), 8 * (size_t)ntri, hipMemcpyHostToDevice, st));
+  }
+  ABX_HIP(hipMemsetAsync(cnt.p, 0, 4 * (size_t)(ncol + 1), st));
+  if (total > 0) {
+    hipLaunchKernelGGL(k_tri_pairs<0>, dim3(nb(total, 256)), dim3(256), 0, st, a, tc.as<TriCols>(),
+                       ps.as<int64_t>(), total, cnt.as<int>(), (int*)nullptr, (float*)nullptr);
+    ABX_HIP(hipGetLastError());
+  }
+  std::vector<int> off((size_t)(ncol + 1));
+  ABX_HIP(hipMemcpyAsync(off.data(), cnt.p, 4 * (size_t)ncol, hipMemcpyDeviceToHost, st));
+  ABX_HIP(hipStreamSynchronize(st));
+  int64_t run = 0;
+  for (int64_t c = 0; c < ncol; ++c) {
+    const int64_t v = off[(size_t)c];
+    off[(size_t)c] = (int)run;
+    run += v;
+    if (run > INT32_MAX) {
+      set_error("voxelize: too many ray hits");
+      return -1;
+    }
+  }
+  off[(size_t)ncol] = (int)run;
+  ABX_HIP(hipMemcpyAsync(cnt.p, off.data(), 4 * (size_t)(ncol + 1), hipMemcpyHostToDevice, st));
+  ABX_HIP(hipMemcpyAsync(pos.p, off.data(), 4 * (size_t)(ncol + 1), hipMemcpyHostToDevice, st));
+  if (hits.alloc(4 * (size_t)std::max<int64_t>(run, 1)) || sorted.alloc(4 * (size_t)std::max<int64_t>(run, 1)))
+    return -1;
+  if (total > 0) {
+    hipLaunchKernelGGL(k_tri_pairs<1>, dim3(nb(total, 256)), dim3(256), 0, st, a, tc.as<TriCols>(),
+                       ps.as<int64_t>(), total, (int*)nullptr, pos.as<int>(), hits.as<float>());
+    ABX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_fill_cols, dim3(nb(ncol, 4)), dim3(256), 0, st, cnt.as<int>(), (int)ncol, hits.as<float>(),
+                     sorted.as<float>(), nz, h, vox_dev);
+  ABX_HIP(hipGetLastError());
+  ABX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ---- host side of prepare --------------------------------------------------
+struct Frame {
+  int dAx, iAx, oAx, sd, si;
+  double seal, cy, sy, yaw;
+  void to_local(const double q[3], double L[3]) const {
+    const double a = q[oAx], b = -si * q[iAx];
+    L[0] = cy * a + sy * b;
+    L[1] = -sd * (q[dAx] - seal);
+    L[2] = -sy * a + cy * b;
+  }
+  void to_model(double X, double D, double Z, double q[3]) const {
+    q[oAx] = cy * X - sy * Z;
+    q[dAx] = seal - sd * D;
+    q[iAx] = -si * (sy * X + cy * Z);
+  }
+};
+
+// Number#toString for the cell sizes the leak message quotes.
+std::string js_num(double v) {
+  char buf[64];
+  for (int prec = 1; prec <= 17; ++prec) {
+    snprintf(buf, sizeof buf, "%.*g", prec, v);
+    if (strtod(buf, nullptr) == v) break;
+  }
+  return buf;
+}
+
+// Number#toFixed(1): the nearest one-decimal value, ties away from zero for
+// positive x.
+std::string js_fixed1(double x) {
+  const double y = x * 10, fl = floor(y);
+  const double n = (y - fl == 0.5) ? fl + 1 : nearbyint(y);
+  char buf[64];
+  snprintf(buf, sizeof buf, "%.1f", n / 10);
+  return buf;
+}
+
+int first_ge(const std::vector<float>& a, double v, bool or_last) {
+  for (size_t i = 0; i < a.size(); ++i)
+    if ((double)a[i] >= v) return (int)i;
+  return or_last ? (int)a.size() - 1 : -1;
+}
+
+}  // namespace abx
+
+using namespace abx;
+
+struct abx_geom {
+  abx_geom_info info;
+  std::vector<uint8_t> solid, fluid, opening, outlet;
+  std::vector<float> xc, dc, zc;
+  std::vector<double> openPts, filterPts, outletPts;
+  uint8_t* fine_dev = nullptr;
+  int64_t nfine = 0;
+  int ordinal = 0;
+  ~abx_geom() {
+    if (fine_dev) hipFree(fine_dev);
+  }
+};
+
+extern "C" int abx_voxelize(abx_dev* dev, const float* T, int64_t ntri, const double origin[3], int nx, int ny,
+                            int nz, double h, uint8_t* vox_out) {
+  if (!dev || (!T && ntri > 0) || !origin || !vox_out || nx < 1 || ny < 1 || nz < 1 || ntri < 0) {
+    set_error("abx_voxelize: bad argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(dev->ordinal));
+  hipStream_t st;
+  ABX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  DBuf dT, dv;
+  const size_t nvox = (size_t)nx * ny * nz;
+  int rc = dT.alloc(36 * (size_t)std::max<int64_t>(ntri, 1)) || dv.alloc(nvox);
+  if (!rc && ntri > 0 && hipMemcpyAsync(dT.p, T, 36 * (size_t)ntri, hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("abx_voxelize: upload failed");
+    rc = -1;
+  }
+  if (!rc) rc = voxelize_device(dT.as<float>(), ntri, origin, nx, ny, nz, h, dv.as<uint8_t>(), st);
+  if (!rc && hipMemcpyAsync(vox_out, dv.p, nvox, hipMemcpyDeviceToHost, st) != hipSuccess) {
+    set_error("abx_voxelize: download failed");
+    rc = -1;
+  }
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) {
+    set_error("abx_voxelize: kernel failure");
+    rc = -1;
+  }
+  hipStreamDestroy(st);
+  return rc ? -1 : 0;
+}
+
+extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const abx_prep_params* p,
+                           abx_geom** out) {
+  if (!dev || !p || !out || (!T0 && nfloat > 0) || nfloat < 0) {
+    set_error("abx_prepare: bad argument");
+    return -1;
+  }
+  *out = nullptr;
+  Frame fr;
+  fr.dAx = p->sealAxis;
+  fr.iAx = p->inletAxis;
+  if (fr.dAx < 0 || fr.dAx > 2 || fr.iAx < 0 || fr.iAx > 2) {
+    set_error("abx_prepare: axis out of range");
+    return -1;
+  }
+  if (fr.dAx == fr.iAx) {
+    set_error("Einlassseite und Filterebene liegen auf derselben Achse.");
+    return -1;
+  }
+  fr.oAx = 3 - fr.dAx - fr.iAx;
+  fr.sd = p->sealSign;
+  fr.si = p->inletSign;
+  fr.seal = p->seal;
+  fr.yaw = p->inletYaw;
+  const double yr = (std::isnan(p->inletYaw) ? 0.0 : p->inletYaw) * 3.141592653589793 / 180;
+  fr.cy = std::isnan(p->cy) ? cos(yr) : p->cy;
+  fr.sy = std::isnan(p->sy) ? sin(yr) : p->sy;
+  ABX_HIP(hipSetDevice(dev->ordinal));
+
+  // local frame and bounding box (engine.js:112-118)
+  const int64_t ntri = nfloat / 9;
+  std::vector<float> T((size_t)std::max<int64_t>(nfloat, 1));
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t v = 0; v < ntri * 3; ++v) {
+    const double q[3] = {T0[3 * v], T0[3 * v + 1], T0[3 * v + 2]};
+    double L[3];
+    fr.to_local(q, L);
+    for (int a = 0; a < 3; ++a) {
+      T[(size_t)(3 * v + a)] = (float)L[a];
+      if (L[a] < mn[a]) mn[a] = L[a];
+      if (L[a] > mx[a]) mx[a] = L[a];
+    }
+  }
+  // grid (engine.js:119-125)
+  const double h = p->dx, hf = h / 3;
+  const bool has_out = p->has_outlet != 0;
+  double origin[3] = {mn[0] - h, (has_out ? std::min(mn[1], -p->out_height) : mn[1]) - h, mn[2] - 10 * h};
+  const double ndx = ceil((mx[0] - origin[0] + h) / h), ndy = ceil((mx[1] - origin[1] + h) / h),
+               ndz = ceil((mx[2] - origin[2] + h) / h);
+  if (!(ndx >= 1 && ndy >= 1 && ndz >= 1) || ndx * ndy * ndz > 4.0e9) {
+    set_error("abx_prepare: degenerate or unbounded grid");
+    return -1;
+  }
+  const int nx = (int)ndx, ny = (int)ndy, nz = (int)ndz;
+  const int64_t N = (int64_t)nx * ny * nz;
+  if (p->max_cells > 0 && (double)N > p->max_cells) {
+    set_errorf("Gitter zu groß (%s Mio. Zellen) – Zellgröße erhöhen.", js_fixed1((double)N / 1e6).c_str());
+    return -1;
+  }
+  const int nxf = 3 * nx, nyf = 3 * ny, nzf = 3 * nz;
+  abx_geom* g = new abx_geom();
+  g->ordinal = dev->ordinal;
+  abx_geom_info& I = g->info;
+  memset(&I, 0, sizeof I);
+  I.nx = nx;
+  I.ny = ny;
+  I.nz = nz;
+  I.N = N;
+  I.h = h;
+  for (int a = 0; a < 3; ++a) {
+    I.origin[a] = origin[a];
+    I.mn[a] = mn[a];
+    I.mx[a] = mx[a];
+  }
+  I.dAx = fr.dAx;
+  I.iAx = fr.iAx;
+  I.oAx = fr.oAx;
+  I.sd = fr.sd;
+  I.si = fr.si;
+  I.seal = fr.seal;
+  I.yaw = std::isnan(p->inletYaw) ? 0.0 : p->inletYaw;
+  I.cy = fr.cy;
+  I.sy = fr.sy;
+  I.pleat = p->pleat;
+  I.has_outlet = has_out ? 1 : 0;
+  auto fail = [&](const std::string& m) {
+    set_error(m);
+    delete g;
+    return -1;
+  };
+
+  // voxelize and vote on the GPU (engine.js:126-135), slice kf for the openings
+  hipStream_t st;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail("abx_prepare: stream");
+  const int kf = (int)floor((mn[2] + 1.5 - origin[2]) / hf);
+  std::vector<uint8_t> empty((size_t)nxf * nyf);
+  g->solid.assign((size_t)N, 0);
+  {
+    DBuf dT, dsolid, dempty;
+    g->nfine = (int64_t)nxf * nyf * nzf;
+    int rc = dT.alloc(36 * (size_t)std::max<int64_t>(ntri, 1)) || dsolid.alloc((size_t)N) ||
+             dempty.alloc(empty.size());
+    if (!rc && hipMalloc((void**)&g->fine_dev, (size_t)g->nfine) != hipSuccess) {
+      g->fine_dev = nullptr;
+      set_errorf("hipMalloc of %lld bytes (fine voxel grid) failed", (long long)g->nfine);
+      rc = -1;
+    }
+    if (!rc && ntri > 0 && hipMemcpyAsync(dT.p, T.data(), 36 * (size_t)ntri, hipMemcpyHostToDevice, st)) {
+      set_error("abx_prepare: triangle upload failed");
+      rc = -1;
+    }
+    if (!rc) rc = voxelize_device(dT.as<float>(), ntri, origin, nxf, nyf, nzf, hf, g->fine_dev, st);
+    if (!rc) {
+      hipLaunchKernelGGL(k_vote, dim3(nb(N, 256)), dim3(256), 0, st, g->fine_dev, nx, ny, nz, dsolid.as<uint8_t>());
+      hipLaunchKernelGGL(k_slice, dim3(nb((int64_t)empty.size(), 256)), dim3(256), 0, st, g->fine_dev, nxf, nyf,
+                         nzf, kf, dempty.as<uint8_t>());
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(g->solid.data(), dsolid.p, (size_t)N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipMemcpyAsync(empty.data(), dempty.p, empty.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        set_error("abx_prepare: voxel vote failed on the GPU");
+        rc = -1;
+      }
+    }
+    hipStreamDestroy(st);
+    if (rc) {
+      std::string m = last_error();
+      return fail(m);
+    }
+  }
+  std::vector<uint8_t>& solid = g->solid;
+  g->xc.resize((size_t)nx);
+  g->dc.resize((size_t)ny);
+  g->zc.resize((size_t)nz);
+  for (int i = 0; i < nx; ++i) g->xc[(size_t)i] = (float)(origin[0] + (i + 0.5) * h);
+  for (int j = 0; j < ny; ++j) g->dc[(size_t)j] = (float)(origin[1] + (j + 0.5) * h);
+  for (int k = 0; k < nz; ++k) g->zc[(size_t)k] = (float)(origin[2] + (k + 0.5) * h);
+  const std::vector<float>&xc = g->xc, &dc = g->dc, &zc = g->zc;
+
+  // openings: holes of the first material layer not connected to the border
+  // (engine.js:141-154)
+  const int64_t n2 = (int64_t)nxf * nyf;
+  std::vector<uint8_t> outside((size_t)n2, 0);
+  std::vector<int32_t> q((size_t)n2);
+  int64_t qh = 0, qt = 0;
+  auto push = [&](int64_t c) {
+    if (empty[(size_t)c] && !outside[(size_t)c]) {
+      outside[(size_t)c] = 1;
+      q[(size_t)qt++] = (int32_t)c;
+    }
+  };
+  for (int i = 0; i < nxf; ++i) {
+    push((int64_t)i * nyf);
+    push((int64_t)i * nyf + nyf - 1);
+  }
+  for (int j = 0; j < nyf; ++j) {
+    push(j);
+    push((int64_t)(nxf - 1) * nyf + j);
+  }
+  while (qh < qt) {
+    const int64_t c = q[(size_t)qh++];
+    const int i = (int)(c / nyf), j = (int)(c % nyf);
+    if (i > 0) push(c - nyf);
+    if (i < nxf - 1) push(c + nyf);
+    if (j > 0) push(c - 1);
+    if (j < nyf - 1) push(c + 1);
+  }
+  std::vector<uint8_t> holes((size_t)n2);
+  for (int64_t c = 0; c < n2; ++c)
+    holes[(size_t)c] = (empty[(size_t)c] && !outside[(size_t)c] && (origin[1] + ((double)(c % nyf) + 0.5) * hf > 0));
+  // 4-connected components in scan order, cells in BFS order (engine.js:95-108)
+  std::vector<int32_t> lab((size_t)n2, -1);
+  std::vector<std::vector<int32_t>> comps;
+  for (int64_t s0 = 0; s0 < n2; ++s0) {
+    if (!holes[(size_t)s0] || lab[(size_t)s0] >= 0) continue;
+    const int32_t id = (int32_t)comps.size();
+    std::vector<int32_t> cells;
+    cells.push_back((int32_t)s0);
+    lab[(size_t)s0] = id;
+    for (size_t h0 = 0; h0 < cells.size(); ++h0) {
+      const int32_t c = cells[h0];
+      const int i = c / nyf, j = c % nyf;
+      const int32_t nbs[4] = {i > 0 ? c - nyf : -1, i < nxf - 1 ? c + nyf : -1, j > 0 ? c - 1 : -1,
+                              j < nyf - 1 ? c + 1 : -1};
+      for (int32_t d : nbs)
+        if (d >= 0 && holes[(size_t)d] && lab[(size_t)d] < 0) {
+          lab[(size_t)d] = id;
+          cells.push_back(d);
+        }
+    }
+    if ((double)cells.size() * hf * hf > 25) comps.push_back(std::move(cells));
+  }
+  I.nHoles = (int)comps.size();
+  if (comps.empty()) return fail("Auf der gewählten Einlassseite wurde keine Öffnung gefunden.");
+  size_t best = 0;
+  if (p->has_click) {
+    double L[3];
+    fr.to_local(p->click, L);
+    double bd = INFINITY;
+    for (size_t ci = 0; ci < comps.size(); ++ci)
+      for (size_t s = 0; s < comps[ci].size(); s += 3) {
+        const int32_t c = comps[ci][s];
+        const double x = origin[0] + ((double)(c / nyf) + 0.5) * hf, y = origin[1] + ((double)(c % nyf) + 0.5) * hf;
+        const double ex = x - L[0], ey = y - L[1];
+        const double d = ex * ex + ey * ey;
+        if (d < bd) {
+          bd = d;
+          best = ci;
+        }
+      }
+  } else {
+    for (size_t ci = 1; ci < comps.size(); ++ci)
+      if (comps[ci].size() > comps[best].size()) best = ci;
+  }
+  std::vector<uint8_t> sel((size_t)nx * ny, 0);
+  for (int32_t c : comps[best]) sel[(size_t)((c / nyf) / 3) * ny + (size_t)((c % nyf) / 3)]++;
+  g->opening.assign((size_t)nx * ny, 0);
+  int nOpen = 0;
+  for (size_t c = 0; c < sel.size(); ++c)
+    if (sel[c] >= 5) {
+      g->opening[c] = 1;
+      nOpen++;
+    }
+  I.nOpen = nOpen;
+  if (nOpen < 8) return fail("Einlassöffnung ist zu klein für die gewählte Zellgröße.");
+  const std::vector<uint8_t>& opening = g->opening;
+
+  // inlet duct, floor, flood fill of the air (engine.js:173-189)
+  auto IDX = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
+  const int kw0 = first_ge(zc, mn[2], false);
+  I.kw0 = kw0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < kw0; ++k) solid[(size_t)IDX(i, j, k)] = opening[(size_t)i * ny + j] ? 0 : 1;
+  for (int j = 0; j < ny; ++j)
+    if ((double)dc[(size_t)j] > mx[1] - 0.5 * h)
+      for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) solid[(size_t)IDX(i, j, k)] = 1;
+  const int top = first_ge(dc, 0.0, false);
+  if (top < 0) return fail("Dichtebene liegt außerhalb des Modells.");
+  I.top = top;
+  g->fluid.assign((size_t)N, 0);
+  std::vector<uint8_t>& fluid = g->fluid;
+  std::vector<int32_t> Q((size_t)N);
+  qh = qt = 0;
+  auto visit = [&](int64_t c, int j) {
+    if (!fluid[(size_t)c] && !solid[(size_t)c] && j >= top) {
+      fluid[(size_t)c] = 1;
+      Q[(size_t)qt++] = (int32_t)c;
+    }
+  };
+  for (int64_t c = 0; c < (int64_t)nx * ny; ++c)
+    if (opening[(size_t)c]) visit(c * nz, (int)(c % ny));
+  while (qh < qt) {
+    const int64_t c = Q[(size_t)qh++];
+    const int k = (int)(c % nz);
+    const int64_t ij = c / nz;
+    const int j = (int)(ij % ny), i = (int)(ij / ny);
+    if (i > 0) visit(c - (int64_t)ny * nz, j);
+    if (i < nx - 1) visit(c + (int64_t)ny * nz, j);
+    if (j > 0) visit(c - nz, j - 1);
+    if (j < ny - 1) visit(c + nz, j + 1);
+    if (k > 0) visit(c - 1, j);
+    if (k < nz - 1) visit(c + 1, j);
+  }
+  // leak check (engine.js:191-195)
+  bool leaked = false;
+  for (int i = 0; i < nx && !leaked; ++i)
+    for (int j = top; j < ny && !leaked; ++j)
+      for (int k = kw0 < 0 ? 0 : kw0; k < nz; ++k)
+        if ((i == 0 || i == nx - 1 || k == nz - 1) && fluid[(size_t)IDX(i, j, k)]) {
+          leaked = true;
+          break;
+        }
+  if (leaked)
+    return fail("Bei " + js_num(h) +
+                " mm Zellgröße läuft die Luft durch eine zu dünne Wand nach außen (Wände sollten mindestens so "
+                "dick wie eine Zelle sein). Bitte eine kleinere Zellgröße wählen.");
+  // clean-air side: lid plenum and outlet disc, or the whole top row (engine.js:197-213)
+  g->outlet.assign((size_t)N, 0);
+  int nOut = 0;
+  if (has_out) {
+    const int jpt = first_ge(dc, -p->out_height, true);
+    if (jpt >= top) return fail("Deckelhöhe ist kleiner als eine Zelle.");
+    double L[3];
+    fr.to_local(p->out_center, L);
+    const double rr = p->out_diameter / 2, r2 = rr * rr;
+    for (int i = 0; i < nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        if (!fluid[(size_t)IDX(i, top, k)]) continue;
+        for (int j = jpt; j < top; ++j) {
+          const int64_t c = IDX(i, j, k);
+          fluid[(size_t)c] = 1;
+          solid[(size_t)c] = 0;
+          qt++;
+        }
+        const double ex = (double)xc[(size_t)i] - L[0], ez = (double)zc[(size_t)k] - L[2];
+        if (ex * ex + ez * ez <= r2) {
+          g->outlet[(size_t)IDX(i, jpt, k)] = 1;
+          nOut++;
+          double m[3];
+          fr.to_model(xc[(size_t)i], dc[(size_t)jpt], zc[(size_t)k], m);
+          g->outletPts.insert(g->outletPts.end(), m, m + 3);
+        }
+      }
+    if (nOut < 4) return fail("Der Auslass liegt nicht über der Filterfläche oder ist zu klein.");
+  } else {
+    for (int i = 0; i < nx; ++i)
+      for (int k = 0; k < nz; ++k) {
+        const int64_t c = IDX(i, top, k);
+        if (fluid[(size_t)c]) {
+          g->outlet[(size_t)c] = 1;
+          nOut++;
+        }
+      }
+  }
+  I.nOut = nOut;
+  I.nf = qt;
+  const int jm = first_ge(dc, p->pleat / 2, true);
+  I.jm = jm;
+  int Af = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int k = 0; k < nz; ++k) Af += fluid[(size_t)IDX(i, jm, k)];
+  I.Af = Af;
+  for (int i = 0; i < nx; ++i)
+    for (int k = 0; k < nz; ++k)
+      if (fluid[(size_t)IDX(i, jm, k)]) {
+        double m[3];
+        fr.to_model(xc[(size_t)i], dc[(size_t)jm], zc[(size_t)k], m);
+        g->filterPts.insert(g->filterPts.end(), m, m + 3);
+      }
+  for (int64_t c = 0; c < (int64_t)nx * ny; ++c)
+    if (opening[(size_t)c]) {
+      double m[3];
+      fr.to_model(xc[(size_t)(c / ny)], dc[(size_t)(c % ny)], mn[2], m);
+      g->openPts.insert(g->openPts.end(), m, m + 3);
+    }
+  int leak = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int k = 0; k < nz; ++k) leak += fluid[(size_t)IDX(i, top, k)];
+  I.leak = leak;
+  I.nOpenPts = (int64_t)g->openPts.size() / 3;
+  I.nFilterPts = (int64_t)g->filterPts.size() / 3;
+  I.nOutletPts = (int64_t)g->outletPts.size() / 3;
+  I.volume_l = (double)I.nf * pow(h, 3) / 1e6;
+  I.inlet_cm2 = nOpen * h * h / 100;
+  I.filter_cm2 = Af * h * h / 100;
+  I.outlet_cm2 = leak * h * h / 100;
+  I.outlet_cm2_real = has_out ? nOut * h * h / 100 : 0;
+  I.pleatOk = jm > top;
+  *out = g;
+  return 0;
+}
+
+extern "C" int abx_geom_info_get(const abx_geom* g, abx_geom_info* out) {
+  if (!g || !out) {
+    set_error("abx_geom_info_get: null argument");
+    return -1;
+  }
+  *out = g->info;
+  return 0;
+}
+
+extern "C" int abx_geom_read(const abx_geom* g, int what, void* out, int64_t cap) {
+  if (!g || !out) {
+    set_error("abx_geom_read: null argument");
+    return -1;
+  }
+  auto cp = [&](const void* src, int64_t n, size_t esz) {
+    if (cap < n) {
+      set_errorf("abx_geom_read: capacity %lld < %lld", (long long)cap, (long long)n);
+      return -1;
+    }
+    if (n) memcpy(out, src, (size_t)n * esz);
+    return 0;
+  };
+  switch (what) {
+    case ABX_G_SOLID:
+      return cp(g->solid.data(), (int64_t)g->solid.size(), 1);
+    case ABX_G_FLUID:
+      return cp(g->fluid.data(), (int64_t)g->fluid.size(), 1);
+    case ABX_G_OPENING:
+      return cp(g->opening.data(), (int64_t)g->opening.size(), 1);
+    case ABX_G_OUTLET:
+      return cp(g->outlet.data(), (int64_t)g->outlet.size(), 1);
+    case ABX_G_XC:
+      return cp(g->xc.data(), (int64_t)g->xc.size(), 4);
+    case ABX_G_DC:
+      return cp(g->dc.data(), (int64_t)g->dc.size(), 4);
+    case ABX_G_ZC:
+      return cp(g->zc.data(), (int64_t)g->zc.size(), 4);
+    case ABX_G_OPENPTS:
+      return cp(g->openPts.data(), (int64_t)g->openPts.size(), 8);
+    case ABX_G_FILTERPTS:
+      return cp(g->filterPts.data(), (int64_t)g->filterPts.size(), 8);
+    case ABX_G_OUTLETPTS:
+      return cp(g->outletPts.data(), (int64_t)g->outletPts.size(), 8);
+    case ABX_G_FINE: {
+      if (!g->fine_dev) {
+        set_error("abx_geom_read: no fine voxel grid");
+        return -1;
+      }
+      if (cap < g->nfine) {
+        set_error("abx_geom_read: capacity too small for the fine grid");
+        return -1;
+      }
+      ABX_HIP(hipSetDevice(g->ordinal));
+      ABX_HIP(hipMemcpy(out, g->fine_dev, (size_t)g->nfine, hipMemcpyDeviceToHost));
+      return 0;
+    }
+    default:
+      set_errorf("abx_geom_read: unknown array %d", what);
+      return -1;
+  }
+}
+
+extern "C" int abx_geom_grid(const abx_geom* g, abx_grid* out) {
+  if (!g || !out) {
+    set_error("abx_geom_grid: null argument");
+    return -1;
+  }
+  out->nx = g->info.nx;
+  out->ny = g->info.ny;
+  out->nz = g->info.nz;
+  out->fluid = g->fluid.data();
+  out->opening = g->opening.data();
+  out->outlet = g->outlet.data();
+  out->dc = g->dc.data();
+  out->top = g->info.top;
+  out->jm = g->info.jm;
+  out->pleat = g->info.pleat;
+  return 0;
+}
+
+extern "C" void abx_geom_free(abx_geom* g) { delete g; }

@@ -1,0 +1,166 @@
+"""Throughput of the D3Q19 stream-collide on one MI355X.
+
+Workload: the roofline airbox (300 x 150 x 200 mm, 6 mm walls, 70 mm inlet on
+the Z-min face, seal plane at Y=0 with a 40 mm pleat layer K=3 aniso=10, lid
+outlet 60 mm, 50 mm lid plenum) at dx = 1 mm, seeded internal baffles and an
+oblique inlet (seed 1), FP32, u = 0.06, tau = 0.51.  One step is one fused
+stream-collide pass over every fluid cell.  The timed region is the
+reference's time loop (simulateGPU, engine.js:613-630) over --steps steps:
+chunks of 100 steps, a gamma/mass-ratio probe after each chunk, the last third
+of the steps accumulating the averaged fields, and the final read-back of the
+averaged fields.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  MLUPS counts fluid cells only (engine.js:338).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_UPDATE = {"f32": 152, "f32_f64": 152, "f64": 304}
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def workload(dx: float, seed: int):
+    from airboxcalc_amd import geometry as G
+    spec, _ = G.seeded_variant(seed, "roofline")
+    p = G.prepare_params(spec, dx, outlet=True, click=True)
+    return spec, G.airbox_triangles(spec), p
+
+
+def cpu_baseline(tris, p, run, seconds: float):
+    """The oracle's single-threaded C restatement of cpuStep on the same
+    geometry, timed on this host for a bounded number of steps."""
+    from oracle import oracle as O
+    g = O.prepare(tris, p, max_cells=0)
+    S = O.setup_sim(g, run["K"], run["aniso"], run["u"], run["tau"])
+    f = S.f0()
+    f2 = np.empty_like(f)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        O.cpu_step(S, f, f2)
+        f, f2 = f2, f
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 200:
+            break
+    return dict(value=S.NF * n / el / 1e6, unit="MLUPS", cores=1, kind="port",
+                sample=f"{n} cpuStep steps (f32 storage, f64 arithmetic) of the same {S.NF}-cell lattice, "
+                       f"{el:.1f} s, oracle/engine_oracle.c single thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--dx", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f32_f64", "f64"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
+              "(the single-grid solve runs on one MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from airboxcalc_amd import abx
+    dev = abx.device(0)
+    log("device", dev.name)
+    spec, tris, p = workload(a.dx, a.seed)
+    run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
+    t = time.perf_counter()
+    g = abx.Geometry(dev, tris, p, max_cells=0)
+    log(f"prepare {time.perf_counter() - t:.2f}s grid {g.nx}x{g.ny}x{g.nz} nf={g.nf} ntri={tris.size // 9}")
+    sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
+    NF = sim.NF
+    log(f"sim NF={NF} device bytes {sim.info.device_bytes / 2**30:.2f} GiB")
+
+    sim.steps(a.warmup)
+    sim.sync()
+    sim.clear_acc()
+    sim.sync()
+
+    steps, every = a.steps, a.every
+    avg = steps // 3
+    start_acc = steps - avg
+    ev_ms = {0: 0.0, 1: 0.0}
+    ev_n = {0: 0, 1: 0}
+    hist = []
+    t0 = time.perf_counter()
+    step = nacc = 0
+    while step < steps:
+        n = min(every, steps - step)
+        if step < start_acc < step + n:
+            n = start_acc - step
+        doacc = 1 if step >= start_acc else 0
+        ev_ms[doacc] += sim.time_steps(n, bool(doacc))
+        ev_n[doacc] += n
+        step += n
+        if doacc:
+            nacc += n
+        if step % every == 0 or step == steps:
+            gam, mr, _ = sim.probe(step)
+            hist.append((step, gam, mr))
+    acc = sim.read_acc(max(nacc, 1))
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} steps in {wall:.3f}s; last gamma {hist[-1][1]:.4f} massRatio {hist[-1][2]:.4f}")
+    assert np.isfinite(acc).all()
+
+    bpu = BYTES_PER_UPDATE[a.precision]
+    mlups = NF * steps / wall / 1e6
+    kern_ms = ev_ms[0] / max(ev_n[0], 1)
+    achieved = NF * bpu / (kern_ms * 1e-3) / 1e9
+    kern_acc_ms = ev_ms[1] / max(ev_n[1], 1)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"pmc_stream_collide_dx{a.dx:g}_{a.precision}.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "MLUPS + HBM GB/s (D3Q19 stream-collide, fluid-cell updates)",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f32": "f32", "f32_f64": "f32 storage, f64 arithmetic", "f64": "f64"}[a.precision],
+        "data": "synthetic (procedural airbox STL, seeded)",
+        "config": {"workload": f"airbox 300x150x200 mm, dx={a.dx:g} mm, seed {a.seed} "
+                               f"(baffles {len(spec.baffles)}, inlet yaw {spec.yaw_deg} deg), "
+                               f"{steps} steps averaging {avg}, probe every {every}",
+                   "grid": [g.nx, g.ny, g.nz], "fluid_cells": NF, "triangles": int(tris.size // 9)},
+        "hbm_gbps": round(NF * steps * bpu / wall / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": "k_stream_collide (no averaging)", "avg_launch_ms": round(kern_ms, 5),
+                     "bytes_per_launch": NF * bpu, "avg_launch_ms_with_acc": round(kern_acc_ms, 5)},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(tris, p, run, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

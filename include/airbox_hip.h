@@ -1,0 +1,209 @@
+/*
+ * airbox_hip.h - C ABI of the MI355X (gfx950) AirboxCalc compute core.
+ *
+ * The reference compute core is src/engine.js: a browser/Node module whose
+ * backend API is   prepare(p) -> g,   simulate*(g, p, onProgress, ns) -> result,
+ * evaluate(g, p, acc, cells, cid, hist, Ain, Af, ns) -> result
+ * (engine.js:110, 341, 613, 632; exported at engine.js:803).  This library is
+ * the native layer beneath a drop-in backend beside the reference's WebGPU
+ * (GpuSim, engine.js:493-594) and CPU (cpuStep, engine.js:266-321) backends.
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Inputs are borrowed for the duration of
+ *    the call; outputs are caller-allocated with explicit lengths.
+ *  - Every function returning int returns 0 on success and -1 on failure;
+ *    abx_last_error() then holds the message of the calling thread.  Messages
+ *    that the reference raises are reproduced with the same text.
+ *  - Grids use the reference's flat index c = (i*ny + j)*nz + k (k fastest,
+ *    engine.js:128-134); per-fluid-cell arrays use its compact order
+ *    ("cid order": fluid cells by ascending c, engine.js:241-242) and its
+ *    structure-of-arrays layout f[q*NF + a], acc[comp*NF + a]
+ *    (engine.js:274, 319).
+ *  - One HIP stream per simulation handle; calls on one handle are not
+ *    thread-safe, calls on different handles are.
+ */
+#ifndef AIRBOX_HIP_H
+#define AIRBOX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct abx_dev abx_dev;
+typedef struct abx_geom abx_geom;
+typedef struct abx_sim abx_sim;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* abx_last_error(void);
+
+/* Library version string. */
+const char* abx_version(void);
+
+/* ---- device ---------------------------------------------------------------
+ * Replaces gpuDevice() (engine.js:479-491): opens GPU `ordinal`, checks that
+ * it is a gfx950 part and writes its name ("AMD Instinct MI355X ...") into
+ * name[name_len].  Fails (no silent fallback) when no such GPU exists. */
+int abx_device_open(int ordinal, abx_dev** dev, char* name, size_t name_len);
+void abx_device_close(abx_dev* dev);
+
+/* ---- STL ------------------------------------------------------------------
+ * Replaces parseSTL (engine.js:15-28).  Binary when 84 + 50*n == len, else
+ * the ASCII "vertex x y z" records.  Writes at most cap floats to out (may be
+ * NULL with cap 0 to query) and the number of floats (9 per triangle) to
+ * *nfloat.  Error text as engine.js:26. */
+int abx_parse_stl(const uint8_t* buf, int64_t len, float* out, int64_t cap, int64_t* nfloat);
+
+/* ---- voxelizer ------------------------------------------------------------
+ * Replaces voxelize (engine.js:45-93), run on the GPU: parity fill along Z of
+ * the triangle soup T (9 floats per triangle, local frame) on the grid
+ * nx*ny*nz of spacing h at origin.  vox_out (nx*ny*nz bytes, i-major, k
+ * fastest) receives 0/1, bit-identical to engine.js. */
+int abx_voxelize(abx_dev* dev, const float* T, int64_t ntri, const double origin[3], int nx, int ny, int nz,
+                 double h, uint8_t* vox_out);
+
+/* ---- geometry preparation -------------------------------------------------
+ * Replaces prepare(p) (engine.js:110-234).  Parameters as the reference's
+ * prepare object (ui.html:477-479): cy/sy are cos/sin of the inlet yaw
+ * (engine.js:36); pass NaN to have the library compute them.  max_cells is
+ * the grid-size cap (engine.js:124 uses 4.5e6); <= 0 lifts it. */
+typedef struct {
+  double dx;
+  int sealAxis, sealSign;
+  double seal;
+  int inletAxis, inletSign;
+  double inletYaw, cy, sy;
+  int has_click;
+  double click[3];
+  double pleat;
+  int has_outlet;
+  double out_center[3], out_diameter, out_height;
+  double max_cells;
+} abx_prep_params;
+
+/* Scalar part of the reference's geometry object g and its summary
+ * (engine.js:226-233). */
+typedef struct {
+  int nx, ny, nz;
+  int64_t N, nf;
+  int top, jm, kw0;
+  double h, origin[3], mn[3], mx[3];
+  int nHoles, nOpen, nOut, Af, leak, pleatOk, has_outlet;
+  double volume_l, inlet_cm2, filter_cm2, outlet_cm2, outlet_cm2_real;
+  int64_t nOpenPts, nFilterPts, nOutletPts;
+  int dAx, iAx, oAx, sd, si;
+  double seal, yaw, cy, sy, pleat;
+} abx_geom_info;
+
+int abx_prepare(abx_dev* dev, const float* tris, int64_t nfloat, const abx_prep_params* p, abx_geom** g);
+int abx_geom_info_get(const abx_geom* g, abx_geom_info* out);
+
+/* Arrays of g: ABX_G_SOLID/FLUID/OUTLET (N bytes), ABX_G_OPENING (nx*ny
+ * bytes), ABX_G_XC/DC/ZC (nx/ny/nz floats), ABX_G_OPENPTS/FILTERPTS/
+ * OUTLETPTS (3*n doubles, model frame), ABX_G_FINE (the fine voxel mask,
+ * 27*N bytes).  cap is in elements. */
+enum {
+  ABX_G_SOLID = 0, ABX_G_FLUID = 1, ABX_G_OPENING = 2, ABX_G_OUTLET = 3, ABX_G_XC = 4, ABX_G_DC = 5,
+  ABX_G_ZC = 6, ABX_G_OPENPTS = 7, ABX_G_FILTERPTS = 8, ABX_G_OUTLETPTS = 9, ABX_G_FINE = 10
+};
+int abx_geom_read(const abx_geom* g, int what, void* out, int64_t cap);
+void abx_geom_free(abx_geom* g);
+
+/* ---- simulation -----------------------------------------------------------
+ * The lattice a simulation runs on, as plain arrays.  abx_geom_grid() fills
+ * it from a prepared geometry; a caller holding the reference's g object
+ * fills it from g.fluid, g.opening, g.outlet, g.dc, g.top, g.jm, g.pleat. */
+typedef struct {
+  int nx, ny, nz;
+  const uint8_t* fluid;   /* N */
+  const uint8_t* opening; /* nx*ny */
+  const uint8_t* outlet;  /* N */
+  const float* dc;        /* ny */
+  int top, jm;
+  double pleat;
+} abx_grid;
+
+int abx_geom_grid(const abx_geom* g, abx_grid* out);
+
+/* Run parameters of setupSim (engine.js:239-264, p = {K, aniso, u, tau}). */
+typedef struct {
+  double K, aniso, u, tau;
+} abx_run_params;
+
+/* Arithmetic of the stream-collide kernel:
+ *  ABX_F32        float storage, float arithmetic (the WGSL lbm_step,
+ *                 engine.js:378-450);
+ *  ABX_F32_F64    float storage, double arithmetic in the reference's
+ *                 operation order (cpuStep, engine.js:266-321); bit-identical
+ *                 to the CPU backend;
+ *  ABX_F64        double storage and arithmetic (error reference). */
+enum { ABX_F32 = 0, ABX_F32_F64 = 1, ABX_F64 = 2 };
+
+/* setupSim scalars (engine.js:246-263). */
+typedef struct {
+  int64_t NF;
+  int Ain, Af, Lp, nM, nT;
+  double u0, tau0, sig, aniso, cs2s;
+  int precision, B;
+  int64_t device_bytes;
+} abx_sim_info;
+
+/* Replaces setupSim + new GpuSim(dev, S, B, ns) (engine.js:239-264,
+ * 495-545): builds the dense device lattice, B variants that share the
+ * geometry, all at the cold start f = WT (engine.js:260-261).  ns is NULL or
+ * B*NF floats in cid order (partial bounce-back, engine.js:238). */
+int abx_sim_create(abx_dev* dev, const abx_grid* grid, const abx_run_params* rp, int precision, int B,
+                   const float* ns, abx_sim** sim);
+int abx_sim_info_get(const abx_sim* sim, abx_sim_info* out);
+
+/* Replaces GpuSim.steps(n, doacc) (engine.js:552-557): n fused
+ * stream-collide steps on every variant; with doacc the step adds the
+ * physical velocity and density to acc (engine.js:319, 445-449).
+ * Asynchronous on the handle's stream. */
+int abx_sim_steps(abx_sim* sim, int n, int doacc);
+
+/* Replaces GpuSim.probe() + progressInfo (engine.js:566-572, 330-339) for
+ * variant 0: out = {gamma, massRatio, qo}.  Fails with the reference's
+ * instability message (engine.js:335) when the outflow is not finite. */
+int abx_sim_probe(abx_sim* sim, int step, double out[3]);
+
+/* Replaces GpuSim.read(acc) and the division by nacc (engine.js:627-628):
+ * acc of variant b in cid order (4*NF), each value divided by divisor in
+ * double and stored as float (as the reference's Float32Array does). */
+int abx_sim_read_acc(abx_sim* sim, int b, double divisor, float* out);
+/* The same in double (ABX_F64 handles keep acc in double). */
+int abx_sim_read_acc_f64(abx_sim* sim, int b, double divisor, double* out);
+
+/* Replaces GpuSim.read(curBuf()) (engine.js:580-585, 603): populations of
+ * variant b in cid order (19*NF); float for ABX_F32/ABX_F32_F64, double for
+ * ABX_F64 handles. */
+int abx_sim_read_f(abx_sim* sim, int b, void* out);
+/* Replaces resetFrom (engine.js:561-565): overwrite the populations of
+ * variant b (b < 0: every variant) from 19*NF values in cid order. */
+int abx_sim_write_f(abx_sim* sim, int b, const void* in);
+/* Equilibrium start f_q = feq_q(rho, u) from rho_u = {rho[NF], ux[NF],
+ * uy[NF], uz[NF]} in cid order, for every variant (feq as engine.js:296).
+ * With rho = 1, u = 0 this is the cold start f0 = WT exactly. */
+int abx_sim_init_eq(abx_sim* sim, const float* rho_u);
+/* Replaces clearAcc (engine.js:559). */
+int abx_sim_clear_acc(abx_sim* sim);
+/* Replaces setNS (engine.js:558): B*NF floats in cid order, or NULL for 0. */
+int abx_sim_set_ns(abx_sim* sim, const float* ns);
+/* Wait for the handle's stream. */
+int abx_sim_sync(abx_sim* sim);
+/* Tuning knobs of the stream-collide launch (no reference counterpart):
+ * xcd_map 1 = each XCD sweeps one contiguous part of the lattice. */
+int abx_sim_set_launch(abx_sim* sim, int xcd_map);
+/* Average duration in ms of the stream-collide launches of the last
+ * abx_sim_steps call with timing on (HIP events on the handle's stream). */
+int abx_sim_time_steps(abx_sim* sim, int n, int doacc, double* ms_total);
+/* Replaces GpuSim.destroy() (engine.js:593). */
+void abx_sim_destroy(abx_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

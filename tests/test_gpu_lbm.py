@@ -1,0 +1,177 @@
+"""Stream-collide on the GPU against the reference's CPU backend.
+
+ABX_F32_F64 (float storage, double arithmetic in cpuStep's order) must give
+engine.js's cpuStep populations bit for bit: the fixtures hold the SHA-256 of
+engine.js's own Float32Arrays after 1, 2 and 12 steps.  ABX_F32 (the WGSL
+kernel's float arithmetic) must pass the reference's own GPU/CPU gate: max
+|f_GPU - f_CPU| < 1e-3 after 12 steps over every 7th element
+(engine.js:597-611, 772).  ABX_F64 must equal the oracle's f64-storage step
+bit for bit.
+"""
+import numpy as np
+import pytest
+
+from airboxcalc_amd import abx, engine
+from oracle import oracle as O
+from tests import golden_util as GU
+from tests.gpu_util import dev, gpu_geometry, grid_of_oracle, oracle_geometry, say, selftest_ns
+
+pytestmark = pytest.mark.gpu
+
+RUN = GU.names(lambda r: "setup" in r)
+
+
+def _sim(fx, precision, B=1, ns=None):
+    g = gpu_geometry(fx)
+    r = fx["desc"]["run"]
+    return g, abx.Sim(dev(), g.grid(), r["K"], r["aniso"], r["u"], r["tau"], precision=precision, B=B, ns=ns)
+
+
+@pytest.mark.parametrize("name", RUN)
+def test_setup_scalars(name):
+    fx = GU.load(name)
+    say("setup", name)
+    _, s = _sim(fx, "f32")
+    st = fx["setup"]
+    I = s.info
+    assert (I.NF, I.Ain, I.Af, I.nM, I.nT) == (st["NF"], st["Ain"], st["Af"], st["nM"], st["nT"])
+    assert I.sig == GU.x64(st["sig"]) and I.cs2s == GU.x64(st["cs2s"])
+
+
+@pytest.mark.parametrize("name", RUN)
+def test_f32_f64_matches_engine_js_bit_for_bit(name):
+    fx = GU.load(name)
+    say("f32_f64 trajectories", name)
+    og = oracle_geometry(fx)
+    r = fx["desc"]["run"]
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    ns = selftest_ns(S.flag)
+    _, sim = _sim(fx, "f32_f64", B=2, ns=np.concatenate([np.zeros_like(ns), ns]))
+    f0 = sim.read_f(0)
+    assert np.array_equal(f0, S.f0())
+    for s in range(1, 13):
+        sim.steps(1, doacc=s > 6)
+        if s in (1, 2, 12):
+            for b, vn in enumerate(("plain", "ns")):
+                assert GU.sha(sim.read_f(b)) == fx["sha"][f"f_{vn}_{s}"], (vn, s)
+    for b, vn in enumerate(("plain", "ns")):
+        assert GU.sha(sim.read_acc(1.0, b)) == fx["sha"][f"acc_{vn}"], vn
+
+
+@pytest.mark.parametrize("name", RUN)
+def test_f32_passes_reference_gate(name):
+    fx = GU.load(name)
+    say("f32 gate", name)
+    og = oracle_geometry(fx)
+    r = fx["desc"]["run"]
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    ns = selftest_ns(S.flag)
+    _, sim = _sim(fx, "f32", B=2, ns=np.concatenate([np.zeros_like(ns), ns]))
+    sim.steps(12)
+    md = 0.0
+    for b, nsb in enumerate((None, ns)):
+        ref = O.run_steps(S, 12, ns=nsb)
+        got = sim.read_f(b)
+        md = max(md, float(np.abs(got[::7].astype(np.float64) - ref[::7]).max()))
+        # all elements, not only every 7th, and far inside the gate
+        assert np.abs(got.astype(np.float64) - ref).max() < 1e-5
+    assert md < 1e-3
+
+
+@pytest.mark.parametrize("name", ["small", "small_yaw"])
+def test_f64_matches_oracle_bit_for_bit(name):
+    fx = GU.load(name)
+    say("f64", name)
+    og = oracle_geometry(fx)
+    r = fx["desc"]["run"]
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    _, sim = _sim(fx, "f64")
+    acc_ref = np.zeros(4 * S.NF)
+    f = S.f0(np.float64)
+    f2 = np.empty_like(f)
+    for s in range(1, 21):
+        O.cpu_step(S, f, f2, acc_ref if s > 10 else None)
+        f, f2 = f2, f
+        sim.steps(1, doacc=s > 10)
+    assert np.array_equal(sim.read_f(), f)
+    assert np.array_equal(sim.read_acc(1.0, f64=True), acc_ref)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f32_f64", "f64"])
+def test_probe_matches_oracle(precision):
+    fx = GU.load("small")
+    say("probe", precision)
+    og = oracle_geometry(fx)
+    r = fx["desc"]["run"]
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    _, sim = _sim(fx, precision)
+    sim.steps(200)
+    f = sim.read_f().astype(np.float32)
+    gam, mr, qo = sim.probe(200)
+    g2, m2, q2 = O.probe(S, f, 200)
+    tol = 1e-12 if precision != "f64" else 1e-6
+    assert gam == pytest.approx(g2, rel=tol) and mr == pytest.approx(m2, rel=tol)
+
+
+@pytest.mark.parametrize("name", ["small", "small_noout", "small_yaw"])
+def test_simulate_metrics_equal_engine_js(name):
+    """simulateGPU's loop on the f32_f64 kernel: the averaged acc is the CPU
+    backend's, so evaluate gives engine.js's metrics exactly."""
+    fx = GU.load(name)
+    say("simulate", name)
+    g = gpu_geometry(fx)
+    r = fx["desc"]["run"]
+    res = engine.simulate_hip(dev(), g, r, precision="f32_f64")
+    og = oracle_geometry(fx)
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    ev = O.evaluate(og, S, r["u"], res.acc)
+    sim = fx["sim"]["sim"]
+    for k, e in sim["metrics"].items():
+        assert ev["metrics"][k] == GU.x64(e), k
+    want = np.array([[GU.x64(e) for e in row] for row in sim["hist"]])
+    got = np.array(res.hist)
+    assert np.array_equal(got[:, 0], want[:, 0])
+    np.testing.assert_allclose(got[:, 1:], want[:, 1:], rtol=1e-12, atol=1e-15)
+
+
+def test_init_eq_cold_start_and_moments():
+    fx = GU.load("small")
+    say("init_eq")
+    og = oracle_geometry(fx)
+    r = fx["desc"]["run"]
+    S = O.setup_sim(og, r["K"], r["aniso"], r["u"], r["tau"])
+    _, sim = _sim(fx, "f32_f64")
+    NF = sim.NF
+    ru = np.zeros(4 * NF, np.float32)
+    ru[:NF] = 1
+    sim.init_eq(ru)
+    assert np.array_equal(sim.read_f(), S.f0())
+    rng = np.random.default_rng(1)
+    rho = rng.uniform(0.95, 1.05, NF).astype(np.float32)
+    u = rng.uniform(-0.05, 0.05, (3, NF)).astype(np.float32)
+    sim.init_eq(np.concatenate([rho, u.ravel()]))
+    f = sim.read_f().reshape(19, NF).astype(np.float64)
+    cx = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+    cy = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1])
+    cz = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+    np.testing.assert_allclose(f.sum(0), rho, rtol=2e-6)
+    for c, comp in ((cx, 0), (cy, 1), (cz, 2)):
+        np.testing.assert_allclose((c[:, None] * f).sum(0), rho * u[comp], atol=2e-7)
+
+
+def test_write_read_round_trip_and_xcd_mapping_invariance():
+    fx = GU.load("small_yaw")
+    say("round trip / launch mapping")
+    _, a = _sim(fx, "f32")
+    _, b = _sim(fx, "f32")
+    b.set_launch(0)
+    rng = np.random.default_rng(3)
+    f = (np.repeat([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, a.NF) *
+         rng.uniform(0.99, 1.01, 19 * a.NF)).astype(np.float32)
+    a.write_f(f)
+    assert np.array_equal(a.read_f(), f)
+    b.write_f(f)
+    a.steps(10, doacc=True)
+    b.steps(10, doacc=True)
+    assert np.array_equal(a.read_f(), b.read_f())
+    assert np.array_equal(a.read_acc(), b.read_acc())
