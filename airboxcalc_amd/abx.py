@@ -66,6 +66,24 @@ class SimInfo(C.Structure):
                 ("device_bytes", C.c_int64)]
 
 
+class EvalGeom(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("top", C.c_int), ("jm", C.c_int),
+                ("h", C.c_double), ("pleat", C.c_double), ("mn_z", C.c_double), ("opening", C.c_void_p),
+                ("xc", C.c_void_p), ("dc", C.c_void_p), ("zc", C.c_void_p), ("oAx", C.c_int), ("dAx", C.c_int),
+                ("iAx", C.c_int), ("sd", C.c_int), ("si", C.c_int), ("seal", C.c_double), ("cy", C.c_double),
+                ("sy", C.c_double)]
+
+
+class Metrics(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("gamma", "gammaCore", "lowCore", "highCore", "reverse", "maxToMean",
+                                          "zeta", "inlet_cm2", "filter_cm2")] + \
+               [(k, C.c_int) for k in ("ic", "jc", "jlow", "jb", "kB")] + \
+               [("rhoRef", C.c_double), ("pin", C.c_double), ("n", C.c_int), ("cn", C.c_int)]
+
+
+METRIC_KEYS = ("gamma", "gammaCore", "lowCore", "highCore", "reverse", "maxToMean", "zeta", "inlet_cm2",
+               "filter_cm2")
+
 G_SOLID, G_FLUID, G_OPENING, G_OUTLET, G_XC, G_DC, G_ZC, G_OPENPTS, G_FILTERPTS, G_OUTLETPTS, G_FINE = range(11)
 
 _lib = None
@@ -112,6 +130,10 @@ def load() -> C.CDLL:
         "abx_sim_set_launch": (i32, [vp, i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
+        "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
+        "abx_evaluate": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp, vp, vp, vp, vp, vp, vp]),
+        "abx_streamlines": (i32, [vp, C.POINTER(EvalGeom), i32, d, i32, vp, i64, vp, i64, vp, i64,
+                                  C.POINTER(i64), C.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -248,6 +270,12 @@ class Geometry:
             self._cache[name] = a
         return a
 
+    def eval_view(self) -> "EvalGeom":
+        e = EvalGeom()
+        _check(load().abx_geom_eval_view(self.h, C.byref(e)))
+        e._owner = self
+        return e
+
     def grid(self) -> "GridArrays":
         g = Grid()
         _check(load().abx_geom_grid(self.h, C.byref(g)))
@@ -349,6 +377,37 @@ class Sim:
         r = np.ascontiguousarray(rho_u, np.float32)
         assert r.size == 4 * self.NF
         _check(load().abx_sim_init_eq(self.h, _p(r)))
+
+    def evaluate(self, eg: "EvalGeom", divisor: float = 1.0, b: int = 0, field3d: bool = True) -> dict:
+        """evaluate + field3d (engine.js:632-719) on the GPU."""
+        nx, ny, nz = eg.nx, eg.ny, eg.nz
+        N = nx * ny * nz
+        M = Metrics()
+        fmap = np.empty(nx * nz, np.float32)
+        pI = np.empty(3 * nx * nz, np.float32)
+        pL = np.empty(3 * nx * nz, np.float32)
+        lon = np.empty(3 * nz * ny, np.float32)
+        vol = [np.empty(N, np.float32) for _ in range(3)] if field3d else [None, None, None]
+        _check(load().abx_evaluate(self.h, C.byref(eg), b, divisor, C.byref(M), _p(fmap), _p(pI), _p(pL), _p(lon),
+                                   *[_p(v) for v in vol]))
+        out = dict(metrics={k: getattr(M, k) for k in METRIC_KEYS}, fmap=fmap, planInlet=pI.reshape(3, -1),
+                   planLow=pL.reshape(3, -1), lon=lon.reshape(3, -1), ic=M.ic, jc=M.jc, jlow=M.jlow, jb=M.jb,
+                   kB=M.kB, rhoRef=M.rhoRef, pin=M.pin)
+        if field3d:
+            out["vol"] = dict(spd=vol[0], pst=vol[1], loss=vol[2], kB=M.kB)
+        return out
+
+    def streamlines(self, eg: "EvalGeom", divisor: float = 1.0, b: int = 0, max_seeds: int = 220) -> dict:
+        """streamlines (engine.js:721-758) on the GPU."""
+        cap = 2002 * max_seeds + 16
+        pts = np.empty(3 * cap, np.float32)
+        spd = np.empty(cap, np.float32)
+        off = np.zeros(max_seeds + 2, np.uint32)
+        npts, nl = C.c_int64(), C.c_int64()
+        _check(load().abx_streamlines(self.h, C.byref(eg), b, divisor, max_seeds, _p(pts), pts.size, _p(spd),
+                                      spd.size, _p(off), off.size, C.byref(npts), C.byref(nl)))
+        n = npts.value
+        return dict(pts=pts[:3 * n].copy(), spd=spd[:n].copy(), off=off[:nl.value + 1].copy())
 
     def clear_acc(self) -> None:
         _check(load().abx_sim_clear_acc(self.h))

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "abx_common.hpp"
+#include "sim_internal.hpp"
 
 namespace abx {
 
@@ -883,3 +884,26 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int xcd_map) {
 }
 
 extern "C" void abx_sim_destroy(abx_sim* s) { sim_free(s); }
+
+namespace abx {
+SimView sim_view(abx_sim* s) {
+  SimView v;
+  v.ordinal = s->dev->ordinal;
+  v.stream = s->stream;
+  v.B = s->B;
+  v.nx = s->nx;
+  v.ny = s->ny;
+  v.nz = s->nz;
+  v.nyp = s->nyp;
+  v.nzp = s->nzp;
+  v.plane = s->plane;
+  v.ssz = s->ssz;
+  v.acc = s->acc;
+  v.ns = s->ns;
+  v.fluid_bits = s->fluid_bits;
+  v.u0 = s->u0;
+  v.Ain = s->Ain;
+  v.Af = s->Af;
+  return v;
+}
+}  // namespace abx

@@ -810,4 +810,33 @@ extern "C" int abx_geom_grid(const abx_geom* g, abx_grid* out) {
   return 0;
 }
 
+extern "C" int abx_geom_eval_view(const abx_geom* g, abx_eval_geom* out) {
+  if (!g || !out) {
+    set_error("abx_geom_eval_view: null argument");
+    return -1;
+  }
+  const abx_geom_info& I = g->info;
+  out->nx = I.nx;
+  out->ny = I.ny;
+  out->nz = I.nz;
+  out->top = I.top;
+  out->jm = I.jm;
+  out->h = I.h;
+  out->pleat = I.pleat;
+  out->mn_z = I.mn[2];
+  out->opening = g->opening.data();
+  out->xc = g->xc.data();
+  out->dc = g->dc.data();
+  out->zc = g->zc.data();
+  out->oAx = I.oAx;
+  out->dAx = I.dAx;
+  out->iAx = I.iAx;
+  out->sd = I.sd;
+  out->si = I.si;
+  out->seal = I.seal;
+  out->cy = I.cy;
+  out->sy = I.sy;
+  return 0;
+}
+
 extern "C" void abx_geom_free(abx_geom* g) { delete g; }

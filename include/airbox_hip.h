@@ -203,6 +203,47 @@ int abx_sim_time_steps(abx_sim* sim, int n, int doacc, double* ms_total);
 /* Replaces GpuSim.destroy() (engine.js:593). */
 void abx_sim_destroy(abx_sim* sim);
 
+/* ---- evaluation -------------------------------------------------------------
+ * The geometry fields evaluate/field3d/streamlines read from the reference's
+ * g object (engine.js:633, 699, 722): grid, rows top and jm, pleat depth,
+ * inlet plane mn[2], opening mask, cell centres and the local frame. */
+typedef struct {
+  int nx, ny, nz, top, jm;
+  double h, pleat, mn_z;
+  const uint8_t* opening; /* nx*ny */
+  const float *xc, *dc, *zc;
+  int oAx, dAx, iAx, sd, si;
+  double seal, cy, sy;
+} abx_eval_geom;
+
+int abx_geom_eval_view(const abx_geom* g, abx_eval_geom* out);
+
+/* Result metrics of evaluate (engine.js:682-685) and the section indices and
+ * reference values it derives (engine.js:657-665, 704-710). */
+typedef struct {
+  double gamma, gammaCore, lowCore, highCore, reverse, maxToMean, zeta, inlet_cm2, filter_cm2;
+  int ic, jc, jlow, jb, kB;
+  double rhoRef, pin;
+  int n, cn;
+} abx_metrics;
+
+/* Replaces evaluate + field3d (engine.js:632-719) on the GPU, reading the
+ * averaged fields of variant b (acc / divisor) in place.  Cells with ns > 0.5
+ * leave the fluid set as in engine.js:634.  Outputs (any may be NULL):
+ * fmap nx*nz; planInlet, planLow 3*nx*nz {speed, ua, ub}; lon 3*nz*ny
+ * {speed, ua, ub}; spd, pst, loss N each (reference flat order, NaN outside
+ * the fluid), all three or none. */
+int abx_evaluate(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, abx_metrics* m, float* fmap,
+                 float* planInlet, float* planLow, float* lon, float* spd, float* pst, float* loss);
+
+/* Replaces streamlines (engine.js:721-758): RK2 traces of the averaged field
+ * from up to maxSeeds seeds over the inlet opening, one GPU thread per seed.
+ * pts (3 floats per point, model frame), spd (one per point) and off (line
+ * starts, nlines+1 entries) as the reference's {pts, spd, off}. */
+int abx_streamlines(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, int maxSeeds, float* pts,
+                    int64_t pts_cap, float* spd, int64_t spd_cap, uint32_t* off, int64_t off_cap,
+                    int64_t* npts, int64_t* nlines);
+
 #ifdef __cplusplus
 }
 #endif

@@ -175,3 +175,35 @@ def test_write_read_round_trip_and_xcd_mapping_invariance():
     b.steps(10, doacc=True)
     assert np.array_equal(a.read_f(), b.read_f())
     assert np.array_equal(a.read_acc(), b.read_acc())
+
+
+@pytest.mark.parametrize("name", ["small", "small_yaw", "c1_seed3"])
+def test_gpu_evaluate_field3d_streamlines(name):
+    """evaluate/field3d/streamlines on the GPU against engine.js's outputs on
+    the same averaged field (f32_f64 kernel, so acc is engine.js's)."""
+    fx = GU.load(name)
+    say("evaluate", name)
+    g = gpu_geometry(fx)
+    r = fx["desc"]["run"]
+    res = engine.simulate_hip(dev(), g, r, precision="f32_f64", keep_sim=True)
+    sim = res.sim
+    ev = sim.evaluate(g.eval_view(), divisor=max(res.nacc, 1))
+    want = fx["sim"]["sim"]
+    for k, e in want["metrics"].items():
+        assert ev["metrics"][k] == pytest.approx(GU.x64(e), rel=1e-12, abs=1e-14), k
+    arr = GU.arrays(name) if name != "c1_seed3" else None
+    # per-cell maps: bit-exact where no sum enters, one float ulp where one does
+    for pl in ("planInlet", "planLow", "lon"):
+        for q, k in enumerate(("speed", "ua", "ub")):
+            assert GU.sha(ev[pl][q]) == fx["sha"][f"sim_{pl}_{k}"], (pl, k)
+    assert GU.sha(ev["vol"]["spd"]) == fx["sha"]["sim_vol_spd"]
+    if arr is not None:
+        for k in ("fmap",):
+            np.testing.assert_array_max_ulp(ev["fmap"], arr["sim_fmap"], maxulp=1)
+        np.testing.assert_array_max_ulp(np.nan_to_num(ev["vol"]["pst"]), np.nan_to_num(arr["sim_vol_pst"]), maxulp=2)
+        np.testing.assert_array_max_ulp(np.nan_to_num(ev["vol"]["loss"]), np.nan_to_num(arr["sim_vol_loss"]),
+                                        maxulp=2)
+    L = sim.streamlines(g.eval_view(), divisor=max(res.nacc, 1))
+    for k in ("pts", "spd", "off"):
+        assert GU.sha(L[k]) == fx["sha"][f"sim_lines_{k}"], k
+    sim.destroy()
