@@ -108,6 +108,7 @@ def load() -> C.CDLL:
         "abx_version": (C.c_char_p, []),
         "abx_device_open": (i32, [i32, C.POINTER(vp), C.c_char_p, C.c_size_t]),
         "abx_device_close": (None, [vp]),
+        "abx_copy_bandwidth": (i32, [vp, i64, i32, C.POINTER(d), C.POINTER(d)]),
         "abx_parse_stl": (i32, [C.c_char_p, i64, vp, i64, C.POINTER(i64)]),
         "abx_voxelize": (i32, [vp, vp, i64, vp, i32, i32, i32, d, vp]),
         "abx_prepare": (i32, [vp, vp, i64, C.POINTER(PrepParams), C.POINTER(vp)]),
@@ -127,7 +128,7 @@ def load() -> C.CDLL:
         "abx_sim_clear_acc": (i32, [vp]),
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
-        "abx_sim_set_launch": (i32, [vp, i32]),
+        "abx_sim_set_launch": (i32, [vp, i32, i32, i32, i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
         "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
@@ -179,6 +180,13 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+def copy_bandwidth(dev: "Device", nbytes: int, iters: int = 20):
+    """(GB/s, ms per copy) of a plain float4 device copy of nbytes."""
+    g, ms = C.c_double(), C.c_double()
+    _check(load().abx_copy_bandwidth(dev.h, int(nbytes), iters, C.byref(g), C.byref(ms)))
+    return g.value, ms.value
 
 
 _DEV = {}
@@ -239,10 +247,10 @@ class Geometry:
         L = load()
         tris = np.ascontiguousarray(tris, np.float32).ravel()
         P = prep_params(p, max_cells, cy, sy)
-        self.h = C.c_void_p()
-        _check(L.abx_prepare(dev.h, _p(tris), tris.size, C.byref(P), C.byref(self.h)))
+        self.ptr = C.c_void_p()
+        _check(L.abx_prepare(dev.h, _p(tris), tris.size, C.byref(P), C.byref(self.ptr)))
         I = GeomInfo()
-        _check(L.abx_geom_info_get(self.h, C.byref(I)))
+        _check(L.abx_geom_info_get(self.ptr, C.byref(I)))
         self.info = I
         for k, _ in GeomInfo._fields_:
             v = getattr(I, k)
@@ -251,7 +259,7 @@ class Geometry:
 
     def _read(self, what: int, n: int, dtype) -> np.ndarray:
         out = np.empty(n, dtype)
-        _check(load().abx_geom_read(self.h, what, _p(out), n))
+        _check(load().abx_geom_read(self.ptr, what, _p(out), n))
         return out
 
     def array(self, name: str) -> np.ndarray:
@@ -272,19 +280,19 @@ class Geometry:
 
     def eval_view(self) -> "EvalGeom":
         e = EvalGeom()
-        _check(load().abx_geom_eval_view(self.h, C.byref(e)))
+        _check(load().abx_geom_eval_view(self.ptr, C.byref(e)))
         e._owner = self
         return e
 
     def grid(self) -> "GridArrays":
         g = Grid()
-        _check(load().abx_geom_grid(self.h, C.byref(g)))
+        _check(load().abx_geom_grid(self.ptr, C.byref(g)))
         return GridArrays.from_struct(g, self)
 
     def free(self):
-        if self.h:
-            load().abx_geom_free(self.h)
-            self.h = C.c_void_p()
+        if self.ptr:
+            load().abx_geom_free(self.ptr)
+            self.ptr = C.c_void_p()
 
     def __del__(self):
         try:
@@ -416,8 +424,9 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, xcd_map: int) -> None:
-        _check(load().abx_sim_set_launch(self.h, xcd_map))
+    def set_launch(self, kernel: int = 0, xcd_map: int = 0, chunk: int = 256, nt: int = 0) -> None:
+        """kernel 0 = wide (default), 1 = narrow (chunk 64/256, nt bits)."""
+        _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
 
     def sync(self) -> None:
         _check(load().abx_sim_sync(self.h))

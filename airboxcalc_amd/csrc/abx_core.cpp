@@ -60,7 +60,8 @@ extern "C" int abx_device_open(int ordinal, abx_dev** dev, char* name, size_t na
   ABX_HIP(hipSetDevice(ordinal));
   abx_dev* d = new abx_dev();
   d->ordinal = ordinal;
-  snprintf(d->name, sizeof d->name, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  snprintf(d->name, sizeof d->name, "%s (%s, %d CUs)", p.name[0] ? p.name : "AMD Instinct GPU", p.gcnArchName,
+           p.multiProcessorCount);
   d->ncu = p.multiProcessorCount;
   d->xcds = 8;
   if (name && name_len) snprintf(name, name_len, "%s", d->name);

@@ -1,0 +1,54 @@
+// Practical HBM ceiling: a plain streaming copy of float4 (read + write), the
+// yardstick the stream-collide kernel's achieved bandwidth is compared with.
+#include "abx_common.hpp"
+
+namespace abx {
+
+__global__ __launch_bounds__(256) void k_copy4(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+}  // namespace abx
+
+using namespace abx;
+
+extern "C" int abx_copy_bandwidth(abx_dev* dev, int64_t bytes, int iters, double* gbps, double* ms_per_copy) {
+  if (!dev || bytes < 1024 || iters < 1 || !gbps) {
+    set_error("abx_copy_bandwidth: bad argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(dev->ordinal));
+  const int64_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  ABX_HIP(hipMalloc((void**)&a, (size_t)n * 16));
+  if (hipMalloc((void**)&b, (size_t)n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    set_error("abx_copy_bandwidth: hipMalloc failed");
+    return -1;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemset(a, 0, (size_t)n * 16);
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(256), 0, 0, a, b, n);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const hipError_t err = hipGetLastError();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) {
+    set_errorf("abx_copy_bandwidth: %s", hipGetErrorString(err));
+    return -1;
+  }
+  *gbps = 2.0 * (double)n * 16 * iters / (ms * 1e-3) / 1e9;
+  if (ms_per_copy) *ms_per_copy = ms / iters;
+  return 0;
+}

@@ -54,56 +54,69 @@ __device__ __forceinline__ M min09(M x) {
   return x > (M)0.9 ? (M)0.9 : x;
 }
 
-// One fused pull-stream + collide step.  S = storage type, M = arithmetic
-// type.  The arithmetic is the reference's, term by term and in its order
-// (engine.js:269-320 for M = double; the WGSL port engine.js:386-449 for
-// M = float), so <float, double> reproduces cpuStep bit for bit.
-template <typename S, typename M, bool ACC, bool NS>
-__global__ __launch_bounds__(CHUNK) void k_stream_collide(StepArgs a) {
-  __shared__ uint32_t win[9][10];
-  const int w = blockIdx.x;
-  const int ci = a.xcd_map ? (w & 7) * a.per_xcd + (w >> 3) : w;
-  if (ci >= a.nchunks) return;  // whole workgroup leaves together
-  const int b = blockIdx.y;
-  const int d0 = a.chunks[ci];
-  const int t = threadIdx.x;
-  // Fluid bits of the nine (di, dj) neighbour rows of the chunk, each window
-  // covering positions [d0 - (di*nyp + dj)*nzp - 32, ... + 320).
-  if (t < 90) {
-    const int wi = t / 10, wo = t - wi * 10;
-    const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-    const int64_t base = (int64_t)d0 - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-    win[wi][wo] = a.fluid_bits[(base >> 5) + wo];
-  }
-  __syncthreads();
-  if (!((win[4][(t + 32) >> 5] >> ((t + 32) & 31)) & 1u)) return;
-  const int d = d0 + t;
-  const uint32_t tb = (uint32_t)d & 31u;
-  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
-  const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
+template <typename T>
+__device__ __forceinline__ T ld(const T* p, bool nt) {
+  return nt ? __builtin_nontemporal_load(p) : *p;
+}
+template <typename T>
+__device__ __forceinline__ void st(T* p, T v, bool nt) {
+  if (nt)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
 
-  M fi[19];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const int p = t + 32 - CZ[q];
-    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const bool nbf = (win[wi][p >> 5] >> (p & 31)) & 1u;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-    const S* src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
-    fi[q] = (M)*src;
+// c*x for a lattice coefficient c in {-1, 0, 1}, as the reference's
+// product but without the multiply: the term is dropped when c == 0 (see
+// csum below), negated when c == -1.
+template <typename M>
+__device__ __forceinline__ M cterm(int c, M x) {
+  return c > 0 ? x : -x;
+}
+// (cx*ux + cy*uy) + cz*uz (engine.js:288, 295) with the zero-coefficient
+// products left out.  For finite operands this equals the reference's value
+// except possibly in the sign of a zero result, which never reaches a stored
+// value (it only enters 1 + cu and cu*cu).
+template <typename M>
+__device__ __forceinline__ M dot3(int cx, int cy, int cz, M ux, M uy, M uz) {
+  M r = 0;
+  bool have = false;
+  if (cx) {
+    r = cterm(cx, ux);
+    have = true;
   }
-  M rho = 0, mx = 0, my = 0, mz = 0;
+  if (cy) {
+    r = have ? r + cterm(cy, uy) : cterm(cy, uy);
+    have = true;
+  }
+  if (cz) r = have ? r + cterm(cz, uz) : cterm(cz, uz);
+  return r;
+}
+
+// Collision of one cell from its pulled populations fi (engine.js:277-318):
+// writes the post-collision populations to fo and the moments the averaging
+// accumulates (physical velocity, density) to ux, uy, uz, rho.  The
+// arithmetic is the reference's in its order (cpuStep for M = double; the
+// WGSL port engine.js:395-443 for M = float), so <float storage, double M>
+// reproduces cpuStep bit for bit.  The only liberty taken: a product by a
+// lattice coefficient 0 or +-1 is not formed (adding c*x with c = 0 to a
+// running sum, or multiplying by +-1, changes no finite result), which halves
+// the instruction count of the kernel.
+template <typename M, bool NS>
+__device__ __forceinline__ void collide(const M (&fi)[19], bool inlet, bool outlet, bool porous, M s,
+                                        const StepArgs& a, M (&fo)[19], M& ux, M& uy, M& uz, M& rho) {
+  rho = 0;
+  M mx = 0, my = 0, mz = 0;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     rho += fi[q];
-    mx += (M)CX[q] * fi[q];
-    my += (M)CY[q] * fi[q];
-    mz += (M)CZ[q] * fi[q];
+    if (CX[q]) mx += cterm(CX[q], fi[q]);
+    if (CY[q]) my += cterm(CY[q], fi[q]);
+    if (CZ[q]) mz += cterm(CZ[q], fi[q]);
   }
-  M ux = mx / rho, uy = my / rho, uz = mz / rho;
+  ux = mx / rho;
+  uy = my / rho;
+  uz = mz / rho;
   const M u0 = (M)a.u0;
   if (inlet || outlet) {
     if (inlet) {
@@ -126,68 +139,403 @@ __global__ __launch_bounds__(CHUNK) void k_stream_collide(StepArgs a) {
     const M usq = (M)1.5 * (ux * ux + uy * uy + uz * uz);
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      const M cu = (M)3 * ((M)CX[q] * ux + (M)CY[q] * uy + (M)CZ[q] * uz);
-      fout[q * P + d] = (S)((M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq));
+      const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ux, uy, uz);
+      fo[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
     }
-  } else {
-    M usq = (M)1.5 * (ux * ux + uy * uy + uz * uz);
-    M pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+    return;
+  }
+  M usq = (M)1.5 * (ux * ux + uy * uy + uz * uz);
+  M pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+  M fe[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ux, uy, uz);
+    fe[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
+    const M ne = fi[q] - fe[q];
+    if (CX[q]) pxx += ne;
+    if (CY[q]) pyy += ne;
+    if (CZ[q]) pzz += ne;
+    if (CX[q] * CY[q]) pxy += cterm(CX[q] * CY[q], ne);
+    if (CX[q] * CZ[q]) pxz += cterm(CX[q] * CZ[q], ne);
+    if (CY[q] * CZ[q]) pyz += cterm(CY[q] * CZ[q], ne);
+  }
+  const M Qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + (M)2 * (pxy * pxy + pxz * pxz + pyz * pyz));
+  const M tau0 = (M)a.tau0;
+  const M tau = (M)0.5 * (tau0 + sqrt(tau0 * tau0 + (M)a.cs2s * Qn / rho));
+  if (porous) {
+    // equilibrium at the Darcy-shifted velocity (engine.js:304-312)
+    const M sg = (M)a.sig;
+    const M sl = (M)1 - min09(tau * sg * (M)a.aniso), sv = (M)1 - min09(tau * sg);
+    const M ex = ux * sl, ey = uy * sv, ez = uz * sl;
+    usq = (M)1.5 * (ex * ex + ey * ey + ez * ez);
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      const M cu = (M)3 * ((M)CX[q] * ux + (M)CY[q] * uy + (M)CZ[q] * uz);
-      const M e = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-      const M ne = fi[q] - e;
-      pxx += (M)(CX[q] * CX[q]) * ne;
-      pyy += (M)(CY[q] * CY[q]) * ne;
-      pzz += (M)(CZ[q] * CZ[q]) * ne;
-      pxy += (M)(CX[q] * CY[q]) * ne;
-      pxz += (M)(CX[q] * CZ[q]) * ne;
-      pyz += (M)(CY[q] * CZ[q]) * ne;
+      const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ex, ey, ez);
+      fe[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
     }
-    const M Qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + (M)2 * (pxy * pxy + pxz * pxz + pyz * pyz));
-    const M tau0 = (M)a.tau0;
-    const M tau = (M)0.5 * (tau0 + sqrt(tau0 * tau0 + (M)a.cs2s * Qn / rho));
-    // velocity of the equilibrium the cell relaxes to (engine.js:304-312)
-    M ex = ux, ey = uy, ez = uz;
+    ux = (M)0.5 * (ux + ex);
+    uy = (M)0.5 * (uy + ey);
+    uz = (M)0.5 * (uz + ez);
+  }
+  const M it = (M)1 / tau;
+  if (NS && s > (M)0) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = ((M)1 - s) * (fi[q] - (fi[q] - fe[q]) * it) + s * fi[OPP[q]];
+    ux *= (M)1 - s;
+    uy *= (M)1 - s;
+    uz *= (M)1 - s;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = fi[q] - (fi[q] - fe[q]) * it;
+  }
+}
+
+// FP32 collision: the same model (BGK + Smagorinsky + anisotropic Darcy
+// shift + partial bounce-back, engine.js:277-318) in a cheaper but
+// mathematically identical formulation, for the float kernel:
+//  - the equilibria of a pair +c/-c share their even part:
+//      feq(+-c) = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u);
+//  - the non-equilibrium stress comes from the second moments of f, using
+//    the D3Q19 identity sum_q c_a c_b feq_q = rho (u_a u_b + delta_ab / 3);
+//  - relaxation is f + (feq - f) / tau as one explicit FMA (contraction is
+//    otherwise off, so every launch variant rounds identically).
+// Rounding differs from the reference's order at the 1e-7 level; the result
+// is held to the reference's GPU/CPU gate (max |df| < 1e-3 after 12 steps).
+template <bool NS>
+__device__ __forceinline__ void collide_fast(const float (&fi)[19], bool inlet, bool outlet, bool porous, float s,
+                                             const StepArgs& a, float (&fo)[19], float& ux, float& uy, float& uz,
+                                             float& rho) {
+  rho = 0;
+  float mx = 0, my = 0, mz = 0;
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    rho += fi[q];
+    if (CX[q]) mx += cterm(CX[q], fi[q]);
+    if (CY[q]) my += cterm(CY[q], fi[q]);
+    if (CZ[q]) mz += cterm(CZ[q], fi[q]);
+  }
+  const float ir = 1.0f / rho;
+  ux = mx * ir;
+  uy = my * ir;
+  uz = mz * ir;
+  const float u0 = (float)a.u0;
+  float ex = ux, ey = uy, ez = uz, omega = 1.0f;
+  if (inlet || outlet) {
+    if (inlet) {
+      float s0 = 0, sm = 0;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        if (CZ[q] == 0)
+          s0 += fi[q];
+        else if (CZ[q] < 0)
+          sm += fi[q];
+      }
+      rho = (s0 + 2.0f * sm) / (1.0f - u0);
+      ux = 0;
+      uy = 0;
+      uz = u0;
+    } else {
+      rho = 1.0f;
+    }
+    ex = ux;
+    ey = uy;
+    ez = uz;
+  } else {
+    // non-equilibrium second moments
+    float sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      if (CX[q]) sxx += fi[q];
+      if (CY[q]) syy += fi[q];
+      if (CZ[q]) szz += fi[q];
+      if (CX[q] * CY[q]) sxy += cterm(CX[q] * CY[q], fi[q]);
+      if (CX[q] * CZ[q]) sxz += cterm(CX[q] * CZ[q], fi[q]);
+      if (CY[q] * CZ[q]) syz += cterm(CY[q] * CZ[q], fi[q]);
+    }
+    const float r3 = rho * (1.0f / 3.0f);
+    const float pxx = sxx - rho * ux * ux - r3, pyy = syy - rho * uy * uy - r3, pzz = szz - rho * uz * uz - r3;
+    const float pxy = sxy - rho * ux * uy, pxz = sxz - rho * ux * uz, pyz = syz - rho * uy * uz;
+    const float Qn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float tau0 = (float)a.tau0;
+    const float tau = 0.5f * (tau0 + sqrtf(tau0 * tau0 + (float)a.cs2s * Qn * ir));
     if (porous) {
-      const M sg = (M)a.sig;
-      const M sl = (M)1 - min09(tau * sg * (M)a.aniso), sv = (M)1 - min09(tau * sg);
+      const float sg = (float)a.sig;
+      const float sl = 1.0f - min09(tau * sg * (float)a.aniso), sv = 1.0f - min09(tau * sg);
       ex = ux * sl;
       ey = uy * sv;
       ez = uz * sl;
-      usq = (M)1.5 * (ex * ex + ey * ey + ez * ez);
-      ux = (M)0.5 * (ux + ex);
-      uy = (M)0.5 * (uy + ey);
-      uz = (M)0.5 * (uz + ez);
+      ux = 0.5f * (ux + ex);
+      uy = 0.5f * (uy + ey);
+      uz = 0.5f * (uz + ez);
     }
-    const M it = (M)1 / tau;
-    M s = 0;
-    if (NS) s = (M)a.ns[(int64_t)b * P + d];
-    if (NS && s > (M)0) {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const M cu = (M)3 * ((M)CX[q] * ex + (M)CY[q] * ey + (M)CZ[q] * ez);
-        const M fe = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-        fout[q * P + d] = (S)(((M)1 - s) * (fi[q] - (fi[q] - fe) * it) + s * fi[OPP[q]]);
-      }
-      ux *= (M)1 - s;
-      uy *= (M)1 - s;
-      uz *= (M)1 - s;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const M cu = (M)3 * ((M)CX[q] * ex + (M)CY[q] * ey + (M)CZ[q] * ez);
-        const M fe = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-        fout[q * P + d] = (S)(fi[q] - (fi[q] - fe) * it);
-      }
-    }
+    omega = 1.0f / tau;
   }
+  // equilibria at (rho, e): rest, then the nine +c/-c pairs (q, q+1)
+  const float usq = 1.5f * (ex * ex + ey * ey + ez * ez);
+  const bool eqonly = inlet || outlet;
+  float feq[19];
+  feq[0] = (float)wt(0) * rho * (1.0f - usq);
+#pragma unroll
+  for (int q = 1; q < 19; q += 2) {
+    const float wr = (float)wt(q) * rho;
+    const float cu = dot3(CX[q], CY[q], CZ[q], ex, ey, ez);
+    const float even = wr * (1.0f - usq + 4.5f * cu * cu), odd = 3.0f * wr * cu;
+    feq[q] = even + odd;
+    feq[q + 1] = even - odd;
+  }
+  if (eqonly) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = feq[q];
+  } else if (NS && s > 0.0f) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = (1.0f - s) * fmaf(feq[q] - fi[q], omega, fi[q]) + s * fi[OPP[q]];
+    ux *= 1.0f - s;
+    uy *= 1.0f - s;
+    uz *= 1.0f - s;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = fmaf(feq[q] - fi[q], omega, fi[q]);
+  }
+}
+
+// Dispatch: float arithmetic takes the FP32 formulation, double the
+// reference's order.
+template <typename M, bool NS>
+__device__ __forceinline__ void collide_any(const M (&fi)[19], bool inlet, bool outlet, bool porous, M s,
+                                            const StepArgs& a, M (&fo)[19], M& ux, M& uy, M& uz, M& rho) {
+  if constexpr (sizeof(M) == 4)
+    collide_fast<NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+  else
+    collide<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+}
+
+// Narrow variant: one lattice position per lane, CH positions per
+// workgroup (64 or 256); NTL / NTS = non-temporal population loads / stores.
+template <typename S, typename M, bool ACC, bool NS, int CH, bool NTL, bool NTS>
+__global__ __launch_bounds__(CH) void k_stream_collide(StepArgs a) {
+  constexpr int WW = CH / 32 + 2;  // words per neighbour-row window
+  __shared__ uint32_t win[9][WW];
+  const int w = blockIdx.x;
+  const int ci = a.xcd_map ? (w & 7) * a.per_xcd + (w >> 3) : w;
+  if (ci >= a.nchunks) return;  // whole workgroup leaves together
+  const int b = blockIdx.y;
+  const int d0 = a.chunks[ci];
+  const int t = threadIdx.x;
+  // Fluid bits of the nine (di, dj) neighbour rows of the chunk, each window
+  // covering positions [d0 - (di*nyp + dj)*nzp - 32, ... + CH + 64).
+  for (int u = t; u < 9 * WW; u += CH) {
+    const int wi = u / WW, wo = u - wi * WW;
+    const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
+    const int64_t base = (int64_t)d0 - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
+    win[wi][wo] = a.fluid_bits[(base >> 5) + wo];
+  }
+  __syncthreads();
+  if (!((win[4][(t + 32) >> 5] >> ((t + 32) & 31)) & 1u)) return;
+  const int d = d0 + t;
+  const uint32_t tb = (uint32_t)d & 31u;
+  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
+  const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
+  const int64_t P = a.plane;
+  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
+  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
+  M fi[19], fo[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const int p = t + 32 - CZ[q];
+    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
+    const bool nbf = (win[wi][p >> 5] >> (p & 31)) & 1u;
+    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+    const S* src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
+    fi[q] = (M)ld(src, NTL);
+  }
+  M s = 0;
+  if (NS) s = (M)a.ns[(int64_t)b * P + d];
+  M ux, uy, uz, rho;
+  collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) st(fout + q * P + d, (S)fo[q], NTS);
   if (ACC) {
     S* ac = (S*)a.acc + (int64_t)b * 4 * P;
     ac[d] = (S)((M)ac[d] + ux);
     ac[P + d] = (S)((M)ac[P + d] + uy);
     ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
     ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
+  }
+}
+
+// Segment variant: one position per lane, a workgroup of 256 lanes takes
+// four independent active 64-position segments (one per wavefront), so only
+// segments holding fluid occupy lanes.  PERSIST: a fixed grid whose
+// workgroups stride over the segment list.
+template <typename S, typename M, bool ACC, bool NS, bool PERSIST>
+__global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
+  __shared__ uint32_t win[4][9][4];
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int b = blockIdx.y;
+  const int64_t P = a.plane;
+  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
+  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
+  const int nblk = (a.nchunks + 3) >> 2;
+  for (int blk = blockIdx.x; blk < nblk; blk += PERSIST ? gridDim.x : nblk) {
+    if (t < 4 * 9 * 4) {
+      const int ss = t / 36, r = t - ss * 36, wi = r >> 2, wo = r & 3;
+      uint32_t val = 0;
+      const int sg = 4 * blk + ss;
+      if (sg < a.nchunks) {
+        const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
+        const int64_t base = (int64_t)a.chunks[sg] - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
+        val = a.fluid_bits[(base >> 5) + wo];
+      }
+      win[ss][wi][wo] = val;
+    }
+    __syncthreads();
+    const int sg = 4 * blk + wv;
+    const int p0 = l + 32;
+    if (sg < a.nchunks && ((win[wv][4][p0 >> 5] >> (p0 & 31)) & 1u)) {
+      const int d = a.chunks[sg] + l;
+      const uint32_t tb = (uint32_t)d & 31u;
+      const uint32_t* tw = a.type_bits + 3 * (d >> 5);
+      const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
+      M fi[19], fo[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) {
+        const int p = p0 - CZ[q];
+        const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
+        const bool nbf = (win[wv][wi][p >> 5] >> (p & 31)) & 1u;
+        const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+        const S* src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
+        fi[q] = (M)*src;
+      }
+      M s = 0;
+      if (NS) s = (M)a.ns[(int64_t)b * P + d];
+      M ux, uy, uz, rho;
+      collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+#pragma unroll
+      for (int q = 0; q < 19; ++q) fout[q * P + d] = (S)fo[q];
+      if (ACC) {
+        S* ac = (S*)a.acc + (int64_t)b * 4 * P;
+        ac[d] = (S)((M)ac[d] + ux);
+        ac[P + d] = (S)((M)ac[P + d] + uy);
+        ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
+        ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
+      }
+    }
+    if (!PERSIST) break;
+    __syncthreads();
+  }
+}
+
+// V consecutive storage values (16 bytes) at an address aligned to the
+// element only: one global_load_dwordx4 / global_store_dwordx4.
+template <typename S, int V>
+__device__ __forceinline__ void vload(const S* p, S (&v)[V]) {
+  __builtin_memcpy(v, p, sizeof(S) * V);
+}
+template <typename S, int V>
+__device__ __forceinline__ void vstore(S* p, const S (&v)[V]) {
+  __builtin_memcpy(p, v, sizeof(S) * V);
+}
+
+// Wide variant (the default): every lane owns V = 16 / sizeof(S)
+// consecutive positions along k and moves each population as one 16-byte
+// access, so a wavefront reads and writes 1 KiB per population.  A
+// workgroup (256 lanes) takes 256*V/64 active 64-position segments from the
+// segment list, each staging its nine neighbour-row bit windows in LDS.
+// Directions with cz = +-1 read 4-byte-aligned (not 16-byte-aligned)
+// vectors.  A population whose pulled source is a wall for some of the lane's
+// cells is read both ways and selected per cell (halfway bounce-back).
+template <typename S, typename M, bool ACC, bool NS>
+__global__ __launch_bounds__(256) void k_stream_collide_wide(StepArgs a) {
+  constexpr int V = 16 / (int)sizeof(S);  // positions per lane
+  constexpr int LPS = 64 / V;             // lanes per 64-position segment
+  constexpr int SPB = 256 / LPS;          // segments per workgroup
+  constexpr int WW = 5;                   // window words: positions seg-32 .. seg+128
+  constexpr uint32_t FULL = (1u << V) - 1;
+  __shared__ uint32_t win[SPB][9][WW];
+  const int t = threadIdx.x, sl = t / LPS, l = t - sl * LPS;
+  const int blk = a.xcd_map ? (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3) : (int)blockIdx.x;
+  const int seg0 = blk * SPB;
+  if (seg0 >= a.nchunks) return;  // whole workgroup leaves together
+  for (int u = t; u < SPB * 9 * WW; u += 256) {
+    const int ss = u / (9 * WW), r = u - ss * (9 * WW), wi = r / WW, wo = r - wi * WW;
+    uint32_t val = 0;
+    if (seg0 + ss < a.nchunks) {
+      const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
+      const int64_t base = (int64_t)a.chunks[seg0 + ss] - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
+      val = a.fluid_bits[(base >> 5) + wo];
+    }
+    win[ss][wi][wo] = val;
+  }
+  __syncthreads();
+  if (seg0 + sl >= a.nchunks) return;
+  // V neighbour bits starting at window position p
+  auto bits = [&](int wi, int p) -> uint32_t {
+    const uint64_t w = ((uint64_t)win[sl][wi][(p >> 5) + 1] << 32) | win[sl][wi][p >> 5];
+    return (uint32_t)(w >> (p & 31)) & FULL;
+  };
+  const int pl = V * l + 32;
+  const uint32_t own = bits(4, pl);
+  if (!own) return;
+  const int b = blockIdx.y;
+  const int d = a.chunks[seg0 + sl] + V * l;
+  const uint32_t tb = (uint32_t)d & 31u;
+  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
+  const uint32_t inl = (tw[0] >> tb) & FULL, outl = (tw[1] >> tb) & FULL, por = (tw[2] >> tb) & FULL;
+  const int64_t P = a.plane;
+  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
+  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
+
+  S f[19][V];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
+    const uint32_t nb = bits(wi, pl - CZ[q]) & own;
+    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+    if (nb == own) {
+      vload<S, V>(fin + q * P + (d - off), f[q]);
+    } else if (nb == 0) {
+      vload<S, V>(fin + OPP[q] * P + d, f[q]);
+    } else {
+      S bb[V];
+      vload<S, V>(fin + q * P + (d - off), f[q]);
+      vload<S, V>(fin + OPP[q] * P + d, bb);
+#pragma unroll
+      for (int c = 0; c < V; ++c)
+        if (!((nb >> c) & 1u)) f[q][c] = bb[c];
+    }
+  }
+  float nsv[V];
+  if (NS) vload<float, V>(a.ns + (int64_t)b * P + d, nsv);
+  S acc_u[4][V];
+  if (ACC) {
+    const S* ac = (const S*)a.acc + (int64_t)b * 4 * P;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) vload<S, V>(ac + m * P + d, acc_u[m]);
+  }
+#pragma unroll
+  for (int c = 0; c < V; ++c) {
+    if (!((own >> c) & 1u)) continue;
+    M fi[19], fo[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fi[q] = (M)f[q][c];
+    M ux, uy, uz, rho;
+    collide_any<M, NS>(fi, (inl >> c) & 1u, (outl >> c) & 1u, (por >> c) & 1u, NS ? (M)nsv[c] : (M)0, a, fo, ux,
+                       uy, uz, rho);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q][c] = (S)fo[q];
+    if (ACC) {
+      acc_u[0][c] = (S)((M)acc_u[0][c] + ux);
+      acc_u[1][c] = (S)((M)acc_u[1][c] + uy);
+      acc_u[2][c] = (S)((M)acc_u[2][c] + uz);
+      acc_u[3][c] = (S)((M)acc_u[3][c] + rho);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 19; ++q) vstore<S, V>(fout + q * P + d, f[q]);
+  if (ACC) {
+    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) vstore<S, V>(ac + m * P + d, acc_u[m]);
   }
 }
 
@@ -322,7 +670,7 @@ struct abx_sim {
   abx_dev* dev = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int prec = ABX_F32, B = 1, xcd_map = 1;
+  int prec = ABX_F32, B = 1, xcd_map = 0;
   int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
   int64_t ntot = 0, plane = 0, NF = 0;
   size_t ssz = 4;  // bytes per stored value
@@ -333,8 +681,11 @@ struct abx_sim {
   uint32_t* fluid_alloc = nullptr;
   const uint32_t* fluid_bits = nullptr;
   uint32_t* type_bits = nullptr;
-  int32_t* chunks = nullptr;
+  int32_t* chunks = nullptr;      // active 256-position chunks
   int nchunks = 0;
+  int32_t* chunks64 = nullptr;    // active 64-position chunks
+  int nchunks64 = 0;
+  int kernel = 2, chunk = 256, nt = 0;  // 0 wide, 1 narrow, 2 segment (default), 3 persistent
   int32_t* cells = nullptr;
   int32_t* plist = nullptr;
   double* vy = nullptr;
@@ -370,9 +721,10 @@ StepArgs step_args(abx_sim* s, int doacc) {
   a.ns = s->ns;
   a.fluid_bits = s->fluid_bits;
   a.type_bits = s->type_bits;
-  a.chunks = s->chunks;
-  a.nchunks = s->nchunks;
-  a.per_xcd = (s->nchunks + 7) / 8;
+  const bool seg64 = s->kernel != 1 || s->chunk == 64;
+  a.chunks = seg64 ? s->chunks64 : s->chunks;
+  a.nchunks = seg64 ? s->nchunks64 : s->nchunks;
+  a.per_xcd = (a.nchunks + 7) / 8;
   a.xcd_map = s->xcd_map;
   a.nyp = s->nyp;
   a.nzp = s->nzp;
@@ -385,26 +737,96 @@ StepArgs step_args(abx_sim* s, int doacc) {
   return a;
 }
 
-template <typename S, typename M>
-void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
+template <typename S, typename M, int CH, bool NTL, bool NTS>
+void launch_step_c(abx_sim* s, const StepArgs& a, int doacc) {
   const unsigned gx = a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)a.nchunks;
-  dim3 grid(gx, (unsigned)s->B), block(CHUNK);
+  dim3 grid(gx, (unsigned)s->B), block(CH);
   const bool ns = a.ns != nullptr;
   if (doacc) {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide<S, M, true, true>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide<S, M, true, true, CH, NTL, NTS>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide<S, M, true, false>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide<S, M, true, false, CH, NTL, NTS>), grid, block, 0, s->stream, a);
   } else {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide<S, M, false, true>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide<S, M, false, true, CH, NTL, NTS>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide<S, M, false, false>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide<S, M, false, false, CH, NTL, NTS>), grid, block, 0, s->stream, a);
+  }
+}
+
+template <typename S, typename M>
+void launch_wide(abx_sim* s, StepArgs a, int doacc) {
+  constexpr int SPB = 256 / (64 / (16 / (int)sizeof(S)));
+  const int nblk = (a.nchunks + SPB - 1) / SPB;
+  a.per_xcd = (nblk + 7) / 8;
+  const unsigned gx = a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk;
+  dim3 grid(gx, (unsigned)s->B), block(256);
+  const bool ns = a.ns != nullptr;
+  if (doacc) {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_wide<S, M, true, true>), grid, block, 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide_wide<S, M, true, false>), grid, block, 0, s->stream, a);
+  } else {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_wide<S, M, false, true>), grid, block, 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide_wide<S, M, false, false>), grid, block, 0, s->stream, a);
+  }
+}
+
+template <typename S, typename M, bool PERSIST>
+void launch_seg(abx_sim* s, const StepArgs& a, int doacc) {
+  const int nblk = (a.nchunks + 3) / 4;
+  const unsigned gx = PERSIST ? (unsigned)std::min(nblk, s->dev->ncu * 8) : (unsigned)nblk;
+  dim3 grid(gx, (unsigned)s->B), block(256);
+  const bool ns = a.ns != nullptr;
+  if (doacc) {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, PERSIST>), grid, block, 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, PERSIST>), grid, block, 0, s->stream, a);
+  } else {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, PERSIST>), grid, block, 0, s->stream, a);
+    else
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, PERSIST>), grid, block, 0, s->stream, a);
+  }
+}
+
+template <typename S, typename M>
+void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
+  const bool ntl = s->nt & 1, nts = (s->nt >> 1) & 1;
+  if (s->kernel == 0) {
+    launch_wide<S, M>(s, a, doacc);
+  } else if (s->kernel == 2) {
+    launch_seg<S, M, false>(s, a, doacc);
+  } else if (s->kernel == 3) {
+    launch_seg<S, M, true>(s, a, doacc);
+  } else if (s->chunk == 64) {
+    if (ntl && nts)
+      launch_step_c<S, M, 64, true, true>(s, a, doacc);
+    else if (ntl)
+      launch_step_c<S, M, 64, true, false>(s, a, doacc);
+    else if (nts)
+      launch_step_c<S, M, 64, false, true>(s, a, doacc);
+    else
+      launch_step_c<S, M, 64, false, false>(s, a, doacc);
+  } else {
+    if (ntl && nts)
+      launch_step_c<S, M, 256, true, true>(s, a, doacc);
+    else if (ntl)
+      launch_step_c<S, M, 256, true, false>(s, a, doacc);
+    else if (nts)
+      launch_step_c<S, M, 256, false, true>(s, a, doacc);
+    else
+      launch_step_c<S, M, 256, false, false>(s, a, doacc);
   }
 }
 
 int launch_step(abx_sim* s, int doacc) {
-  if (s->nchunks == 0) return 0;
+  if ((s->kernel != 1 || s->chunk == 64 ? s->nchunks64 : s->nchunks) == 0) return 0;
   StepArgs a = step_args(s, doacc);
   switch (s->prec) {
     case ABX_F32:
@@ -467,8 +889,8 @@ int upload_ns(abx_sim* s, const float* ns_cid, const std::vector<int32_t>& cells
 void sim_free(abx_sim* s) {
   if (!s) return;
   if (s->stream) hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->f[0], s->f[1], s->acc, s->ns, s->fluid_alloc, s->type_bits, s->chunks, s->cells, s->plist,
-                  s->vy, s->red};
+  void* ptrs[] = {s->f[0], s->f[1], s->acc,  s->ns, s->fluid_alloc, s->type_bits, s->chunks, s->chunks64,
+                  s->cells, s->plist, s->vy, s->red};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (s->red_host) hipHostFree(s->red_host);
@@ -578,13 +1000,16 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->sig = rp->K * 0.5 * s->u0 * s->u0 / (uface * std::max(Lp, 1));
   s->cs2s = 18 * 1.4142135623730951 * 0.17 * 0.17;
 
-  std::vector<int32_t> chunks;
+  std::vector<int32_t> chunks, chunks64;
   for (int64_t c0 = 0; c0 < s->plane; c0 += CHUNK) {
     uint32_t any = 0;
     for (int w = 0; w < CHUNK / 32; ++w) any |= fbits[(c0 >> 5) + w];
     if (any) chunks.push_back((int32_t)c0);
   }
+  for (int64_t c0 = 0; c0 < s->plane; c0 += 64)
+    if (fbits[c0 >> 5] | fbits[(c0 >> 5) + 1]) chunks64.push_back((int32_t)c0);
   s->nchunks = (int)chunks.size();
+  s->nchunks64 = (int)chunks64.size();
   std::vector<int32_t> plist((size_t)(s->nM + s->nT));
   for (int n = 0; n < s->nM; ++n) plist[(size_t)n] = cells[(size_t)lM[(size_t)n]];
   for (int n = 0; n < s->nT; ++n) plist[(size_t)(s->nM + n)] = cells[(size_t)lT[(size_t)n]];
@@ -609,6 +1034,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   rc |= rc ? 0 : dalloc(s, &s->fluid_alloc, fb.size());
   rc |= rc ? 0 : dalloc(s, &s->type_bits, typ.size());
   rc |= rc ? 0 : dalloc(s, &s->chunks, std::max<size_t>(chunks.size(), 1));
+  rc |= rc ? 0 : dalloc(s, &s->chunks64, std::max<size_t>(chunks64.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->cells, std::max<size_t>(cells.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->plist, std::max<size_t>(plist.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->vy, std::max<size_t>(plist.size(), 1));
@@ -627,7 +1053,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
   if (up(s->fluid_alloc, fb.data(), fb.size() * 4) != hipSuccess || up(s->type_bits, typ.data(), typ.size() * 4) ||
-      up(s->chunks, chunks.data(), chunks.size() * 4) || up(s->cells, cells.data(), cells.size() * 4) ||
+      up(s->chunks, chunks.data(), chunks.size() * 4) || up(s->chunks64, chunks64.data(), chunks64.size() * 4) || up(s->cells, cells.data(), cells.size() * 4) ||
       up(s->plist, plist.data(), plist.size() * 4)) {
     set_error("abx_sim_create: upload failed");
     sim_free(s);
@@ -874,12 +1300,15 @@ extern "C" int abx_sim_sync(abx_sim* s) {
   return 0;
 }
 
-extern "C" int abx_sim_set_launch(abx_sim* s, int xcd_map) {
-  if (!s) {
-    set_error("abx_sim_set_launch: null argument");
+extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk, int nt) {
+  if (!s || kernel < 0 || kernel > 3 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
+    set_error("abx_sim_set_launch: bad argument (kernel 0..3, chunk 64 or 256, nt 0..3)");
     return -1;
   }
+  s->kernel = kernel;
   s->xcd_map = xcd_map ? 1 : 0;
+  s->chunk = chunk;
+  s->nt = nt;
   return 0;
 }
 

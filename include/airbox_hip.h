@@ -50,6 +50,10 @@ const char* abx_version(void);
 int abx_device_open(int ordinal, abx_dev** dev, char* name, size_t name_len);
 void abx_device_close(abx_dev* dev);
 
+/* Practical HBM ceiling (no reference counterpart): GB/s of a plain float4
+ * copy of `bytes` (read + write counted), averaged over iters launches. */
+int abx_copy_bandwidth(abx_dev* dev, int64_t bytes, int iters, double* gbps, double* ms_per_copy);
+
 /* ---- STL ------------------------------------------------------------------
  * Replaces parseSTL (engine.js:15-28).  Binary when 84 + 50*n == len, else
  * the ASCII "vertex x y z" records.  Writes at most cap floats to out (may be
@@ -194,9 +198,16 @@ int abx_sim_clear_acc(abx_sim* sim);
 int abx_sim_set_ns(abx_sim* sim, const float* ns);
 /* Wait for the handle's stream. */
 int abx_sim_sync(abx_sim* sim);
-/* Tuning knobs of the stream-collide launch (no reference counterpart):
- * xcd_map 1 = each XCD sweeps one contiguous part of the lattice. */
-int abx_sim_set_launch(abx_sim* sim, int xcd_map);
+/* Tuning knobs of the stream-collide launch (no reference counterpart).
+ * kernel: 0 = wide (16-byte accesses, several positions per lane),
+ *         1 = narrow (one position per lane, chunk = 64 or 256 positions per
+ *             workgroup, nt bit 0 / bit 1 = non-temporal loads / stores),
+ *         2 = segment (one position per lane, four active 64-position
+ *             segments per workgroup; the default),
+ *         3 = segment, persistent grid.
+ * xcd_map 1 = each XCD sweeps one contiguous part of the lattice.  Every
+ * variant computes the same values bit for bit. */
+int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
 /* Average duration in ms of the stream-collide launches of the last
  * abx_sim_steps call with timing on (HIP events on the handle's stream). */
 int abx_sim_time_steps(abx_sim* sim, int n, int doacc, double* ms_total);

@@ -109,7 +109,9 @@ def test_probe_matches_oracle(precision):
     f = sim.read_f().astype(np.float32)
     gam, mr, qo = sim.probe(200)
     g2, m2, q2 = O.probe(S, f, 200)
-    tol = 1e-12 if precision != "f64" else 1e-6
+    # f32_f64 and f64 form u_y in double like probeCPU; f32 in float like the
+    # WGSL probe (engine.js:455-461), so its u_y carries float rounding
+    tol = {"f32": 1e-5, "f32_f64": 1e-12, "f64": 1e-6}[precision]
     assert gam == pytest.approx(g2, rel=tol) and mr == pytest.approx(m2, rel=tol)
 
 
@@ -159,22 +161,30 @@ def test_init_eq_cold_start_and_moments():
         np.testing.assert_allclose((c[:, None] * f).sum(0), rho * u[comp], atol=2e-7)
 
 
-def test_write_read_round_trip_and_xcd_mapping_invariance():
+@pytest.mark.parametrize("precision", ["f32", "f32_f64"])
+def test_write_read_round_trip_and_launch_variants_agree(precision):
+    """Every launch variant of the stream-collide (wide / narrow / segment /
+    persistent, XCD mapping, non-temporal accesses) computes the same step,
+    bit for bit, on an arbitrary state with averaging on."""
     fx = GU.load("small_yaw")
-    say("round trip / launch mapping")
-    _, a = _sim(fx, "f32")
-    _, b = _sim(fx, "f32")
-    b.set_launch(0)
+    say("round trip / launch variants", precision)
+    _, a = _sim(fx, precision)
     rng = np.random.default_rng(3)
     f = (np.repeat([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, a.NF) *
          rng.uniform(0.99, 1.01, 19 * a.NF)).astype(np.float32)
     a.write_f(f)
     assert np.array_equal(a.read_f(), f)
-    b.write_f(f)
     a.steps(10, doacc=True)
-    b.steps(10, doacc=True)
-    assert np.array_equal(a.read_f(), b.read_f())
-    assert np.array_equal(a.read_acc(), b.read_acc())
+    want_f, want_acc = a.read_f(), a.read_acc()
+    for kern, xm, chunk, nt in ((1, 0, 256, 0), (1, 1, 256, 3), (1, 0, 64, 1), (0, 0, 256, 0), (3, 0, 256, 0),
+                                (0, 1, 256, 0)):
+        _, b = _sim(fx, precision)
+        b.set_launch(kern, xm, chunk, nt)
+        b.write_f(f)
+        b.steps(10, doacc=True)
+        assert np.array_equal(b.read_f(), want_f), (kern, xm, chunk, nt)
+        assert np.array_equal(b.read_acc(), want_acc), (kern, xm, chunk, nt)
+        b.destroy()
 
 
 @pytest.mark.parametrize("name", ["small", "small_yaw", "c1_seed3"])
