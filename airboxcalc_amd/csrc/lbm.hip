@@ -367,7 +367,19 @@ __global__ __launch_bounds__(CH) void k_stream_collide(StepArgs a) {
 // four independent active 64-position segments (one per wavefront), so only
 // segments holding fluid occupy lanes.  PERSIST: a fixed grid whose
 // workgroups stride over the segment list.
-template <typename S, typename M, bool ACC, bool NS, bool PERSIST>
+//
+// MODE selects the propagation scheme:
+//  0  two lattices, pull: read fin[q](x - c_q) (or fin[opp q](x) behind a
+//     wall), write fout[q](x);
+//  1  AA pattern, even step, one lattice in place: read F[q](x), write the
+//     post-collision f_q to F[opp q](x);
+//  2  AA pattern, odd step: read F[opp q](x - c_q) (behind a wall: F[q](x)),
+//     write f_q to F[q](x + c_q) (towards a wall: F[opp q](x)).
+// Both schemes apply the same halfway bounce-back and give identical values;
+// the AA steps touch each population slot once for reading and once for
+// writing in place, which halves the lattice footprint and streams far more
+// efficiently (even steps are fully aligned).
+template <typename S, typename M, bool ACC, bool NS, bool PERSIST, int MODE>
 __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   __shared__ uint32_t win[4][9][4];
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
@@ -397,13 +409,21 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
       const uint32_t* tw = a.type_bits + 3 * (d >> 5);
       const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
       M fi[19], fo[19];
+      uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
 #pragma unroll
       for (int q = 0; q < 19; ++q) {
         const int p = p0 - CZ[q];
         const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
         const bool nbf = (win[wv][wi][p >> 5] >> (p & 31)) & 1u;
+        nbm |= (uint32_t)nbf << q;
         const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-        const S* src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
+        const S* src;
+        if (MODE == 0)
+          src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
+        else if (MODE == 1)
+          src = fin + q * P + d;
+        else
+          src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
         fi[q] = (M)*src;
       }
       M s = 0;
@@ -411,7 +431,18 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
       M ux, uy, uz, rho;
       collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
 #pragma unroll
-      for (int q = 0; q < 19; ++q) fout[q * P + d] = (S)fo[q];
+      for (int q = 0; q < 19; ++q) {
+        S* dst;
+        if (MODE == 0) {
+          dst = fout + q * P + d;
+        } else if (MODE == 1) {
+          dst = fout + OPP[q] * P + d;
+        } else {
+          const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+          dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
+        }
+        *dst = (S)fo[q];
+      }
       if (ACC) {
         S* ac = (S*)a.acc + (int64_t)b * 4 * P;
         ac[d] = (S)((M)ac[d] + ux);
@@ -423,6 +454,73 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
     if (!PERSIST) break;
     __syncthreads();
   }
+}
+
+// Per-cell geometry word of the metadata variant: bit q (q < 19) = the cell
+// x - c_q is fluid, bits 19/20/21 = inlet/outlet/porous, bit 31 = fluid.
+constexpr uint32_t META_INLET = 1u << 19, META_OUTLET = 1u << 20, META_POROUS = 1u << 21, META_FLUID = 1u << 31;
+
+// AA-pattern step reading one 32-bit geometry word per position instead of
+// staging bit windows in LDS: no barrier, no dependent load before the
+// population loads.  MODE 1 / 2 = even / odd AA step (as k_stream_collide_seg).
+template <typename S, typename M, bool ACC, bool NS, int MODE>
+__global__ __launch_bounds__(256) void k_stream_collide_meta(StepArgs a, const uint32_t* __restrict__ meta) {
+  const int t = threadIdx.x;
+  const int sg = 4 * blockIdx.x + (t >> 6);
+  if (sg >= a.nchunks) return;
+  const int d = a.chunks[sg] + (t & 63);
+  const uint32_t m = meta[d];
+  if (!(m & META_FLUID)) return;
+  const int b = blockIdx.y;
+  const int64_t P = a.plane;
+  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
+  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
+  M fi[19], fo[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    const bool nbf = (m >> q) & 1u;
+    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+    const S* src;
+    if (MODE == 1)
+      src = fin + q * P + d;
+    else
+      src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
+    fi[q] = (M)*src;
+  }
+  M s = 0;
+  if (NS) s = (M)a.ns[(int64_t)b * P + d];
+  M ux, uy, uz, rho;
+  collide_any<M, NS>(fi, m & META_INLET, m & META_OUTLET, m & META_POROUS, s, a, fo, ux, uy, uz, rho);
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    S* dst;
+    if (MODE == 1) {
+      dst = fout + OPP[q] * P + d;
+    } else {
+      const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+      dst = ((m >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
+    }
+    *dst = (S)fo[q];
+  }
+  if (ACC) {
+    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
+    ac[d] = (S)((M)ac[d] + ux);
+    ac[P + d] = (S)((M)ac[P + d] + uy);
+    ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
+    ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
+  }
+}
+
+// Post-collision population q of the cell at d, whatever the stored form:
+// two lattices / AA after an even step ("swapped": F[opp q](x)) or AA after
+// an odd step ("streamed": F[q](x + c_q), or F[opp q](x) towards a wall).
+template <typename S>
+__device__ __forceinline__ S post_f(const S* F, int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
+                                    int nzp) {
+  if (form == 0) return F[q * P + d];
+  if (form == 1) return F[OPP[q] * P + d];
+  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
+  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? F[q * P + dn] : F[OPP[q] * P + d];
 }
 
 // V consecutive storage values (16 bytes) at an address aligned to the
@@ -543,14 +641,15 @@ __global__ __launch_bounds__(256) void k_stream_collide_wide(StepArgs a) {
 // engine.js:455-461); the value is rounded to float as the reference's
 // Float32Array holds it, except for double storage.
 template <typename S, typename M>
-__global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy) {
+__global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy, const uint32_t* fb, int form,
+                        int nyp, int nzp) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= np) return;
   const int d = plist[n];
   M r = 0, my = 0;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const M v = (M)f[q * P + d];
+    const M v = (M)post_f(f, P, fb, d, q, form, nyp, nzp);
     r += v;
     my += (M)CY[q] * v;
   }
@@ -604,14 +703,16 @@ __global__ __launch_bounds__(1024) void k_gamma(const double* vy, int nM, int nT
   }
 }
 
-// out[c*NF + a] = src[c*P + cells[a]] (the reference's gather kernel,
-// engine.js:468-476, over whole components).
+// out[q*NF + a] = post-collision f_q of fluid cell a (the reference's gather
+// kernel, engine.js:468-476, over whole components), one thread per cell.
 template <typename S>
-__global__ void k_gather(const S* src, int64_t P, int ncomp, const int32_t* cells, int64_t NF, S* out) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= NF * ncomp) return;
-  const int64_t c = n / NF, a = n - c * NF;
-  out[n] = src[c * P + cells[a]];
+__global__ void k_gather_f(const S* F, int64_t P, const int32_t* cells, int64_t NF, S* out, const uint32_t* fb,
+                           int form, int nyp, int nzp) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= NF) return;
+  const int d = cells[a];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) out[q * NF + a] = post_f(F, P, fb, d, q, form, nyp, nzp);
 }
 
 // acc / divisor in double, stored in T (engine.js:628: Float32Array /= nacc).
@@ -623,17 +724,31 @@ __global__ void k_gather_div(const S* src, int64_t P, const int32_t* cells, int6
   out[n] = (T)((double)src[c * P + cells[a]] / div);
 }
 
+// Slot that holds post-collision population q of the cell at d in a given
+// form (the inverse of post_f).
+__device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
+                                             int nzp) {
+  if (form == 0) return q * P + d;
+  if (form == 1) return OPP[q] * P + d;
+  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
+  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? q * P + dn : OPP[q] * P + d;
+}
+
+// Store post-collision populations given in cid order in the stored form.
 template <typename S>
-__global__ void k_scatter(const S* in, int64_t P, int ncomp, const int32_t* cells, int64_t NF, S* dst) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= NF * ncomp) return;
-  const int64_t c = n / NF, a = n - c * NF;
-  dst[c * P + cells[a]] = in[n];
+__global__ void k_scatter_f(const S* in, int64_t P, const int32_t* cells, int64_t NF, S* F, const uint32_t* fb,
+                            int form, int nyp, int nzp) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= NF) return;
+  const int d = cells[a];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) F[post_slot(P, fb, d, q, form, nyp, nzp)] = in[q * NF + a];
 }
 
 // f_q = feq_q(rho, u) at the fluid cells (feq as engine.js:296).
 template <typename S, typename M>
-__global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, int64_t P, S* f) {
+__global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, int64_t P, S* f, const uint32_t* fb,
+                          int form, int nyp, int nzp) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= NF) return;
   const M rho = (M)rho_u[a], ux = (M)rho_u[NF + a], uy = (M)rho_u[2 * NF + a], uz = (M)rho_u[3 * NF + a];
@@ -642,7 +757,7 @@ __global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, 
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     const M cu = (M)3 * ((M)CX[q] * ux + (M)CY[q] * uy + (M)CZ[q] * uz);
-    f[q * P + d] = (S)((M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq));
+    f[post_slot(P, fb, d, q, form, nyp, nzp)] = (S)((M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq));
   }
 }
 
@@ -684,8 +799,14 @@ struct abx_sim {
   int32_t* chunks = nullptr;      // active 256-position chunks
   int nchunks = 0;
   int32_t* chunks64 = nullptr;    // active 64-position chunks
+  uint32_t* meta = nullptr;       // per-position geometry words (kernel 5)
   int nchunks64 = 0;
-  int kernel = 2, chunk = 256, nt = 0;  // 0 wide, 1 narrow, 2 segment (default), 3 persistent
+  // kernel 0 wide, 1 narrow, 2 segment, 3 segment persistent (two lattices),
+  // 4 segment AA pattern (one lattice), 5 AA pattern with per-cell geometry words
+  int kernel = 4, chunk = 256, nt = 0;
+  // stored form of the populations: 0 plain (f[cur][q](x) = f_q), 1 AA
+  // swapped (F[opp q](x) = f_q), 2 AA streamed (after an odd AA step)
+  int form = 1;
   int32_t* cells = nullptr;
   int32_t* plist = nullptr;
   double* vy = nullptr;
@@ -713,10 +834,13 @@ int dalloc(abx_sim* s, T** p, size_t count) {
   return 0;
 }
 
+int ensure_second_lattice(abx_sim* s);
+int ensure_meta(abx_sim* s);
+
 StepArgs step_args(abx_sim* s, int doacc) {
   StepArgs a;
   a.fin = s->f[s->cur];
-  a.fout = s->f[s->cur ^ 1];
+  a.fout = s->kernel >= 4 ? s->f[s->cur] : s->f[s->cur ^ 1];
   a.acc = doacc ? s->acc : nullptr;
   a.ns = s->ns;
   a.fluid_bits = s->fluid_bits;
@@ -776,7 +900,7 @@ void launch_wide(abx_sim* s, StepArgs a, int doacc) {
   }
 }
 
-template <typename S, typename M, bool PERSIST>
+template <typename S, typename M, bool PERSIST, int MODE>
 void launch_seg(abx_sim* s, const StepArgs& a, int doacc) {
   const int nblk = (a.nchunks + 3) / 4;
   const unsigned gx = PERSIST ? (unsigned)std::min(nblk, s->dev->ncu * 8) : (unsigned)nblk;
@@ -784,26 +908,54 @@ void launch_seg(abx_sim* s, const StepArgs& a, int doacc) {
   const bool ns = a.ns != nullptr;
   if (doacc) {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, PERSIST>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, PERSIST, MODE>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, PERSIST>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, PERSIST, MODE>), grid, block, 0, s->stream, a);
   } else {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, PERSIST>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, PERSIST, MODE>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, PERSIST>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, PERSIST, MODE>), grid, block, 0, s->stream, a);
+  }
+}
+
+template <typename S, typename M, int MODE>
+void launch_meta(abx_sim* s, const StepArgs& a, int doacc) {
+  dim3 grid((unsigned)((a.nchunks + 3) / 4), (unsigned)s->B), block(256);
+  const bool ns = a.ns != nullptr;
+  if (doacc) {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_meta<S, M, true, true, MODE>), grid, block, 0, s->stream, a, s->meta);
+    else
+      hipLaunchKernelGGL((k_stream_collide_meta<S, M, true, false, MODE>), grid, block, 0, s->stream, a, s->meta);
+  } else {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_meta<S, M, false, true, MODE>), grid, block, 0, s->stream, a, s->meta);
+    else
+      hipLaunchKernelGGL((k_stream_collide_meta<S, M, false, false, MODE>), grid, block, 0, s->stream, a, s->meta);
   }
 }
 
 template <typename S, typename M>
 void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
   const bool ntl = s->nt & 1, nts = (s->nt >> 1) & 1;
-  if (s->kernel == 0) {
+  if (s->kernel == 5) {
+    if (s->form == 2)
+      launch_meta<S, M, 1>(s, a, doacc);
+    else
+      launch_meta<S, M, 2>(s, a, doacc);
+  } else if (s->kernel == 4) {
+    // AA pattern: an odd step follows the swapped form, an even one the streamed form
+    if (s->form == 2)
+      launch_seg<S, M, false, 1>(s, a, doacc);
+    else
+      launch_seg<S, M, false, 2>(s, a, doacc);
+  } else if (s->kernel == 0) {
     launch_wide<S, M>(s, a, doacc);
   } else if (s->kernel == 2) {
-    launch_seg<S, M, false>(s, a, doacc);
+    launch_seg<S, M, false, 0>(s, a, doacc);
   } else if (s->kernel == 3) {
-    launch_seg<S, M, true>(s, a, doacc);
+    launch_seg<S, M, true, 0>(s, a, doacc);
   } else if (s->chunk == 64) {
     if (ntl && nts)
       launch_step_c<S, M, 64, true, true>(s, a, doacc);
@@ -827,6 +979,8 @@ void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
 
 int launch_step(abx_sim* s, int doacc) {
   if ((s->kernel != 1 || s->chunk == 64 ? s->nchunks64 : s->nchunks) == 0) return 0;
+  if (s->kernel < 4 && ensure_second_lattice(s)) return -1;
+  if (s->kernel == 5 && ensure_meta(s)) return -1;
   StepArgs a = step_args(s, doacc);
   switch (s->prec) {
     case ABX_F32:
@@ -839,13 +993,17 @@ int launch_step(abx_sim* s, int doacc) {
       launch_step_t<double, double>(s, a, doacc);
   }
   ABX_HIP(hipGetLastError());
-  s->cur ^= 1;
+  if (s->kernel >= 4)
+    s->form = s->form == 2 ? 1 : 2;
+  else
+    s->cur ^= 1;
   return 0;
 }
 
 int fill_cold(abx_sim* s) {
   const int64_t n = s->plane * 19 * s->B;
   for (int l = 0; l < 2; ++l) {
+    if (!s->f[l]) continue;
     if (s->prec == ABX_F64)
       hipLaunchKernelGGL(k_fill_wt<double>, dim3(nblk(n, 256)), dim3(256), 0, s->stream, (double*)s->f[l],
                          s->plane, (int)(19 * s->B));
@@ -854,7 +1012,46 @@ int fill_cold(abx_sim* s) {
                          s->plane, (int)(19 * s->B));
   }
   ABX_HIP(hipGetLastError());
+  // WT is symmetric under q -> opp q, so the fill is valid in either form
+  s->form = s->kernel >= 4 ? 1 : 0;
+  s->cur = 0;
   return 0;
+}
+
+// Per-position geometry words for kernel 5, built once from the fluid and
+// type bits.
+__global__ void k_build_meta(const uint32_t* fb, const uint32_t* tb, int64_t plane, int nyp, int nzp,
+                             uint32_t* meta) {
+  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= plane) return;
+  auto bit = [&](int64_t x) { return (fb[x >> 5] >> (x & 31)) & 1u; };
+  uint32_t m = 0;
+  if (bit(d)) {
+    m = META_FLUID;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) m |= bit(d - ((int64_t)(CX[q] * nyp + CY[q]) * nzp + CZ[q])) << q;
+    const uint32_t* w = tb + 3 * (d >> 5);
+    const uint32_t sh = (uint32_t)(d & 31);
+    m |= ((w[0] >> sh) & 1u) << 19 | ((w[1] >> sh) & 1u) << 20 | ((w[2] >> sh) & 1u) << 21;
+  }
+  meta[d] = m;
+}
+
+int ensure_meta(abx_sim* s) {
+  if (s->meta) return 0;
+  if (dalloc(s, &s->meta, (size_t)s->plane)) return -1;
+  hipLaunchKernelGGL(k_build_meta, dim3(nblk(s->plane, 256)), dim3(256), 0, s->stream, s->fluid_bits, s->type_bits,
+                     s->plane, s->nyp, s->nzp, s->meta);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+// The second lattice, needed only by the two-lattice kernels.
+int ensure_second_lattice(abx_sim* s) {
+  if (s->f[1]) return 0;
+  const size_t fcount = (size_t)(s->plane * 19 * s->B);
+  if (s->prec == ABX_F64) return dalloc(s, (double**)&s->f[1], fcount);
+  return dalloc(s, (float**)&s->f[1], fcount);
 }
 
 int zero_acc(abx_sim* s) {
@@ -871,7 +1068,7 @@ int zero_acc(abx_sim* s) {
 int upload_ns(abx_sim* s, const float* ns_cid, const std::vector<int32_t>& cells_host) {
   if (!ns_cid) {
     if (s->ns) {
-      hipFree(s->ns);
+      (void)hipFree(s->ns);
       s->device_bytes -= (int64_t)(s->plane * s->B * sizeof(float));
       s->ns = nullptr;
     }
@@ -888,15 +1085,15 @@ int upload_ns(abx_sim* s, const float* ns_cid, const std::vector<int32_t>& cells
 
 void sim_free(abx_sim* s) {
   if (!s) return;
-  if (s->stream) hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->f[0], s->f[1], s->acc,  s->ns, s->fluid_alloc, s->type_bits, s->chunks, s->chunks64,
-                  s->cells, s->plist, s->vy, s->red};
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void* ptrs[] = {s->f[0],   s->f[1],  s->acc,   s->ns,    s->fluid_alloc, s->type_bits, s->chunks,
+                  s->chunks64, s->meta, s->cells, s->plist, s->vy,    s->red};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (s->red_host) hipHostFree(s->red_host);
-  if (s->ev0) hipEventDestroy(s->ev0);
-  if (s->ev1) hipEventDestroy(s->ev1);
-  if (s->stream) hipStreamDestroy(s->stream);
+    if (p) (void)hipFree(p);
+  if (s->red_host) (void)hipHostFree(s->red_host);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -1024,11 +1221,9 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   int rc = 0;
   if (precision == ABX_F64) {
     rc |= dalloc(s, (double**)&s->f[0], fcount);
-    rc |= rc ? 0 : dalloc(s, (double**)&s->f[1], fcount);
     rc |= rc ? 0 : dalloc(s, (double**)&s->acc, (size_t)(s->plane * 4 * B));
   } else {
     rc |= dalloc(s, (float**)&s->f[0], fcount);
-    rc |= rc ? 0 : dalloc(s, (float**)&s->f[1], fcount);
     rc |= rc ? 0 : dalloc(s, (float**)&s->acc, (size_t)(s->plane * 4 * B));
   }
   rc |= rc ? 0 : dalloc(s, &s->fluid_alloc, fb.size());
@@ -1132,13 +1327,16 @@ extern "C" int abx_sim_probe(abx_sim* s, int step, double out[3]) {
   if (np > 0) {
     if (s->prec == ABX_F32)
       hipLaunchKernelGGL((k_probe<float, float>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy);
+                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
+                         s->nzp);
     else if (s->prec == ABX_F32_F64)
       hipLaunchKernelGGL((k_probe<float, double>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy);
+                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
+                         s->nzp);
     else
       hipLaunchKernelGGL((k_probe<double, double>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const double*)s->f[s->cur], s->plane, s->plist, np, s->vy);
+                         (const double*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
+                         s->nzp);
     hipLaunchKernelGGL(k_gamma, dim3(1), dim3(1024), 0, s->stream, s->vy, s->nM, s->nT, (double)s->Ain * s->u0,
                        s->red);
     ABX_HIP(hipGetLastError());
@@ -1202,13 +1400,13 @@ extern "C" int abx_sim_read_f(abx_sim* s, int b, void* out) {
   void* dout = nullptr;
   ABX_HIP(hipMalloc(&dout, (size_t)n * s->ssz));
   if (s->prec == ABX_F64)
-    hipLaunchKernelGGL(k_gather<double>, dim3(nblk(n, 256)), dim3(256), 0, s->stream,
-                       (const double*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, 19, s->cells, s->NF,
-                       (double*)dout);
+    hipLaunchKernelGGL(k_gather_f<double>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream,
+                       (const double*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, s->cells, s->NF,
+                       (double*)dout, s->fluid_bits, s->form, s->nyp, s->nzp);
   else
-    hipLaunchKernelGGL(k_gather<float>, dim3(nblk(n, 256)), dim3(256), 0, s->stream,
-                       (const float*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, 19, s->cells, s->NF,
-                       (float*)dout);
+    hipLaunchKernelGGL(k_gather_f<float>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream,
+                       (const float*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, s->cells, s->NF,
+                       (float*)dout, s->fluid_bits, s->form, s->nyp, s->nzp);
   ABX_HIP(hipGetLastError());
   ABX_HIP(hipMemcpyAsync(out, dout, (size_t)n * s->ssz, hipMemcpyDeviceToHost, s->stream));
   ABX_HIP(hipStreamSynchronize(s->stream));
@@ -1229,11 +1427,13 @@ extern "C" int abx_sim_write_f(abx_sim* s, int b, const void* in) {
   ABX_HIP(hipMemcpyAsync(din, in, (size_t)n * s->ssz, hipMemcpyHostToDevice, s->stream));
   for (int v = (b < 0 ? 0 : b); v < (b < 0 ? s->B : b + 1); ++v) {
     if (s->prec == ABX_F64)
-      hipLaunchKernelGGL(k_scatter<double>, dim3(nblk(n, 256)), dim3(256), 0, s->stream, (const double*)din,
-                         s->plane, 19, s->cells, s->NF, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane);
+      hipLaunchKernelGGL(k_scatter_f<double>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream, (const double*)din,
+                         s->plane, s->cells, s->NF, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane,
+                         s->fluid_bits, s->form, s->nyp, s->nzp);
     else
-      hipLaunchKernelGGL(k_scatter<float>, dim3(nblk(n, 256)), dim3(256), 0, s->stream, (const float*)din,
-                         s->plane, 19, s->cells, s->NF, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane);
+      hipLaunchKernelGGL(k_scatter_f<float>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream, (const float*)din,
+                         s->plane, s->cells, s->NF, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane,
+                         s->fluid_bits, s->form, s->nyp, s->nzp);
   }
   ABX_HIP(hipGetLastError());
   ABX_HIP(hipStreamSynchronize(s->stream));
@@ -1255,13 +1455,16 @@ extern "C" int abx_sim_init_eq(abx_sim* s, const float* rho_u) {
   for (int v = 0; v < s->B; ++v) {
     if (s->prec == ABX_F32)
       hipLaunchKernelGGL((k_init_eq<float, float>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells, NF,
-                         s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane);
+                         s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
+                         s->nyp, s->nzp);
     else if (s->prec == ABX_F32_F64)
       hipLaunchKernelGGL((k_init_eq<float, double>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells,
-                         NF, s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane);
+                         NF, s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
+                         s->nyp, s->nzp);
     else
       hipLaunchKernelGGL((k_init_eq<double, double>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells,
-                         NF, s->plane, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane);
+                         NF, s->plane, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
+                         s->nyp, s->nzp);
   }
   ABX_HIP(hipGetLastError());
   ABX_HIP(hipStreamSynchronize(s->stream));
@@ -1301,9 +1504,25 @@ extern "C" int abx_sim_sync(abx_sim* s) {
 }
 
 extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk, int nt) {
-  if (!s || kernel < 0 || kernel > 3 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
-    set_error("abx_sim_set_launch: bad argument (kernel 0..3, chunk 64 or 256, nt 0..3)");
+  if (!s || kernel < 0 || kernel > 5 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
+    set_error("abx_sim_set_launch: bad argument (kernel 0..5, chunk 64 or 256, nt 0..3)");
     return -1;
+  }
+  ABX_HIP(hipSetDevice(s->dev->ordinal));
+  const bool was_aa = s->kernel >= 4, to_aa = kernel >= 4;
+  if (!to_aa && ensure_second_lattice(s)) return -1;
+  if (kernel == 5 && ensure_meta(s)) return -1;
+  if (was_aa != to_aa) {
+    // carry the state over: read it in the old form, store it in the new one
+    const size_t n = (size_t)(19 * s->NF);
+    std::vector<char> host(std::max<size_t>(n * s->ssz, 1) * (size_t)s->B);
+    for (int b = 0; b < s->B; ++b)
+      if (abx_sim_read_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
+    s->kernel = kernel;
+    s->form = to_aa ? 1 : 0;
+    s->cur = 0;
+    for (int b = 0; b < s->B; ++b)
+      if (abx_sim_write_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
   }
   s->kernel = kernel;
   s->xcd_map = xcd_map ? 1 : 0;

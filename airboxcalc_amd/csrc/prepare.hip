@@ -199,7 +199,7 @@ inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 struct DBuf {
   void* p = nullptr;
   ~DBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   int alloc(size_t bytes) {
     if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) {
@@ -344,7 +344,7 @@ struct abx_geom {
   int64_t nfine = 0;
   int ordinal = 0;
   ~abx_geom() {
-    if (fine_dev) hipFree(fine_dev);
+    if (fine_dev) (void)hipFree(fine_dev);
   }
 };
 
@@ -373,7 +373,7 @@ extern "C" int abx_voxelize(abx_dev* dev, const float* T, int64_t ntri, const do
     set_error("abx_voxelize: kernel failure");
     rc = -1;
   }
-  hipStreamDestroy(st);
+  (void)hipStreamDestroy(st);
   return rc ? -1 : 0;
 }
 
@@ -500,7 +500,7 @@ extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const 
         rc = -1;
       }
     }
-    hipStreamDestroy(st);
+    (void)hipStreamDestroy(st);
     if (rc) {
       std::string m = last_error();
       return fail(m);

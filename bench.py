@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f32_f64", "f64"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
@@ -82,12 +85,19 @@ def main():
     from airboxcalc_amd import abx
     dev = abx.device(0)
     log("device", dev.name)
+    copy_bytes = 1_600_000_000
+    if a.copy_only:
+        gbps, ms = abx.copy_bandwidth(dev, copy_bytes, 20)
+        print(json.dumps({"copy_bytes": copy_bytes, "gbps": round(gbps, 1), "ms_per_copy": round(ms, 5)}))
+        return
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
     t = time.perf_counter()
     g = abx.Geometry(dev, tris, p, max_cells=0)
     log(f"prepare {time.perf_counter() - t:.2f}s grid {g.nx}x{g.ny}x{g.nz} nf={g.nf} ntri={tris.size // 9}")
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
+    if a.kernel is not None:
+        sim.set_launch(kernel=a.kernel)
     NF = sim.NF
     log(f"sim NF={NF} device bytes {sim.info.device_bytes / 2**30:.2f} GiB")
 
@@ -122,6 +132,7 @@ def main():
     log(f"timed {steps} steps in {wall:.3f}s; last gamma {hist[-1][1]:.4f} massRatio {hist[-1][2]:.4f}")
     assert np.isfinite(acc).all()
 
+    copy_gbps, _ = abx.copy_bandwidth(dev, copy_bytes, 10)
     bpu = BYTES_PER_UPDATE[a.precision]
     mlups = NF * steps / wall / 1e6
     kern_ms = ev_ms[0] / max(ev_n[0], 1)
@@ -152,8 +163,11 @@ def main():
         "hbm_gbps": round(NF * steps * bpu / wall / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "k_stream_collide (no averaging)", "avg_launch_ms": round(kern_ms, 5),
-                     "bytes_per_launch": NF * bpu, "avg_launch_ms_with_acc": round(kern_acc_ms, 5)},
+                     "kernel": "k_stream_collide_seg, AA pattern (steps without averaging)",
+                     "avg_launch_ms": round(kern_ms, 5), "bytes_per_launch": NF * bpu,
+                     "avg_launch_ms_with_acc": round(kern_acc_ms, 5),
+                     "copy_ceiling_gbps": round(copy_gbps, 1),
+                     "frac_of_copy_ceiling": round(achieved / copy_gbps, 4)},
         "cpu_baseline": None,
     }
     if not a.no_cpu:

@@ -176,8 +176,8 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     assert np.array_equal(a.read_f(), f)
     a.steps(10, doacc=True)
     want_f, want_acc = a.read_f(), a.read_acc()
-    for kern, xm, chunk, nt in ((1, 0, 256, 0), (1, 1, 256, 3), (1, 0, 64, 1), (0, 0, 256, 0), (3, 0, 256, 0),
-                                (0, 1, 256, 0)):
+    for kern, xm, chunk, nt in ((5, 0, 256, 0), (1, 0, 256, 0), (1, 1, 256, 3), (1, 0, 64, 1), (0, 0, 256, 0), (2, 0, 256, 0),
+                                (3, 0, 256, 0), (0, 1, 256, 0)):
         _, b = _sim(fx, precision)
         b.set_launch(kern, xm, chunk, nt)
         b.write_f(f)
@@ -185,6 +185,18 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
         assert np.array_equal(b.read_f(), want_f), (kern, xm, chunk, nt)
         assert np.array_equal(b.read_acc(), want_acc), (kern, xm, chunk, nt)
         b.destroy()
+    # switching scheme mid-run (AA pattern <-> two lattices) carries the state over
+    _, c = _sim(fx, precision)
+    c.write_f(f)
+    c.steps(3, doacc=True)
+    c.set_launch(kernel=2)
+    c.steps(4, doacc=True)
+    c.set_launch(kernel=5)
+    c.steps(1, doacc=True)
+    c.set_launch(kernel=4)
+    c.steps(2, doacc=True)
+    assert np.array_equal(c.read_f(), want_f)
+    assert np.array_equal(c.read_acc(), want_acc)
 
 
 @pytest.mark.parametrize("name", ["small", "small_yaw", "c1_seed3"])
