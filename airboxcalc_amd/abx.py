@@ -131,6 +131,16 @@ def load() -> C.CDLL:
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32, i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
+        "abx_sim_reset": (i32, [vp]),
+        "abx_sim_set_graph": (i32, [vp, i32]),
+        "abx_batch_create": (i32, [vp, C.POINTER(vp), i32, C.POINTER(vp)]),
+        "abx_batch_steps": (i32, [vp, i32, i32]),
+        "abx_batch_probe": (i32, [vp, i32, vp]),
+        "abx_batch_sync": (i32, [vp]),
+        "abx_batch_set_launch": (i32, [vp, i32, i32, i32]),
+        "abx_batch_tune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(d)]),
+        "abx_batch_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
+        "abx_batch_destroy": (None, [vp]),
         "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
         "abx_evaluate": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp, vp, vp, vp, vp, vp, vp]),
         "abx_streamlines": (i32, [vp, C.POINTER(EvalGeom), i32, d, i32, vp, i64, vp, i64, vp, i64,
@@ -424,9 +434,17 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, kernel: int = 0, xcd_map: int = 0, chunk: int = 256, nt: int = 0) -> None:
-        """kernel 0 = wide (default), 1 = narrow (chunk 64/256, nt bits)."""
+    def set_launch(self, kernel: int = 4, xcd_map: int = 0, chunk: int = 256, nt: int = 0) -> None:
+        """kernel 4 = AA pattern (default); 0 wide, 1 narrow, 2 segment, 3
+        persistent segment (two lattices); 5 AA with per-cell geometry words."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
+
+    def set_graph(self, enable: bool = True) -> None:
+        _check(load().abx_sim_set_graph(self.h, 1 if enable else 0))
+
+    def reset(self) -> None:
+        """Back to the cold start f0 with empty averages."""
+        _check(load().abx_sim_reset(self.h))
 
     def sync(self) -> None:
         _check(load().abx_sim_sync(self.h))
@@ -434,6 +452,53 @@ class Sim:
     def destroy(self):
         if self.h:
             load().abx_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent simulations (different geometries, grid sizes and
+    parameters) stepped together, one launch per step (abx_batch_*)."""
+
+    def __init__(self, dev: Device, sims):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h for s in self.sims])
+        self.h = C.c_void_p()
+        _check(load().abx_batch_create(dev.h, arr, len(self.sims), C.byref(self.h)))
+
+    def steps(self, n: int, doacc: bool = False) -> None:
+        _check(load().abx_batch_steps(self.h, n, 1 if doacc else 0))
+
+    def probe(self, step: int = 0) -> np.ndarray:
+        out = np.zeros(3 * len(self.sims))
+        _check(load().abx_batch_probe(self.h, step, _p(out)))
+        return out.reshape(-1, 3)
+
+    def sync(self) -> None:
+        _check(load().abx_batch_sync(self.h))
+
+    def set_launch(self, spb: int = 4, persistent: bool = False, graph: bool = False) -> None:
+        _check(load().abx_batch_set_launch(self.h, spb, 1 if persistent else 0, 1 if graph else 0))
+
+    def tune(self, trial_steps: int = 20):
+        """Autotune the launch shape; members restart from f0 afterwards."""
+        spb, pers, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(load().abx_batch_tune(self.h, trial_steps, C.byref(spb), C.byref(pers), C.byref(ms)))
+        return dict(spb=spb.value, persistent=bool(pers.value), ms_per_step=ms.value)
+
+    def time_steps(self, n: int, doacc: bool = False) -> float:
+        ms = C.c_double()
+        _check(load().abx_batch_time_steps(self.h, n, 1 if doacc else 0, C.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        if self.h:
+            load().abx_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -24,290 +24,16 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <cstring>
 #include <vector>
 
 #include "abx_common.hpp"
+#include "lbm_device.hpp"
 #include "sim_internal.hpp"
 
 namespace abx {
-
-constexpr int CHUNK = 256;
-
-struct StepArgs {
-  const void* fin;
-  void* fout;
-  void* acc;
-  const float* ns;
-  const uint32_t* fluid_bits;  // word w holds positions 32w..32w+31; readable from -margin
-  const uint32_t* type_bits;   // per 32 positions: inlet, outlet, porous words
-  const int32_t* chunks;       // first position of each active chunk
-  int nchunks, per_xcd, xcd_map;
-  int nyp, nzp;
-  int64_t plane;
-  double u0, tau0, sig, aniso, cs2s;
-};
-
-// min(x, 0.9) that, like Math.min, returns NaN for a NaN argument.
-template <typename M>
-__device__ __forceinline__ M min09(M x) {
-  return x > (M)0.9 ? (M)0.9 : x;
-}
-
-template <typename T>
-__device__ __forceinline__ T ld(const T* p, bool nt) {
-  return nt ? __builtin_nontemporal_load(p) : *p;
-}
-template <typename T>
-__device__ __forceinline__ void st(T* p, T v, bool nt) {
-  if (nt)
-    __builtin_nontemporal_store(v, p);
-  else
-    *p = v;
-}
-
-// c*x for a lattice coefficient c in {-1, 0, 1}, as the reference's
-// product but without the multiply: the term is dropped when c == 0 (see
-// csum below), negated when c == -1.
-template <typename M>
-__device__ __forceinline__ M cterm(int c, M x) {
-  return c > 0 ? x : -x;
-}
-// (cx*ux + cy*uy) + cz*uz (engine.js:288, 295) with the zero-coefficient
-// products left out.  For finite operands this equals the reference's value
-// except possibly in the sign of a zero result, which never reaches a stored
-// value (it only enters 1 + cu and cu*cu).
-template <typename M>
-__device__ __forceinline__ M dot3(int cx, int cy, int cz, M ux, M uy, M uz) {
-  M r = 0;
-  bool have = false;
-  if (cx) {
-    r = cterm(cx, ux);
-    have = true;
-  }
-  if (cy) {
-    r = have ? r + cterm(cy, uy) : cterm(cy, uy);
-    have = true;
-  }
-  if (cz) r = have ? r + cterm(cz, uz) : cterm(cz, uz);
-  return r;
-}
-
-// Collision of one cell from its pulled populations fi (engine.js:277-318):
-// writes the post-collision populations to fo and the moments the averaging
-// accumulates (physical velocity, density) to ux, uy, uz, rho.  The
-// arithmetic is the reference's in its order (cpuStep for M = double; the
-// WGSL port engine.js:395-443 for M = float), so <float storage, double M>
-// reproduces cpuStep bit for bit.  The only liberty taken: a product by a
-// lattice coefficient 0 or +-1 is not formed (adding c*x with c = 0 to a
-// running sum, or multiplying by +-1, changes no finite result), which halves
-// the instruction count of the kernel.
-template <typename M, bool NS>
-__device__ __forceinline__ void collide(const M (&fi)[19], bool inlet, bool outlet, bool porous, M s,
-                                        const StepArgs& a, M (&fo)[19], M& ux, M& uy, M& uz, M& rho) {
-  rho = 0;
-  M mx = 0, my = 0, mz = 0;
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    rho += fi[q];
-    if (CX[q]) mx += cterm(CX[q], fi[q]);
-    if (CY[q]) my += cterm(CY[q], fi[q]);
-    if (CZ[q]) mz += cterm(CZ[q], fi[q]);
-  }
-  ux = mx / rho;
-  uy = my / rho;
-  uz = mz / rho;
-  const M u0 = (M)a.u0;
-  if (inlet || outlet) {
-    if (inlet) {
-      // density from the populations that arrive from the interior (engine.js:282-284)
-      M s0 = 0, sm = 0;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        if (CZ[q] == 0)
-          s0 += fi[q];
-        else if (CZ[q] < 0)
-          sm += fi[q];
-      }
-      rho = (s0 + (M)2 * sm) / ((M)1 - u0);
-      ux = 0;
-      uy = 0;
-      uz = u0;
-    } else {
-      rho = 1;
-    }
-    const M usq = (M)1.5 * (ux * ux + uy * uy + uz * uz);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ux, uy, uz);
-      fo[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-    }
-    return;
-  }
-  M usq = (M)1.5 * (ux * ux + uy * uy + uz * uz);
-  M pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-  M fe[19];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ux, uy, uz);
-    fe[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-    const M ne = fi[q] - fe[q];
-    if (CX[q]) pxx += ne;
-    if (CY[q]) pyy += ne;
-    if (CZ[q]) pzz += ne;
-    if (CX[q] * CY[q]) pxy += cterm(CX[q] * CY[q], ne);
-    if (CX[q] * CZ[q]) pxz += cterm(CX[q] * CZ[q], ne);
-    if (CY[q] * CZ[q]) pyz += cterm(CY[q] * CZ[q], ne);
-  }
-  const M Qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + (M)2 * (pxy * pxy + pxz * pxz + pyz * pyz));
-  const M tau0 = (M)a.tau0;
-  const M tau = (M)0.5 * (tau0 + sqrt(tau0 * tau0 + (M)a.cs2s * Qn / rho));
-  if (porous) {
-    // equilibrium at the Darcy-shifted velocity (engine.js:304-312)
-    const M sg = (M)a.sig;
-    const M sl = (M)1 - min09(tau * sg * (M)a.aniso), sv = (M)1 - min09(tau * sg);
-    const M ex = ux * sl, ey = uy * sv, ez = uz * sl;
-    usq = (M)1.5 * (ex * ex + ey * ey + ez * ez);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      const M cu = (M)3 * dot3(CX[q], CY[q], CZ[q], ex, ey, ez);
-      fe[q] = (M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq);
-    }
-    ux = (M)0.5 * (ux + ex);
-    uy = (M)0.5 * (uy + ey);
-    uz = (M)0.5 * (uz + ez);
-  }
-  const M it = (M)1 / tau;
-  if (NS && s > (M)0) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fo[q] = ((M)1 - s) * (fi[q] - (fi[q] - fe[q]) * it) + s * fi[OPP[q]];
-    ux *= (M)1 - s;
-    uy *= (M)1 - s;
-    uz *= (M)1 - s;
-  } else {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fo[q] = fi[q] - (fi[q] - fe[q]) * it;
-  }
-}
-
-// FP32 collision: the same model (BGK + Smagorinsky + anisotropic Darcy
-// shift + partial bounce-back, engine.js:277-318) in a cheaper but
-// mathematically identical formulation, for the float kernel:
-//  - the equilibria of a pair +c/-c share their even part:
-//      feq(+-c) = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u);
-//  - the non-equilibrium stress comes from the second moments of f, using
-//    the D3Q19 identity sum_q c_a c_b feq_q = rho (u_a u_b + delta_ab / 3);
-//  - relaxation is f + (feq - f) / tau as one explicit FMA (contraction is
-//    otherwise off, so every launch variant rounds identically).
-// Rounding differs from the reference's order at the 1e-7 level; the result
-// is held to the reference's GPU/CPU gate (max |df| < 1e-3 after 12 steps).
-template <bool NS>
-__device__ __forceinline__ void collide_fast(const float (&fi)[19], bool inlet, bool outlet, bool porous, float s,
-                                             const StepArgs& a, float (&fo)[19], float& ux, float& uy, float& uz,
-                                             float& rho) {
-  rho = 0;
-  float mx = 0, my = 0, mz = 0;
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    rho += fi[q];
-    if (CX[q]) mx += cterm(CX[q], fi[q]);
-    if (CY[q]) my += cterm(CY[q], fi[q]);
-    if (CZ[q]) mz += cterm(CZ[q], fi[q]);
-  }
-  const float ir = 1.0f / rho;
-  ux = mx * ir;
-  uy = my * ir;
-  uz = mz * ir;
-  const float u0 = (float)a.u0;
-  float ex = ux, ey = uy, ez = uz, omega = 1.0f;
-  if (inlet || outlet) {
-    if (inlet) {
-      float s0 = 0, sm = 0;
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        if (CZ[q] == 0)
-          s0 += fi[q];
-        else if (CZ[q] < 0)
-          sm += fi[q];
-      }
-      rho = (s0 + 2.0f * sm) / (1.0f - u0);
-      ux = 0;
-      uy = 0;
-      uz = u0;
-    } else {
-      rho = 1.0f;
-    }
-    ex = ux;
-    ey = uy;
-    ez = uz;
-  } else {
-    // non-equilibrium second moments
-    float sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) {
-      if (CX[q]) sxx += fi[q];
-      if (CY[q]) syy += fi[q];
-      if (CZ[q]) szz += fi[q];
-      if (CX[q] * CY[q]) sxy += cterm(CX[q] * CY[q], fi[q]);
-      if (CX[q] * CZ[q]) sxz += cterm(CX[q] * CZ[q], fi[q]);
-      if (CY[q] * CZ[q]) syz += cterm(CY[q] * CZ[q], fi[q]);
-    }
-    const float r3 = rho * (1.0f / 3.0f);
-    const float pxx = sxx - rho * ux * ux - r3, pyy = syy - rho * uy * uy - r3, pzz = szz - rho * uz * uz - r3;
-    const float pxy = sxy - rho * ux * uy, pxz = sxz - rho * ux * uz, pyz = syz - rho * uy * uz;
-    const float Qn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-    const float tau0 = (float)a.tau0;
-    const float tau = 0.5f * (tau0 + sqrtf(tau0 * tau0 + (float)a.cs2s * Qn * ir));
-    if (porous) {
-      const float sg = (float)a.sig;
-      const float sl = 1.0f - min09(tau * sg * (float)a.aniso), sv = 1.0f - min09(tau * sg);
-      ex = ux * sl;
-      ey = uy * sv;
-      ez = uz * sl;
-      ux = 0.5f * (ux + ex);
-      uy = 0.5f * (uy + ey);
-      uz = 0.5f * (uz + ez);
-    }
-    omega = 1.0f / tau;
-  }
-  // equilibria at (rho, e): rest, then the nine +c/-c pairs (q, q+1)
-  const float usq = 1.5f * (ex * ex + ey * ey + ez * ez);
-  const bool eqonly = inlet || outlet;
-  float feq[19];
-  feq[0] = (float)wt(0) * rho * (1.0f - usq);
-#pragma unroll
-  for (int q = 1; q < 19; q += 2) {
-    const float wr = (float)wt(q) * rho;
-    const float cu = dot3(CX[q], CY[q], CZ[q], ex, ey, ez);
-    const float even = wr * (1.0f - usq + 4.5f * cu * cu), odd = 3.0f * wr * cu;
-    feq[q] = even + odd;
-    feq[q + 1] = even - odd;
-  }
-  if (eqonly) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fo[q] = feq[q];
-  } else if (NS && s > 0.0f) {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fo[q] = (1.0f - s) * fmaf(feq[q] - fi[q], omega, fi[q]) + s * fi[OPP[q]];
-    ux *= 1.0f - s;
-    uy *= 1.0f - s;
-    uz *= 1.0f - s;
-  } else {
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fo[q] = fmaf(feq[q] - fi[q], omega, fi[q]);
-  }
-}
-
-// Dispatch: float arithmetic takes the FP32 formulation, double the
-// reference's order.
-template <typename M, bool NS>
-__device__ __forceinline__ void collide_any(const M (&fi)[19], bool inlet, bool outlet, bool porous, M s,
-                                            const StepArgs& a, M (&fo)[19], M& ux, M& uy, M& uz, M& rho) {
-  if constexpr (sizeof(M) == 4)
-    collide_fast<NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
-  else
-    collide<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
-}
 
 // Narrow variant: one lattice position per lane, CH positions per
 // workgroup (64 or 256); NTL / NTS = non-temporal population loads / stores.
@@ -384,73 +110,15 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   __shared__ uint32_t win[4][9][4];
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   const int b = blockIdx.y;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
   const int nblk = (a.nchunks + 3) >> 2;
   for (int blk = blockIdx.x; blk < nblk; blk += PERSIST ? gridDim.x : nblk) {
-    if (t < 4 * 9 * 4) {
-      const int ss = t / 36, r = t - ss * 36, wi = r >> 2, wo = r & 3;
-      uint32_t val = 0;
-      const int sg = 4 * blk + ss;
-      if (sg < a.nchunks) {
-        const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-        const int64_t base = (int64_t)a.chunks[sg] - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-        val = a.fluid_bits[(base >> 5) + wo];
-      }
-      win[ss][wi][wo] = val;
+    if (t < 4 * 36) {
+      const int ss = t / 36, sg = 4 * blk + ss;
+      (&win[ss][0][0])[t - 36 * ss] = sg < a.nchunks ? seg_window_word(a, a.chunks[sg], t - 36 * ss) : 0u;
     }
     __syncthreads();
     const int sg = 4 * blk + wv;
-    const int p0 = l + 32;
-    if (sg < a.nchunks && ((win[wv][4][p0 >> 5] >> (p0 & 31)) & 1u)) {
-      const int d = a.chunks[sg] + l;
-      const uint32_t tb = (uint32_t)d & 31u;
-      const uint32_t* tw = a.type_bits + 3 * (d >> 5);
-      const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
-      M fi[19], fo[19];
-      uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        const int p = p0 - CZ[q];
-        const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-        const bool nbf = (win[wv][wi][p >> 5] >> (p & 31)) & 1u;
-        nbm |= (uint32_t)nbf << q;
-        const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-        const S* src;
-        if (MODE == 0)
-          src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
-        else if (MODE == 1)
-          src = fin + q * P + d;
-        else
-          src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
-        fi[q] = (M)*src;
-      }
-      M s = 0;
-      if (NS) s = (M)a.ns[(int64_t)b * P + d];
-      M ux, uy, uz, rho;
-      collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
-#pragma unroll
-      for (int q = 0; q < 19; ++q) {
-        S* dst;
-        if (MODE == 0) {
-          dst = fout + q * P + d;
-        } else if (MODE == 1) {
-          dst = fout + OPP[q] * P + d;
-        } else {
-          const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-          dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
-        }
-        *dst = (S)fo[q];
-      }
-      if (ACC) {
-        S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-        ac[d] = (S)((M)ac[d] + ux);
-        ac[P + d] = (S)((M)ac[P + d] + uy);
-        ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
-        ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
-      }
-    }
+    if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE>(a, win[wv], a.chunks[sg], l, b);
     if (!PERSIST) break;
     __syncthreads();
   }
@@ -511,135 +179,6 @@ __global__ __launch_bounds__(256) void k_stream_collide_meta(StepArgs a, const u
   }
 }
 
-// Post-collision population q of the cell at d, whatever the stored form:
-// two lattices / AA after an even step ("swapped": F[opp q](x)) or AA after
-// an odd step ("streamed": F[q](x + c_q), or F[opp q](x) towards a wall).
-template <typename S>
-__device__ __forceinline__ S post_f(const S* F, int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
-                                    int nzp) {
-  if (form == 0) return F[q * P + d];
-  if (form == 1) return F[OPP[q] * P + d];
-  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
-  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? F[q * P + dn] : F[OPP[q] * P + d];
-}
-
-// V consecutive storage values (16 bytes) at an address aligned to the
-// element only: one global_load_dwordx4 / global_store_dwordx4.
-template <typename S, int V>
-__device__ __forceinline__ void vload(const S* p, S (&v)[V]) {
-  __builtin_memcpy(v, p, sizeof(S) * V);
-}
-template <typename S, int V>
-__device__ __forceinline__ void vstore(S* p, const S (&v)[V]) {
-  __builtin_memcpy(p, v, sizeof(S) * V);
-}
-
-// Wide variant (the default): every lane owns V = 16 / sizeof(S)
-// consecutive positions along k and moves each population as one 16-byte
-// access, so a wavefront reads and writes 1 KiB per population.  A
-// workgroup (256 lanes) takes 256*V/64 active 64-position segments from the
-// segment list, each staging its nine neighbour-row bit windows in LDS.
-// Directions with cz = +-1 read 4-byte-aligned (not 16-byte-aligned)
-// vectors.  A population whose pulled source is a wall for some of the lane's
-// cells is read both ways and selected per cell (halfway bounce-back).
-template <typename S, typename M, bool ACC, bool NS>
-__global__ __launch_bounds__(256) void k_stream_collide_wide(StepArgs a) {
-  constexpr int V = 16 / (int)sizeof(S);  // positions per lane
-  constexpr int LPS = 64 / V;             // lanes per 64-position segment
-  constexpr int SPB = 256 / LPS;          // segments per workgroup
-  constexpr int WW = 5;                   // window words: positions seg-32 .. seg+128
-  constexpr uint32_t FULL = (1u << V) - 1;
-  __shared__ uint32_t win[SPB][9][WW];
-  const int t = threadIdx.x, sl = t / LPS, l = t - sl * LPS;
-  const int blk = a.xcd_map ? (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3) : (int)blockIdx.x;
-  const int seg0 = blk * SPB;
-  if (seg0 >= a.nchunks) return;  // whole workgroup leaves together
-  for (int u = t; u < SPB * 9 * WW; u += 256) {
-    const int ss = u / (9 * WW), r = u - ss * (9 * WW), wi = r / WW, wo = r - wi * WW;
-    uint32_t val = 0;
-    if (seg0 + ss < a.nchunks) {
-      const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-      const int64_t base = (int64_t)a.chunks[seg0 + ss] - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-      val = a.fluid_bits[(base >> 5) + wo];
-    }
-    win[ss][wi][wo] = val;
-  }
-  __syncthreads();
-  if (seg0 + sl >= a.nchunks) return;
-  // V neighbour bits starting at window position p
-  auto bits = [&](int wi, int p) -> uint32_t {
-    const uint64_t w = ((uint64_t)win[sl][wi][(p >> 5) + 1] << 32) | win[sl][wi][p >> 5];
-    return (uint32_t)(w >> (p & 31)) & FULL;
-  };
-  const int pl = V * l + 32;
-  const uint32_t own = bits(4, pl);
-  if (!own) return;
-  const int b = blockIdx.y;
-  const int d = a.chunks[seg0 + sl] + V * l;
-  const uint32_t tb = (uint32_t)d & 31u;
-  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
-  const uint32_t inl = (tw[0] >> tb) & FULL, outl = (tw[1] >> tb) & FULL, por = (tw[2] >> tb) & FULL;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
-
-  S f[19][V];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const uint32_t nb = bits(wi, pl - CZ[q]) & own;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-    if (nb == own) {
-      vload<S, V>(fin + q * P + (d - off), f[q]);
-    } else if (nb == 0) {
-      vload<S, V>(fin + OPP[q] * P + d, f[q]);
-    } else {
-      S bb[V];
-      vload<S, V>(fin + q * P + (d - off), f[q]);
-      vload<S, V>(fin + OPP[q] * P + d, bb);
-#pragma unroll
-      for (int c = 0; c < V; ++c)
-        if (!((nb >> c) & 1u)) f[q][c] = bb[c];
-    }
-  }
-  float nsv[V];
-  if (NS) vload<float, V>(a.ns + (int64_t)b * P + d, nsv);
-  S acc_u[4][V];
-  if (ACC) {
-    const S* ac = (const S*)a.acc + (int64_t)b * 4 * P;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) vload<S, V>(ac + m * P + d, acc_u[m]);
-  }
-#pragma unroll
-  for (int c = 0; c < V; ++c) {
-    if (!((own >> c) & 1u)) continue;
-    M fi[19], fo[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fi[q] = (M)f[q][c];
-    M ux, uy, uz, rho;
-    collide_any<M, NS>(fi, (inl >> c) & 1u, (outl >> c) & 1u, (por >> c) & 1u, NS ? (M)nsv[c] : (M)0, a, fo, ux,
-                       uy, uz, rho);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) f[q][c] = (S)fo[q];
-    if (ACC) {
-      acc_u[0][c] = (S)((M)acc_u[0][c] + ux);
-      acc_u[1][c] = (S)((M)acc_u[1][c] + uy);
-      acc_u[2][c] = (S)((M)acc_u[2][c] + uz);
-      acc_u[3][c] = (S)((M)acc_u[3][c] + rho);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 19; ++q) vstore<S, V>(fout + q * P + d, f[q]);
-  if (ACC) {
-    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) vstore<S, V>(ac + m * P + d, acc_u[m]);
-  }
-}
-
-// u_y = sum c_y f / sum f on the probe cells (engine.js:324-327, WGSL probe
-// engine.js:455-461); the value is rounded to float as the reference's
-// Float32Array holds it, except for double storage.
 template <typename S, typename M>
 __global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy, const uint32_t* fb, int form,
                         int nyp, int nzp) {
@@ -724,16 +263,6 @@ __global__ void k_gather_div(const S* src, int64_t P, const int32_t* cells, int6
   out[n] = (T)((double)src[c * P + cells[a]] / div);
 }
 
-// Slot that holds post-collision population q of the cell at d in a given
-// form (the inverse of post_f).
-__device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
-                                             int nzp) {
-  if (form == 0) return q * P + d;
-  if (form == 1) return OPP[q] * P + d;
-  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
-  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? q * P + dn : OPP[q] * P + d;
-}
-
 // Store post-collision populations given in cid order in the stored form.
 template <typename S>
 __global__ void k_scatter_f(const S* in, int64_t P, const int32_t* cells, int64_t NF, S* F, const uint32_t* fb,
@@ -815,6 +344,9 @@ struct abx_sim {
   int nM = 0, nT = 0, Ain = 0, Af = 0, Lp = 0;
   double u0 = 0, tau0 = 0, sig = 0, aniso = 0, cs2s = 0;
   int64_t device_bytes = 0;
+  int use_graph = 0;
+  // captured chunks of steps: (n, doacc, form, cur, kernel) -> executable graph
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -1083,9 +615,15 @@ int upload_ns(abx_sim* s, const float* ns_cid, const std::vector<int32_t>& cells
   return 0;
 }
 
+void sim_drop_graphs(abx_sim* s) {
+  for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+  s->graphs.clear();
+}
+
 void sim_free(abx_sim* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  sim_drop_graphs(s);
   void* ptrs[] = {s->f[0],   s->f[1],  s->acc,   s->ns,    s->fluid_alloc, s->type_bits, s->chunks,
                   s->chunks64, s->meta, s->cells, s->plist, s->vy,    s->red};
   for (void* p : ptrs)
@@ -1295,8 +833,55 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
-  for (int i = 0; i < n; ++i)
-    if (launch_step(s, doacc ? 1 : 0)) return -1;
+  doacc = doacc ? 1 : 0;
+  if (!s->use_graph || n < 2) {
+    for (int i = 0; i < n; ++i)
+      if (launch_step(s, doacc)) return -1;
+    return 0;
+  }
+  // a chunk of n steps depends only on (n, doacc, stored form, current
+  // lattice, kernel): capture it once, replay it afterwards
+  if (s->kernel < 4 && ensure_second_lattice(s)) return -1;
+  if (s->kernel == 5 && ensure_meta(s)) return -1;
+  const int form0 = s->form, cur0 = s->cur;
+  const auto key = std::make_tuple(n, doacc, form0, cur0, s->kernel * 1000 + s->chunk + 10 * s->nt + 7 * s->xcd_map);
+  auto it = s->graphs.find(key);
+  if (it == s->graphs.end()) {
+    hipGraph_t g;
+    ABX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int i = 0; i < n && !rc; ++i) rc = launch_step(s, doacc);
+    const hipError_t e = hipStreamEndCapture(s->stream, &g);
+    s->form = form0;
+    s->cur = cur0;
+    if (rc || e != hipSuccess) {
+      if (!rc) set_errorf("abx_sim_steps: graph capture failed: %s", hipGetErrorString(e));
+      return -1;
+    }
+    hipGraphExec_t ex;
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      set_errorf("abx_sim_steps: graph instantiation failed: %s", hipGetErrorString(e2));
+      return -1;
+    }
+    it = s->graphs.emplace(key, ex).first;
+  }
+  ABX_HIP(hipGraphLaunch(it->second, s->stream));
+  if (s->kernel >= 4) {
+    if (n % 2) s->form = form0 == 2 ? 1 : 2;
+  } else {
+    s->cur = (n % 2) ? cur0 ^ 1 : cur0;
+  }
+  return 0;
+}
+
+extern "C" int abx_sim_set_graph(abx_sim* s, int enable) {
+  if (!s) {
+    set_error("abx_sim_set_graph: null argument");
+    return -1;
+  }
+  s->use_graph = enable ? 1 : 0;
   return 0;
 }
 
@@ -1472,6 +1057,17 @@ extern "C" int abx_sim_init_eq(abx_sim* s, const float* rho_u) {
   return 0;
 }
 
+extern "C" int abx_sim_reset(abx_sim* s) {
+  if (!s) {
+    set_error("abx_sim_reset: null argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(s->dev->ordinal));
+  if (fill_cold(s) || zero_acc(s)) return -1;
+  ABX_HIP(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
 extern "C" int abx_sim_clear_acc(abx_sim* s) {
   if (!s) {
     set_error("abx_sim_clear_acc: null argument");
@@ -1486,6 +1082,7 @@ extern "C" int abx_sim_set_ns(abx_sim* s, const float* ns) {
     set_error("abx_sim_set_ns: null argument");
     return -1;
   }
+  sim_drop_graphs(s);
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   ABX_HIP(hipStreamSynchronize(s->stream));
   // cells in cid order are needed on the host to place ns
@@ -1509,6 +1106,7 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
+  sim_drop_graphs(s);
   const bool was_aa = s->kernel >= 4, to_aa = kernel >= 4;
   if (!to_aa && ensure_second_lattice(s)) return -1;
   if (kernel == 5 && ensure_meta(s)) return -1;
@@ -1534,6 +1132,25 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
 extern "C" void abx_sim_destroy(abx_sim* s) { sim_free(s); }
 
 namespace abx {
+StepArgs sim_step_args(abx_sim* s, int doacc) { return step_args(s, doacc); }
+int sim_form(abx_sim* s) { return s->form; }
+void sim_set_form(abx_sim* s, int form) { s->form = form; }
+int sim_normalize(abx_sim* s) {
+  if (s->kernel < 4 || s->form == 1) return 0;
+  const size_t n = (size_t)(19 * s->NF) * s->ssz;
+  std::vector<char> host(std::max<size_t>(n, 1) * (size_t)s->B);
+  for (int b = 0; b < s->B; ++b)
+    if (abx_sim_read_f(s, b, host.data() + (size_t)b * n)) return -1;
+  s->form = 1;
+  for (int b = 0; b < s->B; ++b)
+    if (abx_sim_write_f(s, b, host.data() + (size_t)b * n)) return -1;
+  return 0;
+}
+std::vector<int32_t> sim_segments(abx_sim* s) {
+  std::vector<int32_t> v((size_t)s->nchunks64);
+  if (s->nchunks64) (void)hipMemcpy(v.data(), s->chunks64, v.size() * 4, hipMemcpyDeviceToHost);
+  return v;
+}
 SimView sim_view(abx_sim* s) {
   SimView v;
   v.ordinal = s->dev->ordinal;

@@ -1,6 +1,8 @@
 // Read-only view of a simulation handle for the post-processing kernels.
 #pragma once
 
+#include <vector>
+
 #include "abx_common.hpp"
 
 namespace abx {
@@ -20,5 +22,16 @@ struct SimView {
 };
 
 SimView sim_view(abx_sim* s);
+
+struct StepArgs;
+// Launch arguments of the handle's next step (its current lattice pointers).
+StepArgs sim_step_args(abx_sim* s, int doacc);
+// Stored form of the populations (0 plain, 1 AA swapped, 2 AA streamed).
+int sim_form(abx_sim* s);
+void sim_set_form(abx_sim* s, int form);
+// Bring an AA-pattern handle to the swapped form (host round trip if needed).
+int sim_normalize(abx_sim* s);
+// Host copy of the handle's active 64-position segments.
+std::vector<int32_t> sim_segments(abx_sim* s);
 
 }  // namespace abx

@@ -36,6 +36,7 @@ extern "C" {
 typedef struct abx_dev abx_dev;
 typedef struct abx_geom abx_geom;
 typedef struct abx_sim abx_sim;
+typedef struct abx_batch abx_batch;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* abx_last_error(void);
@@ -194,6 +195,9 @@ int abx_sim_write_f(abx_sim* sim, int b, const void* in);
 int abx_sim_init_eq(abx_sim* sim, const float* rho_u);
 /* Replaces clearAcc (engine.js:559). */
 int abx_sim_clear_acc(abx_sim* sim);
+/* Cold start again: every variant back to f0 = WT (engine.js:260-261) and
+ * the averaged fields cleared. */
+int abx_sim_reset(abx_sim* sim);
 /* Replaces setNS (engine.js:558): B*NF floats in cid order, or NULL for 0. */
 int abx_sim_set_ns(abx_sim* sim, const float* ns);
 /* Wait for the handle's stream. */
@@ -208,11 +212,38 @@ int abx_sim_sync(abx_sim* sim);
  * xcd_map 1 = each XCD sweeps one contiguous part of the lattice.  Every
  * variant computes the same values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
+/* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
+ * (fewer launch overheads on small grids).  Results are unchanged. */
+int abx_sim_set_graph(abx_sim* sim, int enable);
 /* Average duration in ms of the stream-collide launches of the last
  * abx_sim_steps call with timing on (HIP events on the handle's stream). */
 int abx_sim_time_steps(abx_sim* sim, int n, int doacc, double* ms_total);
 /* Replaces GpuSim.destroy() (engine.js:593). */
 void abx_sim_destroy(abx_sim* sim);
+
+/* ---- batched solves ---------------------------------------------------------
+ * Generalises GpuSim's B variants of one geometry (engine.js:495-545) to n
+ * independent handles of different geometries, grid sizes and parameters,
+ * stepped by one launch per step.  Members must share the precision, hold one
+ * variant each and no ns; they are switched to the AA-pattern launch.  Each
+ * member's result is bit-identical to stepping it alone.  The member handles
+ * stay usable (probe, read, evaluate) after abx_batch_sync. */
+int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_batch** out);
+/* n steps of every member; with doacc every member accumulates its fields. */
+int abx_batch_steps(abx_batch* batch, int n, int doacc);
+/* out[3*m .. 3*m+2] = {gamma, massRatio, qo} of member m (abx_sim_probe). */
+int abx_batch_probe(abx_batch* batch, int step, double* out);
+int abx_batch_sync(abx_batch* batch);
+/* Launch shape: spb = 64-position segments per workgroup (2, 4 or 8),
+ * persistent grid, graph = capture each chunk of steps once as a HIP graph
+ * and replay it. */
+int abx_batch_set_launch(abx_batch* batch, int spb, int persistent, int graph);
+/* Autotuner: times every launch shape over trial_steps steps, keeps the
+ * fastest, then resets every member to the cold start (abx_sim_reset). */
+int abx_batch_tune(abx_batch* batch, int trial_steps, int* best_spb, int* best_persistent, double* best_ms);
+/* Time n steps with HIP events on the batch's stream. */
+int abx_batch_time_steps(abx_batch* batch, int n, int doacc, double* ms);
+void abx_batch_destroy(abx_batch* batch);
 
 /* ---- evaluation -------------------------------------------------------------
  * The geometry fields evaluate/field3d/streamlines read from the reference's
