@@ -62,6 +62,29 @@ def cpu_baseline(tris, p, run, seconds: float):
                        f"{el:.1f} s, oracle/engine_oracle.c single thread")
 
 
+def bench_fp64(a, dev):
+    """The roofline airbox (seed 1) at dx = 2 mm, 6000 steps averaging 2000,
+    in FP32 and in FP64: the error of every FP32 result against FP64, with
+    the FP64 kernel's own throughput."""
+    from airboxcalc_amd import abx, steady
+    spec, tris, p = workload(2.0, a.seed)
+    g = abx.Geometry(dev, tris, p, max_cells=0)
+    run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51, steps=a.steps, avg=a.steps // 3, every=a.every)
+    t = time.perf_counter()
+    e = steady.fp64_errors(dev, g, run, precision=a.precision)
+    wall = time.perf_counter() - t
+    sim = abx.Sim(dev, g.grid(), precision="f64", K=3.0, aniso=10.0, u=0.06, tau=0.51)
+    sim.steps(20)
+    ms = sim.time_steps(200) / 200
+    NF = sim.NF
+    sim.destroy()
+    out = {"workload": f"airbox 300x150x200 mm, dx=2 mm, seed {a.seed}, {a.steps} steps averaging {a.steps // 3}",
+           "grid": [g.nx, g.ny, g.nz], "fluid_cells": NF, "precision": a.precision, "wall_s_both_runs": round(wall, 2),
+           "fp64_kernel_ms_per_step": round(ms, 5), "fp64_glups": round(NF / ms / 1e6, 2),
+           "fp64_hbm_gbps_algorithmic": round(NF * 304 / ms / 1e6, 1), "errors": e}
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(a, dev):
     """Batched variants: 64 seeded airboxes (dims 200-360 mm, dx in {2, 2.5,
     3, 4} mm, K 1.5-6, aniso 5-20) solved together, 2000 steps averaging 500
@@ -116,8 +139,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
-    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch"],
-                    help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch")
+    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch", "fp64"],
+                    help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch; "
+                         "fp64: the FP64 error report of the roofline airbox at dx=2 mm")
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
@@ -137,6 +161,8 @@ def main():
         return
     if a.workload == "batch":
         return bench_batch(a, dev)
+    if a.workload == "fp64":
+        return bench_fp64(a, dev)
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
     t = time.perf_counter()
