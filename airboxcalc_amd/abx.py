@@ -145,6 +145,19 @@ def load() -> C.CDLL:
         "abx_evaluate": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp, vp, vp, vp, vp, vp, vp]),
         "abx_streamlines": (i32, [vp, C.POINTER(EvalGeom), i32, d, i32, vp, i64, vp, i64, vp, i64,
                                   C.POINTER(i64), C.POINTER(i64)]),
+        # flow surrogate kernels (stream first; tensors as device addresses)
+        "abx_conv3d_pack": (i32, [vp, vp, i32, i32, i32, i32, vp]),
+        "abx_conv3d_forward": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, i32, i32, i32, i32, vp, i32]),
+        "abx_conv3d_wgrad_workspace": (i64, [i32, i32, i32, i32, i32, i32, i32]),
+        "abx_conv3d_wgrad": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp, i32]),
+        "abx_maxpool2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
+        "abx_maxpool2_backward": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),
+        "abx_upsample2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
+        "abx_upsample2_backward": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
+        "abx_relu_backward": (i32, [vp, vp, i32, vp, i32, i64, i32]),
+        "abx_mse_partials": (i64, [i64]),
+        "abx_mse_loss": (i32, [vp, vp, i32, vp, i32, vp, i64, i32, vp, C.c_float, vp, vp, i32]),
+        "abx_adam": (i32, [vp, vp, vp, vp, vp, i64, C.c_float, C.c_float, C.c_float, C.c_float, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

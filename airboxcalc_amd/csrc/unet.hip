@@ -1,0 +1,709 @@
+// BF16 kernels of the 3-D U-Net flow surrogate on gfx950.
+//
+// The surrogate predicts the averaged steady fields (ux, uy, uz, rho) of a box
+// from its voxel masks; the prediction seeds the lattice at equilibrium
+// instead of the reference's rest state f = w_q (engine.js:260-261).
+//
+// Tensors are NDHWC bf16: voxel (n, z, y, x) of an N x D x H x W grid sits at
+// ((n*D + z)*H + y)*W + x, its channels contiguous.  Every tensor argument has
+// its own per-voxel stride, so a channel slice of a wider tensor (one half of
+// a skip concatenation) is read and written in place.  Channel counts of the
+// convolutions are multiples of 16, strides and channel offsets multiples of 8
+// (16-byte vector accesses).
+//
+// Convolutions ('same' zero padding, 3x3x3 or 1x1x1) are implicit GEMMs on the
+// matrix cores, v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+//   forward   Y[v][co]  = sum_{tap,ci} W[co][tap][ci] X[v+tap][ci]   rows co,  cols voxels, k (tap,ci)
+//   input     dX        = forward over dZ with W'[ci][tap][co] = W[co][26-tap][ci]
+//   weights   dW[co][tap][ci] = sum_v dZ[v][co] X[v+tap][ci]        rows co,  cols ci,     k voxels
+// A workgroup owns a 16x4x2 tile of output voxels; its input brick with the
+// one-voxel halo (18x6x4) is staged in LDS once per channel chunk and read by
+// all 27 taps.  Weight gradients are summed per workgroup in registers over a
+// strided range of tiles and written as partial sums that one more kernel
+// reduces, so the result does not depend on scheduling.
+#include "abx_common.hpp"
+
+namespace abx {
+namespace {
+
+typedef uint16_t u16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef u16 u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Round to nearest even; NaN stays NaN.
+__host__ __device__ __forceinline__ u16 f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 as_bf8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ f32x4 mfma16(u32x4 a, u32x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf8(a), as_bf8(b), c, 0, 0, 0);
+}
+
+constexpr int TX = 16, TY = 4, TZ = 2, TV = TX * TY * TZ;  // output tile, 128 voxels
+
+template <int TAPS>
+struct Brick {
+  static constexpr int HALO = TAPS == 27 ? 1 : 0;
+  static constexpr int X = TX + 2 * HALO, Y = TY + 2 * HALO, Z = TZ + 2 * HALO, V = X * Y * Z;
+};
+
+struct Tile {
+  int n, z0, y0, x0;
+};
+
+__device__ __forceinline__ Tile tile_of(int t, int D, int H, int W) {
+  const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY, ntz = (D + TZ - 1) / TZ;
+  Tile r;
+  r.x0 = (t % ntx) * TX;
+  t /= ntx;
+  r.y0 = (t % nty) * TY;
+  t /= nty;
+  r.z0 = (t % ntz) * TZ;
+  r.n = t / ntz;
+  return r;
+}
+
+__device__ __forceinline__ int64_t vox(const Tile& t, int D, int H, int W, int z, int y, int x) {
+  return (((int64_t)t.n * D + z) * H + y) * W + x;
+}
+
+// Stage channels [c0, c0+ck) of the tile's brick (zero outside the grid) as
+// s[brick voxel * sp + c].
+template <int TAPS>
+__device__ __forceinline__ void load_brick(u16* s, int sp, const u16* __restrict__ x, int xs, const Tile& tl, int D,
+                                           int H, int W, int c0, int ck) {
+  using Bk = Brick<TAPS>;
+  const int groups = ck >> 3;
+  for (int e = threadIdx.x; e < Bk::V * groups; e += blockDim.x) {
+    const int vb = e / groups, g = e - vb * groups;
+    const int bx = vb % Bk::X, r = vb / Bk::X, by = r % Bk::Y, bz = r / Bk::Y;
+    const int gx = tl.x0 + bx - Bk::HALO, gy = tl.y0 + by - Bk::HALO, gz = tl.z0 + bz - Bk::HALO;
+    u32x4 v = {0u, 0u, 0u, 0u};
+    if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
+      v = *(const u32x4*)(x + vox(tl, D, H, W, gz, gy, gx) * xs + c0 + 8 * g);
+    *(u32x4*)(s + vb * sp + 8 * g) = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward convolution (also the input gradient with repacked weights).
+// grid (tiles, Co / (16*NCT)), 256 threads; wave w computes voxel rows
+// j = 2w, 2w+1 (16 voxels each) for NCT output-channel tiles of 16.
+// wt: bf16 [Co][TAPS][Ci].  ck: channels per LDS chunk (divides Ci, <= 64).
+template <int TAPS, int NCT>
+__global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
+                                                  const float* __restrict__ bias, u16* __restrict__ y, int ys, int D,
+                                                  int H, int W, int Ci, int Co, int ck, int relu, int accum) {
+  using Bk = Brick<TAPS>;
+  __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * 72];
+  const int sp = ck + 8;
+  const Tile tl = tile_of(blockIdx.x, D, H, W);
+  const int co0 = blockIdx.y * 16 * NCT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  f32x4 acc[NCT][2];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) acc[ct][jj] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  for (int c0 = 0; c0 < Ci; c0 += ck) {
+    __syncthreads();
+    load_brick<TAPS>(sx, sp, x, xs, tl, D, H, W, c0, ck);
+    __syncthreads();
+    const int K = TAPS * ck;
+    for (int kk = 0; kk < K; kk += 32) {
+      const int k = kk + 8 * lg;
+      const int tap = k / ck, cil = k - tap * ck;
+      u32x4 av[NCT], bv[2];
+      if (tap < TAPS) {
+        const int dz = TAPS == 27 ? tap / 9 : 0, dy = TAPS == 27 ? (tap / 3) % 3 : 0, dx = TAPS == 27 ? tap % 3 : 0;
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) {
+          const int j = 2 * wave + jj;
+          const int vb = (((j >> 2) + dz) * Bk::Y + ((j & 3) + dy)) * Bk::X + lr + dx;
+          bv[jj] = *(const u32x4*)(sx + vb * sp + cil);
+        }
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          av[ct] = *(const u32x4*)(wt + ((int64_t)(co0 + ct * 16 + lr) * TAPS + tap) * Ci + c0 + cil);
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) bv[jj] = (u32x4){0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) av[ct] = (u32x4){0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj) acc[ct][jj] = mfma16(av[ct], bv[jj], acc[ct][jj]);
+    }
+  }
+
+  // C element i of lane l: output channel 4*lg + i of the tile, voxel lr of row j
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj) {
+    const int j = 2 * wave + jj;
+    const int gx = tl.x0 + lr, gy = tl.y0 + (j & 3), gz = tl.z0 + (j >> 2);
+    if (gx >= W || gy >= H || gz >= D) continue;
+    u16* yp = y + vox(tl, D, H, W, gz, gy, gx) * ys;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int co = co0 + ct * 16 + 4 * lg;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[ct][jj][i] + (bias ? bias[co + i] : 0.f);
+      if (accum) {
+        const u32x2 old = *(const u32x2*)(yp + co);
+        v[0] += bf2f((u16)(old.x & 0xffffu));
+        v[1] += bf2f((u16)(old.x >> 16));
+        v[2] += bf2f((u16)(old.y & 0xffffu));
+        v[3] += bf2f((u16)(old.y >> 16));
+      }
+      if (relu)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      u32x2 o;
+      o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+      o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      *(u32x2*)(yp + co) = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: partial sums over a strided range of tiles.
+// grid (S, co blocks * ci chunks * tap groups), 256 threads.  A workgroup owns
+// output channels [co0, co0+16*NCT), input channels [ci0, ci0+16*NCI) and taps
+// [t0, t0+tg); its tg*NCT*NCI tiles of 16x16 are dealt to the 4 waves, at most
+// 8 each.  part: fp32 [S][Co][TAPS][Ci]; partb: fp32 [S][Co] (bias gradient,
+// written by the workgroups of ci chunk 0 and tap group 0).
+template <int TAPS, int NCT, int NCI>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, int xs, const u16* __restrict__ dz,
+                                                    int dzs, float* __restrict__ part, float* __restrict__ partb,
+                                                    int D, int H, int W, int Ci, int Co, int ntiles, int tg) {
+  using Bk = Brick<TAPS>;
+  constexpr int CO = 16 * NCT, CI = 16 * NCI, SZP = CO + 8, SXP = CI + 8, MAXW = 8;
+  __shared__ __attribute__((aligned(16))) u16 sz[TV * SZP];
+  __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * SXP];
+  const int nci = Ci / CI, ngr = (TAPS + tg - 1) / tg;
+  int r = blockIdx.y;
+  const int g = r % ngr;
+  r /= ngr;
+  const int cc = r % nci, cb = r / nci;
+  const int co0 = cb * CO, ci0 = cc * CI, t0 = g * tg;
+  const int ntap = min(tg, TAPS - t0);
+  const int nq = ntap * NCT * NCI;
+  const bool do_bias = partb && cc == 0 && g == 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  f32x4 acc[MAXW];
+#pragma unroll
+  for (int u = 0; u < MAXW; ++u) acc[u] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tl = tile_of(t, D, H, W);
+    __syncthreads();
+    // dZ tile (zero outside the grid): sz[v * SZP + co]
+    for (int e = threadIdx.x; e < TV * NCT * 2; e += blockDim.x) {
+      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (gx < W && gy < H && gz < D) val = *(const u32x4*)(dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q);
+      *(u32x4*)(sz + v * SZP + 8 * q) = val;
+    }
+    load_brick<TAPS>(sx, SXP, x, xs, tl, D, H, W, ci0, CI);
+    __syncthreads();
+    if (do_bias && threadIdx.x < CO)
+      for (int v = 0; v < TV; ++v) bsum += bf2f(sz[v * SZP + threadIdx.x]);
+#pragma unroll 1
+    for (int s = 0; s < TV / 32; ++s) {
+      // k = voxels 32s + 8lg + e (e = 0..7), consecutive in x
+      const int vb0 = 32 * s + 8 * lg;
+      const int tx0 = vb0 & 15, ty = (vb0 >> 4) & 3, tz = vb0 >> 6;
+      u32x4 av[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        u16x8 a;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = sz[(vb0 + e) * SZP + ct * 16 + lr];
+        av[ct] = __builtin_bit_cast(u32x4, a);
+      }
+#pragma unroll
+      for (int u = 0; u < MAXW; ++u) {
+        const int q = wave + 4 * u;
+        if (q >= nq) break;
+        const int tap = t0 + q / (NCT * NCI), ct = (q / NCI) % NCT, it = q % NCI;
+        const int dzb = TAPS == 27 ? tap / 9 : 0, dyb = TAPS == 27 ? (tap / 3) % 3 : 0,
+                  dxb = TAPS == 27 ? tap % 3 : 0;
+        const int base = ((tz + dzb) * Bk::Y + (ty + dyb)) * Bk::X + tx0 + dxb;
+        u16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = sx[(base + e) * SXP + it * 16 + lr];
+        u32x4 avc = av[0];
+#pragma unroll
+        for (int c = 1; c < NCT; ++c)
+          if (ct == c) avc = av[c];
+        acc[u] = mfma16(avc, __builtin_bit_cast(u32x4, b), acc[u]);
+      }
+    }
+  }
+
+  // C element i of lane l: output channel 4*lg + i, input channel lr of the tile
+  float* pp = part + (int64_t)blockIdx.x * Co * TAPS * Ci;
+#pragma unroll
+  for (int u = 0; u < MAXW; ++u) {
+    const int q = wave + 4 * u;
+    if (q >= nq) break;
+    const int tap = t0 + q / (NCT * NCI), ct = (q / NCI) % NCT, it = q % NCI;
+    const int ci = ci0 + it * 16 + lr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int co = co0 + ct * 16 + 4 * lg + i;
+      pp[((int64_t)co * TAPS + tap) * Ci + ci] = acc[u][i];
+    }
+  }
+  if (do_bias && threadIdx.x < CO) partb[(int64_t)blockIdx.x * Co + co0 + threadIdx.x] = bsum;
+}
+
+// out[i] = (accum ? out[i] : 0) + sum_s part[s*n + i]
+__global__ void k_reduce_parts(const float* __restrict__ part, int S, int64_t n, float* __restrict__ out, int accum) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = accum ? out[i] : 0.f;
+  for (int k = 0; k < S; ++k) s += part[(int64_t)k * n + i];
+  out[i] = s;
+}
+
+// fp32 [Co][T][Ci] -> bf16 [Co][T][Ci], or for the input gradient the flipped
+// transpose bf16 [Ci][T][Co] with tap T-1-t.
+__global__ void k_pack_w(const float* __restrict__ w, int Co, int T, int Ci, int flip, u16* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)Co * T * Ci;
+  if (i >= n) return;
+  const int ci = (int)(i % Ci), t = (int)((i / Ci) % T), co = (int)(i / ((int64_t)Ci * T));
+  if (!flip)
+    out[i] = f2bf(w[i]);
+  else
+    out[((int64_t)ci * T + (T - 1 - t)) * Co + co] = f2bf(w[i]);
+}
+
+// ---------------------------------------------------------------------------
+// 2x2x2 max pooling; one thread per (coarse voxel, 8-channel group).
+// D, H, W: fine grid (even).
+__global__ void k_maxpool2(const u16* __restrict__ x, int xs, int N, int D, int H, int W, int C, u16* __restrict__ y,
+                           int ys) {
+  const int G = C >> 3, d = D >> 1, h = H >> 1, w = W >> 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * d * h * w * G) return;
+  const int g = (int)(i % G);
+  int64_t v = i / G;
+  const int cx = (int)(v % w), cy = (int)((v / w) % h), cz = (int)((v / ((int64_t)w * h)) % d);
+  const int n = (int)(v / ((int64_t)w * h * d));
+  float m[8];
+  for (int c = 0; c < 8; ++c) m[c] = -__builtin_inff();
+  for (int o = 0; o < 8; ++o) {
+    const int64_t fv = (((int64_t)n * D + 2 * cz + (o >> 2)) * H + 2 * cy + ((o >> 1) & 1)) * W + 2 * cx + (o & 1);
+    const u16x8 a = *(const u16x8*)(x + fv * xs + 8 * g);
+    for (int c = 0; c < 8; ++c) m[c] = fmaxf(m[c], bf2f(a[c]));
+  }
+  u16x8 o8;
+  for (int c = 0; c < 8; ++c) o8[c] = f2bf(m[c]);
+  *(u16x8*)(y + v * ys + 8 * g) = o8;
+}
+
+// Gradient of k_maxpool2: dy goes to the first maximum (fine order z, y, x).
+__global__ void k_maxpool2_bwd(const u16* __restrict__ x, int xs, const u16* __restrict__ dy, int dys, int N, int D,
+                               int H, int W, int C, u16* __restrict__ dx, int dxs, int accum) {
+  const int G = C >> 3, d = D >> 1, h = H >> 1, w = W >> 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * d * h * w * G) return;
+  const int g = (int)(i % G);
+  int64_t v = i / G;
+  const int cx = (int)(v % w), cy = (int)((v / w) % h), cz = (int)((v / ((int64_t)w * h)) % d);
+  const int n = (int)(v / ((int64_t)w * h * d));
+  int64_t fv[8];
+  float m[8];
+  int am[8];
+  for (int c = 0; c < 8; ++c) {
+    m[c] = -__builtin_inff();
+    am[c] = 0;
+  }
+  for (int o = 0; o < 8; ++o) {
+    fv[o] = (((int64_t)n * D + 2 * cz + (o >> 2)) * H + 2 * cy + ((o >> 1) & 1)) * W + 2 * cx + (o & 1);
+    const u16x8 a = *(const u16x8*)(x + fv[o] * xs + 8 * g);
+    for (int c = 0; c < 8; ++c)
+      if (bf2f(a[c]) > m[c]) {
+        m[c] = bf2f(a[c]);
+        am[c] = o;
+      }
+  }
+  const u16x8 gy = *(const u16x8*)(dy + v * dys + 8 * g);
+  for (int o = 0; o < 8; ++o) {
+    u16* p = dx + fv[o] * dxs + 8 * g;
+    u16x8 cur = accum ? *(const u16x8*)p : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    u16x8 out;
+    for (int c = 0; c < 8; ++c) {
+      float val = accum ? bf2f(cur[c]) : 0.f;
+      if (am[c] == o) val += bf2f(gy[c]);
+      out[c] = f2bf(val);
+    }
+    *(u16x8*)p = out;
+  }
+}
+
+// Nearest 2x upsampling; one thread per (fine voxel, 8-channel group).
+// D, H, W: coarse grid.
+__global__ void k_upsample2(const u16* __restrict__ x, int xs, int N, int D, int H, int W, int C, u16* __restrict__ y,
+                            int ys) {
+  const int G = C >> 3, Df = 2 * D, Hf = 2 * H, Wf = 2 * W;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * Df * Hf * Wf * G) return;
+  const int g = (int)(i % G);
+  const int64_t v = i / G;
+  const int fx = (int)(v % Wf), fy = (int)((v / Wf) % Hf), fz = (int)((v / ((int64_t)Wf * Hf)) % Df);
+  const int n = (int)(v / ((int64_t)Wf * Hf * Df));
+  const int64_t cv = (((int64_t)n * D + (fz >> 1)) * H + (fy >> 1)) * W + (fx >> 1);
+  *(u16x8*)(y + v * ys + 8 * g) = *(const u16x8*)(x + cv * xs + 8 * g);
+}
+
+// Gradient of k_upsample2: dx[coarse] = sum of the 8 fine dy (FP32 sum).
+__global__ void k_upsample2_bwd(const u16* __restrict__ dy, int dys, int N, int D, int H, int W, int C,
+                                u16* __restrict__ dx, int dxs) {
+  const int G = C >> 3, Hf = 2 * H, Wf = 2 * W;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N * D * H * W * G) return;
+  const int g = (int)(i % G);
+  const int64_t v = i / G;
+  const int cx = (int)(v % W), cy = (int)((v / W) % H), cz = (int)((v / ((int64_t)W * H)) % D);
+  const int n = (int)(v / ((int64_t)W * H * D));
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int o = 0; o < 8; ++o) {
+    const int64_t fv = (((int64_t)n * 2 * D + 2 * cz + (o >> 2)) * Hf + 2 * cy + ((o >> 1) & 1)) * Wf + 2 * cx + (o & 1);
+    const u16x8 a = *(const u16x8*)(dy + fv * dys + 8 * g);
+    for (int c = 0; c < 8; ++c) s[c] += bf2f(a[c]);
+  }
+  u16x8 o8;
+  for (int c = 0; c < 8; ++c) o8[c] = f2bf(s[c]);
+  *(u16x8*)(dx + v * dxs + 8 * g) = o8;
+}
+
+// dy *= (y > 0), in place; one thread per (voxel, 8-channel group).
+__global__ void k_relu_bwd(u16* __restrict__ dy, int dys, const u16* __restrict__ y, int ys, int64_t nvox, int C) {
+  const int G = C >> 3;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvox * G) return;
+  const int64_t v = i / G;
+  const int g = (int)(i % G);
+  const u16x8 a = *(const u16x8*)(y + v * ys + 8 * g);
+  u16x8 d = *(const u16x8*)(dy + v * dys + 8 * g);
+  for (int c = 0; c < 8; ++c)
+    if (!(bf2f(a[c]) > 0.f)) d[c] = 0;
+  *(u16x8*)(dy + v * dys + 8 * g) = d;
+}
+
+// Masked, channel-weighted squared error and its gradient.  One thread per
+// voxel; loss partial sums per workgroup into loss[blockIdx.x].
+__global__ __launch_bounds__(256) void k_mse(const u16* __restrict__ pred, int ps, const u16* __restrict__ tgt, int ts,
+                                             const uint8_t* __restrict__ mask, int64_t nvox, int C,
+                                             const float* __restrict__ cw, float scale, float* __restrict__ loss,
+                                             u16* __restrict__ dpred, int dps) {
+  __shared__ float red[256];
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (v < nvox) {
+    const bool m = mask[v] != 0;
+    for (int c = 0; c < dps; ++c) {
+      float gr = 0.f;
+      if (c < C && m) {
+        const float e = bf2f(pred[v * ps + c]) - bf2f(tgt[v * ts + c]);
+        l += cw[c] * e * e;
+        gr = 2.f * scale * cw[c] * e;
+      }
+      dpred[v * dps + c] = f2bf(gr);
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = red[0] * scale;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int64_t n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+int64_t tiles_of(int n, int d, int h, int w) {
+  return (int64_t)n * ((d + TZ - 1) / TZ) * ((h + TY - 1) / TY) * ((w + TX - 1) / TX);
+}
+
+int fwd_chunk(int ci) {
+  for (int c : {64, 48, 32, 16})
+    if (ci % c == 0) return c;
+  return 0;
+}
+
+// Weight-gradient launch shape: channel tiles, taps per workgroup, splits.
+struct WgShape {
+  int nct, nci, tg, S, blocks_y;
+};
+
+WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
+  WgShape s;
+  s.nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
+  s.nci = (ci % 32 == 0) ? 2 : 1;
+  s.tg = std::min(taps, 32 / (s.nct * s.nci));
+  const int ngr = (taps + s.tg - 1) / s.tg;
+  s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci)) * ngr;
+  const int64_t nt = tiles_of(n, d, h, w);
+  s.S = (int)std::max<int64_t>(1, std::min<int64_t>(nt, std::max(1, 2048 / s.blocks_y)));
+  return s;
+}
+
+bool conv_args_ok(const char* fn, const void* x, int xs, int n, int d, int h, int w, int ci, int co, int ksize) {
+  if (!x || n < 1 || d < 1 || h < 1 || w < 1 || ci < 16 || co < 16 || ci % 16 || co % 16 || (ksize != 1 && ksize != 3) ||
+      xs < ci || xs % 8 || !aligned16(x) || (int64_t)n * d * h * w > ((int64_t)1 << 31) / 2) {
+    set_errorf("%s: bad argument (channels must be multiples of 16, strides of 8, pointers 16-byte aligned)", fn);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace abx
+
+using namespace abx;
+
+extern "C" int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, int ci, int for_input_grad,
+                               uint16_t* out) {
+  if (!w || !out || co < 1 || ci < 1 || (ksize != 1 && ksize != 3)) {
+    set_error("abx_conv3d_pack: bad argument");
+    return -1;
+  }
+  const int T = ksize == 3 ? 27 : 1;
+  const int64_t n = (int64_t)co * T * ci;
+  hipLaunchKernelGGL(k_pack_w, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, co, T, ci, for_input_grad,
+                     out);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                                  const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
+                                  uint16_t* y, int ys) {
+  if (!conv_args_ok("abx_conv3d_forward", x, xs, n, d, h, w, ci, co, ksize)) return -1;
+  if (!wpack || !y || ys < co || ys % 8 || !aligned16(wpack) || !aligned16(y)) {
+    set_error("abx_conv3d_forward: bad output or weight argument");
+    return -1;
+  }
+  const int ck = fwd_chunk(ci);
+  const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
+  const dim3 grid((unsigned)tiles_of(n, d, h, w), (unsigned)(co / (16 * nct)));
+  hipStream_t s = (hipStream_t)stream;
+#define ABX_FWD(T, NC)                                                                                              \
+  hipLaunchKernelGGL((k_conv_fwd<T, NC>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h, w, ci, co, ck, \
+                     relu, accumulate)
+  if (ksize == 3) {
+    if (nct == 4) ABX_FWD(27, 4);
+    else if (nct == 2) ABX_FWD(27, 2);
+    else ABX_FWD(27, 1);
+  } else {
+    if (nct == 4) ABX_FWD(1, 4);
+    else if (nct == 2) ABX_FWD(1, 2);
+    else ABX_FWD(1, 1);
+  }
+#undef ABX_FWD
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci, int co, int ksize) {
+  if (n < 1 || d < 1 || h < 1 || w < 1 || ci < 16 || co < 16 || ci % 16 || co % 16 || (ksize != 1 && ksize != 3))
+    return -1;
+  const int T = ksize == 3 ? 27 : 1;
+  const WgShape s = wgrad_shape(n, d, h, w, ci, co, T);
+  return (int64_t)s.S * ((int64_t)co * T * ci + co) * 4;
+}
+
+extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d,
+                                int h, int w, int ci, int co, int ksize, float* workspace, int64_t ws_bytes, float* dw,
+                                float* db, int accumulate) {
+  if (!conv_args_ok("abx_conv3d_wgrad", x, xs, n, d, h, w, ci, co, ksize)) return -1;
+  if (!dz || !dw || !workspace || dzs < co || dzs % 8 || !aligned16(dz)) {
+    set_error("abx_conv3d_wgrad: bad gradient or output argument");
+    return -1;
+  }
+  const int T = ksize == 3 ? 27 : 1;
+  const WgShape sh = wgrad_shape(n, d, h, w, ci, co, T);
+  const int64_t nw = (int64_t)co * T * ci;
+  if (ws_bytes < (int64_t)sh.S * (nw + co) * 4) {
+    set_errorf("abx_conv3d_wgrad: workspace of %lld bytes is smaller than the %lld needed", (long long)ws_bytes,
+               (long long)((int64_t)sh.S * (nw + co) * 4));
+    return -1;
+  }
+  float* part = workspace;
+  float* partb = workspace + (int64_t)sh.S * nw;
+  const int nt = (int)tiles_of(n, d, h, w);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)sh.S, (unsigned)sh.blocks_y);
+#define ABX_WG(T_, A, B)                                                                                        \
+  hipLaunchKernelGGL((k_conv_wgrad<T_, A, B>), grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, \
+                     d, h, w, ci, co, nt, sh.tg)
+#define ABX_WG_T(T_)                          \
+  if (sh.nct == 4 && sh.nci == 2) ABX_WG(T_, 4, 2); \
+  else if (sh.nct == 4) ABX_WG(T_, 4, 1);     \
+  else if (sh.nct == 2 && sh.nci == 2) ABX_WG(T_, 2, 2); \
+  else if (sh.nct == 2) ABX_WG(T_, 2, 1);     \
+  else if (sh.nci == 2) ABX_WG(T_, 1, 2);     \
+  else ABX_WG(T_, 1, 1);
+  if (T == 27) {
+    ABX_WG_T(27)
+  } else {
+    ABX_WG_T(1)
+  }
+#undef ABX_WG_T
+#undef ABX_WG
+  ABX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
+  if (db) hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(co)), dim3(256), 0, s, partb, sh.S, (int64_t)co, db, accumulate);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+bool ew_ok(const char* fn, std::initializer_list<const void*> ptrs, std::initializer_list<int> strides, int c) {
+  for (const void* p : ptrs)
+    if (!p || !aligned16(p)) {
+      set_errorf("%s: null or unaligned tensor", fn);
+      return false;
+    }
+  for (int s : strides)
+    if (s < c || s % 8) {
+      set_errorf("%s: stride must be >= channels and a multiple of 8", fn);
+      return false;
+    }
+  if (c < 8 || c % 8) {
+    set_errorf("%s: channels must be a multiple of 8", fn);
+    return false;
+  }
+  return true;
+}
+}  // namespace
+
+extern "C" int abx_maxpool2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y,
+                            int ys) {
+  if (!ew_ok("abx_maxpool2", {x, y}, {xs, ys}, c)) return -1;
+  if (n < 1 || d < 2 || h < 2 || w < 2 || d % 2 || h % 2 || w % 2) {
+    set_error("abx_maxpool2: the grid must have even, positive sizes");
+    return -1;
+  }
+  const int64_t tot = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * (c / 8);
+  hipLaunchKernelGGL(k_maxpool2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y, ys);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_maxpool2_backward(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d,
+                                     int h, int w, int c, uint16_t* dx, int dxs, int accumulate) {
+  if (!ew_ok("abx_maxpool2_backward", {x, dy, dx}, {xs, dys, dxs}, c)) return -1;
+  if (n < 1 || d < 2 || h < 2 || w < 2 || d % 2 || h % 2 || w % 2) {
+    set_error("abx_maxpool2_backward: the grid must have even, positive sizes");
+    return -1;
+  }
+  const int64_t tot = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * (c / 8);
+  hipLaunchKernelGGL(k_maxpool2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, dy, dys, n, d,
+                     h, w, c, dx, dxs, accumulate);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y,
+                             int ys) {
+  if (!ew_ok("abx_upsample2", {x, y}, {xs, ys}, c)) return -1;
+  if (n < 1 || d < 1 || h < 1 || w < 1) {
+    set_error("abx_upsample2: bad grid");
+    return -1;
+  }
+  const int64_t tot = (int64_t)n * 8 * d * h * w * (c / 8);
+  hipLaunchKernelGGL(k_upsample2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y,
+                     ys);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_upsample2_backward(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c,
+                                      uint16_t* dx, int dxs) {
+  if (!ew_ok("abx_upsample2_backward", {dy, dx}, {dys, dxs}, c)) return -1;
+  if (n < 1 || d < 1 || h < 1 || w < 1) {
+    set_error("abx_upsample2_backward: bad grid");
+    return -1;
+  }
+  const int64_t tot = (int64_t)n * d * h * w * (c / 8);
+  hipLaunchKernelGGL(k_upsample2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d, h, w,
+                     c, dx, dxs);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint16_t* y, int ys, int64_t nvox, int c) {
+  if (!ew_ok("abx_relu_backward", {dy, y}, {dys, ys}, c) || nvox < 1) return -1;
+  hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(nvox * (c / 8))), dim3(256), 0, (hipStream_t)stream, dy, dys, y, ys,
+                     nvox, c);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int64_t abx_mse_partials(int64_t nvox) { return nvox < 1 ? -1 : (int64_t)blocks_for(nvox); }
+
+extern "C" int abx_mse_loss(void* stream, const uint16_t* pred, int ps, const uint16_t* target, int ts,
+                            const uint8_t* mask, int64_t nvox, int c, const float* chan_w, float scale, float* loss_parts,
+                            uint16_t* dpred, int dps) {
+  if (!pred || !target || !mask || !chan_w || !loss_parts || !dpred || nvox < 1 || c < 1 || ps < c || ts < c ||
+      dps < c) {
+    set_error("abx_mse_loss: bad argument");
+    return -1;
+  }
+  hipLaunchKernelGGL(k_mse, dim3(blocks_for(nvox)), dim3(256), 0, (hipStream_t)stream, pred, ps, target, ts, mask,
+                     nvox, c, chan_w, scale, loss_parts, dpred, dps);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_adam(void* stream, float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                        float b2, float eps, int step) {
+  if (!w || !g || !m || !v || n < 1 || step < 1) {
+    set_error("abx_adam: bad argument");
+    return -1;
+  }
+  const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, n, lr, b1, b2, eps,
+                     bc1, bc2);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}

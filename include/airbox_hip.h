@@ -286,6 +286,72 @@ int abx_streamlines(abx_sim* sim, const abx_eval_geom* g, int b, double divisor,
                     int64_t pts_cap, float* spd, int64_t spd_cap, uint32_t* off, int64_t off_cap,
                     int64_t* npts, int64_t* nlines);
 
+/* ---- Flow surrogate (3-D U-Net) kernels ---------------------------------
+ * The reference has no surrogate: every solve starts from rest, f = w_q
+ * (engine.js:260-261).  These kernels train and run a U-Net whose predicted
+ * (rho, u) seeds abx_sim_init_eq.  All launch on `stream` (a hipStream_t,
+ * NULL = the default stream) and return at once; errors are argument errors
+ * found before the launch.
+ *
+ * Tensors are bf16 (uint16_t bit patterns) in NDHWC order: voxel (n, z, y, x)
+ * of an n x d x h x w grid at ((n*d + z)*h + y)*w + x, channels contiguous,
+ * `*s` voxel strides in elements.  Convolution channel counts are multiples of
+ * 16; strides and channel offsets multiples of 8; pointers 16-byte aligned.
+ * Convolutions are 'same' (zero padding), ksize 3 (27 taps) or 1. */
+
+/* FP32 weights [co][taps][ci] -> bf16 for abx_conv3d_forward; with
+ * for_input_grad the flipped transpose [ci][taps][co] (tap t <- taps-1-t),
+ * which makes abx_conv3d_forward over dZ compute the input gradient. */
+int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, int ci, int for_input_grad, uint16_t* out);
+
+/* y = act(conv(x, w) + bias) [+ y if accumulate]; act = ReLU if relu.  bias
+ * may be NULL.  MFMA implicit GEMM, FP32 accumulation, bf16 result. */
+int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                       const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
+                       uint16_t* y, int ys);
+
+/* Bytes of FP32 workspace abx_conv3d_wgrad needs for this shape (-1 on a bad
+ * shape). */
+int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci, int co, int ksize);
+
+/* dw[co][taps][ci] (FP32) = sum over voxels of dz[v][co] * x[v + tap][ci];
+ * db[co] = sum dz[v][co] (db may be NULL); both added to the old contents if
+ * accumulate.  Deterministic: per-workgroup partial sums in `workspace`,
+ * reduced in a fixed order. */
+int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d, int h,
+                     int w, int ci, int co, int ksize, float* workspace, int64_t ws_bytes, float* dw, float* db,
+                     int accumulate);
+
+/* 2x2x2 max pooling of an n x d x h x w grid (even sizes), c channels
+ * (multiple of 8), and its gradient (to the first maximum in z, y, x order;
+ * dx overwritten, or added to if accumulate). */
+int abx_maxpool2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y, int ys);
+int abx_maxpool2_backward(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d, int h,
+                          int w, int c, uint16_t* dx, int dxs, int accumulate);
+
+/* Nearest-neighbour 2x upsampling of an n x d x h x w grid and its gradient
+ * (sum of the 8 fine values, FP32). */
+int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y, int ys);
+int abx_upsample2_backward(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c, uint16_t* dx,
+                           int dxs);
+
+/* dy *= (y > 0) in place over nvox voxels of c channels. */
+int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint16_t* y, int ys, int64_t nvox, int c);
+
+/* Number of per-workgroup loss partial sums abx_mse_loss writes. */
+int64_t abx_mse_partials(int64_t nvox);
+
+/* Masked, channel-weighted squared error over the first c channels:
+ * loss = scale * sum_v mask[v] sum_c chan_w[c] (pred - target)^2, written as
+ * abx_mse_partials(nvox) partial sums; dpred (all dps channels) = its
+ * gradient, zero outside the mask and for channels >= c. */
+int abx_mse_loss(void* stream, const uint16_t* pred, int ps, const uint16_t* target, int ts, const uint8_t* mask,
+                 int64_t nvox, int c, const float* chan_w, float scale, float* loss_parts, uint16_t* dpred, int dps);
+
+/* One Adam step (bias-corrected, step >= 1) on n FP32 weights. */
+int abx_adam(void* stream, float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+             float eps, int step);
+
 #ifdef __cplusplus
 }
 #endif

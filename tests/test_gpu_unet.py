@@ -1,0 +1,403 @@
+"""U-Net surrogate kernels on the GPU against FP32/FP64 references.
+
+Operands are rounded to bf16 before the reference sees them, so the only
+differences left are FP32 accumulation order and the bf16 rounding of the
+kernels' bf16 outputs.  Bars (written per test): bf16 outputs within
+2^-7 relative plus 2^-9 of the output's range; FP32 weight gradients within
+1e-3 relative of the range.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from airboxcalc_amd import abx
+from tests.gpu_util import say
+from tests import unet_ref as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    abx.device(0)   # gfx950 check; raises without the GPU
+    return torch.device("cuda:0")
+
+
+def bf(a) -> "torch.Tensor":
+    """float tensor -> bf16 tensor on the GPU."""
+    return torch.as_tensor(a, dtype=torch.float32).to(_dev()).to(torch.bfloat16)
+
+
+def f32(t) -> np.ndarray:
+    return t.float().cpu().numpy()
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ck(rc):
+    if rc != 0:
+        raise abx.AbxError(abx.load().abx_last_error().decode())
+
+
+def pack(W, ksize, flip=False):
+    co, taps, ci = W.shape
+    w32 = torch.as_tensor(W, dtype=torch.float32).to(_dev()).contiguous()
+    out = torch.empty(co * taps * ci, dtype=torch.bfloat16, device=_dev())
+    ck(abx.load().abx_conv3d_pack(stream(), w32.data_ptr(), co, ksize, ci, int(flip), out.data_ptr()))
+    return out, w32
+
+
+def close_bf16(got, ref, what):
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    rng = max(np.abs(ref).max(), 1e-30)
+    tol = 2.0 ** -7 * np.abs(ref) + 2.0 ** -9 * rng
+    bad = np.abs(got - ref) > tol
+    assert not bad.any(), f"{what}: {bad.sum()} of {bad.size} outside bf16 bar, max err {np.abs(got - ref).max():.3e} (range {rng:.3e})"
+
+
+CASES = [
+    # n, d, h, w, ci, co, ksize  (d, h, w not multiples of the 2x4x16 tile)
+    (2, 6, 5, 20, 16, 32, 3),
+    (1, 4, 8, 16, 48, 16, 3),
+    (1, 3, 3, 12, 192, 64, 3),
+    (1, 2, 2, 8, 64, 128, 3),
+    (2, 5, 4, 17, 16, 16, 1),
+    (1, 3, 7, 33, 96, 32, 3),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv_forward_matches_torch(case):
+    n, d, h, w, ci, co, k = case
+    say("conv forward", case)
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((n, d, h, w, ci)).astype(np.float32)
+    W = (rng.standard_normal((co, k ** 3, ci)) / np.sqrt(k ** 3 * ci)).astype(np.float32)
+    b = rng.standard_normal(co).astype(np.float32)
+    xt = bf(X)
+    wp, _ = pack(W, k)
+    bt = torch.as_tensor(b).to(_dev())
+    y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device=_dev())
+    ck(abx.load().abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(), co,
+                                     k, 1, 0, y.data_ptr(), co))
+    torch.cuda.synchronize()
+    Xr = xt.float()
+    Wr = wp.float().reshape(co, k ** 3, ci)
+    import torch.nn.functional as F
+    ref = F.relu(F.conv3d(Xr.permute(0, 4, 1, 2, 3), R.to_torch_weight(Wr, k), bt, padding=k // 2))
+    close_bf16(f32(y), ref.permute(0, 2, 3, 4, 1).cpu().numpy(), f"forward {case}")
+
+
+def test_conv_forward_brute_force():
+    """Tiny case against a float64 direct convolution (no torch in the reference)."""
+    n, d, h, w, ci, co = 1, 3, 3, 5, 16, 16
+    say("conv forward brute force")
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((n, d, h, w, ci)).astype(np.float32)
+    W = rng.standard_normal((co, 27, ci)).astype(np.float32) * 0.1
+    b = rng.standard_normal(co).astype(np.float32)
+    xt = bf(X)
+    wp, _ = pack(W, 3)
+    bt = torch.as_tensor(b).to(_dev())
+    y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device=_dev())
+    ck(abx.load().abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(), co,
+                                     3, 0, 0, y.data_ptr(), co))
+    torch.cuda.synchronize()
+    ref = R.direct_conv3d(f32(xt), f32(wp).reshape(co, 27, ci), b, 3, relu=False)
+    close_bf16(f32(y), ref, "brute force")
+
+
+def test_conv_forward_slices_and_accumulate():
+    """Input and output are channel slices of wider tensors; accumulate adds."""
+    n, d, h, w, ci, co = 1, 4, 4, 16, 32, 16
+    say("conv forward slices")
+    rng = np.random.default_rng(3)
+    big_in = bf(rng.standard_normal((n, d, h, w, ci + 24)))
+    big_out = bf(rng.standard_normal((n, d, h, w, co + 16)))
+    before = f32(big_out)
+    W = (rng.standard_normal((co, 27, ci)) / 20).astype(np.float32)
+    wp, _ = pack(W, 3)
+    xoff, yoff = 16, 8
+    ck(abx.load().abx_conv3d_forward(stream(), big_in.data_ptr() + 2 * xoff, ci + 24, n, d, h, w, ci, wp.data_ptr(),
+                                     None, co, 3, 0, 1, big_out.data_ptr() + 2 * yoff, co + 16))
+    torch.cuda.synchronize()
+    after = f32(big_out)
+    X = f32(big_in)[..., xoff:xoff + ci]
+    ref = R.direct_conv3d(X, f32(wp).reshape(co, 27, ci)) + before[..., yoff:yoff + co]
+    close_bf16(after[..., yoff:yoff + co], ref, "accumulated slice")
+    # channels outside the slice are untouched
+    assert np.array_equal(after[..., :yoff], before[..., :yoff])
+    assert np.array_equal(after[..., yoff + co:], before[..., yoff + co:])
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv_backward_matches_autograd(case):
+    n, d, h, w, ci, co, k = case
+    say("conv backward", case)
+    rng = np.random.default_rng(4)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    W = (rng.standard_normal((co, k ** 3, ci)) / np.sqrt(k ** 3 * ci)).astype(np.float32)
+    dzt = bf(rng.standard_normal((n, d, h, w, co)))
+    wp, _ = pack(W, k)
+    wb, _ = pack(W, k, flip=True)
+    lib = abx.load()
+    # input gradient: forward over dZ with the flipped transpose
+    dx = torch.empty((n, d, h, w, ci), dtype=torch.bfloat16, device=_dev())
+    ck(lib.abx_conv3d_forward(stream(), dzt.data_ptr(), co, n, d, h, w, co, wb.data_ptr(), None, ci, k, 0, 0,
+                              dx.data_ptr(), ci))
+    # weight gradient
+    ws_bytes = lib.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, k)
+    assert ws_bytes > 0
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=_dev())
+    dw = torch.empty(co * k ** 3 * ci, dtype=torch.float32, device=_dev())
+    db = torch.empty(co, dtype=torch.float32, device=_dev())
+    ck(lib.abx_conv3d_wgrad(stream(), xt.data_ptr(), ci, dzt.data_ptr(), co, n, d, h, w, ci, co, k, ws.data_ptr(),
+                            ws_bytes, dw.data_ptr(), db.data_ptr(), 0))
+    torch.cuda.synchronize()
+    import torch.nn.functional as F
+    Xr = xt.float().permute(0, 4, 1, 2, 3).contiguous().requires_grad_(True)
+    Wr = R.to_torch_weight(wp.float().reshape(co, k ** 3, ci), k).requires_grad_(True)
+    Br = torch.zeros(co, device=_dev(), requires_grad=True)
+    out = F.conv3d(Xr, Wr, Br, padding=k // 2)
+    out.backward(dzt.float().permute(0, 4, 1, 2, 3).contiguous())
+    close_bf16(f32(dx), Xr.grad.permute(0, 2, 3, 4, 1).cpu().numpy(), f"input grad {case}")
+    ref_dw = Wr.grad.permute(0, 2, 3, 4, 1).reshape(co, k ** 3, ci).cpu().numpy()
+    got_dw = dw.cpu().numpy().reshape(co, k ** 3, ci)
+    rng_w = np.abs(ref_dw).max()
+    err = np.abs(got_dw - ref_dw).max()
+    assert err <= 1e-3 * rng_w, f"weight grad {case}: max err {err:.3e} vs range {rng_w:.3e}"
+    ref_db = Br.grad.cpu().numpy()
+    assert np.abs(db.cpu().numpy() - ref_db).max() <= 1e-3 * max(np.abs(ref_db).max(), 1e-30)
+    # accumulate adds to the previous contents (old + sum of partials: FP32 order differs from 2 * sum)
+    ck(lib.abx_conv3d_wgrad(stream(), xt.data_ptr(), ci, dzt.data_ptr(), co, n, d, h, w, ci, co, k, ws.data_ptr(),
+                            ws_bytes, dw.data_ptr(), None, 1))
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(dw.cpu().numpy().reshape(co, k ** 3, ci), 2 * got_dw, rtol=0, atol=1e-5 * rng_w)
+    # deterministic: the same call gives the same bits
+    ck(lib.abx_conv3d_wgrad(stream(), xt.data_ptr(), ci, dzt.data_ptr(), co, n, d, h, w, ci, co, k, ws.data_ptr(),
+                            ws_bytes, dw.data_ptr(), None, 0))
+    torch.cuda.synchronize()
+    assert np.array_equal(dw.cpu().numpy().reshape(co, k ** 3, ci), got_dw)
+
+
+def test_wgrad_workspace_too_small_is_an_error():
+    say("wgrad workspace check")
+    lib = abx.load()
+    n, d, h, w, ci, co = 1, 2, 4, 16, 16, 16
+    need = lib.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
+    x = torch.zeros((n, d, h, w, ci), dtype=torch.bfloat16, device=_dev())
+    ws = torch.empty(need // 4, dtype=torch.float32, device=_dev())
+    dw = torch.empty(co * 27 * ci, dtype=torch.float32, device=_dev())
+    rc = lib.abx_conv3d_wgrad(stream(), x.data_ptr(), ci, x.data_ptr(), co, n, d, h, w, ci, co, 3, ws.data_ptr(),
+                              need - 4, dw.data_ptr(), None, 0)
+    assert rc != 0 and b"workspace" in lib.abx_last_error()
+
+
+def _distinct_windows(n, d, h, w, c, rng):
+    """bf16-exact values, distinct inside every 2x2x2 window."""
+    z, y, x = np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")
+    pos = ((z & 1) * 4 + (y & 1) * 2 + (x & 1))[None, ..., None]
+    base = rng.integers(0, 24, size=(n, d, h, w, c))
+    sign = np.where(rng.random((n, d, h, w, c)) < 0.5, -1, 1)
+    return (sign * (base * 8 + pos)).astype(np.float32)
+
+
+def test_pool_and_upsample_match_torch():
+    import torch.nn.functional as F
+    n, d, h, w, c = 2, 4, 6, 10, 24
+    say("pool / upsample")
+    rng = np.random.default_rng(5)
+    lib = abx.load()
+    X = bf(_distinct_windows(n, d, h, w, c, rng))
+    Y = torch.empty((n, d // 2, h // 2, w // 2, c), dtype=torch.bfloat16, device=_dev())
+    ck(lib.abx_maxpool2(stream(), X.data_ptr(), c, n, d, h, w, c, Y.data_ptr(), c))
+    dY = bf(rng.standard_normal((n, d // 2, h // 2, w // 2, c)))
+    dX = bf(rng.standard_normal((n, d, h, w, c)))      # accumulate onto this
+    dX0 = f32(dX)
+    ck(lib.abx_maxpool2_backward(stream(), X.data_ptr(), c, dY.data_ptr(), c, n, d, h, w, c, dX.data_ptr(), c, 1))
+    U = torch.empty((n, 2 * d, 2 * h, 2 * w, c), dtype=torch.bfloat16, device=_dev())
+    ck(lib.abx_upsample2(stream(), X.data_ptr(), c, n, d, h, w, c, U.data_ptr(), c))
+    dU = bf(rng.standard_normal((n, 2 * d, 2 * h, 2 * w, c)))
+    dXu = torch.empty_like(X)
+    ck(lib.abx_upsample2_backward(stream(), dU.data_ptr(), c, n, d, h, w, c, dXu.data_ptr(), c))
+    torch.cuda.synchronize()
+    Xr = X.float().permute(0, 4, 1, 2, 3).contiguous().requires_grad_(True)
+    P = F.max_pool3d(Xr, 2)
+    assert np.array_equal(f32(Y), P.detach().permute(0, 2, 3, 4, 1).cpu().numpy())
+    P.backward(dY.float().permute(0, 4, 1, 2, 3).contiguous())
+    close_bf16(f32(dX), dX0 + Xr.grad.permute(0, 2, 3, 4, 1).cpu().numpy(), "maxpool backward")
+    Xr2 = X.float().permute(0, 4, 1, 2, 3).contiguous().requires_grad_(True)
+    Ur = F.interpolate(Xr2, scale_factor=2, mode="nearest")
+    assert np.array_equal(f32(U), Ur.detach().permute(0, 2, 3, 4, 1).cpu().numpy())
+    Ur.backward(dU.float().permute(0, 4, 1, 2, 3).contiguous())
+    close_bf16(f32(dXu), Xr2.grad.permute(0, 2, 3, 4, 1).cpu().numpy(), "upsample backward")
+
+
+def _bfr(a):
+    """Round float64 values to bf16 (nearest even) and back."""
+    return torch.as_tensor(np.asarray(a, np.float32)).to(torch.bfloat16).double().numpy()
+
+
+def test_unet_forward_backward_stepwise():
+    """Every stored activation and gradient buffer of the hand-written forward
+    and backward is recomputed in float64 from the network's own stored inputs
+    (so rounding differences cannot cascade) and must agree to the bf16 bar:
+    this checks each kernel call and all the wiring (slices, skips, masks,
+    which buffer feeds which)."""
+    from airboxcalc_amd import unet as U
+    say("unet stepwise forward/backward")
+    net = U.UNet(seed=3)
+    rng = np.random.default_rng(6)
+    n, d, h, w = 2, 8, 16, 24
+    x = bf(rng.standard_normal((n, d, h, w, U.IN_CH)))
+    net.forward(x)
+    dout = bf(np.concatenate([rng.standard_normal((n, d, h, w, U.OUT_CH)),
+                              np.zeros((n, d, h, w, U.HEAD_CH - U.OUT_CH))], -1))
+    net.backward(dout)
+    torch.cuda.synchronize()
+    A = {k: f32(v).astype(np.float64) for k, v in net.A.items()}
+    G = {k: f32(v).astype(np.float64) for k, v in net.G.items()}
+    X, DO = f32(x).astype(np.float64), f32(dout).astype(np.float64)
+    Wm = net.w.cpu().numpy().astype(np.float64)
+    Gw = net.g.cpu().numpy().astype(np.float64)
+    L = net.by
+    Wb = {nm: f32(net.packs[nm][0]).astype(np.float64).reshape(l.co, l.taps, l.ci) for nm, l in L.items()}
+    B = {nm: Wm[l.b_off:l.b_off + l.co] for nm, l in L.items()}
+    w1, w2, w3, w4 = U.WIDTHS
+
+    def conv(nm, xin):
+        l = L[nm]
+        return _bfr(R.direct_conv3d(xin, Wb[nm], B[nm], l.ksize, l.relu))
+
+    def dgrad(nm, dz):
+        l = L[nm]
+        flipped = np.transpose(Wb[nm][:, ::-1, :], (2, 1, 0))
+        return _bfr(R.direct_conv3d(dz, flipped, None, l.ksize))
+
+    def check(tag, got, ref, ulps=1):
+        rng_ = max(np.abs(ref).max(), 1e-30)
+        tol = ulps * 2.0 ** -7 * np.abs(ref) + 1e-5 * rng_     # one bf16 ulp is at most 2^-7 |x|
+        bad = np.abs(got - ref) > tol
+        assert not bad.any(), f"{tag}: {bad.sum()} of {bad.size} off, max err {np.abs(got - ref).max():.3e}"
+
+    c1, c2, c3 = A["cat1"], A["cat2"], A["cat3"]
+    e1, e2, e3 = c1[..., w2:], c2[..., w3:], c3[..., w4:]
+    # forward
+    check("a1", A["a1"], conv("enc1a", X))
+    check("e1", e1, conv("enc1b", A["a1"]))
+    check("p1", A["p1"], R.pool_ref(e1)[0], 0)
+    check("a2", A["a2"], conv("enc2a", A["p1"]))
+    check("e2", e2, conv("enc2b", A["a2"]))
+    check("p2", A["p2"], R.pool_ref(e2)[0], 0)
+    check("a3", A["a3"], conv("enc3a", A["p2"]))
+    check("e3", e3, conv("enc3b", A["a3"]))
+    check("p3", A["p3"], R.pool_ref(e3)[0], 0)
+    check("a4", A["a4"], conv("bota", A["p3"]))
+    check("b4", A["b4"], conv("botb", A["a4"]))
+    check("up3", c3[..., :w4], R.up_ref(A["b4"]), 0)
+    check("d3a", A["d3a"], conv("dec3a", c3))
+    check("d3b", A["d3b"], conv("dec3b", A["d3a"]))
+    check("up2", c2[..., :w3], R.up_ref(A["d3b"]), 0)
+    check("d2a", A["d2a"], conv("dec2a", c2))
+    check("d2b", A["d2b"], conv("dec2b", A["d2a"]))
+    check("up1", c1[..., :w2], R.up_ref(A["d2b"]), 0)
+    check("d1a", A["d1a"], conv("dec1a", c1))
+    check("d1b", A["d1b"], conv("dec1b", A["d1a"]))
+    check("out", A["out"], conv("head", A["d1b"]))
+    # backward: activation gradients (ReLU-masked where the network masks them)
+    m = lambda a: (a > 0).astype(np.float64)
+    gc1, gc2, gc3 = G["cat1"], G["cat2"], G["cat3"]
+    check("g d1b", G["d1b"], dgrad("head", DO) * m(A["d1b"]))
+    check("g d1a", G["d1a"], dgrad("dec1b", G["d1b"]) * m(A["d1a"]))
+    ref_c1 = dgrad("dec1a", G["d1a"])
+    check("g up1", gc1[..., :w2], ref_c1[..., :w2])
+    check("g d2b", G["d2b"], _bfr(R.up_bwd_ref(gc1[..., :w2])) * m(A["d2b"]))
+    check("g d2a", G["d2a"], dgrad("dec2b", G["d2b"]) * m(A["d2a"]))
+    ref_c2 = dgrad("dec2a", G["d2a"])
+    check("g up2", gc2[..., :w3], ref_c2[..., :w3])
+    check("g d3b", G["d3b"], _bfr(R.up_bwd_ref(gc2[..., :w3])) * m(A["d3b"]))
+    check("g d3a", G["d3a"], dgrad("dec3b", G["d3b"]) * m(A["d3a"]))
+    ref_c3 = dgrad("dec3a", G["d3a"])
+    check("g up3", gc3[..., :w4], ref_c3[..., :w4])
+    check("g b4", G["b4"], _bfr(R.up_bwd_ref(gc3[..., :w4])) * m(A["b4"]))
+    check("g a4", G["a4"], dgrad("botb", G["b4"]) * m(A["a4"]))
+    check("g p3", G["p3"], dgrad("bota", G["a4"]))
+    # skip gradient + pooling gradient, then the ReLU mask
+    check("g e3", gc3[..., w4:], _bfr(ref_c3[..., w4:] + R.pool_bwd_ref(e3, G["p3"])) * m(e3), 2)
+    check("g a3", G["a3"], dgrad("enc3b", gc3[..., w4:]) * m(A["a3"]))
+    check("g p2", G["p2"], dgrad("enc3a", G["a3"]))
+    check("g e2", gc2[..., w3:], _bfr(ref_c2[..., w3:] + R.pool_bwd_ref(e2, G["p2"])) * m(e2), 2)
+    check("g a2", G["a2"], dgrad("enc2b", gc2[..., w3:]) * m(A["a2"]))
+    check("g p1", G["p1"], dgrad("enc2a", G["a2"]))
+    check("g e1", gc1[..., w2:], _bfr(ref_c1[..., w2:] + R.pool_bwd_ref(e1, G["p1"])) * m(e1), 2)
+    check("g a1", G["a1"], dgrad("enc1b", gc1[..., w2:]) * m(A["a1"]))
+    # weight and bias gradients from the same stored tensors (FP32 sums)
+    pairs = {"head": (A["d1b"], DO), "dec1b": (A["d1a"], G["d1b"]), "dec1a": (c1, G["d1a"]),
+             "dec2b": (A["d2a"], G["d2b"]), "dec2a": (c2, G["d2a"]), "dec3b": (A["d3a"], G["d3b"]),
+             "dec3a": (c3, G["d3a"]), "botb": (A["a4"], G["b4"]), "bota": (A["p3"], G["a4"]),
+             "enc3b": (A["a3"], gc3[..., w4:]), "enc3a": (A["p2"], G["a3"]), "enc2b": (A["a2"], gc2[..., w3:]),
+             "enc2a": (A["p1"], G["a2"]), "enc1b": (A["a1"], gc1[..., w2:]), "enc1a": (X, G["a1"])}
+    for nm, (xin, dz) in pairs.items():
+        l = L[nm]
+        rw, rb = R.wgrad_ref(xin, dz, l.ksize)
+        gw = Gw[l.w_off:l.w_off + l.nw].reshape(rw.shape)
+        gb = Gw[l.b_off:l.b_off + l.co]
+        assert np.abs(gw - rw).max() <= 1e-5 * max(np.abs(rw).max(), 1e-30), nm
+        assert np.abs(gb - rb).max() <= 1e-5 * max(np.abs(rb).max(), 1e-30), nm
+
+
+def test_unet_close_to_plain_fp32_network():
+    """End to end against the FP32 network in torch: bf16 storage costs a few
+    parts in a thousand of the output and, through ReLU masks that flip near
+    zero, a few percent of the weight gradients."""
+    from airboxcalc_amd import unet as U
+    say("unet vs FP32 network")
+    net = U.UNet(seed=3)
+    rng = np.random.default_rng(6)
+    n, d, h, w = 2, 8, 16, 24
+    x = bf(rng.standard_normal((n, d, h, w, U.IN_CH)))
+    out = net.forward(x)
+    dout = bf(np.concatenate([rng.standard_normal((n, d, h, w, U.OUT_CH)),
+                              np.zeros((n, d, h, w, U.HEAD_CH - U.OUT_CH))], -1))
+    net.backward(dout)
+    torch.cuda.synchronize()
+    Wn = net.w.cpu().numpy()
+    P = {}
+    for l in net.L:
+        ww = torch.as_tensor(Wn[l.w_off:l.w_off + l.nw].reshape(l.co, l.taps, l.ci)).to(torch.bfloat16).double()
+        P[l.name] = (ww.requires_grad_(True), torch.as_tensor(Wn[l.b_off:l.b_off + l.co]).double().requires_grad_(True))
+    ref = R.torch_unet(x.double().cpu(), P, net.L)
+    o, r = f32(out), ref.detach().numpy()
+    rel = float(np.linalg.norm(o - r) / np.linalg.norm(r))
+    assert rel < 1e-2, rel
+    ref.backward(dout.double().cpu())
+    G = net.g.cpu().numpy()
+    for l in net.L:
+        gw = G[l.w_off:l.w_off + l.nw]
+        rw = P[l.name][0].grad.numpy().ravel()
+        if l.name == "head":
+            gw, rw = gw.reshape(l.co, -1)[:U.OUT_CH].ravel(), rw.reshape(l.co, -1)[:U.OUT_CH].ravel()
+        cos = float(gw @ rw / (np.linalg.norm(gw) * np.linalg.norm(rw)))
+        assert cos > 0.98, (l.name, cos)
+
+
+def test_unet_training_reduces_loss():
+    from airboxcalc_amd import unet as U
+    say("unet training smoke")
+    net = U.UNet(seed=0)
+    rng = np.random.default_rng(7)
+    n, d, h, w = 4, 8, 16, 16
+    X = np.zeros((n, d, h, w, U.IN_CH), np.float32)
+    X[..., :5] = rng.standard_normal((n, d, h, w, 5))
+    # a target the network can represent: a smoothed linear map of the inputs
+    Y = np.stack([X[..., 0] + 0.5 * X[..., 1], X[..., 2] - X[..., 3], 0.3 * X[..., 4], X[..., 0] * 0.2], -1)
+    x, y = bf(X), bf(Y)
+    mask = torch.ones((n, d, h, w), dtype=torch.uint8, device=_dev())
+    cw = torch.ones(U.OUT_CH, dtype=torch.float32, device=_dev())
+    losses = [float(net.train_step(x, y, mask, cw, lr=1e-3).item()) for _ in range(60)]
+    assert all(np.isfinite(losses))
+    assert losses[-1] < 0.5 * losses[0], losses[::10]
