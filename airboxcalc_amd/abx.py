@@ -101,6 +101,15 @@ def load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise AbxError(f"{LIB_PATH} is missing: build it with `make -C airboxcalc_amd/csrc` "
                        "or __graft_entry__.build()")
+    # One HIP runtime per process: torch's ROCm wheel carries its own
+    # libamdhip64 (soname libamdhip64.so.7, but requested as libamdhip64.so).
+    # Loaded after ours it becomes a second runtime that finds no GPU; loaded
+    # first it also satisfies this library's libamdhip64.so.7.  So torch, when
+    # installed, is imported before the library is opened.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, d, f32p = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_void_p
     sig = {
@@ -192,6 +201,7 @@ class Device:
         name = C.create_string_buffer(256)
         _check(L.abx_device_open(ordinal, C.byref(self.h), name, 256))
         self.name = name.value.decode()
+        self.ordinal = int(ordinal)
 
     def close(self):
         if self.h:

@@ -86,15 +86,17 @@ class AirboxSpec:
 
 class _Mesh:
     def __init__(self, subdiv: int):
-        self.tris: List[Sequence[float]] = []
+        self.chunks: List[np.ndarray] = []
         self.subdiv = max(1, int(subdiv))
 
     def tri(self, a: Point, b: Point, c: Point) -> None:
-        self.tris.append((*a, *b, *c))
+        self.chunks.append(np.array([(*a, *b, *c)], dtype=np.float64))
 
     def quad(self, p00: Point, p10: Point, p11: Point, p01: Point, n: int = None) -> None:
         """Planar quad p00-p10-p11-p01, subdivided n x n, split the same way
-        whatever corner it is given from (canonical order)."""
+        whatever corner it is given from (canonical order).  Vertex (iu, iv) is
+        the bilinear point at u = iu/n, v = iv/n (corners exact); cells in
+        (iu, iv) order, two triangles each."""
         n = self.subdiv if n is None else n
         corners = [p00, p10, p11, p01]
         # canonical start: the lexicographically smallest corner, then the
@@ -106,27 +108,20 @@ class _Mesh:
         else:
             order = [corners[(s - q) % 4] for q in range(4)]
         a, b, c, d = (np.array(v, dtype=np.float64) for v in order)
-        for iu in range(n):
-            for iv in range(n):
-                u0, u1, v0, v1 = iu / n, (iu + 1) / n, iv / n, (iv + 1) / n
-
-                def P(u, v):
-                    if u == 0.0 and v == 0.0:
-                        return tuple(a)
-                    if u == 1.0 and v == 0.0:
-                        return tuple(b)
-                    if u == 1.0 and v == 1.0:
-                        return tuple(c)
-                    if u == 0.0 and v == 1.0:
-                        return tuple(d)
-                    q = (1 - u) * (1 - v) * a + u * (1 - v) * b + u * v * c + (1 - u) * v * d
-                    return tuple(q)
-                q00, q10, q11, q01 = P(u0, v0), P(u1, v0), P(u1, v1), P(u0, v1)
-                self.tri(q00, q10, q11)
-                self.tri(q00, q11, q01)
+        t = np.arange(n + 1) / n
+        u = t[:, None, None]
+        v = t[None, :, None]
+        Q = (1 - u) * (1 - v) * a + u * (1 - v) * b + u * v * c + (1 - u) * v * d
+        Q[0, 0], Q[n, 0], Q[n, n], Q[0, n] = a, b, c, d
+        q00, q10, q11, q01 = Q[:-1, :-1], Q[1:, :-1], Q[1:, 1:], Q[:-1, 1:]
+        t1 = np.concatenate([q00, q10, q11], -1)
+        t2 = np.concatenate([q00, q11, q01], -1)
+        self.chunks.append(np.stack([t1, t2], 2).reshape(-1, 9))
 
     def array(self) -> np.ndarray:
-        return np.asarray(self.tris, dtype=np.float64).astype(np.float32).reshape(-1, 9)
+        if not self.chunks:
+            return np.zeros((0, 9), np.float32)
+        return np.concatenate(self.chunks).astype(np.float32)
 
 
 def _prism(m: _Mesh, poly_xz: Sequence[Tuple[float, float]], y0: float, y1: float) -> None:
@@ -308,13 +303,18 @@ def seeded_variant(seed: int, kind: str = "roofline") -> Tuple[AirboxSpec, dict]
     kind="roofline": 300x150x200 mm, 0-3 baffles (2-4 mm), yaw uniform 0-30 deg.
     kind="batch":    outer dims uniform 200-360 mm per axis, dx in {2,2.5,3,4},
                      K uniform 1.5-6, aniso uniform 5-20.
+    kind="surrogate": the "roofline" box of the same seed (same draws, same
+                     geometry) plus K uniform 1.5-6 and aniso uniform 5-20.
     Returns (spec, extra) where extra holds per-variant run parameters."""
     rng = np.random.default_rng(seed)
     extra = {}
-    if kind == "roofline":
+    if kind in ("roofline", "surrogate"):
         yaw = round(float(rng.uniform(0.0, 30.0)), 1)
         spec = AirboxSpec(yaw_deg=yaw)
         spec.baffles = random_baffles(rng, spec)
+        if kind == "surrogate":
+            extra["K"] = float(rng.uniform(1.5, 6.0))
+            extra["aniso"] = float(rng.uniform(5.0, 20.0))
     elif kind == "batch":
         LX, LY, LZ = (float(v) for v in rng.uniform(200.0, 360.0, size=3))
         spec = AirboxSpec(LX=LX, LY=LY, LZ=LZ, inlet_d=min(70.0, 0.4 * min(LX, LY)))
@@ -349,14 +349,17 @@ def run_params(steps: int, avg: int, every: int, K: float = 3.0, aniso: float = 
             "steps": int(steps), "avg": int(avg), "every": int(every), "dx": float(dx), "outlet": ""}
 
 
-def write_stl_binary(path: str, tris: np.ndarray) -> None:
+def stl_binary_bytes(tris: np.ndarray) -> bytes:
+    """Binary STL (80-byte header, count, 50-byte records) of a triangle soup."""
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rec = np.zeros(len(tris), dtype=[("n", "<f4", 3), ("v", "<f4", 9), ("a", "<u2")])
     rec["v"] = tris
+    return b"airboxcalc_amd procedural airbox".ljust(80, b" ") + struct.pack("<I", len(tris)) + rec.tobytes()
+
+
+def write_stl_binary(path: str, tris: np.ndarray) -> None:
     with open(path, "wb") as fh:
-        fh.write(b"airboxcalc_amd procedural airbox".ljust(80, b" "))
-        fh.write(struct.pack("<I", len(tris)))
-        fh.write(rec.tobytes())
+        fh.write(stl_binary_bytes(tris))
 
 
 def write_stl_ascii(path: str, tris: np.ndarray) -> None:

@@ -106,3 +106,25 @@ def fp64_errors(dev: abx.Device, g: abx.Geometry, p: dict, precision: str = "f32
     fin = np.isfinite(h64)
     out["hist_gamma_max_abs_err"] = float(np.max(np.abs(h32[fin] - h64[fin]))) if fin.any() else None
     return out
+
+
+def fixed_run(dev: abx.Device, g: abx.Geometry, p: dict, steps: int = 6000, avg: int = 2000, precision: str = "f32",
+              init_rho_u: Optional[np.ndarray] = None) -> SteadyResult:
+    """The reference's protocol (simulate, engine.js:341-358 / 613-630): a
+    fixed number of steps from the start state, averaging the last `avg`;
+    evaluated on the GPU.  converged is True by definition."""
+    sim = abx.Sim(dev, g.grid(), p["K"], p["aniso"], p["u"], p["tau"], precision=precision)
+    try:
+        if init_rho_u is not None:
+            sim.init_eq(init_rho_u)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.steps(steps - avg, False)
+        sim.steps(avg, True)
+        sim.probe(steps)
+        m = sim.evaluate(g.eval_view(), divisor=avg, field3d=False)["metrics"]
+        sim.sync()
+        wall = time.perf_counter() - t0
+        return SteadyResult(True, steps, wall, m, [(steps, m["gammaCore"], m["zeta"])], sim.read_acc(avg))
+    finally:
+        sim.destroy()

@@ -1,0 +1,517 @@
+"""Flow surrogate for warm starts: dataset from the LBM, training, inference.
+
+The reference solves every box from rest, f = w_q (engine.js:260-261), for a
+fixed number of steps.  Here a 3-D U-Net (unet.py) trained on converged LBM
+solutions of seeded boxes predicts the averaged steady fields of a new box;
+the prediction seeds the lattice at equilibrium (abx_sim_init_eq) and the
+solve then runs to the steady criterion of steady.py.
+
+Data layout.  A box is described on a fixed volume of GRID = 96 x 48 x 64
+lattice cells (i, j, k in the reference's lattice order, k fastest,
+engine.js:128-134), at the cell size FAMILY_DX that makes every box of the
+family fit; cells beyond the box's own lattice are solid.  Input channels
+(bf16, IN_CH wide, the rest zero):
+
+  0 solid          1 - fluid
+  1 porous         filter layer cells (setupSim flag 3, engine.js:254)
+  2 inlet          inlet cells (flag 1: k = 0 and opening, engine.js:252)
+  3 K              K / 6 on porous cells
+  4 wall distance  signed Euclidean distance to the wall in cells / 8
+                   (positive in the fluid)
+  5 outlet         outlet cells (flag 2, engine.js:253)
+  6 aniso          aniso / 20 on porous cells
+
+Targets: ux/u0, uy/u0, uz/u0, (rho - 1)/RHO_SCALE of the averaged fields,
+masked to fluid cells, then standardised per channel with the dataset's
+fluid-cell mean and standard deviation (kept with the weights, undone by
+predict).
+"""
+from __future__ import annotations
+
+import copy
+import os
+import time
+from dataclasses import dataclass
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import abx
+from . import geometry as G
+from . import unet as U
+
+GRID = (96, 48, 64)
+FAMILY_DX = 4.8          # mm; the largest box of the family (yaw 30 deg) is 86 x 45 x 63 cells
+U0, TAU = 0.06, 0.51
+RHO_SCALE = 0.1          # (rho - 1) of these boxes is of order 0.1
+N_FEAT = 7
+
+
+@dataclass
+class Box:
+    seed: int
+    spec: G.AirboxSpec
+    K: float
+    aniso: float
+
+
+def family(seeds) -> List[Box]:
+    """The surrogate family: the roofline boxes (300 x 150 x 200 mm, 0-3
+    baffles, inlet yaw 0-30 deg) with K uniform 1.5-6 and aniso uniform 5-20."""
+    out = []
+    for s in seeds:
+        spec, ex = G.seeded_variant(int(s), "surrogate")
+        out.append(Box(int(s), spec, ex["K"], ex["aniso"]))
+    return out
+
+
+def cell_types(grid: abx.GridArrays):
+    """fluid, inlet, outlet, porous masks (nx, ny, nz) in setupSim's order of
+    precedence (engine.js:250-255)."""
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    fluid = grid.fluid.reshape(nx, ny, nz).astype(bool)
+    inlet = np.zeros_like(fluid)
+    inlet[:, :, 0] = grid.opening.reshape(nx, ny).astype(bool)
+    inlet &= fluid
+    outlet = grid.outlet.reshape(nx, ny, nz).astype(bool) & fluid & ~inlet
+    dc = grid.dc
+    pj = (dc > dc[grid.top]) & (dc.astype(np.float64) <= grid.pleat)
+    porous = fluid & pj[None, :, None] & ~inlet & ~outlet
+    return fluid, inlet, outlet, porous
+
+
+def _place(a: np.ndarray, shape, fill=0):
+    out = np.full(shape + a.shape[3:], fill, a.dtype)
+    nx, ny, nz = a.shape[:3]
+    if nx > shape[0] or ny > shape[1] or nz > shape[2]:
+        raise abx.AbxError(f"lattice {nx}x{ny}x{nz} does not fit the surrogate volume {shape}")
+    out[:nx, :ny, :nz] = a
+    return out
+
+
+def features(grid: abx.GridArrays, K: float, aniso: float, shape=GRID) -> np.ndarray:
+    """Input volume (shape + (IN_CH,)) float32."""
+    from scipy import ndimage
+    fluid, inlet, outlet, porous = cell_types(grid)
+    fl = _place(fluid, shape, False)
+    d_in = ndimage.distance_transform_edt(fl)
+    d_out = ndimage.distance_transform_edt(~fl)
+    x = np.zeros(shape + (U.IN_CH,), np.float32)
+    x[..., 0] = ~fl
+    por = _place(porous, shape, False)
+    x[..., 1] = por
+    x[..., 2] = _place(inlet, shape, False)
+    x[..., 3] = por * (K / 6.0)
+    x[..., 4] = np.clip((d_in - d_out) / 8.0, -4.0, 4.0)
+    x[..., 5] = _place(outlet, shape, False)
+    x[..., 6] = por * (aniso / 20.0)
+    return x
+
+
+def targets(grid: abx.GridArrays, acc: np.ndarray, shape=GRID):
+    """(target volume shape + (4,) float32, fluid mask shape uint8) from the
+    averaged fields acc (4*NF, components ux, uy, uz, rho, cid order)."""
+    fluid = grid.fluid.reshape(grid.nx, grid.ny, grid.nz).astype(bool)
+    NF = int(fluid.sum())
+    a = np.asarray(acc, np.float32).reshape(4, NF)
+    y = np.zeros((grid.nx, grid.ny, grid.nz, 4), np.float32)
+    y[fluid, 0] = a[0] / U0
+    y[fluid, 1] = a[1] / U0
+    y[fluid, 2] = a[2] / U0
+    y[fluid, 3] = (a[3] - 1.0) / RHO_SCALE
+    return _place(y, shape), _place(fluid.astype(np.uint8), shape)
+
+
+def rho_u_of_prediction(pred: np.ndarray, grid: abx.GridArrays) -> np.ndarray:
+    """(rho, ux, uy, uz) in cid order (abx_sim_init_eq) from a predicted volume
+    on the same lattice (pred: shape + (>= 4,) float)."""
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    fluid = grid.fluid.reshape(nx, ny, nz).astype(bool)
+    p = np.asarray(pred[:nx, :ny, :nz, :4], np.float32)[fluid]
+    return np.concatenate([1.0 + RHO_SCALE * p[:, 3], U0 * p[:, 0], U0 * p[:, 1], U0 * p[:, 2]]).astype(np.float32)
+
+
+def resample(pred: np.ndarray, gc: abx.Geometry, gf: abx.Geometry) -> np.ndarray:
+    """Trilinear resampling of a prediction on the lattice of gc (same box, any
+    cell size) to the fluid cells of gf, in cid order of gf: (rho, ux, uy, uz).
+    Both lattices share the local frame; cell centres are origin + (i+0.5) h
+    (engine.js:137-139).  Only fluid cells of gc contribute (weights
+    renormalised); a cell with no fluid neighbour keeps rest (rho 1, u 0)."""
+    cg, fg = gc.grid(), gf.grid()
+    cflu = cg.fluid.reshape(cg.nx, cg.ny, cg.nz).astype(np.float32)
+    P = np.asarray(pred[:cg.nx, :cg.ny, :cg.nz, :4], np.float32) * cflu[..., None]
+    fflu = fg.fluid.reshape(fg.nx, fg.ny, fg.nz).astype(bool)
+    idx = np.nonzero(fflu)
+    oc, of = np.array(gc.origin), np.array(gf.origin)
+    coords = []
+    for ax, n in enumerate((cg.nx, cg.ny, cg.nz)):
+        pos = of[ax] + (idx[ax] + 0.5) * gf.h
+        coords.append(np.clip((pos - oc[ax]) / gc.h - 0.5, 0.0, n - 1.0))
+    i0 = [np.minimum(np.floor(c).astype(np.int64), n - 2 if n > 1 else 0) for c, n in
+          zip(coords, (cg.nx, cg.ny, cg.nz))]
+    t = [c - i for c, i in zip(coords, i0)]
+    acc = np.zeros((idx[0].size, 4), np.float64)
+    wsum = np.zeros(idx[0].size, np.float64)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                ii, jj, kk = i0[0] + dx, i0[1] + dy, i0[2] + dz
+                w = (t[0] if dx else 1 - t[0]) * (t[1] if dy else 1 - t[1]) * (t[2] if dz else 1 - t[2])
+                w = w * cflu[ii, jj, kk]
+                acc += w[:, None] * P[ii, jj, kk]
+                wsum += w
+    v = np.where(wsum[:, None] > 0, acc / np.maximum(wsum, 1e-30)[:, None], 0.0)
+    return np.concatenate([1.0 + RHO_SCALE * v[:, 3], U0 * v[:, 0], U0 * v[:, 1], U0 * v[:, 2]]).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk: int = 64,
+                 log: Optional[Callable[[str], None]] = None) -> dict:
+    """Solve every box (batched, one launch per step for `chunk` boxes, the
+    last `avg` of `steps` steps averaged) and build the training volumes on
+    the device: X (n, 96, 48, 64, IN_CH) bf16, Y (n, ..., 4) bf16, M uint8."""
+    import torch
+    n = len(boxes)
+    D, H, W = GRID
+    cuda = torch.device("cuda", dev.ordinal)
+    X = torch.zeros((n, D, H, W, U.IN_CH), dtype=torch.bfloat16, device=cuda)
+    Y = torch.zeros((n, D, H, W, 4), dtype=torch.bfloat16, device=cuda)
+    M = torch.zeros((n, D, H, W), dtype=torch.uint8, device=cuda)
+    stats = dict(cells=0, solve_s=0.0, prep_s=0.0, lups=0.0)
+    for c0 in range(0, n, chunk):
+        part = boxes[c0:c0 + chunk]
+        t0 = time.perf_counter()
+        geoms, grids, sims = [], [], []
+        for b in part:
+            g = abx.Geometry(dev, G.airbox_triangles(b.spec), G.prepare_params(b.spec, FAMILY_DX), max_cells=0)
+            gr = g.grid()
+            geoms.append(g)
+            grids.append(gr)
+            sims.append(abx.Sim(dev, gr, K=b.K, aniso=b.aniso, u=U0, tau=TAU, precision="f32"))
+        for k, (b, gr) in enumerate(zip(part, grids)):
+            X[c0 + k] = torch.from_numpy(features(gr, b.K, b.aniso)).to(torch.bfloat16)
+        t1 = time.perf_counter()
+        bt = abx.Batch(dev, sims)
+        tuned = bt.tune(10)
+        bt.set_launch(tuned["spb"], tuned["persistent"], True)
+        bt.steps(steps - avg, False)
+        bt.steps(avg, True)
+        bt.probe(steps)
+        bt.sync()
+        t2 = time.perf_counter()
+        for k, (s, gr) in enumerate(zip(sims, grids)):
+            y, m = targets(gr, s.read_acc(avg))
+            Y[c0 + k] = torch.from_numpy(y).to(torch.bfloat16)
+            M[c0 + k] = torch.from_numpy(m)
+        nf = sum(s.NF for s in sims)
+        stats["cells"] += nf
+        stats["prep_s"] += t1 - t0
+        stats["solve_s"] += t2 - t1
+        stats["lups"] += nf * steps
+        bt.destroy()
+        for s in sims:
+            s.destroy()
+        for g in geoms:
+            g.free()
+        if log:
+            log(f"dataset {c0 + len(part)}/{n}: prepare+features {t1 - t0:.1f} s, solve {t2 - t1:.1f} s "
+                f"({nf * steps / max(t2 - t1, 1e-9) / 1e9:.1f} G fluid-LUPS)")
+    # per-channel standardisation over the fluid cells
+    s1 = torch.zeros(4, dtype=torch.float64, device=cuda)
+    s2 = torch.zeros(4, dtype=torch.float64, device=cuda)
+    cnt = 0
+    for c0 in range(0, n, chunk):
+        y, m = Y[c0:c0 + chunk].double(), M[c0:c0 + chunk].bool()
+        v = y[m]
+        s1 += v.sum(0)
+        s2 += (v * v).sum(0)
+        cnt += v.shape[0]
+    mean = (s1 / max(cnt, 1)).float()
+    std = torch.sqrt(torch.clamp(s2 / max(cnt, 1) - (s1 / max(cnt, 1)) ** 2, min=1e-12)).float()
+    for c0 in range(0, n, chunk):
+        y = (Y[c0:c0 + chunk].float() - mean) / std
+        Y[c0:c0 + chunk] = (y * M[c0:c0 + chunk, ..., None]).to(torch.bfloat16)
+    torch.cuda.synchronize(cuda)
+    return dict(X=X, Y=Y, M=M, seeds=[b.seed for b in boxes], stats=stats, steps=steps, avg=avg,
+                norm=dict(mean=mean.cpu().numpy(), std=std.cpu().numpy()))
+
+
+def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3, val_frac: float = 0.05,
+          seed: int = 0, log: Optional[Callable[[str], None]] = None, max_seconds: float = 1e30) -> dict:
+    """Adam on the masked MSE of the normalised targets; the last val_frac of
+    the samples are held out.  Returns the per-epoch losses and timings."""
+    import torch
+    X, Y, M = data["X"], data["Y"], data["M"]
+    n = X.shape[0]
+    nval = max(1, int(round(n * val_frac))) if n > 1 else 0
+    ntr = n - nval
+    rng = np.random.default_rng(seed)
+    cw = torch.ones(U.OUT_CH, dtype=torch.float32, device=X.device)
+    hist = []
+    t_start = time.perf_counter()
+    step_times = []
+    for ep in range(epochs):
+        order = rng.permutation(ntr)
+        tot, cnt = 0.0, 0
+        losses = []
+        for b0 in range(0, ntr - batch + 1, batch):
+            idx = torch.as_tensor(order[b0:b0 + batch], device=X.device)
+            xb, yb, mb = X[idx].contiguous(), Y[idx].contiguous(), M[idx].contiguous()
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            loss = net.train_step(xb, yb, mb, cw, lr=lr)
+            losses.append(loss)
+            torch.cuda.synchronize()
+            step_times.append(time.perf_counter() - ts)
+        tr = float(torch.stack(losses).mean().item()) if losses else float("nan")
+        va = evaluate_loss(net, data, range(ntr, n), batch) if nval else float("nan")
+        hist.append(dict(epoch=ep + 1, train=tr, val=va, wall_s=time.perf_counter() - t_start))
+        if log:
+            log(f"epoch {ep + 1}: train {tr:.4f}  val {va:.4f}  ({time.perf_counter() - t_start:.0f} s, "
+                f"{1e3 * np.median(step_times):.1f} ms/step)")
+        if time.perf_counter() - t_start > max_seconds:
+            break
+    return dict(history=hist, ms_per_step=1e3 * float(np.median(step_times)) if step_times else None,
+                n_train=ntr, n_val=nval)
+
+
+def evaluate_loss(net: U.UNet, data: dict, indices, batch: int = 8) -> float:
+    import torch
+    X, Y, M = data["X"], data["Y"], data["M"]
+    cw = torch.ones(U.OUT_CH, dtype=torch.float32, device=X.device)
+    idx = list(indices)
+    tot = 0.0
+    for b0 in range(0, len(idx), batch):
+        sel = torch.as_tensor(idx[b0:b0 + batch], device=X.device)
+        net.forward(X[sel].contiguous())
+        loss, _ = net.loss_and_grad(Y[sel].contiguous(), M[sel].contiguous(), cw)
+        tot += float(loss.item()) * len(sel)
+    return tot / max(len(idx), 1)
+
+
+def predict(net: U.UNet, x: np.ndarray, norm: dict) -> np.ndarray:
+    """Forward of one input volume (GRID + (IN_CH,)) -> targets (GRID + (4,))
+    float32 in the units of targets() (standardisation undone)."""
+    import torch
+    xt = torch.from_numpy(np.ascontiguousarray(x[None])).to(net.dev).to(torch.bfloat16)
+    out = net.forward(xt)
+    y = out[0, ..., :U.OUT_CH].float().cpu().numpy()
+    return y * np.asarray(norm["std"], np.float32) + np.asarray(norm["mean"], np.float32)
+
+
+def warm_start_fields(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: abx.Geometry,
+                      tris: Optional[np.ndarray] = None) -> np.ndarray:
+    """(rho, ux, uy, uz) in cid order for g_fine's lattice from the surrogate:
+    the box is prepared at FAMILY_DX, predicted there and resampled."""
+    tris = G.airbox_triangles(box.spec) if tris is None else tris
+    gc = abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
+    try:
+        pred = predict(net, features(gc.grid(), box.K, box.aniso), norm)
+        return resample(pred, gc, g_fine)
+    finally:
+        gc.free()
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_unet_96x48x64.npz")
+
+
+def save(path: str, net: U.UNet, norm: dict, meta: dict) -> None:
+    """Weights (FP32), Adam state and target standardisation as one .npz."""
+    import json
+    st = net.state()
+    np.savez_compressed(path, w=st["w"], m=st["m"], v=st["v"], step=st["step"], norm_mean=norm["mean"],
+                        norm_std=norm["std"], meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
+
+
+def load(path: str, net: U.UNet) -> tuple:
+    """Loads weights into net; returns (norm, meta)."""
+    import json
+    z = np.load(path)      # plain arrays only (allow_pickle stays off)
+    if "m" in z:
+        net.load_state({"w": z["w"], "m": z["m"], "v": z["v"], "step": z["step"]})
+    else:                  # weights-only file (inference)
+        net.load_state({"w": z["w"]})
+    return dict(mean=z["norm_mean"], std=z["norm_std"]), json.loads(bytes(z["meta"]).decode())
+
+
+# ---------------------------------------------------------------------------
+STUDY_SUBDIV = 57        # quads split 57 x 57: about 200 k triangles per box
+
+
+def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box], dx: float = 2.0,
+                     window: int = 1000, max_steps: int = 60000, fixed_steps: int = 6000, fixed_avg: int = 2000,
+                     fp64_every: int = 4, subdiv: int = STUDY_SUBDIV,
+                     log: Optional[Callable[[str], None]] = None) -> dict:
+    """Cold against surrogate-warm time to steady for held-out boxes.
+
+    Each box is written as a binary STL of about 200 k triangles, parsed and
+    prepared on the GPU at dx.  Runs per box:
+      fixed   from rest, the reference's fixed-step protocol (fixed_steps,
+              averaging the last fixed_avg): the result the others are judged by;
+      cold    from rest to the steady criterion (steady.solve_to_steady);
+      warm    from the surrogate's equilibrium to the same criterion;
+      fp64    (every fp64_every-th box) the fixed protocol in FP64.
+    Velocity-field errors are relative L2 norms over the fluid cells."""
+    from . import steady
+    rows = []
+    for bi, box in enumerate(boxes):
+        spec = copy.deepcopy(box.spec)
+        spec.subdiv = subdiv
+        t0 = time.perf_counter()
+        stl = G.stl_binary_bytes(G.airbox_triangles(spec))
+        t1 = time.perf_counter()
+        tris = abx.parse_stl(stl)
+        g = abx.Geometry(dev, tris, G.prepare_params(spec, dx), max_cells=0)
+        t2 = time.perf_counter()
+        p = dict(K=box.K, aniso=box.aniso, u=U0, tau=TAU)
+        try:
+            try:
+                fixed = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg)
+                cold = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, keep_acc=True)
+                t3 = time.perf_counter()
+                rho_u = warm_start_fields(net, norm, dev, box, g, tris)
+                t4 = time.perf_counter()
+                warm = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, init_rho_u=rho_u,
+                                              keep_acc=True)
+                f64 = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg, precision="f64") \
+                    if fp64_every and bi % fp64_every == 0 else None
+            except abx.AbxError as e:       # e.g. the reference's instability error
+                rows.append(dict(seed=box.seed, error=str(e), grid=[g.nx, g.ny, g.nz]))
+                if log:
+                    log(f"box {bi + 1}/{len(boxes)} seed {box.seed}: failed: {e}")
+                continue
+        finally:
+            g.free()
+
+        def uerr(a, b):
+            a = np.asarray(a, np.float64).reshape(4, -1)[:3]
+            b = np.asarray(b, np.float64).reshape(4, -1)[:3]
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+        def judge(r, ref):
+            dg = abs(r.metrics["gammaCore"] - ref.metrics["gammaCore"])
+            dz = abs(r.metrics["zeta"] - ref.metrics["zeta"]) / abs(ref.metrics["zeta"])
+            return dict(gammaCore_abs=dg, zeta_rel=dz, within=bool(dg <= 0.005 and dz <= 0.02),
+                        u_err=uerr(r.acc, ref.acc))
+
+        def run(r):
+            return dict(converged=r.converged, steps=r.steps, wall_s=r.wall_s,
+                        gammaCore=r.metrics["gammaCore"], zeta=r.metrics["zeta"])
+
+        NF = fixed.acc.size // 4
+        init = np.concatenate([rho_u[NF:], rho_u[:NF]])     # (ux, uy, uz, rho), the order of acc
+        row = dict(seed=box.seed, ntri=int(tris.size // 9), grid=[g.nx, g.ny, g.nz], NF=int(NF), K=box.K,
+                   aniso=box.aniso, stl_s=t1 - t0, prepare_s=t2 - t1, predict_s=t4 - t3,
+                   fixed=run(fixed), cold=run(cold), warm=run(warm),
+                   cold_vs_fixed=judge(cold, fixed), warm_vs_fixed=judge(warm, fixed),
+                   prediction_u_err_vs_fixed=uerr(init, fixed.acc))
+        if f64 is not None:
+            row["fp64"] = run(f64)
+            row["fixed_vs_fp64"] = judge(fixed, f64)
+            row["warm_vs_fp64"] = judge(warm, f64)
+        rows.append(row)
+        if log:
+            log(f"box {bi + 1}/{len(boxes)} seed {box.seed}: {row['ntri']} tris, grid {row['grid']}, "
+                f"cold {cold.steps} steps {cold.wall_s:.2f} s, warm {warm.steps} steps {warm.wall_s:.2f} s "
+                f"(+{row['predict_s']:.2f} s predict); vs fixed: cold dgamma {row['cold_vs_fixed']['gammaCore_abs']:.4f} "
+                f"dzeta {row['cold_vs_fixed']['zeta_rel']:.3f}, warm dgamma {row['warm_vs_fixed']['gammaCore_abs']:.4f} "
+                f"dzeta {row['warm_vs_fixed']['zeta_rel']:.3f}; prediction u err {row['prediction_u_err_vs_fixed']:.3f}")
+    return dict(rows=rows, summary=summarize(rows), dx=dx, window=window, fixed_steps=fixed_steps,
+                fixed_avg=fixed_avg)
+
+
+def summarize(rows: List[dict]) -> dict:
+    ok = [r for r in rows if "error" not in r and r["cold"]["converged"] and r["warm"]["converged"]]
+    out = dict(n=len(rows), n_failed=sum("error" in r for r in rows), n_converged=len(ok))
+    if not ok:
+        return out
+    cs = np.array([r["cold"]["steps"] for r in ok], float)
+    ws = np.array([r["warm"]["steps"] for r in ok], float)
+    cw = np.array([r["cold"]["wall_s"] for r in ok])
+    ww = np.array([r["warm"]["wall_s"] + r["predict_s"] for r in ok])
+    col = lambda key, f: np.array([r[key][f] for r in ok], float)
+    out.update(steps_cold_median=float(np.median(cs)), steps_warm_median=float(np.median(ws)),
+               step_speedup_median=float(np.median(cs / ws)), step_speedup_total=float(cs.sum() / ws.sum()),
+               wall_speedup_total_incl_predict=float(cw.sum() / ww.sum()),
+               warm_within_tol_of_fixed=float(np.mean(col("warm_vs_fixed", "within"))),
+               cold_within_tol_of_fixed=float(np.mean(col("cold_vs_fixed", "within"))),
+               warm_dgamma_vs_fixed_median=float(np.median(col("warm_vs_fixed", "gammaCore_abs"))),
+               cold_dgamma_vs_fixed_median=float(np.median(col("cold_vs_fixed", "gammaCore_abs"))),
+               warm_dzeta_vs_fixed_median=float(np.median(col("warm_vs_fixed", "zeta_rel"))),
+               cold_dzeta_vs_fixed_median=float(np.median(col("cold_vs_fixed", "zeta_rel"))),
+               warm_u_err_vs_fixed_median=float(np.median(col("warm_vs_fixed", "u_err"))),
+               cold_u_err_vs_fixed_median=float(np.median(col("cold_vs_fixed", "u_err"))),
+               prediction_u_err_vs_fixed_median=float(np.median([r["prediction_u_err_vs_fixed"] for r in ok])))
+    f = [r for r in ok if "fp64" in r]
+    if f:
+        out.update(fp64_boxes=len(f),
+                   fixed_vs_fp64_dgamma_max=float(max(r["fixed_vs_fp64"]["gammaCore_abs"] for r in f)),
+                   fixed_vs_fp64_dzeta_max=float(max(r["fixed_vs_fp64"]["zeta_rel"] for r in f)),
+                   fixed_vs_fp64_u_err_median=float(np.median([r["fixed_vs_fp64"]["u_err"] for r in f])),
+                   warm_vs_fp64_dgamma_median=float(np.median([r["warm_vs_fp64"]["gammaCore_abs"] for r in f])),
+                   warm_vs_fp64_dzeta_median=float(np.median([r["warm_vs_fp64"]["zeta_rel"] for r in f])))
+    return out
+
+
+def main(argv=None) -> int:
+    """python -m airboxcalc_amd.surrogate train|study ...: the surrogate
+    dataset + training run, and the warm-start study."""
+    import argparse
+    import json
+    import torch
+    ap = argparse.ArgumentParser(prog="python -m airboxcalc_amd.surrogate")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    tr = sub.add_parser("train", help="generate the LBM dataset on the GPU and train the U-Net")
+    tr.add_argument("--boxes", type=int, default=2000)
+    tr.add_argument("--first-seed", type=int, default=0)
+    tr.add_argument("--steps", type=int, default=4000)
+    tr.add_argument("--avg", type=int, default=1000)
+    tr.add_argument("--epochs", type=int, default=40)
+    tr.add_argument("--batch", type=int, default=8)
+    tr.add_argument("--lr", type=float, default=1e-3)
+    tr.add_argument("--max-train-seconds", type=float, default=600)
+    tr.add_argument("--out", required=True, help="weights .npz")
+    tr.add_argument("--report", help="JSON report")
+    st = sub.add_parser("study", help="cold vs warm time to steady on held-out boxes")
+    st.add_argument("--weights", default=WEIGHTS, help="weights .npz (default: the trained 96x48x64 surrogate)")
+    st.add_argument("--boxes", type=int, default=128)
+    st.add_argument("--first-seed", type=int, default=1_000_000)
+    st.add_argument("--dx", type=float, default=2.0)
+    st.add_argument("--window", type=int, default=1000)
+    st.add_argument("--fp64-every", type=int, default=4)
+    st.add_argument("--subdiv", type=int, default=STUDY_SUBDIV)
+    st.add_argument("--report", help="JSON report")
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    dev = abx.device(0)
+    if a.cmd == "train":
+        boxes = family(range(a.first_seed, a.first_seed + a.boxes))
+        t0 = time.perf_counter()
+        data = make_dataset(dev, boxes, a.steps, a.avg, log=log)
+        t_data = time.perf_counter() - t0
+        net = U.UNet(seed=0)
+        res = train(net, data, a.epochs, a.batch, a.lr, log=log, max_seconds=a.max_train_seconds)
+        meta = dict(boxes=a.boxes, first_seed=a.first_seed, steps=a.steps, avg=a.avg, dx=FAMILY_DX, grid=GRID,
+                    epochs_run=len(res["history"]), batch=a.batch, lr=a.lr)
+        save(a.out, net, data["norm"], meta)
+        rep = dict(meta=meta, dataset_s=t_data, dataset_stats=data["stats"], norm={k: v.tolist() for k, v in
+                                                                                   data["norm"].items()},
+                   train=res, params=U.n_params(), device=dev.name, torch=torch.__version__)
+        log(json.dumps({k: rep[k] for k in ("dataset_s", "params")}))
+    else:
+        net = U.UNet(seed=0)
+        norm, meta = load(a.weights, net)
+        boxes = family(range(a.first_seed, a.first_seed + a.boxes))
+        rep = warm_start_study(dev, net, norm, boxes, dx=a.dx, window=a.window, fp64_every=a.fp64_every,
+                               subdiv=a.subdiv, log=log)
+        rep["weights_meta"] = meta
+        rep["device"] = dev.name
+        log(json.dumps(rep["summary"]))
+    if a.report:
+        with open(a.report, "w") as fh:
+            json.dump(rep, fh, indent=1, default=float)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

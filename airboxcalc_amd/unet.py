@@ -35,8 +35,10 @@ WIDTHS = (16, 32, 64, 128)
 
 def _torch():
     import torch
-    if not torch.cuda.is_available():
-        raise abx.AbxError("the U-Net surrogate needs a GPU (torch.cuda is not available)")
+    try:
+        torch.empty(1, device="cuda")     # initialises torch's HIP context (or fails without a GPU)
+    except Exception as e:
+        raise abx.AbxError(f"the U-Net surrogate needs a GPU visible to torch ({e})") from None
     return torch
 
 

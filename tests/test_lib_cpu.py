@@ -46,8 +46,11 @@ def test_parse_stl_error_text():
 
 
 def test_device_open_fails_loudly_without_gpu():
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
-    with pytest.raises(abx.AbxError):
-        abx.Device(0)
+    # the library's own probe decides (torch's device count can read 0 where HIP works)
+    try:
+        d = abx.Device(0)
+    except abx.AbxError as e:
+        assert "GPU" in str(e)
+        return
+    d.close()
+    pytest.skip("a GPU is present")

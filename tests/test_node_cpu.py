@@ -12,9 +12,12 @@ pytestmark = pytest.mark.skipif(shutil.which("node") is None, reason="node not i
 
 
 def test_addon_and_shim_load_and_fail_loudly_without_gpu():
-    import torch
-    if torch.cuda.is_available():
+    from airboxcalc_amd import abx
+    try:
+        abx.Device(0).close()
         pytest.skip("a GPU is present")
+    except abx.AbxError:
+        pass
     js = ("const H=require('./airboxcalc_amd/js/engine_hip.js');"
           "const k=Object.keys(H.native).sort().join(',');"
           "let msg='';try{H.native.deviceOpen(0)}catch(e){msg=e.message}"

@@ -1,0 +1,107 @@
+"""CPU checks of the surrogate's data preparation (oracle geometry, no GPU)."""
+from __future__ import annotations
+
+import types
+
+import numpy as np
+import pytest
+
+from airboxcalc_amd import abx
+from airboxcalc_amd import geometry as G
+from airboxcalc_amd import surrogate as S
+from oracle import oracle as O
+from tests.gpu_util import grid_of_oracle
+
+
+def _box_geom(seed=3, dx=S.FAMILY_DX):
+    box = S.family([seed])[0]
+    g = O.prepare(G.airbox_triangles(box.spec), G.prepare_params(box.spec, dx), max_cells=0)
+    return box, g
+
+
+def test_family_is_the_roofline_geometry_plus_k_and_aniso():
+    for seed in (0, 7, 42):
+        a, _ = G.seeded_variant(seed, "roofline")
+        b = S.family([seed])[0]
+        assert G.spec_dict(a) == G.spec_dict(b.spec)
+        assert 1.5 <= b.K <= 6.0 and 5.0 <= b.aniso <= 20.0
+
+
+@pytest.mark.parametrize("seed", [0, 11, 133])
+def test_family_fits_the_volume(seed):
+    _, g = _box_geom(seed)
+    assert g.nx <= S.GRID[0] and g.ny <= S.GRID[1] and g.nz <= S.GRID[2]
+
+
+def test_features_and_targets_follow_the_cell_flags():
+    box, g = _box_geom()
+    grid = grid_of_oracle(g)
+    x = S.features(grid, box.K, box.aniso)
+    assert x.shape == S.GRID + (16,)
+    # cell flags of setupSim (engine.js:250-255): 1 inlet, 2 outlet, 3 porous
+    sim = O.setup_sim(g, K=box.K, aniso=box.aniso)
+    flag = np.asarray(sim.flag)
+    fl = g.fluid.reshape(g.nx, g.ny, g.nz).astype(bool)
+    cid_cells = np.nonzero(fl.ravel())[0]
+    vol = lambda ch: x[:g.nx, :g.ny, :g.nz, ch].ravel()[cid_cells]
+    np.testing.assert_array_equal(vol(2) > 0, flag == 1)
+    np.testing.assert_array_equal(vol(5) > 0, flag == 2)
+    np.testing.assert_array_equal(vol(1) > 0, flag == 3)
+    np.testing.assert_allclose(vol(3)[flag == 3], box.K / 6, rtol=1e-6)
+    # solid outside the box's lattice and on its solid cells; distance sign
+    assert (x[g.nx:, ..., 0] == 1).all() and (x[:, g.ny:, :, 0] == 1).all() and (x[:, :, g.nz:, 0] == 1).all()
+    assert ((x[..., 4] > 0) == (x[..., 0] == 0)).all()
+    # targets: the averaged fields scattered to the fluid cells
+    NF = cid_cells.size
+    rng = np.random.default_rng(0)
+    acc = rng.standard_normal(4 * NF).astype(np.float32)
+    acc[3 * NF:] = 1 + 0.01 * acc[3 * NF:]
+    y, m = S.targets(grid, acc)
+    assert m.sum() == NF and y.shape == S.GRID + (4,)
+    got = y[:g.nx, :g.ny, :g.nz].reshape(-1, 4)[cid_cells]
+    np.testing.assert_allclose(got[:, 0], acc[:NF] / S.U0, rtol=1e-6)
+    np.testing.assert_allclose(got[:, 3], (acc[3 * NF:] - 1) / S.RHO_SCALE, rtol=1e-4, atol=1e-4)
+    # and back: the prediction of the targets gives the fields again
+    ru = S.rho_u_of_prediction(y, grid)
+    np.testing.assert_allclose(ru[:NF], acc[3 * NF:], rtol=1e-6)
+    np.testing.assert_allclose(ru[NF:2 * NF], acc[:NF], rtol=1e-6, atol=1e-7)
+
+
+def _geom_view(g):
+    """What resample reads of a geometry: grid(), origin, h."""
+    grid = grid_of_oracle(g)
+    return types.SimpleNamespace(grid=lambda: grid, origin=list(g.origin), h=g.h)
+
+
+def test_resample_same_lattice_is_identity_and_linear_fields_survive():
+    box, g = _box_geom(5)
+    gv = _geom_view(g)
+    grid = gv.grid()
+    fl = g.fluid.reshape(g.nx, g.ny, g.nz).astype(bool)
+    rng = np.random.default_rng(1)
+    pred = rng.standard_normal(S.GRID + (4,)).astype(np.float32)
+    ru = S.resample(pred, gv, gv)
+    np.testing.assert_allclose(ru, S.rho_u_of_prediction(pred, grid), rtol=1e-5, atol=1e-6)
+    # a field linear in position is reproduced on a finer lattice away from walls
+    _, gf = _box_geom(5, dx=3.0)
+    gfv = _geom_view(gf)
+    I, J, K = np.meshgrid(np.arange(S.GRID[0]), np.arange(S.GRID[1]), np.arange(S.GRID[2]), indexing="ij")
+    pos = [g.origin[a] + (idx + 0.5) * g.h for a, idx in enumerate((I, J, K))]
+    lin = np.stack([0.01 * pos[0], 0.02 * pos[1], -0.01 * pos[2], 0.005 * pos[0]], -1).astype(np.float32)
+    ru = S.resample(lin, gv, gfv)
+    ffl = gf.fluid.reshape(gf.nx, gf.ny, gf.nz).astype(bool)
+    idx = np.nonzero(ffl)
+    fpos = [gf.origin[a] + (idx[a] + 0.5) * gf.h for a in range(3)]
+    NFf = idx[0].size
+    ux = ru[NFf:2 * NFf]
+    # cells whose whole coarse interpolation stencil is fluid
+    ci = [np.clip((fpos[a] - g.origin[a]) / g.h - 0.5, 0, n - 1) for a, n in enumerate((g.nx, g.ny, g.nz))]
+    i0 = [np.minimum(np.floor(c).astype(int), n - 2) for c, n in zip(ci, (g.nx, g.ny, g.nz))]
+    full = np.ones(NFf, bool)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                full &= fl[i0[0] + dx, i0[1] + dy, i0[2] + dz]
+    inside = full & np.all([(fpos[a] - g.origin[a]) / g.h - 0.5 >= 0 for a in range(3)], 0)
+    assert inside.sum() > 0.5 * NFf
+    np.testing.assert_allclose(ux[inside], S.U0 * 0.01 * fpos[0][inside], rtol=1e-4, atol=1e-6)
