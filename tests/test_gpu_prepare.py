@@ -91,3 +91,29 @@ def test_voxelize_empty_and_degenerate():
     assert a.sum() == 0
     a = abx.voxelize(dev(), np.zeros(0, np.float32), [0, 0, 0], 4, 4, 4, 1.0)
     assert a.sum() == 0
+
+
+@pytest.mark.parametrize("seed", [1000000, 1000003])
+def test_study_box_200k_triangles_bit_exact(seed):
+    """The warm-start study's input: a box written as a binary STL of about
+    200 k triangles, parsed and prepared on the GPU at dx = 2 mm.  Every array
+    must equal the oracle's (itself bit-exact to engine.js on the fixtures)."""
+    from airboxcalc_amd import geometry as G
+    from airboxcalc_amd import surrogate as S
+    box = S.family([seed])[0]
+    box.spec.subdiv = S.STUDY_SUBDIV
+    stl = G.stl_binary_bytes(G.airbox_triangles(box.spec))
+    tris = abx.parse_stl(stl)
+    assert tris.size // 9 > 150_000
+    assert np.array_equal(tris.view(np.uint32), O.parse_stl(stl).view(np.uint32))
+    p = G.prepare_params(box.spec, 2.0)
+    say("prepare 200k-triangle study box", seed, tris.size // 9)
+    g = abx.Geometry(dev(), tris, p, max_cells=0)
+    o = O.prepare(tris, p, max_cells=0)
+    try:
+        for k in ("nx", "ny", "nz", "N", "nf", "top", "jm"):
+            assert getattr(g, k) == getattr(o, k), k
+        for k in ("solid", "fluid", "opening", "outlet", "fine", "xc", "dc", "zc"):
+            assert np.array_equal(g.array(k), getattr(o, k)), k
+    finally:
+        g.free()
