@@ -118,6 +118,7 @@ def load() -> C.CDLL:
         "abx_device_open": (i32, [i32, C.POINTER(vp), C.c_char_p, C.c_size_t]),
         "abx_device_close": (None, [vp]),
         "abx_copy_bandwidth": (i32, [vp, i64, i32, C.POINTER(d), C.POINTER(d)]),
+        "abx_aa_pattern_bandwidth": (i32, [vp, i64, i32, C.POINTER(d), C.POINTER(d)]),
         "abx_parse_stl": (i32, [C.c_char_p, i64, vp, i64, C.POINTER(i64)]),
         "abx_voxelize": (i32, [vp, vp, i64, vp, i32, i32, i32, d, vp]),
         "abx_prepare": (i32, [vp, vp, i64, C.POINTER(PrepParams), C.POINTER(vp)]),
@@ -213,6 +214,14 @@ class Device:
             self.close()
         except Exception:
             pass
+
+
+def aa_pattern_bandwidth(dev: "Device", positions: int, iters: int = 20):
+    """(GB/s at 152 B per position, ms per pass) of the AA step's memory
+    pattern over `positions` dense positions (abx_aa_pattern_bandwidth)."""
+    g, ms = C.c_double(), C.c_double()
+    _check(load().abx_aa_pattern_bandwidth(dev.h, int(positions), iters, C.byref(g), C.byref(ms)))
+    return g.value, ms.value
 
 
 def copy_bandwidth(dev: "Device", nbytes: int, iters: int = 20):
@@ -457,9 +466,11 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, kernel: int = 4, xcd_map: int = 0, chunk: int = 256, nt: int = 0) -> None:
+    def set_launch(self, kernel: int = 4, xcd_map: int = 8, chunk: int = 256, nt: int = 0) -> None:
         """kernel 4 = AA pattern (default); 0 wide, 1 narrow, 2 segment, 3
-        persistent segment (two lattices); 5 AA with per-cell geometry words."""
+        persistent segment (two lattices); 5 AA with per-cell geometry words.
+        xcd_map: block order across XCDs (0 round robin, 1 contiguous, G > 1
+        groups of G blocks; the library's default is 8)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
 
     def set_graph(self, enable: bool = True) -> None:

@@ -56,3 +56,58 @@ extern "C" int abx_copy_bandwidth(abx_dev* dev, int64_t bytes, int iters, double
   if (ms_per_copy) *ms_per_copy = ms / iters;
   return 0;
 }
+
+namespace abx {
+namespace {
+
+// The memory pattern of an AA even step with every position active: each
+// lane reads the 19 population planes at its position and writes them back
+// permuted (slot q -> slot opp q), in place.  No collision; 152 B per
+// position.  This is the ceiling of the stream-collide's access pattern, as
+// k_copy4 is the ceiling of a plain copy.
+__global__ __launch_bounds__(256) void k_aa_pattern(float* __restrict__ f, int64_t P) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float v[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) v[q] = f[q * P + i];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) f[OPP[q] * P + i] = v[q];
+}
+
+}  // namespace
+}  // namespace abx
+
+extern "C" int abx_aa_pattern_bandwidth(abx_dev* dev, int64_t positions, int iters, double* gbps,
+                                        double* ms_per_pass) {
+  if (!dev || positions < 1024 || iters < 1 || !gbps) {
+    set_error("abx_aa_pattern_bandwidth: bad argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(dev->ordinal));
+  float* f = nullptr;
+  ABX_HIP(hipMalloc((void**)&f, (size_t)positions * 19 * 4));
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemset(f, 0, (size_t)positions * 19 * 4);
+  const unsigned grid = (unsigned)((positions + 255) / 256);
+  hipLaunchKernelGGL(k_aa_pattern, dim3(grid), dim3(256), 0, 0, f, positions);
+  (void)hipEventRecord(e0, 0);
+  for (int i = 0; i < iters; ++i) hipLaunchKernelGGL(k_aa_pattern, dim3(grid), dim3(256), 0, 0, f, positions);
+  (void)hipEventRecord(e1, 0);
+  (void)hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  const hipError_t err = hipGetLastError();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(f);
+  if (err != hipSuccess) {
+    set_errorf("abx_aa_pattern_bandwidth: %s", hipGetErrorString(err));
+    return -1;
+  }
+  *gbps = 152.0 * (double)positions * iters / (ms * 1e-3) / 1e9;
+  if (ms_per_pass) *ms_per_pass = ms / iters;
+  return 0;
+}

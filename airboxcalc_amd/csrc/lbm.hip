@@ -111,7 +111,20 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   const int b = blockIdx.y;
   const int nblk = (a.nchunks + 3) >> 2;
-  for (int blk = blockIdx.x; blk < nblk; blk += PERSIST ? gridDim.x : nblk) {
+  // Block order across the 8 XCDs (workgroup x runs on XCD x % 8 as the
+  // x-th of that XCD's workgroups, s = x / 8):
+  //   xcd_map 0: block x (round robin);
+  //   xcd_map 1: XCD x % 8 owns the contiguous range [(x % 8) per_xcd, ...);
+  //   xcd_map G > 1: groups of G consecutive blocks per XCD, all XCDs moving
+  //   through the lattice together (block (s / G) 8G + (x % 8) G + s % G), so
+  //   neighbouring rows share an L2 while the active region stays narrow.
+  int first = (int)blockIdx.x;
+  if (!PERSIST && a.xcd_map == 1) first = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
+  if (!PERSIST && a.xcd_map > 1) {
+    const int sq = (int)(blockIdx.x >> 3), G = a.xcd_map;
+    first = (sq / G) * 8 * G + (int)(blockIdx.x & 7) * G + sq % G;
+  }
+  for (int blk = first; blk < nblk; blk += PERSIST ? gridDim.x : nblk) {
     if (t < 4 * 36) {
       const int ss = t / 36, sg = 4 * blk + ss;
       (&win[ss][0][0])[t - 36 * ss] = sg < a.nchunks ? seg_window_word(a, a.chunks[sg], t - 36 * ss) : 0u;
@@ -314,7 +327,7 @@ struct abx_sim {
   abx_dev* dev = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int prec = ABX_F32, B = 1, xcd_map = 0;
+  int prec = ABX_F32, B = 1, xcd_map = 8;  // default launch: groups of 8 blocks per XCD
   int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
   int64_t ntot = 0, plane = 0, NF = 0;
   size_t ssz = 4;  // bytes per stored value
@@ -433,9 +446,12 @@ void launch_wide(abx_sim* s, StepArgs a, int doacc) {
 }
 
 template <typename S, typename M, bool PERSIST, int MODE>
-void launch_seg(abx_sim* s, const StepArgs& a, int doacc) {
+void launch_seg(abx_sim* s, StepArgs a, int doacc) {
   const int nblk = (a.nchunks + 3) / 4;
-  const unsigned gx = PERSIST ? (unsigned)std::min(nblk, s->dev->ncu * 8) : (unsigned)nblk;
+  a.per_xcd = (nblk + 7) / 8;
+  unsigned gx = PERSIST ? (unsigned)std::min(nblk, s->dev->ncu * 8)
+                        : (a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk);
+  if (!PERSIST && a.xcd_map > 1) gx = (unsigned)((nblk + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
   dim3 grid(gx, (unsigned)s->B), block(256);
   const bool ns = a.ns != nullptr;
   if (doacc) {
@@ -1123,7 +1139,7 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
       if (abx_sim_write_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
   }
   s->kernel = kernel;
-  s->xcd_map = xcd_map ? 1 : 0;
+  s->xcd_map = std::max(0, std::min(xcd_map, 4096));
   s->chunk = chunk;
   s->nt = nt;
   return 0;

@@ -431,7 +431,16 @@ __device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int 
 template <typename S, typename M, bool ACC, bool NS, int MODE>
 __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win)[9][4], int seg, int l, int b) {
   const int p0 = l + 32;
-  if (!((win[4][p0 >> 5] >> (p0 & 31)) & 1u)) return;
+  // AA steps: a non-fluid position (solid or padding) whose 32-position half
+  // of the segment holds fluid moves its own 19 slots through unchanged (read
+  // slot q, write slot opp q, as a fluid cell walled in on all sides).  No
+  // fluid cell ever reads or writes those slots; this only makes every memory
+  // line that holds fluid fully read and fully written (no partially written
+  // lines at the ragged ends of the fluid runs), while lines without fluid
+  // stay untouched.
+  const uint32_t own = win[4][p0 >> 5];
+  const bool fluid = (own >> (p0 & 31)) & 1u;
+  if (!fluid && (MODE == 0 || own == 0u)) return;
   const int64_t P = a.plane;
   const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
   S* fout = (S*)a.fout + (int64_t)b * 19 * P;
@@ -445,7 +454,7 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
   for (int q = 0; q < 19; ++q) {
     const int p = p0 - CZ[q];
     const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const bool nbf = (win[wi][p >> 5] >> (p & 31)) & 1u;
+    const bool nbf = fluid && ((win[wi][p >> 5] >> (p & 31)) & 1u);
     nbm |= (uint32_t)nbf << q;
     const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
     const S* src;
@@ -472,14 +481,16 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
       const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
       dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
     }
-    *dst = (S)fo[q];
+    *dst = fluid ? (S)fo[q] : (S)fi[q];
   }
   if (ACC) {
     S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-    ac[d] = (S)((M)ac[d] + ux);
-    ac[P + d] = (S)((M)ac[P + d] + uy);
-    ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
-    ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
+    // non-fluid positions rewrite their (unused) sums unchanged
+    const S a0 = ac[d], a1 = ac[P + d], a2 = ac[2 * P + d], a3 = ac[3 * P + d];
+    ac[d] = fluid ? (S)((M)a0 + ux) : a0;
+    ac[P + d] = fluid ? (S)((M)a1 + uy) : a1;
+    ac[2 * P + d] = fluid ? (S)((M)a2 + uz) : a2;
+    ac[3 * P + d] = fluid ? (S)((M)a3 + rho) : a3;
   }
 }
 

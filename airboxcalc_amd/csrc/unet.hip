@@ -100,14 +100,16 @@ __device__ __forceinline__ void load_brick(u16* s, int sp, const u16* __restrict
 // Forward convolution (also the input gradient with repacked weights).
 // grid (tiles, Co / (16*NCT)), 256 threads; wave w computes voxel rows
 // j = 2w, 2w+1 (16 voxels each) for NCT output-channel tiles of 16.
-// wt: bf16 [Co][TAPS][Ci].  ck: channels per LDS chunk (divides Ci, <= 64).
-template <int TAPS, int NCT>
+// wt: bf16 [Co][TAPS][Ci].  CK: channels per LDS chunk (divides Ci); the
+// brick takes (CK + 8) * 2 bytes per voxel, so small chunks leave room for
+// more workgroups per CU.
+template <int TAPS, int NCT, int CK>
 __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
                                                   const float* __restrict__ bias, u16* __restrict__ y, int ys, int D,
-                                                  int H, int W, int Ci, int Co, int ck, int relu, int accum) {
+                                                  int H, int W, int Ci, int Co, int relu, int accum) {
   using Bk = Brick<TAPS>;
-  __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * 72];
-  const int sp = ck + 8;
+  constexpr int sp = CK + 8, ck = CK;
+  __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * sp];
   const Tile tl = tile_of(blockIdx.x, D, H, W);
   const int co0 = blockIdx.y * 16 * NCT;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
@@ -276,13 +278,33 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
   if (do_bias && threadIdx.x < CO) partb[(int64_t)blockIdx.x * Co + co0 + threadIdx.x] = bsum;
 }
 
-// out[i] = (accum ? out[i] : 0) + sum_s part[s*n + i]
-__global__ void k_reduce_parts(const float* __restrict__ part, int S, int64_t n, float* __restrict__ out, int accum) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float s = accum ? out[i] : 0.f;
-  for (int k = 0; k < S; ++k) s += part[(int64_t)k * n + i];
-  out[i] = s;
+// out[i] = (accum ? out[i] : 0) + sum_s part[s*n + i].  A workgroup owns 64
+// consecutive i; its 4 waves sum the partials s = g, g+4, g+8, ... (8
+// independent loads in flight per lane), then wave 0 adds the four sums in a
+// fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, int S, int64_t n,
+                                                      float* __restrict__ out, int accum) {
+  __shared__ float red[4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  float acc = 0.f;
+  if (i < n) {
+    int k = g;
+    for (; k + 28 < S; k += 32) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = part[(int64_t)(k + 4 * u) * n + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; k < S; k += 4) acc += part[(int64_t)k * n + i];
+  }
+  red[g][lane] = acc;
+  __syncthreads();
+  if (g == 0 && i < n) {
+    const float s = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    out[i] = accum ? out[i] + s : s;
+  }
 }
 
 // fp32 [Co][T][Ci] -> bf16 [Co][T][Ci], or for the input gradient the flipped
@@ -482,7 +504,9 @@ WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
   const int ngr = (taps + s.tg - 1) / s.tg;
   s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci)) * ngr;
   const int64_t nt = tiles_of(n, d, h, w);
-  s.S = (int)std::max<int64_t>(1, std::min<int64_t>(nt, std::max(1, 2048 / s.blocks_y)));
+  // enough workgroups to fill the GPU, but at most about 4 M partial sums
+  const int64_t by_size = std::max<int64_t>(1, ((int64_t)4 << 20) / ((int64_t)co * taps * ci + co));
+  s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, (int64_t)std::max(1, 2048 / s.blocks_y), by_size}));
   return s;
 }
 
@@ -526,18 +550,24 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
   const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   const dim3 grid((unsigned)tiles_of(n, d, h, w), (unsigned)(co / (16 * nct)));
   hipStream_t s = (hipStream_t)stream;
-#define ABX_FWD(T, NC)                                                                                              \
-  hipLaunchKernelGGL((k_conv_fwd<T, NC>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h, w, ci, co, ck, \
+#define ABX_FWD(T, NC, C)                                                                                          \
+  hipLaunchKernelGGL((k_conv_fwd<T, NC, C>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h, w, ci, co, \
                      relu, accumulate)
+#define ABX_FWD_CK(T, NC)          \
+  if (ck == 64) ABX_FWD(T, NC, 64); \
+  else if (ck == 48) ABX_FWD(T, NC, 48); \
+  else if (ck == 32) ABX_FWD(T, NC, 32); \
+  else ABX_FWD(T, NC, 16);
   if (ksize == 3) {
-    if (nct == 4) ABX_FWD(27, 4);
-    else if (nct == 2) ABX_FWD(27, 2);
-    else ABX_FWD(27, 1);
+    if (nct == 4) { ABX_FWD_CK(27, 4) }
+    else if (nct == 2) { ABX_FWD_CK(27, 2) }
+    else { ABX_FWD_CK(27, 1) }
   } else {
-    if (nct == 4) ABX_FWD(1, 4);
-    else if (nct == 2) ABX_FWD(1, 2);
-    else ABX_FWD(1, 1);
+    if (nct == 4) { ABX_FWD_CK(1, 4) }
+    else if (nct == 2) { ABX_FWD_CK(1, 2) }
+    else { ABX_FWD_CK(1, 1) }
   }
+#undef ABX_FWD_CK
 #undef ABX_FWD
   ABX_HIP(hipGetLastError());
   return 0;
@@ -590,8 +620,10 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
 #undef ABX_WG_T
 #undef ABX_WG
   ABX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
-  if (db) hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(co)), dim3(256), 0, s, partb, sh.S, (int64_t)co, db, accumulate);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw, 64)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
+  if (db)
+    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(co, 64)), dim3(256), 0, s, partb, sh.S, (int64_t)co, db,
+                       accumulate);
   ABX_HIP(hipGetLastError());
   return 0;
 }

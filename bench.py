@@ -40,6 +40,12 @@ def workload(dx: float, seed: int):
     return spec, G.airbox_triangles(spec), p
 
 
+def dense_positions(g) -> int:
+    """Positions of the padded dense lattice the stream-collide runs on:
+    (nx+2)(ny+2) rows of nz+2 cells rounded up to 32."""
+    return (g.nx + 2) * (g.ny + 2) * (32 * ((g.nz + 2 + 31) // 32))
+
+
 def cpu_baseline(tris, p, run, seconds: float):
     """The oracle's single-threaded C restatement of cpuStep on the same
     geometry, timed on this host for a bounded number of steps."""
@@ -127,6 +133,64 @@ def bench_batch(a, dev):
     print(json.dumps(out), flush=True)
 
 
+def unet_flops(net_layers, n: int, dims) -> dict:
+    """Algorithmic FLOPs of one training step of the surrogate: per conv layer
+    2 * voxels * co * ci * taps for the forward, the input gradient (not for
+    the first layer) and the weight gradient."""
+    from airboxcalc_amd import unet as U
+    fwd = dgrad = wgrad = 0
+    for l in net_layers:
+        d, h, w = dims[U.UNet._level(l)]
+        f = 2 * n * d * h * w * l.co * l.ci * l.taps
+        fwd += f
+        wgrad += f
+        if l.name != "enc1a":
+            dgrad += f
+    return dict(fwd=fwd, dgrad=dgrad, wgrad=wgrad, total=fwd + dgrad + wgrad)
+
+
+def bench_unet(a, dev):
+    """Surrogate training step (the config-4 network: 96 x 48 x 64 volume,
+    batch 8, BF16 MFMA convolutions, Adam) on synthetic inputs."""
+    import torch
+    from airboxcalc_amd import unet as U
+    from airboxcalc_amd import surrogate as S
+    n = 8
+    D, H, W = S.GRID
+    steps = a.steps if a.steps != 6000 else 50
+    warm = min(a.warmup, 10)
+    g = torch.Generator(device="cpu").manual_seed(a.seed)
+    x = torch.randn((n, D, H, W, U.IN_CH), generator=g).to(torch.bfloat16).cuda()
+    y = torch.randn((n, D, H, W, 4), generator=g).to(torch.bfloat16).cuda()
+    m = torch.ones((n, D, H, W), dtype=torch.uint8, device="cuda")
+    cw = torch.ones(4, dtype=torch.float32, device="cuda")
+    net = U.UNet(seed=0)
+    for _ in range(warm):
+        net.train_step(x, y, m, cw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = net.train_step(x, y, m, cw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / steps
+    fl = unet_flops(net.L, n, net.dims)
+    tflops = fl["total"] / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "surrogate training steps/s (U-Net, BF16 MFMA convolutions)", "value": round(1e3 / ms, 2),
+        "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)",
+        "data": "synthetic (random volumes)",
+        "config": {"workload": f"U-Net 16/32/64/128, batch {n}, volume {D}x{H}x{W}, forward + backward + Adam",
+                   "params": U.n_params(), "flops_per_step": fl},
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 5), "traffic": None,
+                     "kernel": "all kernels of one training step (convolutions dominate)"},
+        "loss": float(loss.item()), "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,9 +203,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
-    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch", "fp64"],
+    ap.add_argument("--xcd-map", type=int, default=None, help="block order across XCDs (abx_sim_set_launch)")
+    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch", "fp64", "unet"],
                     help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch; "
-                         "fp64: the FP64 error report of the roofline airbox at dx=2 mm")
+                         "fp64: the FP64 error report of the roofline airbox at dx=2 mm; "
+                         "unet: surrogate training steps")
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
@@ -163,14 +229,16 @@ def main():
         return bench_batch(a, dev)
     if a.workload == "fp64":
         return bench_fp64(a, dev)
+    if a.workload == "unet":
+        return bench_unet(a, dev)
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
     t = time.perf_counter()
     g = abx.Geometry(dev, tris, p, max_cells=0)
     log(f"prepare {time.perf_counter() - t:.2f}s grid {g.nx}x{g.ny}x{g.nz} nf={g.nf} ntri={tris.size // 9}")
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
-    if a.kernel is not None:
-        sim.set_launch(kernel=a.kernel)
+    if a.kernel is not None or a.xcd_map is not None:
+        sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=8 if a.xcd_map is None else a.xcd_map)
     NF = sim.NF
     log(f"sim NF={NF} device bytes {sim.info.device_bytes / 2**30:.2f} GiB")
 
@@ -206,6 +274,8 @@ def main():
     assert np.isfinite(acc).all()
 
     copy_gbps, _ = abx.copy_bandwidth(dev, copy_bytes, 10)
+    # the stream-collide's own access pattern over the dense lattice (all positions active)
+    aa_gbps, _ = abx.aa_pattern_bandwidth(dev, dense_positions(g), 10)
     bpu = BYTES_PER_UPDATE[a.precision]
     mlups = NF * steps / wall / 1e6
     kern_ms = ev_ms[0] / max(ev_n[0], 1)
@@ -240,7 +310,9 @@ def main():
                      "avg_launch_ms": round(kern_ms, 5), "bytes_per_launch": NF * bpu,
                      "avg_launch_ms_with_acc": round(kern_acc_ms, 5),
                      "copy_ceiling_gbps": round(copy_gbps, 1),
-                     "frac_of_copy_ceiling": round(achieved / copy_gbps, 4)},
+                     "frac_of_copy_ceiling": round(achieved / copy_gbps, 4),
+                     "aa_pattern_ceiling_gbps": round(aa_gbps, 1),
+                     "frac_of_aa_pattern_ceiling": round(achieved / aa_gbps, 4)},
         "cpu_baseline": None,
     }
     if not a.no_cpu:

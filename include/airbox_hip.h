@@ -55,6 +55,12 @@ void abx_device_close(abx_dev* dev);
  * copy of `bytes` (read + write counted), averaged over iters launches. */
 int abx_copy_bandwidth(abx_dev* dev, int64_t bytes, int iters, double* gbps, double* ms_per_copy);
 
+/* Ceiling of the stream-collide's access pattern (no reference counterpart):
+ * GB/s (152 B per position) of an in-place pass over 19 planes of `positions`
+ * floats that reads slot q and writes slot opp q at every position, the AA
+ * even step without its collision, averaged over iters launches. */
+int abx_aa_pattern_bandwidth(abx_dev* dev, int64_t positions, int iters, double* gbps, double* ms_per_pass);
+
 /* ---- STL ------------------------------------------------------------------
  * Replaces parseSTL (engine.js:15-28).  Binary when 84 + 50*n == len, else
  * the ASCII "vertex x y z" records.  Writes at most cap floats to out (may be
@@ -209,8 +215,10 @@ int abx_sim_sync(abx_sim* sim);
  *         2 = segment (one position per lane, four active 64-position
  *             segments per workgroup; the default),
  *         3 = segment, persistent grid.
- * xcd_map 1 = each XCD sweeps one contiguous part of the lattice.  Every
- * variant computes the same values bit for bit. */
+ * xcd_map 1 = each XCD sweeps one contiguous part of the lattice; xcd_map
+ * G > 1 (segment kernels) = groups of G consecutive blocks per XCD, all XCDs
+ * moving through the lattice together.  Every variant computes the same
+ * values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
