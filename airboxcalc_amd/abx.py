@@ -21,6 +21,11 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "airbox_hip.h")
 F32, F32_F64, F64 = 0, 1, 2
 PRECISIONS = {"f32": F32, "f32_f64": F32_F64, "f64": F64}
 
+# D3Q19 velocity set in the reference's numbering (engine.js:4-9)
+CX = (0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0)
+CY = (0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1)
+CZ = (0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1)
+
 
 class AbxError(RuntimeError):
     """A failed library call; the message is the library's (the reference's
@@ -135,6 +140,8 @@ def load() -> C.CDLL:
         "abx_sim_read_f": (i32, [vp, i32, vp]),
         "abx_sim_write_f": (i32, [vp, i32, vp]),
         "abx_sim_init_eq": (i32, [vp, vp]),
+        "abx_sim_init_eq_resample": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, i32, vp, C.POINTER(d), d,
+                                           C.POINTER(d), d, C.POINTER(C.c_float)]),
         "abx_sim_clear_acc": (i32, [vp]),
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
@@ -413,6 +420,14 @@ class Sim:
             _check(load().abx_sim_read_acc(self.h, b, divisor, _p(out)))
         return out
 
+    def read_rho_u(self, b: int = 0) -> np.ndarray:
+        """(rho, ux, uy, uz) of the current populations in cid order (the
+        moments of read_f; host side)."""
+        f = self.read_f(b).astype(np.float64).reshape(19, self.NF)
+        rho = f.sum(0)
+        u = [(np.asarray(c, np.float64)[:, None] * f).sum(0) / rho for c in (CX, CY, CZ)]
+        return np.concatenate([rho] + u).astype(np.float32)
+
     def read_f(self, b: int = 0) -> np.ndarray:
         out = np.empty(19 * self.NF, self.sdtype)
         _check(load().abx_sim_read_f(self.h, b, _p(out)))
@@ -427,6 +442,17 @@ class Sim:
         r = np.ascontiguousarray(rho_u, np.float32)
         assert r.size == 4 * self.NF
         _check(load().abx_sim_init_eq(self.h, _p(r)))
+
+    def init_eq_resample(self, vol_ptr: int, cstride: int, vol_dims, coarse_dims, coarse_fluid_ptr: int,
+                         origin_c, h_c: float, origin_f, h_f: float, affine) -> None:
+        """abx_sim_init_eq_resample: equilibrium from a coarse predicted volume
+        already on the device (vol_ptr: bf16, coarse_fluid_ptr: uint8)."""
+        oc = (C.c_double * 3)(*[float(v) for v in origin_c])
+        of = (C.c_double * 3)(*[float(v) for v in origin_f])
+        af = (C.c_float * 8)(*[float(v) for v in affine])
+        _check(load().abx_sim_init_eq_resample(self.h, C.c_void_p(vol_ptr), int(cstride), *[int(v) for v in vol_dims],
+                                               *[int(v) for v in coarse_dims], C.c_void_p(coarse_fluid_ptr), oc,
+                                               float(h_c), of, float(h_f), af))
 
     def evaluate(self, eg: "EvalGeom", divisor: float = 1.0, b: int = 0, field3d: bool = True) -> dict:
         """evaluate + field3d (engine.js:632-719) on the GPU."""

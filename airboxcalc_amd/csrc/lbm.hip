@@ -303,6 +303,60 @@ __global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, 
   }
 }
 
+// Resampling of a coarse volume (bf16, channels (ux, uy, uz, rho) mapped to
+// lattice units by phys_c = a_c * v_c + b_c) onto the fluid cells, in cid
+// order, as (rho, ux, uy, uz) for k_init_eq.  Both lattices share the local
+// frame; cell centres are origin + (i + 0.5) h (engine.js:137-139).
+// Trilinear weights over the coarse fluid cells only, renormalised; a cell
+// with no fluid coarse neighbour gets rest (rho 1, u 0).
+struct ResampleArgs {
+  const uint16_t* vol;
+  const uint8_t* cfluid;  // nxc * nyc * nzc, k fastest
+  int cs, VD, VH, VW, nxc, nyc, nzc;
+  double oc[3], hc, of[3], hf;
+  float a[4], b[4];
+};
+
+__global__ void k_resample_rho_u(ResampleArgs r, const int32_t* cells, int64_t NF, int nyp, int nzp, float* rho_u) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NF) return;
+  const int d = cells[e];
+  const int k = d % nzp - 1, t = d / nzp, j = t % nyp - 1, i = t / nyp - 1;
+  const int n3[3] = {r.nxc, r.nyc, r.nzc};
+  const int idx[3] = {i, j, k};
+  int i0[3];
+  double w1[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const double pos = r.of[ax] + ((double)idx[ax] + 0.5) * r.hf;
+    double c = (pos - r.oc[ax]) / r.hc - 0.5;
+    c = fmin(fmax(c, 0.0), (double)(n3[ax] - 1));
+    int f0 = (int)floor(c);
+    f0 = min(f0, n3[ax] > 1 ? n3[ax] - 2 : 0);
+    i0[ax] = f0;
+    w1[ax] = c - (double)f0;
+  }
+  double acc[4] = {0, 0, 0, 0}, ws = 0;
+  for (int o = 0; o < 8; ++o) {
+    const int ci = i0[0] + (o & 1), cj = i0[1] + ((o >> 1) & 1), ck = i0[2] + (o >> 2);
+    if (ci >= r.nxc || cj >= r.nyc || ck >= r.nzc) continue;
+    if (!r.cfluid[((int64_t)ci * r.nyc + cj) * r.nzc + ck]) continue;
+    const double w = ((o & 1) ? w1[0] : 1.0 - w1[0]) * (((o >> 1) & 1) ? w1[1] : 1.0 - w1[1]) *
+                     ((o >> 2) ? w1[2] : 1.0 - w1[2]);
+    const uint16_t* v = r.vol + (((int64_t)ci * r.VH + cj) * r.VW + ck) * r.cs;
+    for (int c = 0; c < 4; ++c) acc[c] += w * (double)__uint_as_float((uint32_t)v[c] << 16);
+    ws += w;
+  }
+  float rho = 1.f, u[3] = {0.f, 0.f, 0.f};
+  if (ws > 0) {
+    for (int c = 0; c < 3; ++c) u[c] = (float)((double)r.a[c] * (acc[c] / ws) + (double)r.b[c]);
+    rho = (float)((double)r.a[3] * (acc[3] / ws) + (double)r.b[3]);
+  }
+  rho_u[e] = rho;
+  rho_u[NF + e] = u[0];
+  rho_u[2 * NF + e] = u[1];
+  rho_u[3 * NF + e] = u[2];
+}
+
 // Cold start f_q = WT[q] on every position of every plane (f0, engine.js:260-261).
 template <typename S>
 __global__ void k_fill_wt(S* f, int64_t P, int nplanes) {
@@ -1042,17 +1096,10 @@ extern "C" int abx_sim_write_f(abx_sim* s, int b, const void* in) {
   return 0;
 }
 
-extern "C" int abx_sim_init_eq(abx_sim* s, const float* rho_u) {
-  if (!s || !rho_u) {
-    set_error("abx_sim_init_eq: null argument");
-    return -1;
-  }
-  ABX_HIP(hipSetDevice(s->dev->ordinal));
+namespace {
+// Equilibrium of (rho, u) in cid order, already on the device, into every variant.
+int sim_init_eq_dev(abx_sim* s, const float* d) {
   const int64_t NF = s->NF;
-  if (NF == 0) return 0;
-  float* d = nullptr;
-  ABX_HIP(hipMalloc((void**)&d, (size_t)(4 * NF) * sizeof(float)));
-  ABX_HIP(hipMemcpyAsync(d, rho_u, (size_t)(4 * NF) * sizeof(float), hipMemcpyHostToDevice, s->stream));
   for (int v = 0; v < s->B; ++v) {
     if (s->prec == ABX_F32)
       hipLaunchKernelGGL((k_init_eq<float, float>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells, NF,
@@ -1068,8 +1115,76 @@ extern "C" int abx_sim_init_eq(abx_sim* s, const float* rho_u) {
                          s->nyp, s->nzp);
   }
   ABX_HIP(hipGetLastError());
-  ABX_HIP(hipStreamSynchronize(s->stream));
-  ABX_HIP(hipFree(d));
+  return 0;
+}
+}  // namespace
+
+extern "C" int abx_sim_init_eq(abx_sim* s, const float* rho_u) {
+  if (!s || !rho_u) {
+    set_error("abx_sim_init_eq: null argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(s->dev->ordinal));
+  const int64_t NF = s->NF;
+  if (NF == 0) return 0;
+  float* d = nullptr;
+  ABX_HIP(hipMalloc((void**)&d, (size_t)(4 * NF) * sizeof(float)));
+  ABX_HIP(hipMemcpyAsync(d, rho_u, (size_t)(4 * NF) * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  const int rc = sim_init_eq_dev(s, d);
+  const hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  ABX_HIP(e);
+  return 0;
+}
+
+extern "C" int abx_sim_init_eq_resample(abx_sim* s, const uint16_t* vol, int cstride, int vd, int vh, int vw, int nxc,
+                                        int nyc, int nzc, const uint8_t* coarse_fluid, const double origin_c[3],
+                                        double h_c, const double origin_f[3], double h_f, const float affine[8]) {
+  if (!s || !vol || !coarse_fluid || !origin_c || !origin_f || !affine || cstride < 4 || nxc < 1 || nyc < 1 ||
+      nzc < 1 || nxc > vd || nyc > vh || nzc > vw || !(h_c > 0) || !(h_f > 0)) {
+    set_error("abx_sim_init_eq_resample: bad argument");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(s->dev->ordinal));
+  const int64_t NF = s->NF;
+  if (NF == 0) return 0;
+  ResampleArgs r;
+  r.vol = vol;
+  r.cfluid = coarse_fluid;
+  r.cs = cstride;
+  r.VD = vd;
+  r.VH = vh;
+  r.VW = vw;
+  r.nxc = nxc;
+  r.nyc = nyc;
+  r.nzc = nzc;
+  for (int a = 0; a < 3; ++a) {
+    r.oc[a] = origin_c[a];
+    r.of[a] = origin_f[a];
+  }
+  r.hc = h_c;
+  r.hf = h_f;
+  for (int c = 0; c < 4; ++c) {
+    r.a[c] = affine[c];
+    r.b[c] = affine[4 + c];
+  }
+  float* d = nullptr;
+  ABX_HIP(hipMalloc((void**)&d, (size_t)(4 * NF) * sizeof(float)));
+  hipLaunchKernelGGL(k_resample_rho_u, dim3(nblk(NF, 256)), dim3(256), 0, s->stream, r, s->cells, NF, s->nyp, s->nzp,
+                     d);
+  const hipError_t le = hipGetLastError();
+  int rc = 0;
+  if (le != hipSuccess) {
+    set_errorf("abx_sim_init_eq_resample: %s", hipGetErrorString(le));
+    rc = -1;
+  } else {
+    rc = sim_init_eq_dev(s, d);
+  }
+  const hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  if (rc) return rc;
+  ABX_HIP(e);
   return 0;
 }
 

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import Callable, List, Optional
 
 import numpy as np
 
@@ -44,11 +44,16 @@ def rho_u_from_acc(acc: np.ndarray, NF: int) -> np.ndarray:
 
 def solve_to_steady(dev: abx.Device, g: abx.Geometry, p: dict, window: int = 500, tol_gamma: float = 0.005,
                     tol_zeta: float = 0.02, max_steps: int = 60000, precision: str = "f32",
-                    init_rho_u: Optional[np.ndarray] = None, keep_acc: bool = False) -> SteadyResult:
+                    init_rho_u: Optional[np.ndarray] = None, keep_acc: bool = False,
+                    init_fn: Optional[Callable[[abx.Sim], None]] = None) -> SteadyResult:
+    """init_rho_u: equilibrium start from (rho, u) in cid order; init_fn: any
+    start set on the fresh handle (its time is not part of wall_s)."""
     sim = abx.Sim(dev, g.grid(), p["K"], p["aniso"], p["u"], p["tau"], precision=precision)
     try:
         if init_rho_u is not None:
             sim.init_eq(init_rho_u)
+        if init_fn is not None:
+            init_fn(sim)
         eg = g.eval_view()
         wins = []
         prev = None

@@ -165,11 +165,24 @@ def resample(pred: np.ndarray, gc: abx.Geometry, gf: abx.Geometry) -> np.ndarray
 
 
 # ---------------------------------------------------------------------------
+def restrict_targets(acc: np.ndarray, gf: abx.Geometry, gc: abx.Geometry, shape=GRID):
+    """Targets on gc's lattice (as targets()) from the averaged fields acc
+    (4*NF of gf, cid order) of a solve of the same box on the finer lattice
+    gf: trilinear over gf's fluid cells at gc's fluid cell centres."""
+    fg, cg = gf.grid(), gc.grid()
+    vol, _ = targets(fg, acc, shape=(fg.nx, fg.ny, fg.nz))
+    ru = resample(vol, gf, gc)                     # (rho, ux, uy, uz) on gc's fluid cells
+    NF = ru.size // 4
+    return targets(cg, np.concatenate([ru[NF:], ru[:NF]]), shape)
+
+
 def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk: int = 64,
-                 log: Optional[Callable[[str], None]] = None) -> dict:
+                 log: Optional[Callable[[str], None]] = None, target_dx: Optional[float] = None) -> dict:
     """Solve every box (batched, one launch per step for `chunk` boxes, the
     last `avg` of `steps` steps averaged) and build the training volumes on
-    the device: X (n, 96, 48, 64, IN_CH) bf16, Y (n, ..., 4) bf16, M uint8."""
+    the device: X (n, 96, 48, 64, IN_CH) bf16, Y (n, ..., 4) bf16, M uint8.
+    With target_dx the solves run at that (finer) cell size and their fields
+    are restricted to the FAMILY_DX lattice of the inputs."""
     import torch
     n = len(boxes)
     D, H, W = GRID
@@ -181,13 +194,18 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
     for c0 in range(0, n, chunk):
         part = boxes[c0:c0 + chunk]
         t0 = time.perf_counter()
-        geoms, grids, sims = [], [], []
+        geoms, grids, sims, fines = [], [], [], []
         for b in part:
-            g = abx.Geometry(dev, G.airbox_triangles(b.spec), G.prepare_params(b.spec, FAMILY_DX), max_cells=0)
+            tris = G.airbox_triangles(b.spec)
+            g = abx.Geometry(dev, tris, G.prepare_params(b.spec, FAMILY_DX), max_cells=0)
             gr = g.grid()
             geoms.append(g)
             grids.append(gr)
-            sims.append(abx.Sim(dev, gr, K=b.K, aniso=b.aniso, u=U0, tau=TAU, precision="f32"))
+            gs = g
+            if target_dx is not None:
+                gs = abx.Geometry(dev, tris, G.prepare_params(b.spec, target_dx), max_cells=0)
+                fines.append(gs)
+            sims.append(abx.Sim(dev, gs.grid(), K=b.K, aniso=b.aniso, u=U0, tau=TAU, precision="f32"))
         for k, (b, gr) in enumerate(zip(part, grids)):
             X[c0 + k] = torch.from_numpy(features(gr, b.K, b.aniso)).to(torch.bfloat16)
         t1 = time.perf_counter()
@@ -200,7 +218,10 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
         bt.sync()
         t2 = time.perf_counter()
         for k, (s, gr) in enumerate(zip(sims, grids)):
-            y, m = targets(gr, s.read_acc(avg))
+            if target_dx is not None:
+                y, m = restrict_targets(s.read_acc(avg), fines[k], geoms[k])
+            else:
+                y, m = targets(gr, s.read_acc(avg))
             Y[c0 + k] = torch.from_numpy(y).to(torch.bfloat16)
             M[c0 + k] = torch.from_numpy(m)
         nf = sum(s.NF for s in sims)
@@ -211,7 +232,7 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
         bt.destroy()
         for s in sims:
             s.destroy()
-        for g in geoms:
+        for g in geoms + fines:
             g.free()
         if log:
             log(f"dataset {c0 + len(part)}/{n}: prepare+features {t1 - t0:.1f} s, solve {t2 - t1:.1f} s "
@@ -299,6 +320,31 @@ def predict(net: U.UNet, x: np.ndarray, norm: dict) -> np.ndarray:
     return y * np.asarray(norm["std"], np.float32) + np.asarray(norm["mean"], np.float32)
 
 
+def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: abx.Geometry, sim: abx.Sim,
+                    tris: Optional[np.ndarray] = None) -> None:
+    """Seed sim (on g_fine's lattice) at the equilibrium of the surrogate's
+    prediction without leaving the GPU: the box is prepared at FAMILY_DX, its
+    features go through the network, and abx_sim_init_eq_resample maps the
+    network's output (standardisation undone by the affine map) onto sim's
+    fluid cells."""
+    import torch
+    tris = G.airbox_triangles(box.spec) if tris is None else tris
+    gc = abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
+    try:
+        grid = gc.grid()
+        x = torch.from_numpy(features(grid, box.K, box.aniso)[None]).to(net.dev).to(torch.bfloat16)
+        out = net.forward(x)
+        fl = torch.from_numpy(grid.fluid).to(net.dev)
+        mean, std = np.asarray(norm["mean"], np.float64), np.asarray(norm["std"], np.float64)
+        scale = np.array([U0, U0, U0, RHO_SCALE])
+        affine = list(scale * std) + list(scale * mean + np.array([0.0, 0.0, 0.0, 1.0]))
+        torch.cuda.synchronize(net.dev)
+        sim.init_eq_resample(out.data_ptr(), U.HEAD_CH, GRID, (gc.nx, gc.ny, gc.nz), fl.data_ptr(), gc.origin, gc.h,
+                             g_fine.origin, g_fine.h, affine)
+    finally:
+        gc.free()
+
+
 def warm_start_fields(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: abx.Geometry,
                       tris: Optional[np.ndarray] = None) -> np.ndarray:
     """(rho, ux, uy, uz) in cid order for g_fine's lattice from the surrogate:
@@ -368,11 +414,17 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
             try:
                 fixed = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg)
                 cold = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, keep_acc=True)
-                t3 = time.perf_counter()
-                rho_u = warm_start_fields(net, norm, dev, box, g, tris)
-                t4 = time.perf_counter()
-                warm = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, init_rho_u=rho_u,
+                timing = {}
+
+                def init(sim, timing=timing):
+                    t = time.perf_counter()
+                    warm_start_init(net, norm, dev, box, g, sim, tris)
+                    timing["predict_s"] = time.perf_counter() - t
+                    timing["rho_u"] = sim.read_rho_u()
+
+                warm = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, init_fn=init,
                                               keep_acc=True)
+                rho_u = timing["rho_u"]
                 f64 = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg, precision="f64") \
                     if fp64_every and bi % fp64_every == 0 else None
             except abx.AbxError as e:       # e.g. the reference's instability error
@@ -401,7 +453,7 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
         NF = fixed.acc.size // 4
         init = np.concatenate([rho_u[NF:], rho_u[:NF]])     # (ux, uy, uz, rho), the order of acc
         row = dict(seed=box.seed, ntri=int(tris.size // 9), grid=[g.nx, g.ny, g.nz], NF=int(NF), K=box.K,
-                   aniso=box.aniso, stl_s=t1 - t0, prepare_s=t2 - t1, predict_s=t4 - t3,
+                   aniso=box.aniso, stl_s=t1 - t0, prepare_s=t2 - t1, predict_s=timing["predict_s"],
                    fixed=run(fixed), cold=run(cold), warm=run(warm),
                    cold_vs_fixed=judge(cold, fixed), warm_vs_fixed=judge(warm, fixed),
                    prediction_u_err_vs_fixed=uerr(init, fixed.acc))
@@ -470,6 +522,8 @@ def main(argv=None) -> int:
     tr.add_argument("--batch", type=int, default=8)
     tr.add_argument("--lr", type=float, default=1e-3)
     tr.add_argument("--max-train-seconds", type=float, default=600)
+    tr.add_argument("--target-dx", type=float, default=None,
+                    help="solve the boxes at this cell size and restrict the fields to the 96x48x64 inputs")
     tr.add_argument("--out", required=True, help="weights .npz")
     tr.add_argument("--report", help="JSON report")
     st = sub.add_parser("study", help="cold vs warm time to steady on held-out boxes")
@@ -487,12 +541,12 @@ def main(argv=None) -> int:
     if a.cmd == "train":
         boxes = family(range(a.first_seed, a.first_seed + a.boxes))
         t0 = time.perf_counter()
-        data = make_dataset(dev, boxes, a.steps, a.avg, log=log)
+        data = make_dataset(dev, boxes, a.steps, a.avg, log=log, target_dx=a.target_dx)
         t_data = time.perf_counter() - t0
         net = U.UNet(seed=0)
         res = train(net, data, a.epochs, a.batch, a.lr, log=log, max_seconds=a.max_train_seconds)
         meta = dict(boxes=a.boxes, first_seed=a.first_seed, steps=a.steps, avg=a.avg, dx=FAMILY_DX, grid=GRID,
-                    epochs_run=len(res["history"]), batch=a.batch, lr=a.lr)
+                    target_dx=a.target_dx, epochs_run=len(res["history"]), batch=a.batch, lr=a.lr)
         save(a.out, net, data["norm"], meta)
         rep = dict(meta=meta, dataset_s=t_data, dataset_stats=data["stats"], norm={k: v.tolist() for k, v in
                                                                                    data["norm"].items()},

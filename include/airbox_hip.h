@@ -199,6 +199,20 @@ int abx_sim_write_f(abx_sim* sim, int b, const void* in);
  * uy[NF], uz[NF]} in cid order, for every variant (feq as engine.js:296).
  * With rho = 1, u = 0 this is the cold start f0 = WT exactly. */
 int abx_sim_init_eq(abx_sim* sim, const float* rho_u);
+/* Equilibrium start from a predicted field on a coarser lattice of the same
+ * box (the surrogate's warm start; the reference starts from rest,
+ * engine.js:260-261).  vol (device, bf16, vd x vh x vw cells, cstride
+ * channels per cell, channels 0..3 = ux, uy, uz, rho as v) holds the coarse
+ * lattice in its first nxc x nyc x nzc cells; coarse_fluid (device,
+ * nxc*nyc*nzc bytes, k fastest) marks its fluid cells.  Each fluid cell of
+ * sim takes the trilinear average over the fluid coarse cells around its
+ * centre (weights renormalised; rest if none), mapped to lattice units as
+ * phys_c = affine[c] * v_c + affine[4 + c].  Cell centres are origin +
+ * (i + 0.5) h on both lattices (engine.js:137-139).  The caller makes vol
+ * and coarse_fluid ready before the call; the call returns when sim is set. */
+int abx_sim_init_eq_resample(abx_sim* sim, const uint16_t* vol, int cstride, int vd, int vh, int vw, int nxc,
+                             int nyc, int nzc, const uint8_t* coarse_fluid, const double origin_c[3], double h_c,
+                             const double origin_f[3], double h_f, const float affine[8]);
 /* Replaces clearAcc (engine.js:559). */
 int abx_sim_clear_acc(abx_sim* sim);
 /* Cold start again: every variant back to f0 = WT (engine.js:260-261) and

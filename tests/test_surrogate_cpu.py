@@ -105,3 +105,32 @@ def test_resample_same_lattice_is_identity_and_linear_fields_survive():
     inside = full & np.all([(fpos[a] - g.origin[a]) / g.h - 0.5 >= 0 for a in range(3)], 0)
     assert inside.sum() > 0.5 * NFf
     np.testing.assert_allclose(ux[inside], S.U0 * 0.01 * fpos[0][inside], rtol=1e-4, atol=1e-6)
+
+
+def test_trained_weights_file_matches_the_network():
+    from airboxcalc_amd import unet as U
+    z = np.load(S.WEIGHTS)
+    assert z["w"].shape == (U.n_params(),) and np.isfinite(z["w"]).all()
+    assert z["norm_mean"].shape == (4,) and (z["norm_std"] > 0).all()
+    import json
+    meta = json.loads(bytes(z["meta"]).decode())
+    assert meta["grid"] == list(S.GRID) and meta["dx"] == S.FAMILY_DX
+    # the unused head channels stay zero
+    head = U.layers()[-1]
+    W = z["w"][head.w_off:head.w_off + head.nw].reshape(head.co, head.taps, head.ci)
+    assert not W[U.OUT_CH:].any()
+
+
+def test_restricted_targets_of_a_same_lattice_solve_are_the_targets():
+    """Restriction from a lattice onto itself returns targets() of the field."""
+    box, g = _box_geom(7)
+    gv = _geom_view(g)
+    grid = gv.grid()
+    NF = int(g.fluid.sum())
+    rng = np.random.default_rng(3)
+    acc = rng.standard_normal(4 * NF).astype(np.float32) * 0.01
+    acc[3 * NF:] += 1.0
+    y0, m0 = S.targets(grid, acc)
+    y1, m1 = S.restrict_targets(acc, gv, gv)
+    assert np.array_equal(m0, m1)
+    np.testing.assert_allclose(y1, y0, rtol=1e-5, atol=1e-5)
