@@ -78,3 +78,47 @@ def simulate_hip(dev: abx.Device, g, p: dict, on_progress: Optional[Callable] = 
     elif keep_sim:
         res.sim = sim
     return res
+
+
+# ---------------------------------------------------------------------------
+# Output files in the layout of the reference's command-line tool, so that its
+# `--from-npz` report reads results of this backend (airbox_cfd.py:160-166,
+# 287-288, 360-366).
+
+def cli_arrays(g, acc: np.ndarray) -> dict:
+    """The NPZ arrays of airbox_cfd.py:287 from a prepared geometry g and the
+    averaged fields acc (4*NF, components ux, uy, uz, rho, cid order): masks
+    and fields as (nx, ny, nz) arrays, fields 0 outside the fluid."""
+    from .surrogate import cell_types
+    grid = g.grid()
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    fluid, inlet, outlet, porous = cell_types(grid)
+    NF = int(fluid.sum())
+    a = np.asarray(acc, np.float64).reshape(4, NF)
+    out = dict(fluid=fluid, porous=porous, solid=g.array("solid").reshape(nx, ny, nz).astype(bool),
+               xc=g.array("xc").astype(np.float64), dc=g.array("dc").astype(np.float64),
+               zc=g.array("zc").astype(np.float64))
+    for c, k in enumerate(("ux", "uy", "uz", "rho")):
+        v = np.zeros((nx, ny, nz), np.float64)
+        v[fluid] = a[c]
+        out[k] = v
+    return out
+
+
+def save_cli_npz(path: str, g, acc: np.ndarray, u: float, pleat: float, hist) -> None:
+    """airbox_result*.npz of the reference's CLI (airbox_cfd.py:287-288)."""
+    np.savez_compressed(path, **cli_arrays(g, acc), u=u, pleat=pleat, hist=np.array(hist, dtype=np.float64))
+
+
+def cli_report(metrics: dict, K: float, dx: float, steps: int, hist) -> dict:
+    """The JSON result object of the reference's CLI (airbox_cfd.py:360-366),
+    from evaluate's metrics (engine.js:682-685, same definitions)."""
+    zeta = float(metrics["zeta"])
+    return {"gamma": round(float(metrics["gamma"]), 4), "gamma_ohne_randzellen": round(float(metrics["gammaCore"]), 4),
+            "anteil_unter_50pct_kern": round(float(metrics["lowCore"]), 4),
+            "anteil_ueber_150pct_kern": round(float(metrics["highCore"]), 4),
+            "anteil_rueckstroemung": round(float(metrics["reverse"]), 4),
+            "max_zu_mittel": round(float(metrics["maxToMean"]), 3),
+            "zeta_rohluftseite": round(zeta, 3),
+            "dp_rohluft_mbar_bei_30ms": round(zeta * 0.5 * 1.2 * 30 ** 2 / 100, 2),
+            "K": K, "dx": dx, "steps": steps, "verlauf": [list(map(float, r)) for r in hist]}

@@ -71,3 +71,30 @@ def test_node_f32_run_and_prepare_errors(tmp_path):
     for name in ("err_grid", "err_noopen", "err_lid", "err_outlet"):
         fx, out = _run(tmp_path, name, run=False)
         assert out["error"] == fx["errors"]["prepare"], name
+
+
+def test_worker_protocol_run_matches_direct_call(tmp_path):
+    """The HIP worker (airboxcalc_amd/js/worker_hip.js) runs a case through the
+    reference's message protocol; its result message equals runHIP called
+    directly, bit for bit, and carries the reference's device/status texts."""
+    fx = GU.load("small_yaw")
+    d = fx["desc"]
+    stl = str(tmp_path / "box.stl")
+    G.write_stl_binary(stl, G.airbox_triangles(G.spec_from_dict(d["spec"])))
+    cj = str(tmp_path / "case.json")
+    with open(cj, "w") as fh:
+        json.dump({"stl": stl, "prepare": d["prepare"], "run": d["run"]}, fh)
+    print("[gpu] node worker case small_yaw", flush=True)
+    p = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "run_worker_case.js"), cj], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = json.loads(p.stdout)
+    assert out["gpuinfo"]["ok"] and out["gpuinfo"]["name"]
+    assert out["loaded"]["transferred"] and out["prepared"]["cells"] == fx["prepare"]["nf"]
+    assert not out["errors"], out["errors"]
+    r, dr = out["result"], out["direct"]
+    assert r["device"].startswith("GPU · ")
+    assert out["status"] == ["Grafikkarte wird geprüft …", "Rechnet auf der Grafikkarte (" + out["gpuinfo"]["name"] + ")"]
+    assert out["progress"] == len(r["hist"])
+    assert r["metrics"] == dr["metrics"] and r["hist"] == dr["hist"]
+    assert r["sha"] == dr["sha"]

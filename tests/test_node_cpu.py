@@ -41,3 +41,31 @@ def test_make_frame_round_trip():
     assert p.returncode == 0, p.stderr
     import json
     assert max(json.loads(p.stdout)) < 1e-12
+
+
+def test_worker_protocol_without_gpu(tmp_path):
+    """The HIP worker speaks the reference's protocol (engine.js:760-801): it
+    answers gpuinfo (no device), loads an STL (transferring the buffer), and
+    answers prepare with an error message rather than a CPU fallback."""
+    from airboxcalc_amd import abx
+    from airboxcalc_amd import geometry as G
+    try:
+        abx.Device(0).close()
+        pytest.skip("a GPU is present")
+    except abx.AbxError:
+        pass
+    import json
+    spec, _ = G.seeded_variant(3, "roofline")
+    tris = G.airbox_triangles(spec)
+    stl = str(tmp_path / "box.stl")
+    G.write_stl_binary(stl, tris)
+    cj = tmp_path / "case.json"
+    cj.write_text(json.dumps({"stl": stl, "prepare": G.prepare_params(spec, 4.0), "run": None}))
+    p = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "run_worker_case.js"), str(cj)], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    out = json.loads(p.stdout)
+    assert out["gpuinfo"] == {"ok": False, "name": ""}
+    assert out["loaded"]["type"] == "loaded" and out["loaded"]["ntri"] == len(tris)
+    assert out["loaded"]["transferred"]
+    assert out["prepared"] is None and out["errors"][0]["during"] == "prepare"

@@ -134,3 +134,35 @@ def test_restricted_targets_of_a_same_lattice_solve_are_the_targets():
     y1, m1 = S.restrict_targets(acc, gv, gv)
     assert np.array_equal(m0, m1)
     np.testing.assert_allclose(y1, y0, rtol=1e-5, atol=1e-5)
+
+
+def test_cli_npz_layout_matches_the_reference_cli():
+    """The keys, shapes and types airbox_cfd.py's --from-npz reads
+    (airbox_cfd.py:160-166, 287-288): masks and fields (nx, ny, nz), axes 1-D,
+    hist rows (step, gamma, ratio)."""
+    import io
+    from airboxcalc_amd import engine as E
+    box, g = _box_geom(9)
+    grid = grid_of_oracle(g)
+    NF = int(g.fluid.sum())
+    view = types.SimpleNamespace(grid=lambda: grid, array=lambda k: getattr(g, k))
+    acc = np.random.default_rng(0).standard_normal(4 * NF).astype(np.float32)
+    buf = io.BytesIO()
+    np.savez_compressed(buf, **E.cli_arrays(view, acc), u=0.06, pleat=40.0, hist=np.array([[100, 0.9, 0.5]]))
+    buf.seek(0)
+    d = np.load(buf)
+    for k in ("fluid", "porous", "solid", "ux", "uy", "uz", "rho"):
+        assert d[k].shape == (g.nx, g.ny, g.nz), k
+    for k, n in (("xc", g.nx), ("dc", g.ny), ("zc", g.nz)):
+        assert d[k].shape == (n,)
+    assert d["fluid"].dtype == bool and d["fluid"].sum() == NF
+    np.testing.assert_array_equal(d["ux"][d["fluid"]], acc[:NF].astype(np.float64))
+    assert (d["rho"][~d["fluid"]] == 0).all()
+    assert d["fluid"][:, :, 0].shape == (g.nx, g.ny)       # the opening argument of report()
+    assert d["hist"].shape[1] == 3
+    rep = E.cli_report(dict(gamma=0.9, gammaCore=0.95, lowCore=0.01, highCore=0.02, reverse=0.0, maxToMean=1.5,
+                            zeta=1.234), 3.0, 4.0, 2000, [(100, 0.9, 0.5)])
+    assert rep["dp_rohluft_mbar_bei_30ms"] == round(1.234 * 0.5 * 1.2 * 900 / 100, 2)
+    assert set(rep) == {"gamma", "gamma_ohne_randzellen", "anteil_unter_50pct_kern", "anteil_ueber_150pct_kern",
+                        "anteil_rueckstroemung", "max_zu_mittel", "zeta_rohluftseite", "dp_rohluft_mbar_bei_30ms",
+                        "K", "dx", "steps", "verlauf"}
