@@ -146,6 +146,7 @@ def load() -> C.CDLL:
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32, i32]),
+        "abx_sim_set_pipeline": (i32, [vp, i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
         "abx_sim_reset": (i32, [vp]),
@@ -494,10 +495,16 @@ class Sim:
 
     def set_launch(self, kernel: int = 4, xcd_map: int = 8, chunk: int = 256, nt: int = 0) -> None:
         """kernel 4 = AA pattern (default); 0 wide, 1 narrow, 2 segment, 3
-        persistent segment (two lattices); 5 AA with per-cell geometry words.
+        persistent segment (two lattices); 5 AA with per-cell geometry words;
+        6 AA pattern out of place (two lattices).
         xcd_map: block order across XCDs (0 round robin, 1 contiguous, G > 1
         groups of G blocks; the library's default is 8)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
+
+    def set_pipeline(self, planes: int) -> None:
+        """AA pairs issued plane-pipelined over bands of `planes` lattice
+        planes (kernel 4); 0 = one launch per step.  Same values bit for bit."""
+        _check(load().abx_sim_set_pipeline(self.h, planes))
 
     def set_graph(self, enable: bool = True) -> None:
         _check(load().abx_sim_set_graph(self.h, 1 if enable else 0))

@@ -137,6 +137,42 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   }
 }
 
+// Plane-pipelined AA pair: the two steps of an (A, B) pair of AA steps are
+// issued as a sequence of launches that each run step A over one band of
+// lattice planes and step B over an earlier band that trails it by more than
+// one plane, so step B re-reads lines that step A wrote shortly before (they
+// are still in the last-level cache) instead of a whole sweep later.
+// Segments [a0, a0 + nA) run mode MA, segments [b0, b0 + nB) mode 3 - MA; the
+// host picks the bands so that no position touched by one range is touched by
+// the other and every B position's A-step inputs were written by earlier
+// launches.  Block order across XCDs as k_stream_collide_seg's xcd_map G > 1.
+template <typename S, typename M, bool ACC, bool NS, int MA>
+__global__ __launch_bounds__(256) void k_stream_collide_pipe(StepArgs a, int a0, int nA, int b0, int nB) {
+  __shared__ uint32_t win[4][9][4];
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int b = blockIdx.y;
+  const int nbA = (nA + 3) >> 2, nbB = (nB + 3) >> 2;
+  int blk = (int)blockIdx.x;
+  if (a.xcd_map > 1) {
+    const int sq = blk >> 3, G = a.xcd_map;
+    blk = (sq / G) * 8 * G + (blk & 7) * G + sq % G;
+  }
+  if (blk >= nbA + nbB) return;  // whole workgroup leaves together
+  const bool first = blk < nbA;
+  const int base = first ? a0 : b0, n = first ? nA : nB, bb = first ? blk : blk - nbA;
+  if (t < 4 * 36) {
+    const int ss = t / 36, sg = 4 * bb + ss;
+    (&win[ss][0][0])[t - 36 * ss] = sg < n ? seg_window_word(a, a.chunks[base + sg], t - 36 * ss) : 0u;
+  }
+  __syncthreads();
+  const int sg = 4 * bb + wv;
+  if (sg >= n) return;
+  if (first)
+    seg_cell<S, M, ACC, NS, MA>(a, win[wv], a.chunks[base + sg], l, b);
+  else
+    seg_cell<S, M, ACC, NS, 3 - MA>(a, win[wv], a.chunks[base + sg], l, b);
+}
+
 // Per-cell geometry word of the metadata variant: bit q (q < 19) = the cell
 // x - c_q is fluid, bits 19/20/21 = inlet/outlet/porous, bit 31 = fluid.
 constexpr uint32_t META_INLET = 1u << 19, META_OUTLET = 1u << 20, META_POROUS = 1u << 21, META_FLUID = 1u << 31;
@@ -397,7 +433,12 @@ struct abx_sim {
   int32_t* chunks64 = nullptr;    // active 64-position chunks
   uint32_t* meta = nullptr;       // per-position geometry words (kernel 5)
   int nchunks64 = 0;
+  std::vector<int32_t> seg_host;  // host copy of chunks64 (ascending positions)
+  // AA pairs issued plane-pipelined over bands of `pipe` lattice planes
+  // (kernel 4 only; 0 = one launch per step)
+  int pipe = 0;
   // kernel 0 wide, 1 narrow, 2 segment, 3 segment persistent (two lattices),
+  // 6 segment AA pattern out of place (reads f[cur], writes f[cur ^ 1]),
   // 4 segment AA pattern (one lattice), 5 AA pattern with per-cell geometry words
   int kernel = 4, chunk = 256, nt = 0;
   // stored form of the populations: 0 plain (f[cur][q](x) = f_q), 1 AA
@@ -439,7 +480,7 @@ int ensure_meta(abx_sim* s);
 StepArgs step_args(abx_sim* s, int doacc) {
   StepArgs a;
   a.fin = s->f[s->cur];
-  a.fout = s->kernel >= 4 ? s->f[s->cur] : s->f[s->cur ^ 1];
+  a.fout = (s->kernel == 4 || s->kernel == 5) ? s->f[s->cur] : s->f[s->cur ^ 1];
   a.acc = doacc ? s->acc : nullptr;
   a.ns = s->ns;
   a.fluid_bits = s->fluid_bits;
@@ -546,7 +587,7 @@ void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
       launch_meta<S, M, 1>(s, a, doacc);
     else
       launch_meta<S, M, 2>(s, a, doacc);
-  } else if (s->kernel == 4) {
+  } else if (s->kernel == 4 || s->kernel == 6) {
     // AA pattern: an odd step follows the swapped form, an even one the streamed form
     if (s->form == 2)
       launch_seg<S, M, false, 1>(s, a, doacc);
@@ -581,7 +622,7 @@ void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
 
 int launch_step(abx_sim* s, int doacc) {
   if ((s->kernel != 1 || s->chunk == 64 ? s->nchunks64 : s->nchunks) == 0) return 0;
-  if (s->kernel < 4 && ensure_second_lattice(s)) return -1;
+  if ((s->kernel < 4 || s->kernel == 6) && ensure_second_lattice(s)) return -1;
   if (s->kernel == 5 && ensure_meta(s)) return -1;
   StepArgs a = step_args(s, doacc);
   switch (s->prec) {
@@ -595,10 +636,96 @@ int launch_step(abx_sim* s, int doacc) {
       launch_step_t<double, double>(s, a, doacc);
   }
   ABX_HIP(hipGetLastError());
-  if (s->kernel >= 4)
-    s->form = s->form == 2 ? 1 : 2;
+  if (s->kernel >= 4) s->form = s->form == 2 ? 1 : 2;
+  if (s->kernel < 4 || s->kernel == 6) s->cur ^= 1;
+  return 0;
+}
+
+template <typename S, typename M, int MA>
+void launch_pipe_t(abx_sim* s, const StepArgs& a, int doacc, int a0, int nA, int b0, int nB) {
+  const int nb = (nA + 3) / 4 + (nB + 3) / 4;
+  if (nb == 0) return;
+  unsigned gx = (unsigned)nb;
+  if (a.xcd_map > 1) gx = (unsigned)((nb + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
+  dim3 grid(gx, (unsigned)s->B), block(256);
+  const bool ns = a.ns != nullptr;
+  if (doacc) {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, true, true, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
+    else
+      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, true, false, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
+  } else {
+    if (ns)
+      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, false, true, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
+    else
+      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, false, false, MA>), grid, block, 0, s->stream, a, a0, nA, b0,
+                         nB);
+  }
+}
+
+template <typename S, typename M>
+void launch_pipe_m(abx_sim* s, const StepArgs& a, int doacc, int ma, int a0, int nA, int b0, int nB) {
+  if (ma == 1)
+    launch_pipe_t<S, M, 1>(s, a, doacc, a0, nA, b0, nB);
   else
-    s->cur ^= 1;
+    launch_pipe_t<S, M, 2>(s, a, doacc, a0, nA, b0, nB);
+}
+
+// One pair of AA steps issued plane-pipelined (k_stream_collide_pipe).  Band c
+// of step A holds the segments starting in [c W, (c + 1) W), W = pipe lattice
+// planes; launch c runs it together with the segments of step B that start in
+// [c' W - GAP, c W - GAP) (c' = c - 1).  GAP exceeds the largest neighbour
+// offset |off_q| = nyp nzp + nzp + 1 by more than a segment, so (i) no
+// position touched by step B in launch c is touched by step A in launch c,
+// and (ii) every input of step B comes from step A launches issued before.
+// The stored form is the same before and after the pair.
+int launch_pair_pipelined(abx_sim* s, int doacc) {
+  const std::vector<int32_t>& sg = s->seg_host;
+  const int n = (int)sg.size();
+  if (n == 0) return 0;
+  const StepArgs a = step_args(s, doacc);
+  const int64_t ipl = (int64_t)s->nyp * s->nzp;
+  const int64_t W = (int64_t)s->pipe * ipl;
+  const int64_t GAP = ipl + s->nzp + 128;
+  const int ma = s->form == 2 ? 1 : 2;  // streamed form: even step first
+  auto lb = [&](int64_t x) -> int {
+    if (x <= 0) return 0;
+    return (int)(std::lower_bound(sg.begin(), sg.end(), x, [](int32_t v, int64_t y) { return (int64_t)v < y; }) -
+                 sg.begin());
+  };
+  const int nc = (int)((int64_t)sg.back() / W) + 1;
+  int b0 = 0;
+  for (int c = 0; c <= nc; ++c) {
+    int a0 = 0, nA = 0;
+    if (c < nc) {
+      a0 = lb((int64_t)c * W);
+      nA = lb((int64_t)(c + 1) * W) - a0;
+    }
+    const int b1 = c == nc ? n : (c == 0 ? 0 : std::max(b0, lb((int64_t)c * W - GAP)));
+    const int nB = b1 - b0;
+    switch (s->prec) {
+      case ABX_F32:
+        launch_pipe_m<float, float>(s, a, doacc, ma, a0, nA, b0, nB);
+        break;
+      case ABX_F32_F64:
+        launch_pipe_m<float, double>(s, a, doacc, ma, a0, nA, b0, nB);
+        break;
+      default:
+        launch_pipe_m<double, double>(s, a, doacc, ma, a0, nA, b0, nB);
+    }
+    b0 = b1;
+  }
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+// n steps: AA pairs plane-pipelined when enabled, the rest one launch each.
+int launch_steps(abx_sim* s, int n, int doacc) {
+  if (s->kernel == 4 && s->pipe > 0 && s->nchunks64 > 0)
+    for (; n >= 2; n -= 2)
+      if (launch_pair_pipelined(s, doacc)) return -1;
+  for (int i = 0; i < n; ++i)
+    if (launch_step(s, doacc)) return -1;
   return 0;
 }
 
@@ -815,6 +942,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     if (fbits[c0 >> 5] | fbits[(c0 >> 5) + 1]) chunks64.push_back((int32_t)c0);
   s->nchunks = (int)chunks.size();
   s->nchunks64 = (int)chunks64.size();
+  s->seg_host = chunks64;
   std::vector<int32_t> plist((size_t)(s->nM + s->nT));
   for (int n = 0; n < s->nM; ++n) plist[(size_t)n] = cells[(size_t)lM[(size_t)n]];
   for (int n = 0; n < s->nT; ++n) plist[(size_t)(s->nM + n)] = cells[(size_t)lT[(size_t)n]];
@@ -904,23 +1032,19 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   doacc = doacc ? 1 : 0;
-  if (!s->use_graph || n < 2) {
-    for (int i = 0; i < n; ++i)
-      if (launch_step(s, doacc)) return -1;
-    return 0;
-  }
+  if (!s->use_graph || n < 2) return launch_steps(s, n, doacc);
   // a chunk of n steps depends only on (n, doacc, stored form, current
   // lattice, kernel): capture it once, replay it afterwards
-  if (s->kernel < 4 && ensure_second_lattice(s)) return -1;
+  if ((s->kernel < 4 || s->kernel == 6) && ensure_second_lattice(s)) return -1;
   if (s->kernel == 5 && ensure_meta(s)) return -1;
   const int form0 = s->form, cur0 = s->cur;
-  const auto key = std::make_tuple(n, doacc, form0, cur0, s->kernel * 1000 + s->chunk + 10 * s->nt + 7 * s->xcd_map);
+  const auto key = std::make_tuple(n, doacc, form0, cur0,
+                                   s->kernel * 1000 + s->chunk + 10 * s->nt + 7 * s->xcd_map + 100000 * s->pipe);
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
     hipGraph_t g;
     ABX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = 0;
-    for (int i = 0; i < n && !rc; ++i) rc = launch_step(s, doacc);
+    const int rc = launch_steps(s, n, doacc);
     const hipError_t e = hipStreamEndCapture(s->stream, &g);
     s->form = form0;
     s->cur = cur0;
@@ -938,10 +1062,9 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
     it = s->graphs.emplace(key, ex).first;
   }
   ABX_HIP(hipGraphLaunch(it->second, s->stream));
-  if (s->kernel >= 4) {
-    if (n % 2) s->form = form0 == 2 ? 1 : 2;
-  } else {
-    s->cur = (n % 2) ? cur0 ^ 1 : cur0;
+  if (n % 2) {
+    if (s->kernel >= 4) s->form = form0 == 2 ? 1 : 2;
+    if (s->kernel < 4 || s->kernel == 6) s->cur = cur0 ^ 1;
   }
   return 0;
 }
@@ -962,8 +1085,7 @@ extern "C" int abx_sim_time_steps(abx_sim* s, int n, int doacc, double* ms_total
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   ABX_HIP(hipEventRecord(s->ev0, s->stream));
-  for (int i = 0; i < n; ++i)
-    if (launch_step(s, doacc ? 1 : 0)) return -1;
+  if (launch_steps(s, n, doacc ? 1 : 0)) return -1;
   ABX_HIP(hipEventRecord(s->ev1, s->stream));
   ABX_HIP(hipEventSynchronize(s->ev1));
   float ms = 0;
@@ -1232,14 +1354,14 @@ extern "C" int abx_sim_sync(abx_sim* s) {
 }
 
 extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk, int nt) {
-  if (!s || kernel < 0 || kernel > 5 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
-    set_error("abx_sim_set_launch: bad argument (kernel 0..5, chunk 64 or 256, nt 0..3)");
+  if (!s || kernel < 0 || kernel > 6 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
+    set_error("abx_sim_set_launch: bad argument (kernel 0..6, chunk 64 or 256, nt 0..3)");
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   sim_drop_graphs(s);
   const bool was_aa = s->kernel >= 4, to_aa = kernel >= 4;
-  if (!to_aa && ensure_second_lattice(s)) return -1;
+  if ((!to_aa || kernel == 6) && ensure_second_lattice(s)) return -1;
   if (kernel == 5 && ensure_meta(s)) return -1;
   if (was_aa != to_aa) {
     // carry the state over: read it in the old form, store it in the new one
@@ -1257,6 +1379,16 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
   s->xcd_map = std::max(0, std::min(xcd_map, 4096));
   s->chunk = chunk;
   s->nt = nt;
+  return 0;
+}
+
+extern "C" int abx_sim_set_pipeline(abx_sim* s, int planes) {
+  if (!s || planes < 0 || planes > 4096) {
+    set_error("abx_sim_set_pipeline: bad argument (planes 0..4096)");
+    return -1;
+  }
+  sim_drop_graphs(s);
+  s->pipe = planes;
   return 0;
 }
 

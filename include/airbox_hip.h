@@ -227,13 +227,23 @@ int abx_sim_sync(abx_sim* sim);
  *         1 = narrow (one position per lane, chunk = 64 or 256 positions per
  *             workgroup, nt bit 0 / bit 1 = non-temporal loads / stores),
  *         2 = segment (one position per lane, four active 64-position
- *             segments per workgroup; the default),
- *         3 = segment, persistent grid.
+ *             segments per workgroup), two lattices,
+ *         3 = segment, persistent grid, two lattices,
+ *         4 = segment, AA pattern in one lattice (the default),
+ *         5 = AA pattern with a per-position geometry word,
+ *         6 = segment, AA pattern out of place (two lattices).
  * xcd_map 1 = each XCD sweeps one contiguous part of the lattice; xcd_map
  * G > 1 (segment kernels) = groups of G consecutive blocks per XCD, all XCDs
  * moving through the lattice together.  Every variant computes the same
  * values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
+/* Plane-pipelined AA pairs (kernel 4; no reference counterpart): planes > 0
+ * issues each pair of steps as launches over bands of `planes` lattice
+ * planes, the second step trailing the first by a little more than one
+ * plane, so that it re-reads the first step's output from the last-level
+ * cache.  0 (the default) = one launch per step.  The values are the same
+ * bit for bit. */
+int abx_sim_set_pipeline(abx_sim* sim, int planes);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
 int abx_sim_set_graph(abx_sim* sim, int enable);
