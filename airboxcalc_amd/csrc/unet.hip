@@ -739,3 +739,147 @@ extern "C" int abx_adam(void* stream, float* w, const float* g, float* m, float*
   ABX_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Input volume of the surrogate, built on the GPU from a box's grid arrays.
+//
+// A lattice of nx x ny x nz cells (the reference's order, k fastest,
+// engine.js:128-134) sits at the origin of a d x h x w volume; voxels beyond
+// it are solid.  Cell types follow setupSim's precedence (engine.js:250-255):
+// inlet = k == 0 and opening, else outlet, else porous when the row's depth
+// lies in (dc[top], pleat].  Channels (bf16, the rest zero):
+//   0 solid, 1 porous, 2 inlet, 3 K/6 on porous cells,
+//   4 signed Euclidean distance to the wall in cells / 8, clipped to [-4, 4]
+//     (positive in the fluid: distance to the nearest solid voxel; negative in
+//     the solid: minus the distance to the nearest fluid voxel),
+//   5 outlet, 6 aniso/20 on porous cells.
+// Each value is formed in double, rounded to float and then to bf16 (nearest
+// even), the rounding of the host path that builds the same volume in float
+// and converts it.
+//
+// The distances are exact: squared distances are separable,
+//   D(i,j,k) = min_i' [(i-i')^2 + min_j' [(j-j')^2 + min_k' [(k-k')^2 + F(i',j',k')]]],
+// F = 0 on the target set and "infinite" elsewhere, so three passes, one per
+// axis, each take the minimum over the whole line in exact integer
+// arithmetic (at most 96 terms per line at the surrogate's volume).
+namespace abx {
+namespace {
+
+constexpr int32_t EDT_INF = 1 << 30;  // + (line length)^2 stays below 2^31
+
+struct FeatArgs {
+  const uint8_t* fluid;    // nx*ny*nz
+  const uint8_t* opening;  // nx*ny
+  const uint8_t* outlet;   // nx*ny*nz
+  const float* dc;         // ny
+  int nx, ny, nz, top;
+  double pleat, K, aniso;
+  int d, h, w;
+};
+
+__device__ __forceinline__ bool feat_fluid(const FeatArgs& a, int i, int j, int k) {
+  return i < a.nx && j < a.ny && k < a.nz && a.fluid[((int64_t)i * a.ny + j) * a.nz + k] != 0;
+}
+
+// Pass along k: for every voxel the squared distance along its k line to the
+// nearest non-fluid voxel (in) and to the nearest fluid voxel (out).
+__global__ void k_feat_edt_k(FeatArgs a, int32_t* gin, int32_t* gout) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nv = (int64_t)a.d * a.h * a.w;
+  if (v >= nv) return;
+  const int k = (int)(v % a.w), j = (int)((v / a.w) % a.h), i = (int)(v / ((int64_t)a.w * a.h));
+  int32_t bi = EDT_INF, bo = EDT_INF;
+  for (int kk = 0; kk < a.w; ++kk) {
+    const int32_t dd = (k - kk) * (k - kk);
+    if (feat_fluid(a, i, j, kk))
+      bo = min(bo, dd);
+    else
+      bi = min(bi, dd);
+  }
+  gin[v] = bi;
+  gout[v] = bo;
+}
+
+// Pass along j: src holds the k-pass minima.
+__global__ void k_feat_edt_j(FeatArgs a, const int32_t* sin, const int32_t* sout, int32_t* din, int32_t* dout) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nv = (int64_t)a.d * a.h * a.w;
+  if (v >= nv) return;
+  const int k = (int)(v % a.w), j = (int)((v / a.w) % a.h), i = (int)(v / ((int64_t)a.w * a.h));
+  int32_t bi = EDT_INF, bo = EDT_INF;
+  for (int jj = 0; jj < a.h; ++jj) {
+    const int64_t u = ((int64_t)i * a.h + jj) * a.w + k;
+    const int32_t dd = (j - jj) * (j - jj);
+    bi = min(bi, sin[u] + dd);
+    bo = min(bo, sout[u] + dd);
+  }
+  din[v] = min(bi, EDT_INF);
+  dout[v] = min(bo, EDT_INF);
+}
+
+// Pass along i, then every channel of the voxel.
+__global__ void k_feat_write(FeatArgs a, const int32_t* sin, const int32_t* sout, u16* x, int xs, int c) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nv = (int64_t)a.d * a.h * a.w;
+  if (v >= nv) return;
+  const int k = (int)(v % a.w), j = (int)((v / a.w) % a.h), i = (int)(v / ((int64_t)a.w * a.h));
+  int32_t bi = EDT_INF, bo = EDT_INF;
+  for (int ii = 0; ii < a.d; ++ii) {
+    const int64_t u = ((int64_t)ii * a.h + j) * a.w + k;
+    const int32_t dd = (i - ii) * (i - ii);
+    bi = min(bi, sin[u] + dd);
+    bo = min(bo, sout[u] + dd);
+  }
+  const bool fl = feat_fluid(a, i, j, k);
+  // a voxel is at distance 0 of its own set: fluid has bo == 0, solid bi == 0
+  const double din = fl ? sqrt((double)bi) : 0.0, dout = fl ? 0.0 : sqrt((double)bo);
+  bool inlet = false, outlet = false, porous = false;
+  if (fl) {
+    const int64_t c3 = ((int64_t)i * a.ny + j) * a.nz + k;
+    inlet = k == 0 && a.opening[(int64_t)i * a.ny + j] != 0;
+    outlet = !inlet && a.outlet[c3] != 0;
+    porous = !inlet && !outlet && a.dc[j] > a.dc[a.top] && (double)a.dc[j] <= a.pleat;
+  }
+  double sd = (din - dout) / 8.0;
+  sd = sd < -4.0 ? -4.0 : (sd > 4.0 ? 4.0 : sd);
+  float ch[7];
+  ch[0] = fl ? 0.f : 1.f;
+  ch[1] = porous ? 1.f : 0.f;
+  ch[2] = inlet ? 1.f : 0.f;
+  ch[3] = porous ? (float)(a.K / 6.0) : 0.f;
+  ch[4] = (float)sd;
+  ch[5] = outlet ? 1.f : 0.f;
+  ch[6] = porous ? (float)(a.aniso / 20.0) : 0.f;
+  u16* o = x + v * xs;
+  for (int q = 0; q < c; ++q) o[q] = q < 7 ? f2bf(ch[q]) : (u16)0;
+}
+
+}  // namespace
+}  // namespace abx
+
+extern "C" int64_t abx_surrogate_features_workspace(int d, int h, int w) {
+  if (d < 1 || h < 1 || w < 1) return -1;
+  return (int64_t)4 * d * h * w * (int64_t)sizeof(int32_t);
+}
+
+extern "C" int abx_surrogate_features(void* stream, const uint8_t* fluid, const uint8_t* opening,
+                                      const uint8_t* outlet, const float* dc, int nx, int ny, int nz, int top,
+                                      double pleat, double K, double aniso, int d, int h, int w, int32_t* workspace,
+                                      int64_t ws_bytes, uint16_t* x, int xs, int c) {
+  if (!fluid || !opening || !outlet || !dc || !workspace || !x || nx < 1 || ny < 1 || nz < 1 || top < 0 ||
+      top >= ny || d < nx || h < ny || w < nz || d > 1024 || h > 1024 || w > 1024 || c < 7 || xs < c ||
+      ws_bytes < abx_surrogate_features_workspace(d, h, w)) {
+    set_error("abx_surrogate_features: bad argument (the lattice must fit the volume, 7 <= c <= xs, "
+              "workspace of abx_surrogate_features_workspace bytes)");
+    return -1;
+  }
+  FeatArgs a{fluid, opening, outlet, dc, nx, ny, nz, top, pleat, K, aniso, d, h, w};
+  const int64_t nv = (int64_t)d * h * w;
+  int32_t *g0 = workspace, *g1 = workspace + nv, *g2 = workspace + 2 * nv, *g3 = workspace + 3 * nv;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_feat_edt_k, dim3(blocks_for(nv)), dim3(256), 0, s, a, g0, g1);
+  hipLaunchKernelGGL(k_feat_edt_j, dim3(blocks_for(nv)), dim3(256), 0, s, a, g0, g1, g2, g3);
+  hipLaunchKernelGGL(k_feat_write, dim3(blocks_for(nv)), dim3(256), 0, s, a, g2, g3, x, xs, c);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}

@@ -108,6 +108,29 @@ def features(grid: abx.GridArrays, K: float, aniso: float, shape=GRID) -> np.nda
     return x
 
 
+def features_device(out, grid: abx.GridArrays, K: float, aniso: float, shape=GRID) -> None:
+    """features() computed on the GPU (abx_surrogate_features) into `out`, a
+    contiguous bf16 device tensor of shape + (IN_CH,), on torch's current
+    stream: the same values as features() converted to bf16."""
+    import torch
+    D, H, W = shape
+    if tuple(out.shape) != (D, H, W, U.IN_CH) or out.dtype != torch.bfloat16 or not out.is_contiguous():
+        raise ValueError("features_device: out must be a contiguous bf16 tensor of shape GRID + (IN_CH,)")
+    if grid.nx > D or grid.ny > H or grid.nz > W:
+        raise abx.AbxError(f"lattice {grid.nx}x{grid.ny}x{grid.nz} does not fit the surrogate volume {shape}")
+    dev = out.device
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    fl, op, ou, dc = up(grid.fluid), up(grid.opening), up(grid.outlet), up(grid.dc)
+    lib = abx.load()
+    nb = int(lib.abx_surrogate_features_workspace(D, H, W))
+    ws = torch.empty((nb + 3) // 4, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    abx._check(lib.abx_surrogate_features(stream, fl.data_ptr(), op.data_ptr(), ou.data_ptr(), dc.data_ptr(),
+                                          grid.nx, grid.ny, grid.nz, grid.top, float(grid.pleat), float(K),
+                                          float(aniso), D, H, W, ws.data_ptr(), nb, out.data_ptr(), U.IN_CH,
+                                          U.IN_CH))
+
+
 def targets(grid: abx.GridArrays, acc: np.ndarray, shape=GRID):
     """(target volume shape + (4,) float32, fluid mask shape uint8) from the
     averaged fields acc (4*NF, components ux, uy, uz, rho, cid order)."""
@@ -207,7 +230,7 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
                 fines.append(gs)
             sims.append(abx.Sim(dev, gs.grid(), K=b.K, aniso=b.aniso, u=U0, tau=TAU, precision="f32"))
         for k, (b, gr) in enumerate(zip(part, grids)):
-            X[c0 + k] = torch.from_numpy(features(gr, b.K, b.aniso)).to(torch.bfloat16)
+            features_device(X[c0 + k], gr, b.K, b.aniso)
         t1 = time.perf_counter()
         bt = abx.Batch(dev, sims)
         tuned = bt.tune(10)
@@ -332,7 +355,8 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
     gc = abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
     try:
         grid = gc.grid()
-        x = torch.from_numpy(features(grid, box.K, box.aniso)[None]).to(net.dev).to(torch.bfloat16)
+        x = torch.empty((1,) + GRID + (U.IN_CH,), dtype=torch.bfloat16, device=net.dev)
+        features_device(x[0], grid, box.K, box.aniso)
         out = net.forward(x)
         fl = torch.from_numpy(grid.fluid).to(net.dev)
         mean, std = np.asarray(norm["mean"], np.float64), np.asarray(norm["std"], np.float64)
@@ -341,19 +365,6 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
         torch.cuda.synchronize(net.dev)
         sim.init_eq_resample(out.data_ptr(), U.HEAD_CH, GRID, (gc.nx, gc.ny, gc.nz), fl.data_ptr(), gc.origin, gc.h,
                              g_fine.origin, g_fine.h, affine)
-    finally:
-        gc.free()
-
-
-def warm_start_fields(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: abx.Geometry,
-                      tris: Optional[np.ndarray] = None) -> np.ndarray:
-    """(rho, ux, uy, uz) in cid order for g_fine's lattice from the surrogate:
-    the box is prepared at FAMILY_DX, predicted there and resampled."""
-    tris = G.airbox_triangles(box.spec) if tris is None else tris
-    gc = abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
-    try:
-        pred = predict(net, features(gc.grid(), box.K, box.aniso), norm)
-        return resample(pred, gc, g_fine)
     finally:
         gc.free()
 

@@ -380,6 +380,21 @@ int64_t abx_mse_partials(int64_t nvox);
 int abx_mse_loss(void* stream, const uint16_t* pred, int ps, const uint16_t* target, int ts, const uint8_t* mask,
                  int64_t nvox, int c, const float* chan_w, float scale, float* loss_parts, uint16_t* dpred, int dps);
 
+/* Input volume of the surrogate for one box (no reference counterpart):
+ * the lattice nx x ny x nz of a prepared box (device copies of abx_grid's
+ * fluid, opening, outlet and dc; top and pleat) placed at the origin of a
+ * d x h x w volume (nx <= d, ny <= h, nz <= w, each <= 1024), voxels beyond
+ * it solid.  Writes c >= 7 bf16 channels per voxel (stride xs): solid,
+ * porous, inlet, K/6 on porous cells, signed Euclidean wall distance in cells
+ * / 8 clipped to [-4, 4] (exact, positive in the fluid), outlet, aniso/20 on
+ * porous cells, then zeros.  Cell types follow setupSim's precedence
+ * (engine.js:250-255).  workspace: abx_surrogate_features_workspace(d, h, w)
+ * bytes of device memory. */
+int64_t abx_surrogate_features_workspace(int d, int h, int w);
+int abx_surrogate_features(void* stream, const uint8_t* fluid, const uint8_t* opening, const uint8_t* outlet,
+                           const float* dc, int nx, int ny, int nz, int top, double pleat, double K, double aniso,
+                           int d, int h, int w, int32_t* workspace, int64_t ws_bytes, uint16_t* x, int xs, int c);
+
 /* One Adam step (bias-corrected, step >= 1) on n FP32 weights. */
 int abx_adam(void* stream, float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
              float eps, int step);

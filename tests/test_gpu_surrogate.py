@@ -21,6 +21,49 @@ def _net():
     return net, norm
 
 
+@pytest.mark.parametrize("seed", [7, 1000002, 1000005])
+def test_device_features_equal_host_features(seed):
+    """abx_surrogate_features gives the host feature volume (scipy's exact
+    distance transform) converted to bf16, bit for bit."""
+    say("surrogate features on the GPU, seed", seed)
+    dev = abx.device(0)
+    box = S.family([seed])[0]
+    g = abx.Geometry(dev, G.airbox_triangles(box.spec), G.prepare_params(box.spec, S.FAMILY_DX), max_cells=0)
+    try:
+        grid = g.grid()
+        want = torch.from_numpy(S.features(grid, box.K, box.aniso)).to(torch.bfloat16)
+        out = torch.full(S.GRID + (16,), 7.0, dtype=torch.bfloat16, device="cuda")
+        S.features_device(out, grid, box.K, box.aniso)
+        assert torch.equal(out.cpu().view(torch.int16), want.view(torch.int16))
+    finally:
+        g.free()
+
+
+def test_device_features_distance_against_brute_force():
+    """The signed wall distance of a random lattice (smaller than its volume)
+    against the minimum over all voxel pairs."""
+    say("surrogate features: brute-force distances")
+    rng = np.random.default_rng(5)
+    nx, ny, nz, shape = 9, 6, 11, (12, 8, 13)
+    fluid = rng.random((nx, ny, nz)) < 0.6
+    grid = abx.GridArrays(nx, ny, nz, fluid.ravel(), np.zeros(nx * ny, np.uint8), np.zeros(nx * ny * nz, np.uint8),
+                          np.arange(ny, dtype=np.float32), 0, 0, -1.0)
+    out = torch.zeros(shape + (16,), dtype=torch.bfloat16, device="cuda")
+    S.features_device(out, grid, 3.0, 10.0, shape=shape)
+    fl = np.zeros(shape, bool)
+    fl[:nx, :ny, :nz] = fluid
+    idx = np.indices(shape).reshape(3, -1).T
+    d2 = ((idx[:, None, :] - idx[None, :, :]) ** 2).sum(-1)
+    f = fl.ravel()
+    d_in = np.where(f, np.sqrt(np.where(f[None, :], np.iinfo(np.int64).max, d2).min(1).astype(np.float64)), 0.0)
+    d_out = np.where(~f, np.sqrt(np.where(~f[None, :], np.iinfo(np.int64).max, d2).min(1).astype(np.float64)), 0.0)
+    want = torch.from_numpy(np.clip((d_in - d_out) / 8.0, -4.0, 4.0).astype(np.float32)).to(torch.bfloat16)
+    got = out.cpu()
+    assert torch.equal(got[..., 4].reshape(-1).view(torch.int16), want.view(torch.int16))
+    assert torch.equal(got[..., 0].reshape(-1), torch.from_numpy((~f).astype(np.float32)).to(torch.bfloat16))
+    assert not got[..., 7:].any()
+
+
 def test_device_resample_matches_host_path():
     say("warm start: device resample vs host")
     dev = abx.device(0)
