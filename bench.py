@@ -46,26 +46,36 @@ def dense_positions(g) -> int:
     return (g.nx + 2) * (g.ny + 2) * (32 * ((g.nz + 2 + 31) // 32))
 
 
-def cpu_baseline(tris, p, run, seconds: float):
-    """The oracle's single-threaded C restatement of cpuStep on the same
-    geometry, timed on this host for a bounded number of steps."""
+def cpu_baseline(tris, p, run, seconds: float, threads: int):
+    """The oracle's C restatement of cpuStep on the same geometry, timed on
+    this host for a bounded number of steps: on `threads` OpenMP threads (the
+    reported value) and on one thread."""
     from oracle import oracle as O
     g = O.prepare(tris, p, max_cells=0)
     S = O.setup_sim(g, run["K"], run["aniso"], run["u"], run["tau"])
     f = S.f0()
     f2 = np.empty_like(f)
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        O.cpu_step(S, f, f2)
-        f, f2 = f2, f
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds or n >= 200:
-            break
-    return dict(value=S.NF * n / el / 1e6, unit="MLUPS", cores=1, kind="port",
-                sample=f"{n} cpuStep steps (f32 storage, f64 arithmetic) of the same {S.NF}-cell lattice, "
-                       f"{el:.1f} s, oracle/engine_oracle.c single thread")
+
+    def timed(nt, budget):
+        nonlocal f, f2
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            O.cpu_step(S, f, f2, threads=nt)
+            f, f2 = f2, f
+            n += 1
+            el = time.perf_counter() - t0
+            if el > budget or n >= 400:
+                return n, el
+
+    n1, el1 = timed(1, seconds / 3)
+    nt, elt = timed(threads, 2 * seconds / 3)
+    one = S.NF * n1 / el1 / 1e6
+    return dict(value=S.NF * nt / elt / 1e6, unit="MLUPS", cores=threads, kind="port",
+                one_core_mlups=round(one, 2),
+                sample=f"{nt} cpuStep steps (f32 storage, f64 arithmetic) of the same {S.NF}-cell lattice on "
+                       f"{threads} OpenMP threads in {elt:.1f} s, and {n1} steps on one thread in {el1:.1f} s "
+                       f"({one:.2f} MLUPS); oracle/engine_oracle.c")
 
 
 def bench_fp64(a, dev):
@@ -200,7 +210,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--precision", default="f32", choices=["f32", "f32_f64", "f64"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host threads of the CPU baseline (default: up to 16, the host share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
     ap.add_argument("--xcd-map", type=int, default=None, help="block order across XCDs (abx_sim_set_launch)")
@@ -317,7 +329,7 @@ def main():
     }
     if not a.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(tris, p, run, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(tris, p, run, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

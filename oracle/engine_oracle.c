@@ -796,12 +796,13 @@ void orc_sim_free(orc_sim* S) {
 }
 
 /* ---- cpuStep (engine.js:266-321) -------------------------------------- */
+/* Cells a0 <= a < a1 of one step; every cell reads f and writes only its own
+ * entries of f2 and acc, so disjoint ranges may run concurrently. */
 #define CPU_STEP_BODY(LOAD, STORE, ACC_ADD)                                                                  \
-  init_wt();                                                                                                 \
   const int64_t NF = S->NF;                                                                                  \
   const double u0 = S->u0, tau0 = S->tau0, sig = S->sig, aniso = S->aniso, cs2s = S->cs2s;                 \
-  double fi[19], fe[19];                                                                                     \
-  for (int64_t a = 0; a < NF; a++) {                                                                         \
+  for (int64_t a = a0; a < a1; a++) {                                                                        \
+    double fi[19], fe[19];                                                                                   \
     const int64_t base = a * 19;                                                                             \
     double rho = 0, mx = 0, my = 0, mz = 0;                                                                  \
     for (int i = 0; i < 19; i++) {                                                                           \
@@ -888,11 +889,32 @@ void orc_sim_free(orc_sim* S) {
 #define ACC32(arr, idx, val) ((arr)[idx] = (float)((double)(arr)[idx] + (val)))
 #define ACC64(arr, idx, val) ((arr)[idx] = (arr)[idx] + (val))
 
-void orc_cpu_step(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns) {
+static void step_range32(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns, int64_t a0,
+                        int64_t a1) {
   CPU_STEP_BODY(LD, ST32, ACC32)
 }
-void orc_cpu_step_f64(const orc_sim* S, const double* f, double* f2, double* acc, const float* ns) {
+static void step_range64(const orc_sim* S, const double* f, double* f2, double* acc, const float* ns, int64_t a0,
+                        int64_t a1) {
   CPU_STEP_BODY(LD, ST64, ACC64)
+}
+
+void orc_cpu_step(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns) {
+  init_wt();
+  step_range32(S, f, f2, acc, ns, 0, S->NF);
+}
+void orc_cpu_step_f64(const orc_sim* S, const double* f, double* f2, double* acc, const float* ns) {
+  init_wt();
+  step_range64(S, f, f2, acc, ns, 0, S->NF);
+}
+/* The same step split into nthreads contiguous cell ranges run on OpenMP
+ * threads (the host CPU baseline); results are identical to orc_cpu_step. */
+void orc_cpu_step_threads(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns, int nthreads) {
+  init_wt();
+  if (nthreads < 1) nthreads = 1;
+  const int64_t NF = S->NF;
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+  for (int t = 0; t < nthreads; t++)
+    step_range32(S, f, f2, acc, ns, NF * t / nthreads, NF * (t + 1) / nthreads);
 }
 
 /* ---- probeCPU + progressInfo (engine.js:324-339) ---------------------- */

@@ -89,6 +89,8 @@ void orc_sim_free(orc_sim* S);
 void orc_cpu_step(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns);
 /* the same step with f64 storage (FP64 reference for the double kernel) */
 void orc_cpu_step_f64(const orc_sim* S, const double* f, double* f2, double* acc, const float* ns);
+/* orc_cpu_step on nthreads OpenMP threads (contiguous cell ranges); same result. */
+void orc_cpu_step_threads(const orc_sim* S, const float* f, float* f2, float* acc, const float* ns, int nthreads);
 /* probeCPU + progressInfo reduction (engine.js:324-339): out = {gamma, massRatio, qo} */
 int orc_probe(const orc_sim* S, const float* f, int step, double out[3]);
 

@@ -94,6 +94,7 @@ def lib():
         L.orc_sim_free.argtypes = [C.POINTER(SimS)]
         L.orc_cpu_step.argtypes = [C.POINTER(SimS), _f32p, _f32p, C.c_void_p, C.c_void_p]
         L.orc_cpu_step_f64.argtypes = [C.POINTER(SimS), _f64p, _f64p, C.c_void_p, C.c_void_p]
+        L.orc_cpu_step_threads.argtypes = [C.POINTER(SimS), _f32p, _f32p, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_probe.argtypes = [C.POINTER(SimS), _f32p, C.c_int, _f64p]
         L.orc_simulate.argtypes = [C.POINTER(SimS), C.c_int, C.c_int, C.c_int, C.c_void_p, _f32p, _f64p,
                                    C.POINTER(C.c_int), C.c_int64]
@@ -284,9 +285,10 @@ def _ptr(a: Optional[np.ndarray]):
 
 
 def cpu_step(S: Sim, f: np.ndarray, f2: np.ndarray, acc: Optional[np.ndarray] = None,
-             ns: Optional[np.ndarray] = None) -> None:
+             ns: Optional[np.ndarray] = None, threads: int = 1) -> None:
     """cpuStep (engine.js:266-321): f32 storage, f64 arithmetic.  With f64
-    arrays the same step runs with f64 storage (the FP64 reference)."""
+    arrays the same step runs with f64 storage (the FP64 reference).  threads
+    > 1 (f32 only) splits the cells over OpenMP threads; same result."""
     if ns is not None:
         assert ns.dtype == np.float32 and ns.size == S.NF
     if f.dtype == np.float64:
@@ -294,7 +296,10 @@ def cpu_step(S: Sim, f: np.ndarray, f2: np.ndarray, acc: Optional[np.ndarray] = 
         lib().orc_cpu_step_f64(C.byref(S.s), f, f2, _ptr(acc), _ptr(ns))
     else:
         assert acc is None or acc.dtype == np.float32
-        lib().orc_cpu_step(C.byref(S.s), f, f2, _ptr(acc), _ptr(ns))
+        if threads > 1:
+            lib().orc_cpu_step_threads(C.byref(S.s), f, f2, _ptr(acc), _ptr(ns), int(threads))
+        else:
+            lib().orc_cpu_step(C.byref(S.s), f, f2, _ptr(acc), _ptr(ns))
 
 
 def run_steps(S: Sim, n: int, f: Optional[np.ndarray] = None, ns=None, acc=None, dtype=np.float32):

@@ -80,16 +80,17 @@ def test_setup_and_12_step_trajectories(name):
     ns_st = np.zeros(S.NF, np.float32)
     idx = np.arange(0, S.NF, 17)
     ns_st[idx[S.flag[idx] == 0]] = 1
-    for vn, ns in (("plain", None), ("ns", ns_st)):
+    # threads > 1: the multi-core CPU baseline's split of the same step
+    for vn, ns, threads in (("plain", None, 1), ("ns", ns_st, 1), ("plain", None, 5), ("ns", ns_st, 3)):
         f = S.f0()
         f2 = np.empty_like(f)
         acc = np.zeros(4 * S.NF, np.float32)
         for s in range(1, 13):
-            O.cpu_step(S, f, f2, acc if s > 6 else None, ns)
+            O.cpu_step(S, f, f2, acc if s > 6 else None, ns, threads=threads)
             f, f2 = f2, f
             if s in (1, 2, 12):
-                assert GU.sha(f) == fx["sha"][f"f_{vn}_{s}"], (vn, s)
-        assert GU.sha(acc) == fx["sha"][f"acc_{vn}"]
+                assert GU.sha(f) == fx["sha"][f"f_{vn}_{s}"], (vn, s, threads)
+        assert GU.sha(acc) == fx["sha"][f"acc_{vn}"], (vn, threads)
         gam, mr, _ = O.probe(S, f, 12)
         t = fx["traj"][vn]
         assert np.array_equal(np.array([gam, mr]), np.array([GU.x64(t["gamma"]), GU.x64(t["massRatio"])]),
