@@ -1,7 +1,9 @@
 // Geometry preparation: the reference's prepare(p) (engine.js:110-234) with
-// its voxelizer (engine.js:45-93) and 27-subcell vote (engine.js:127-135)
-// on the GPU, and the order-dependent steps (opening choice by BFS order,
-// engine.js:141-171) on the host.
+// its voxelizer (engine.js:45-93), 27-subcell vote (engine.js:127-135),
+// inlet duct, floor, flood fill and leak check (engine.js:173-195) on the
+// GPU, and the order-dependent opening choice (by BFS order over one 2-D
+// slice, engine.js:141-171) and the lid and outlet rows (engine.js:196-214)
+// on the host.
 //
 // Voxelizer.  The reference loops triangle by triangle over the columns of
 // its bounding box, tests the column centre with f64 barycentrics, stores the
@@ -195,6 +197,90 @@ __global__ void k_slice(const uint8_t* fine, int nxf, int nyf, int nzf, int kf, 
 
 inline unsigned nb(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---- inlet duct, floor, flood fill and leak check (engine.js:173-195) ------
+// Duct: cells below kw0 are solid except under the opening; floor: rows
+// whose depth lies beyond mx_d - h/2 are solid (the floor wins).
+__global__ void k_duct_floor(uint8_t* solid, const uint8_t* opening, const uint8_t* floor_row, int nx, int ny, int nz,
+                             int kw0) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)nx * ny * nz) return;
+  const int k = (int)(c % nz);
+  const int64_t ij = c / nz;
+  uint8_t v = solid[c];
+  if (k < kw0) v = opening[ij] ? 0 : 1;
+  if (floor_row[ij % ny]) v = 1;
+  solid[c] = v;
+}
+
+// Breadth-first flood of the air from the opening cells at k = 0, 6-connected,
+// through cells that are not solid and lie at or below the seal (j >= top).
+// The visited set is a bit per cell (set with atomicOr, so every cell enters
+// the queue once); the queue holds cell indices, level by level: the level
+// being expanded is Q[lo, end), new cells are appended at tail.  The final
+// set is the connected part of the allowed cells reachable from the seeds,
+// whatever the order in which a level is expanded.
+struct FloodCtr {
+  unsigned long long lo, end, tail;
+  int leak;
+};
+
+__device__ __forceinline__ void flood_visit(int64_t c, int j, int top, const uint8_t* __restrict__ solid, uint32_t* fw,
+                                            int32_t* Q, FloodCtr* ctr) {
+  if (j < top || solid[c]) return;
+  const uint32_t bit = 1u << (c & 31);
+  if (fw[c >> 5] & bit) return;  // bits are only ever set: a set bit is final
+  if (atomicOr(&fw[c >> 5], bit) & bit) return;
+  Q[atomicAdd(&ctr->tail, 1ull)] = (int32_t)c;
+}
+
+__global__ void k_flood_seed(const uint8_t* __restrict__ opening, const uint8_t* __restrict__ solid, uint32_t* fw,
+                             int32_t* Q, FloodCtr* ctr, int nx, int ny, int nz, int top) {
+  const int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ij >= (int64_t)nx * ny || !opening[ij]) return;
+  flood_visit(ij * nz, (int)(ij % ny), top, solid, fw, Q, ctr);
+}
+
+__global__ void k_flood_level(const uint8_t* __restrict__ solid, uint32_t* fw, int32_t* Q, FloodCtr* ctr, int nx,
+                              int ny, int nz, int top) {
+  const unsigned long long lo = ctr->lo, hi = ctr->end;
+  const int64_t pl = (int64_t)ny * nz;
+  for (unsigned long long q = lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; q < hi;
+       q += (unsigned long long)gridDim.x * blockDim.x) {
+    const int64_t c = Q[q];
+    const int k = (int)(c % nz);
+    const int64_t ij = c / nz;
+    const int j = (int)(ij % ny), i = (int)(ij / ny);
+    if (i > 0) flood_visit(c - pl, j, top, solid, fw, Q, ctr);
+    if (i < nx - 1) flood_visit(c + pl, j, top, solid, fw, Q, ctr);
+    if (j > 0) flood_visit(c - nz, j - 1, top, solid, fw, Q, ctr);
+    if (j < ny - 1) flood_visit(c + nz, j + 1, top, solid, fw, Q, ctr);
+    if (k > 0) flood_visit(c - 1, j, top, solid, fw, Q, ctr);
+    if (k < nz - 1) flood_visit(c + 1, j, top, solid, fw, Q, ctr);
+  }
+}
+
+// The next level is what the last one appended.
+__global__ void k_flood_advance(FloodCtr* ctr) {
+  ctr->lo = ctr->end;
+  ctr->end = ctr->tail;
+}
+
+// Byte mask of the flooded cells, and the leak flag: air on the faces i = 0,
+// i = nx - 1 or k = nz - 1 at or below the seal and not in the duct
+// (engine.js:191-195).
+__global__ void k_flood_finish(const uint32_t* fw, uint8_t* fluid, FloodCtr* ctr, int nx, int ny, int nz, int top,
+                               int k0) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (int64_t)nx * ny * nz) return;
+  const uint8_t f = (fw[c >> 5] >> (c & 31)) & 1u;
+  fluid[c] = f;
+  if (!f) return;
+  const int k = (int)(c % nz);
+  const int64_t ij = c / nz;
+  const int j = (int)(ij % ny), i = (int)(ij / ny);
+  if (j >= top && k >= k0 && (i == 0 || i == nx - 1 || k == nz - 1)) atomicOr(&ctr->leak, 1);
+}
+
 // RAII device buffer for the preparation stages.
 struct DBuf {
   void* p = nullptr;
@@ -329,6 +415,84 @@ int first_ge(const std::vector<float>& a, double v, bool or_last) {
   for (size_t i = 0; i < a.size(); ++i)
     if ((double)a[i] >= v) return (int)i;
   return or_last ? (int)a.size() - 1 : -1;
+}
+
+// Inlet duct, floor, flood fill and leak check on the GPU (engine.js:173-195).
+// solid (N bytes) is updated in place with the duct and floor; fluid receives
+// the flooded cells, *nfluid their number and *leaked whether air reaches an
+// outer face.
+int flood_device(std::vector<uint8_t>& solid, const std::vector<uint8_t>& opening,
+                 const std::vector<uint8_t>& floor_row, int nx, int ny, int nz, int kw0, int top,
+                 std::vector<uint8_t>& fluid, int64_t* nfluid, bool* leaked) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  if (N >= (int64_t)INT32_MAX) {
+    set_error("abx_prepare: grid too large for the flood fill (2^31 cells)");
+    return -1;
+  }
+  hipStream_t st;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("abx_prepare: stream");
+    return -1;
+  }
+  DBuf dsolid, dopen, dfloor, dfw, dQ, dfluid, dctr;
+  const int64_t nw = (N + 31) / 32;
+  int rc = dsolid.alloc((size_t)N) || dopen.alloc(opening.size()) || dfloor.alloc(floor_row.size()) ||
+           dfw.alloc(4 * (size_t)nw) || dQ.alloc(4 * (size_t)N) || dfluid.alloc((size_t)N) ||
+           dctr.alloc(sizeof(FloodCtr));
+  FloodCtr h;
+  memset(&h, 0, sizeof h);
+  if (!rc && (hipMemcpyAsync(dsolid.p, solid.data(), (size_t)N, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(dopen.p, opening.data(), opening.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(dfloor.p, floor_row.data(), floor_row.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemsetAsync(dfw.p, 0, 4 * (size_t)nw, st) != hipSuccess ||
+              hipMemcpyAsync(dctr.p, &h, sizeof h, hipMemcpyHostToDevice, st) != hipSuccess)) {
+    set_error("abx_prepare: flood fill upload failed");
+    rc = -1;
+  }
+  uint8_t* sol = dsolid.as<uint8_t>();
+  uint32_t* fw = dfw.as<uint32_t>();
+  int32_t* Q = dQ.as<int32_t>();
+  FloodCtr* ctr = dctr.as<FloodCtr>();
+  if (!rc) {
+    hipLaunchKernelGGL(k_duct_floor, dim3(nb(N, 256)), dim3(256), 0, st, sol, dopen.as<uint8_t>(),
+                       dfloor.as<uint8_t>(), nx, ny, nz, kw0);
+    hipLaunchKernelGGL(k_flood_seed, dim3(nb((int64_t)nx * ny, 256)), dim3(256), 0, st, dopen.as<uint8_t>(), sol, fw,
+                       Q, ctr, nx, ny, nz, top);
+    hipLaunchKernelGGL(k_flood_advance, dim3(1), dim3(1), 0, st, ctr);
+    // levels are issued 32 at a time; an empty level (lo == end) ends the search
+    for (;;) {
+      for (int r = 0; r < 32; ++r) {
+        hipLaunchKernelGGL(k_flood_level, dim3(1024), dim3(256), 0, st, sol, fw, Q, ctr, nx, ny, nz, top);
+        hipLaunchKernelGGL(k_flood_advance, dim3(1), dim3(1), 0, st, ctr);
+      }
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(&h, dctr.p, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) {
+        set_error("abx_prepare: flood fill failed on the GPU");
+        rc = -1;
+        break;
+      }
+      if (h.lo == h.end) break;
+    }
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(k_flood_finish, dim3(nb(N, 256)), dim3(256), 0, st, fw, dfluid.as<uint8_t>(), ctr, nx, ny, nz,
+                       top, kw0 < 0 ? 0 : kw0);
+    fluid.assign((size_t)N, 0);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(fluid.data(), dfluid.p, (size_t)N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(solid.data(), dsolid.p, (size_t)N, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(&h, dctr.p, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      set_error("abx_prepare: flood fill read-back failed");
+      rc = -1;
+    }
+  }
+  (void)hipStreamDestroy(st);
+  if (rc) return -1;
+  *nfluid = (int64_t)h.tail;
+  *leaked = h.leak != 0;
+  return 0;
 }
 
 }  // namespace abx
@@ -603,53 +767,23 @@ extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const 
   if (nOpen < 8) return fail("Einlassöffnung ist zu klein für die gewählte Zellgröße.");
   const std::vector<uint8_t>& opening = g->opening;
 
-  // inlet duct, floor, flood fill of the air (engine.js:173-189)
+  // inlet duct, floor, flood fill of the air and leak check, on the GPU
+  // (engine.js:173-195).  The reference applies duct and floor before it
+  // checks the seal row; neither changes whether that check fails.
   auto IDX = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
   const int kw0 = first_ge(zc, mn[2], false);
   I.kw0 = kw0;
-  for (int i = 0; i < nx; ++i)
-    for (int j = 0; j < ny; ++j)
-      for (int k = 0; k < kw0; ++k) solid[(size_t)IDX(i, j, k)] = opening[(size_t)i * ny + j] ? 0 : 1;
-  for (int j = 0; j < ny; ++j)
-    if ((double)dc[(size_t)j] > mx[1] - 0.5 * h)
-      for (int i = 0; i < nx; ++i)
-        for (int k = 0; k < nz; ++k) solid[(size_t)IDX(i, j, k)] = 1;
   const int top = first_ge(dc, 0.0, false);
   if (top < 0) return fail("Dichtebene liegt außerhalb des Modells.");
   I.top = top;
-  g->fluid.assign((size_t)N, 0);
+  std::vector<uint8_t> floor_row((size_t)ny);
+  for (int j = 0; j < ny; ++j) floor_row[(size_t)j] = (double)dc[(size_t)j] > mx[1] - 0.5 * h ? 1 : 0;
   std::vector<uint8_t>& fluid = g->fluid;
-  std::vector<int32_t> Q((size_t)N);
-  qh = qt = 0;
-  auto visit = [&](int64_t c, int j) {
-    if (!fluid[(size_t)c] && !solid[(size_t)c] && j >= top) {
-      fluid[(size_t)c] = 1;
-      Q[(size_t)qt++] = (int32_t)c;
-    }
-  };
-  for (int64_t c = 0; c < (int64_t)nx * ny; ++c)
-    if (opening[(size_t)c]) visit(c * nz, (int)(c % ny));
-  while (qh < qt) {
-    const int64_t c = Q[(size_t)qh++];
-    const int k = (int)(c % nz);
-    const int64_t ij = c / nz;
-    const int j = (int)(ij % ny), i = (int)(ij / ny);
-    if (i > 0) visit(c - (int64_t)ny * nz, j);
-    if (i < nx - 1) visit(c + (int64_t)ny * nz, j);
-    if (j > 0) visit(c - nz, j - 1);
-    if (j < ny - 1) visit(c + nz, j + 1);
-    if (k > 0) visit(c - 1, j);
-    if (k < nz - 1) visit(c + 1, j);
-  }
-  // leak check (engine.js:191-195)
   bool leaked = false;
-  for (int i = 0; i < nx && !leaked; ++i)
-    for (int j = top; j < ny && !leaked; ++j)
-      for (int k = kw0 < 0 ? 0 : kw0; k < nz; ++k)
-        if ((i == 0 || i == nx - 1 || k == nz - 1) && fluid[(size_t)IDX(i, j, k)]) {
-          leaked = true;
-          break;
-        }
+  if (flood_device(solid, opening, floor_row, nx, ny, nz, kw0, top, fluid, &qt, &leaked)) {
+    std::string m = last_error();
+    return fail(m);
+  }
   if (leaked)
     return fail("Bei " + js_num(h) +
                 " mm Zellgröße läuft die Luft durch eine zu dünne Wand nach außen (Wände sollten mindestens so "
