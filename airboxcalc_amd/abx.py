@@ -165,6 +165,7 @@ def load() -> C.CDLL:
                                   C.POINTER(i64), C.POINTER(i64)]),
         # flow surrogate kernels (stream first; tensors as device addresses)
         "abx_conv3d_pack": (i32, [vp, vp, i32, i32, i32, i32, vp]),
+        "abx_conv3d_set_kernel": (i32, [i32]),
         "abx_surrogate_features_workspace": (i64, [i32, i32, i32]),
         "abx_surrogate_features": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, d, d, d, i32, i32, i32, vp, i64,
                                          vp, i32, i32]),

@@ -184,6 +184,156 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 }
 
 // ---------------------------------------------------------------------------
+// Forward convolution, 3x3x3 taps, one 16-channel output tile per workgroup
+// (the layers 16 channels wide, where k_conv_fwd has one MFMA per weight
+// load).  A tile is 16 x 4 x 4 (x, y, z) output voxels; wave w computes
+// z-slice w, four rows of 16 voxels.  Input channels are taken 16 at a time:
+// the k-steps of 32 then cover two taps each (k = 32 s + 8 lg: tap
+// 2 s + lg / 2, channels 8 (lg % 2) .. + 8), 14 steps for 27 taps (the last
+// half empty).  Every k-step reads one input fragment per row from the LDS
+// brick, at an offset fixed at compile time up to the lane's tap parity, and
+// issues four MFMAs against a weight fragment held in registers.
+//
+// Workgroups are persistent over tiles.  With 16 input channels the weights
+// are loaded once per workgroup; each thread owns fixed slots of the brick
+// (decoded once), and the global loads of the next (tile, chunk) brick are
+// issued before the MFMAs of the current one, so their latency is hidden.
+constexpr int F16X = 16, F16Y = 4, F16Z = 4;
+constexpr int F16BX = F16X + 2, F16BY = F16Y + 2, F16BZ = F16Z + 2, F16BV = F16BX * F16BY * F16BZ;
+constexpr int F16SP = 24;  // u16 per brick voxel: 16 channels + 8 pad (48-byte stride: conflict-free 16-byte reads)
+constexpr int F16SLOTS = (2 * F16BV + 255) / 256;  // 16-byte brick pieces per thread
+
+__host__ __device__ constexpr int f16_tapoff(int tap) {
+  return tap < 27 ? ((tap / 9) * F16BY + (tap / 3) % 3) * F16BX + tap % 3 : 0;
+}
+
+struct F16Tile {
+  int n, x0, y0, z0;
+};
+
+__device__ __forceinline__ F16Tile f16_tile(int t, int D, int H, int W) {
+  const int ntx = (W + F16X - 1) / F16X, nty = (H + F16Y - 1) / F16Y, ntz = (D + F16Z - 1) / F16Z;
+  F16Tile r;
+  r.x0 = (t % ntx) * F16X;
+  t /= ntx;
+  r.y0 = (t % nty) * F16Y;
+  t /= nty;
+  r.z0 = (t % ntz) * F16Z;
+  r.n = t / ntz;
+  return r;
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats to packed bf16, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+__global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
+                                                    const float* __restrict__ bias, u16* __restrict__ y, int ys,
+                                                    int D, int H, int W, int Ci, int ntiles, int relu, int accum) {
+  __shared__ __attribute__((aligned(16))) u16 sx[F16BV * F16SP];
+  const int co0 = blockIdx.y * 16, nch = Ci / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4, hi = lg >> 1;
+  const int lbase = (wave * F16BY * F16BX + lr) * F16SP + 8 * (lg & 1);
+
+  // brick slots: piece e = thread + 256 m is voxel e / 2 (x fastest), channels 8 (e % 2) .. + 8
+  int sb[F16SLOTS];  // bx | by << 5 | bz << 8 | g << 11, or -1
+#pragma unroll
+  for (int m = 0; m < F16SLOTS; ++m) {
+    const int e = threadIdx.x + 256 * m, vb = e >> 1;
+    const int bx = vb % F16BX, r = vb / F16BX, by = r % F16BY, bz = r / F16BY;
+    sb[m] = e < 2 * F16BV ? bx | by << 5 | bz << 8 | (e & 1) << 11 : -1;
+  }
+  u32x4 pf[F16SLOTS];
+  auto fetch = [&](int t, int c) {
+    const F16Tile tl = f16_tile(t, D, H, W);
+#pragma unroll
+    for (int m = 0; m < F16SLOTS; ++m) {
+      pf[m] = (u32x4){0u, 0u, 0u, 0u};
+      if (sb[m] < 0) continue;
+      const int gx = tl.x0 + (sb[m] & 31) - 1, gy = tl.y0 + ((sb[m] >> 5) & 7) - 1,
+                gz = tl.z0 + ((sb[m] >> 8) & 7) - 1;
+      if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
+        pf[m] = *(const u32x4*)(x + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xs + 16 * c +
+                                8 * (sb[m] >> 11));
+    }
+  };
+  u32x4 wf[14];
+  auto load_weights = [&](int c) {
+#pragma unroll
+    for (int s = 0; s < 14; ++s) {
+      const int tap = 2 * s + hi;
+      wf[s] = tap < 27 ? *(const u32x4*)(wt + ((int64_t)(co0 + lr) * 27 + tap) * Ci + 16 * c + 8 * (lg & 1))
+                       : (u32x4){0u, 0u, 0u, 0u};
+    }
+  };
+
+  if (nch == 1) load_weights(0);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    f32x4 acc[F16Y];
+#pragma unroll
+    for (int r = 0; r < F16Y; ++r) acc[r] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < nch; ++c) {
+      fetch(t, c);
+      __syncthreads();  // the previous chunk's MFMAs are done with the brick
+#pragma unroll
+      for (int m = 0; m < F16SLOTS; ++m)
+        if (sb[m] >= 0) {
+          const int vb = ((sb[m] >> 8) & 7) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
+          *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 11)) = pf[m];
+        }
+      __syncthreads();
+      if (nch > 1) load_weights(c);
+#pragma unroll
+      for (int s = 0; s < 14; ++s) {
+        const int off = (hi ? f16_tapoff(2 * s + 1) : f16_tapoff(2 * s)) * F16SP;
+        const bool none = s == 13 && hi;  // tap 27 does not exist
+#pragma unroll
+        for (int r = 0; r < F16Y; ++r) {
+          u32x4 b = *(const u32x4*)(sx + lbase + off + r * F16BX * F16SP);
+          if (none) b = (u32x4){0u, 0u, 0u, 0u};
+          acc[r] = mfma16(wf[s], b, acc[r]);
+        }
+      }
+    }
+
+    // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
+    const F16Tile tl = f16_tile(t, D, H, W);
+    const int gx = tl.x0 + lr, gz = tl.z0 + wave, co = co0 + 4 * lg;
+    float bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[co + i] : 0.f;
+#pragma unroll
+    for (int r = 0; r < F16Y; ++r) {
+      const int gy = tl.y0 + r;
+      if (gx >= W || gy >= H || gz >= D) continue;
+      u16* yp = y + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * ys + co;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[r][i] + bv[i];
+      if (accum) {
+        const u32x2 old = *(const u32x2*)yp;
+        v[0] += bf2f((u16)(old.x & 0xffffu));
+        v[1] += bf2f((u16)(old.x >> 16));
+        v[2] += bf2f((u16)(old.y & 0xffffu));
+        v[3] += bf2f((u16)(old.y >> 16));
+      }
+      if (relu)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.f ? v[i] : 0.f;
+      u32x2 o;
+      o.x = pack_bf16(v[0], v[1]);
+      o.y = pack_bf16(v[2], v[3]);
+      *(u32x2*)yp = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: partial sums over a strided range of tiles.
 // grid (S, co blocks * ci chunks * tap groups), 256 threads.  A workgroup owns
 // output channels [co0, co0+16*NCT), input channels [ci0, ci0+16*NCI) and taps
@@ -481,6 +631,22 @@ unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t)
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// 3x3x3 convolutions with a 16-channel output tile use k_conv_fwd16
+// (abx_conv3d_set_kernel switches back to k_conv_fwd, for comparison).
+int g_fwd16 = 1;
+
+// Compute units of the current device (cached per device ordinal).
+int device_cus() {
+  static int cache[64] = {0};
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= 64) return 256;
+  if (!cache[d]) {
+    int v = 0;
+    cache[d] = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d) == hipSuccess && v > 0 ? v : 256;
+  }
+  return cache[d];
+}
+
 int64_t tiles_of(int n, int d, int h, int w) {
   return (int64_t)n * ((d + TZ - 1) / TZ) * ((h + TY - 1) / TY) * ((w + TX - 1) / TX);
 }
@@ -550,6 +716,16 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
   const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   const dim3 grid((unsigned)tiles_of(n, d, h, w), (unsigned)(co / (16 * nct)));
   hipStream_t s = (hipStream_t)stream;
+  if (ksize == 3 && nct == 1 && g_fwd16) {
+    const int64_t tiles = (int64_t)n * ((d + F16Z - 1) / F16Z) * ((h + F16Y - 1) / F16Y) * ((w + F16X - 1) / F16X);
+    // persistent workgroups: about four per CU across the co tiles
+    const int64_t per_co = std::max<int64_t>(1, (int64_t)4 * device_cus() / (co / 16));
+    const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
+    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h,
+                       w, ci, (int)tiles, relu, accumulate);
+    ABX_HIP(hipGetLastError());
+    return 0;
+  }
 #define ABX_FWD(T, NC, C)                                                                                          \
   hipLaunchKernelGGL((k_conv_fwd<T, NC, C>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h, w, ci, co, \
                      relu, accumulate)
@@ -570,6 +746,11 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
 #undef ABX_FWD_CK
 #undef ABX_FWD
   ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int abx_conv3d_set_kernel(int fwd16) {
+  g_fwd16 = fwd16 ? 1 : 0;
   return 0;
 }
 

@@ -342,6 +342,12 @@ int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, in
                        const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
                        uint16_t* y, int ys);
 
+/* Process-wide choice of the forward kernel for 3x3x3 convolutions whose
+ * output channels are not a multiple of 32: fwd16 = 1 (the default) uses the
+ * 16-channel kernel (16x4x4 voxel tiles, weights held in registers), 0 the
+ * general one.  Both compute the same sums; for comparison and timing. */
+int abx_conv3d_set_kernel(int fwd16);
+
 /* Bytes of FP32 workspace abx_conv3d_wgrad needs for this shape (-1 on a bad
  * shape). */
 int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci, int co, int ksize);

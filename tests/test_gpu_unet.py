@@ -111,6 +111,32 @@ def test_conv_forward_brute_force():
     close_bf16(f32(y), ref, "brute force")
 
 
+@pytest.mark.parametrize("co", [16, 48])
+def test_conv_forward16_equals_general_kernel(co):
+    """The 16-channel forward kernel (k_conv_fwd16) and the general one take
+    the same k-steps in the same order for 16 input channels: equal bit for
+    bit, on a grid that is not a multiple of either tile."""
+    n, d, h, w, ci = 2, 5, 9, 37, 16
+    say("conv forward16 vs general", co)
+    rng = np.random.default_rng(4)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    W = (rng.standard_normal((co, 27, ci)) / 20).astype(np.float32)
+    wp, _ = pack(W, 3)
+    bt = torch.as_tensor(rng.standard_normal(co).astype(np.float32)).to(_dev())
+    outs = []
+    try:
+        for k in (1, 0):
+            ck(abx.load().abx_conv3d_set_kernel(k))
+            y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device=_dev())
+            ck(abx.load().abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(),
+                                             co, 3, 1, 0, y.data_ptr(), co))
+            outs.append(y)
+        torch.cuda.synchronize()
+    finally:
+        ck(abx.load().abx_conv3d_set_kernel(1))
+    assert torch.equal(outs[0].view(torch.int16), outs[1].view(torch.int16))
+
+
 def test_conv_forward_slices_and_accumulate():
     """Input and output are channel slices of wider tensors; accumulate adds."""
     n, d, h, w, ci, co = 1, 4, 4, 16, 32, 16
