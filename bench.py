@@ -201,6 +201,59 @@ def bench_unet(a, dev):
     print(json.dumps(out), flush=True)
 
 
+def bench_conv(a, dev):
+    """The surrogate's 16-channel 3x3x3 forward convolutions (batch 8, 96 x 48
+    x 64): the 16-channel kernel against the general one, each shape timed
+    with HIP events over --steps calls."""
+    import torch
+    from airboxcalc_amd import abx
+    L = abx.load()
+    s = torch.cuda.current_stream().cuda_stream
+    calls = a.steps if a.steps != 6000 else 20
+    g = torch.Generator(device="cpu").manual_seed(a.seed)
+    rows = []
+    n, d, h, w = 8, 96, 48, 64
+    for ci, co in ((16, 16), (48, 16), (32, 16), (16, 48)):
+        x = torch.randn((n, d, h, w, ci), generator=g).to(torch.bfloat16).cuda()
+        wf = (torch.randn((co * 27 * ci,), generator=g) / 20).float().cuda()
+        wp = torch.empty(co * 27 * ci, dtype=torch.bfloat16, device="cuda")
+        assert L.abx_conv3d_pack(s, wf.data_ptr(), co, 3, ci, 0, wp.data_ptr()) == 0
+        y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device="cuda")
+        us = {}
+        try:
+            for k in (0, 1):
+                L.abx_conv3d_set_kernel(k)
+                for _ in range(3):
+                    assert L.abx_conv3d_forward(s, x.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, 3, 1, 0,
+                                                y.data_ptr(), co) == 0
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(calls):
+                    L.abx_conv3d_forward(s, x.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, 3, 1, 0,
+                                         y.data_ptr(), co)
+                e1.record()
+                torch.cuda.synchronize()
+                us[k] = e0.elapsed_time(e1) / calls * 1e3
+        finally:
+            L.abx_conv3d_set_kernel(1)
+        fl = 2 * n * d * h * w * ci * co * 27
+        rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[1], 1), general_us=round(us[0], 1),
+                         fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1)))
+    r0 = rows[0]
+    out = {
+        "metric": "3x3x3 bf16 convolution TFLOP/s (16 -> 16 channels, 96x48x64, batch 8)",
+        "value": r0["fwd16_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": calls, "warmup": 3,
+        "ms_per_step": round(r0["fwd16_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random volumes)",
+        "config": {"workload": "the surrogate's 16-channel forward convolutions, k_conv_fwd16 against k_conv_fwd",
+                   "shapes": rows},
+        "roofline": {"bound": "mfma", "achieved": r0["fwd16_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(r0["fwd16_tflops"] / 2500.0, 4), "traffic": None, "kernel": "k_conv_fwd16"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,10 +269,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
     ap.add_argument("--xcd-map", type=int, default=None, help="block order across XCDs (abx_sim_set_launch)")
-    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch", "fp64", "unet"],
+    ap.add_argument("--workload", default="roofline", choices=["roofline", "batch", "fp64", "unet", "conv"],
                     help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch; "
                          "fp64: the FP64 error report of the roofline airbox at dx=2 mm; "
-                         "unet: surrogate training steps")
+                         "unet: surrogate training steps; conv: the surrogate's 16-channel convolutions")
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
@@ -243,6 +296,8 @@ def main():
         return bench_fp64(a, dev)
     if a.workload == "unet":
         return bench_unet(a, dev)
+    if a.workload == "conv":
+        return bench_conv(a, dev)
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
     t = time.perf_counter()
