@@ -497,12 +497,13 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, kernel: int = 4, xcd_map: int = 8, chunk: int = 256, nt: int = 0) -> None:
+    def set_launch(self, kernel: int = 4, xcd_map: int = -1, chunk: int = 256, nt: int = 0) -> None:
         """kernel 4 = AA pattern (default); 0 wide, 1 narrow, 2 segment, 3
         persistent segment (two lattices); 5 AA with per-cell geometry words;
         6 AA pattern out of place (two lattices).
         xcd_map: block order across XCDs (0 round robin, 1 contiguous, G > 1
-        groups of G blocks; the library's default is 8)."""
+        groups of G blocks, -1 one band of rows j per XCD: the library's
+        default)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
 
     def set_pipeline(self, planes: int) -> None:

@@ -417,7 +417,7 @@ struct abx_sim {
   abx_dev* dev = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int prec = ABX_F32, B = 1, xcd_map = 8;  // default launch: groups of 8 blocks per XCD
+  int prec = ABX_F32, B = 1, xcd_map = -1;  // default launch: one band of rows j per XCD
   int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
   int64_t ntot = 0, plane = 0, NF = 0;
   size_t ssz = 4;  // bytes per stored value
@@ -434,6 +434,11 @@ struct abx_sim {
   uint32_t* meta = nullptr;       // per-position geometry words (kernel 5)
   int nchunks64 = 0;
   std::vector<int32_t> seg_host;  // host copy of chunks64 (ascending positions)
+  // xcd_map -1: the segments reordered so that XCD x (workgroups x, x + 8, ...)
+  // owns the x-th of eight bands of lattice rows j (equal segment counts) and
+  // sweeps it plane by plane; padded with position 0 (never fluid)
+  int32_t* chunks_slab = nullptr;
+  int nslab = 0;
   // AA pairs issued plane-pipelined over bands of `pipe` lattice planes
   // (kernel 4 only; 0 = one launch per step)
   int pipe = 0;
@@ -488,8 +493,15 @@ StepArgs step_args(abx_sim* s, int doacc) {
   const bool seg64 = s->kernel != 1 || s->chunk == 64;
   a.chunks = seg64 ? s->chunks64 : s->chunks;
   a.nchunks = seg64 ? s->nchunks64 : s->nchunks;
-  a.per_xcd = (a.nchunks + 7) / 8;
   a.xcd_map = s->xcd_map;
+  if (s->xcd_map < 0) {
+    if (s->chunks_slab && (s->kernel == 2 || s->kernel == 4 || s->kernel == 6)) {
+      a.chunks = s->chunks_slab;
+      a.nchunks = s->nslab;
+    }
+    a.xcd_map = 0;
+  }
+  a.per_xcd = (a.nchunks + 7) / 8;
   a.nyp = s->nyp;
   a.nzp = s->nzp;
   a.plane = s->plane;
@@ -683,7 +695,9 @@ int launch_pair_pipelined(abx_sim* s, int doacc) {
   const std::vector<int32_t>& sg = s->seg_host;
   const int n = (int)sg.size();
   if (n == 0) return 0;
-  const StepArgs a = step_args(s, doacc);
+  StepArgs a = step_args(s, doacc);
+  a.chunks = s->chunks64;  // bands index the position-ordered list
+  a.nchunks = s->nchunks64;
   const int64_t ipl = (int64_t)s->nyp * s->nzp;
   const int64_t W = (int64_t)s->pipe * ipl;
   const int64_t GAP = ipl + s->nzp + 128;
@@ -775,6 +789,34 @@ int ensure_meta(abx_sim* s) {
   return 0;
 }
 
+// Segment order of xcd_map -1 (see abx_sim::chunks_slab).
+int ensure_slab_order(abx_sim* s) {
+  if (s->chunks_slab || s->seg_host.empty()) return 0;
+  const std::vector<int32_t>& sg = s->seg_host;
+  std::vector<int64_t> per_j((size_t)s->nyp + 1, 0);
+  for (int32_t d : sg) per_j[(size_t)((d / s->nzp) % s->nyp)]++;
+  // band of row j: cut the cumulative segment count into eight equal parts
+  std::vector<int> band((size_t)s->nyp, 0);
+  int64_t cum = 0;
+  const int64_t tot = (int64_t)sg.size();
+  for (int j = 0; j < s->nyp; ++j) {
+    band[(size_t)j] = (int)std::min<int64_t>(7, (cum * 8) / std::max<int64_t>(tot, 1));
+    cum += per_j[(size_t)j];
+  }
+  std::vector<std::vector<int32_t>> lists(8);
+  for (int32_t d : sg) lists[(size_t)band[(size_t)((d / s->nzp) % s->nyp)]].push_back(d);
+  size_t maxb = 0;
+  for (const auto& l : lists) maxb = std::max(maxb, (l.size() + 3) / 4);
+  // workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th workgroup
+  std::vector<int32_t> perm(maxb * 8 * 4, 0);
+  for (size_t x = 0; x < 8; ++x)
+    for (size_t q = 0; q < lists[x].size(); ++q) perm[((q / 4) * 8 + x) * 4 + q % 4] = lists[x][q];
+  if (dalloc(s, &s->chunks_slab, perm.size())) return -1;
+  s->nslab = (int)perm.size();
+  ABX_HIP(hipMemcpyAsync(s->chunks_slab, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, s->stream));
+  return 0;
+}
+
 // The second lattice, needed only by the two-lattice kernels.
 int ensure_second_lattice(abx_sim* s) {
   if (s->f[1]) return 0;
@@ -822,7 +864,7 @@ void sim_free(abx_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   sim_drop_graphs(s);
   void* ptrs[] = {s->f[0],   s->f[1],  s->acc,   s->ns,    s->fluid_alloc, s->type_bits, s->chunks,
-                  s->chunks64, s->meta, s->cells, s->plist, s->vy,    s->red};
+                  s->chunks64, s->meta, s->cells, s->plist, s->vy,    s->red,   s->chunks_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->red_host) (void)hipHostFree(s->red_host);
@@ -990,7 +1032,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     sim_free(s);
     return -1;
   }
-  if (fill_cold(s) || zero_acc(s) || upload_ns(s, ns, cells)) {
+  if (ensure_slab_order(s) || fill_cold(s) || zero_acc(s) || upload_ns(s, ns, cells)) {
     sim_free(s);
     return -1;
   }
@@ -1363,6 +1405,7 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
   const bool was_aa = s->kernel >= 4, to_aa = kernel >= 4;
   if ((!to_aa || kernel == 6) && ensure_second_lattice(s)) return -1;
   if (kernel == 5 && ensure_meta(s)) return -1;
+  if (xcd_map < 0 && ensure_slab_order(s)) return -1;
   if (was_aa != to_aa) {
     // carry the state over: read it in the old form, store it in the new one
     const size_t n = (size_t)(19 * s->NF);
@@ -1376,7 +1419,7 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
       if (abx_sim_write_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
   }
   s->kernel = kernel;
-  s->xcd_map = std::max(0, std::min(xcd_map, 4096));
+  s->xcd_map = xcd_map < 0 ? -1 : std::min(xcd_map, 4096);
   s->chunk = chunk;
   s->nt = nt;
   return 0;

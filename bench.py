@@ -305,7 +305,7 @@ def main():
     log(f"prepare {time.perf_counter() - t:.2f}s grid {g.nx}x{g.ny}x{g.nz} nf={g.nf} ntri={tris.size // 9}")
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
     if a.kernel is not None or a.xcd_map is not None:
-        sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=8 if a.xcd_map is None else a.xcd_map)
+        sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=-1 if a.xcd_map is None else a.xcd_map)
     NF = sim.NF
     log(f"sim NF={NF} device bytes {sim.info.device_bytes / 2**30:.2f} GiB")
 

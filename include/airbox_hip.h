@@ -234,7 +234,9 @@ int abx_sim_sync(abx_sim* sim);
  *         6 = segment, AA pattern out of place (two lattices).
  * xcd_map 1 = each XCD sweeps one contiguous part of the lattice; xcd_map
  * G > 1 (segment kernels) = groups of G consecutive blocks per XCD, all XCDs
- * moving through the lattice together.  Every variant computes the same
+ * moving through the lattice together; xcd_map -1 (kernels 2, 4, 6; the
+ * default) = each XCD owns one of eight bands of rows j (equal segment
+ * counts) and sweeps it plane by plane.  Every variant computes the same
  * values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
 /* Plane-pipelined AA pairs (kernel 4; no reference counterpart): planes > 0

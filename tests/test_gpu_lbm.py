@@ -179,7 +179,8 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     want_f, want_acc = a.read_f(), a.read_acc()
     for kern, xm, chunk, nt in ((5, 0, 256, 0), (1, 0, 256, 0), (1, 1, 256, 3), (1, 0, 64, 1), (0, 0, 256, 0), (2, 0, 256, 0),
                                 (3, 0, 256, 0), (0, 1, 256, 0), (4, 0, 256, 0), (4, 1, 256, 0), (4, 3, 256, 0),
-                                (2, 5, 256, 0), (6, 8, 256, 0), (6, 0, 256, 0)):
+                                (2, 5, 256, 0), (6, 8, 256, 0), (6, 0, 256, 0), (4, -1, 256, 0), (2, -1, 256, 0),
+                                (6, -1, 256, 0)):
         _, b = _sim(fx, precision)
         b.set_launch(kern, xm, chunk, nt)
         b.write_f(f)
