@@ -172,15 +172,45 @@ extern "C" int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_b
     bt->members.push_back(s);
     bt->cells += I.NF;
   }
-  // work lists: each member's segments padded to whole workgroups
+  // work lists.  Workgroup b runs on XCD b % 8.  Each member's segments are
+  // cut into eight bands of lattice rows j with equal segment counts; XCD x
+  // takes band x of every member, in member order, each band padded to whole
+  // workgroups.  All XCDs move through the members together, and the odd
+  // step's gathers from neighbouring rows stay in the XCD's own L2.
   const int spbs[3] = {2, 4, 8};
+  std::vector<std::vector<int32_t>> segs((size_t)n);
+  std::vector<int> nyp((size_t)n), nzp((size_t)n);
+  for (int m = 0; m < n; ++m) {
+    segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
+    const SimView v = sim_view(bt->members[(size_t)m]);
+    nyp[(size_t)m] = v.nyp;
+    nzp[(size_t)m] = v.nzp;
+  }
   for (int k = 0; k < 3; ++k) {
-    std::vector<int2> w;
+    std::vector<std::vector<int2>> q(8);
     for (int m = 0; m < n; ++m) {
-      const std::vector<int32_t> segs = sim_segments(bt->members[(size_t)m]);
-      for (int32_t sg : segs) w.push_back(make_int2(m, sg));
-      while (w.size() % spbs[k]) w.push_back(make_int2(m, -1));
+      const std::vector<int32_t>& sg = segs[(size_t)m];
+      std::vector<int64_t> per_j((size_t)nyp[(size_t)m], 0);
+      for (int32_t d : sg) per_j[(size_t)((d / nzp[(size_t)m]) % nyp[(size_t)m])]++;
+      std::vector<int> band((size_t)nyp[(size_t)m], 0);
+      int64_t cum = 0;
+      const int64_t tot = std::max<int64_t>((int64_t)sg.size(), 1);
+      for (int j = 0; j < nyp[(size_t)m]; ++j) {
+        band[(size_t)j] = (int)std::min<int64_t>(7, cum * 8 / tot);
+        cum += per_j[(size_t)j];
+      }
+      const size_t start[8] = {q[0].size(), q[1].size(), q[2].size(), q[3].size(),
+                               q[4].size(), q[5].size(), q[6].size(), q[7].size()};
+      for (int32_t d : sg) q[(size_t)band[(size_t)((d / nzp[(size_t)m]) % nyp[(size_t)m])]].push_back(make_int2(m, d));
+      for (int x = 0; x < 8; ++x)
+        if (q[(size_t)x].size() > start[x])
+          while (q[(size_t)x].size() % spbs[k]) q[(size_t)x].push_back(make_int2(m, -1));
     }
+    size_t maxb = 0;
+    for (const auto& qx : q) maxb = std::max(maxb, qx.size() / spbs[k]);
+    std::vector<int2> w(maxb * 8 * spbs[k], make_int2(0, -1));
+    for (size_t x = 0; x < 8; ++x)
+      for (size_t i = 0; i < q[x].size(); ++i) w[((i / spbs[k]) * 8 + x) * spbs[k] + i % spbs[k]] = q[x][i];
     bt->nblocks[k] = (int)(w.size() / spbs[k]);
     if (hipMalloc((void**)&bt->d_work[k], std::max<size_t>(w.size(), 1) * sizeof(int2)) != hipSuccess ||
         hipMemcpy(bt->d_work[k], w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
