@@ -7,7 +7,10 @@ module defines the criterion the warm-start study is judged by:
   run in windows of `window` steps, averaging each window on the GPU, and
   evaluate it (gamma-core and zeta, engine.js:632-691); the solve is steady
   when gamma-core moved by less than tol_gamma and zeta by less than
-  tol_zeta (relative) between two consecutive windows.
+  tol_zeta (relative) between two consecutive windows.  The steady result is
+  the mean of those two windows' metrics, a 2-window estimate like the
+  reference's averaging over its last `avg` steps; zeta is linear in the
+  averaged density, so its mean is exactly zeta of the 2-window average.
 
 A cold start begins from f0 = WT (engine.js:260-261); a warm start from the
 equilibrium of a predicted (rho, u) field (abx_sim_init_eq).  The FP64 report
@@ -31,8 +34,8 @@ class SteadyResult:
     converged: bool
     steps: int
     wall_s: float
-    metrics: dict
-    windows: List[tuple] = field(default_factory=list)   # (steps, gammaCore, zeta)
+    metrics: dict                                        # steady: mean of the last two windows
+    windows: List[tuple] = field(default_factory=list)   # (steps, gammaCore, zeta) of every window
     acc: Optional[np.ndarray] = None                     # last window's averaged fields, cid order
 
 
@@ -71,7 +74,7 @@ def solve_to_steady(dev: abx.Device, g: abx.Geometry, p: dict, window: int = 500
             if prev is not None and abs(m["gammaCore"] - prev["gammaCore"]) < tol_gamma and \
                     abs(m["zeta"] - prev["zeta"]) <= tol_zeta * abs(prev["zeta"]):
                 converged = True
-                prev = m
+                prev = {k: 0.5 * (v + prev[k]) if isinstance(v, float) and k in prev else v for k, v in m.items()}
                 break
             prev = m
         sim.sync()
