@@ -222,7 +222,7 @@ def bench_conv(a, dev):
         us = {}
         try:
             for k in (0, 1):
-                L.abx_conv3d_set_kernel(k)
+                L.abx_conv3d_set_kernel(k | 2)   # bit 0: k_conv_fwd16; bit 1 (default): transposed weight gradient
                 for _ in range(3):
                     assert L.abx_conv3d_forward(s, x.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, 3, 1, 0,
                                                 y.data_ptr(), co) == 0
@@ -235,7 +235,7 @@ def bench_conv(a, dev):
                 torch.cuda.synchronize()
                 us[k] = e0.elapsed_time(e1) / calls * 1e3
         finally:
-            L.abx_conv3d_set_kernel(1)
+            L.abx_conv3d_set_kernel(3)
         fl = 2 * n * d * h * w * ci * co * 27
         rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[1], 1), general_us=round(us[0], 1),
                          fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1)))

@@ -125,7 +125,7 @@ def test_conv_forward16_equals_general_kernel(co):
     bt = torch.as_tensor(rng.standard_normal(co).astype(np.float32)).to(_dev())
     outs = []
     try:
-        for k in (1, 0):
+        for k in (3, 2):   # bit 0: k_conv_fwd16; bit 1: transposed weight gradient (the default)
             ck(abx.load().abx_conv3d_set_kernel(k))
             y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device=_dev())
             ck(abx.load().abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(),
@@ -133,8 +133,40 @@ def test_conv_forward16_equals_general_kernel(co):
             outs.append(y)
         torch.cuda.synchronize()
     finally:
-        ck(abx.load().abx_conv3d_set_kernel(1))
+        ck(abx.load().abx_conv3d_set_kernel(3))
     assert torch.equal(outs[0].view(torch.int16), outs[1].view(torch.int16))
+
+
+@pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 3), (1, 6, 5, 20, 32, 16, 3), (1, 4, 8, 16, 16, 32, 3),
+                                  (1, 3, 7, 33, 32, 32, 3), (2, 5, 4, 17, 16, 32, 1)],
+                         ids=lambda c: "x".join(map(str, c)))
+def test_wgrad_transposed_staging_equals_gathered(case):
+    """The weight gradient with channel-major LDS staging (k_conv_wgrad_t,
+    the default) reads the same fragments as the 16-bit gathering kernel and
+    issues the same MFMAs in the same order: equal bit for bit, weights and
+    bias, on grids that are not multiples of the tile."""
+    n, d, h, w, ci, co, k = case
+    say("wgrad transposed vs gathered", case)
+    rng = np.random.default_rng(6)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    dzt = bf(rng.standard_normal((n, d, h, w, co)))
+    lib = abx.load()
+    ws_bytes = lib.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, k)
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=_dev())
+    outs = []
+    try:
+        for flags in (3, 1):
+            ck(lib.abx_conv3d_set_kernel(flags))
+            dw = torch.empty(co * k ** 3 * ci, dtype=torch.float32, device=_dev())
+            db = torch.empty(co, dtype=torch.float32, device=_dev())
+            ck(lib.abx_conv3d_wgrad(stream(), xt.data_ptr(), ci, dzt.data_ptr(), co, n, d, h, w, ci, co, k,
+                                    ws.data_ptr(), ws_bytes, dw.data_ptr(), db.data_ptr(), 0))
+            torch.cuda.synchronize()
+            outs.append((dw.cpu(), db.cpu()))
+    finally:
+        ck(lib.abx_conv3d_set_kernel(3))
+    assert torch.equal(outs[0][0].view(torch.int32), outs[1][0].view(torch.int32))
+    assert torch.equal(outs[0][1].view(torch.int32), outs[1][1].view(torch.int32))
 
 
 def test_conv_forward_slices_and_accumulate():
