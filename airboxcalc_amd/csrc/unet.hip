@@ -185,8 +185,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 
 // ---------------------------------------------------------------------------
 // Forward convolution, 3x3x3 taps, one 16-channel output tile per workgroup
-// (the layers 16 channels wide, where k_conv_fwd has one MFMA per weight
-// load).  A tile is 16 x 4 x 4 (x, y, z) output voxels; wave w computes
+// (every 3x3x3 layer: on the 16-channel layers k_conv_fwd has one MFMA per
+// weight load, and on the small deep levels its wide channel tiles leave too
+// few workgroups).  A tile is 16 x 4 x 4 (x, y, z) output voxels; wave w computes
 // z-slice w, four rows of 16 voxels.  Input channels are taken 16 at a time:
 // the k-steps of 32 then cover two taps each (k = 32 s + 8 lg: tap
 // 2 s + lg / 2, channels 8 (lg % 2) .. + 8), 14 steps for 27 taps (the last
@@ -784,8 +785,9 @@ unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t)
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-// 3x3x3 convolutions with a 16-channel output tile use k_conv_fwd16
-// (abx_conv3d_set_kernel switches back to k_conv_fwd, for comparison).
+// 3x3x3 convolutions use k_conv_fwd16, one 16-channel output tile per
+// workgroup, at every channel count (abx_conv3d_set_kernel switches back to
+// k_conv_fwd, for comparison).
 int g_fwd16 = 1;
 // Weight gradients use k_conv_wgrad_t (channel-major LDS staging); 0 = k_conv_wgrad.
 int g_wgrad_t = 1;
@@ -871,7 +873,7 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
   const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   const dim3 grid((unsigned)tiles_of(n, d, h, w), (unsigned)(co / (16 * nct)));
   hipStream_t s = (hipStream_t)stream;
-  if (ksize == 3 && nct == 1 && g_fwd16) {
+  if (ksize == 3 && g_fwd16) {
     const int64_t tiles = (int64_t)n * ((d + F16Z - 1) / F16Z) * ((h + F16Y - 1) / F16Y) * ((w + F16X - 1) / F16X);
     // persistent workgroups: about four per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)4 * device_cus() / (co / 16));

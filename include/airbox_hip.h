@@ -345,9 +345,9 @@ int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, in
                        uint16_t* y, int ys);
 
 /* Process-wide choice of convolution kernels, for comparison and timing
- * (no reference counterpart).  flags bit 0: 3x3x3 forward convolutions whose
- * output channels are not a multiple of 32 use the 16-channel kernel (16x4x4
- * voxel tiles, weights held in registers) instead of the general one; bit 1:
+ * (no reference counterpart).  flags bit 0: 3x3x3 forward convolutions use the
+ * 16-channel-tile kernel (16x4x4 voxel tiles, weights held in registers)
+ * instead of the general one; bit 1:
  * weight gradients with 16 or 32 output channels per workgroup stage their
  * tiles channel-major in LDS (16-byte fragment reads) instead of gathering
  * 16-bit values.  Default 3.  The choices compute the same sums. */

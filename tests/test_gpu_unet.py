@@ -111,7 +111,7 @@ def test_conv_forward_brute_force():
     close_bf16(f32(y), ref, "brute force")
 
 
-@pytest.mark.parametrize("co", [16, 48])
+@pytest.mark.parametrize("co", [16, 48, 64])
 def test_conv_forward16_equals_general_kernel(co):
     """The 16-channel forward kernel (k_conv_fwd16) and the general one take
     the same k-steps in the same order for 16 input channels: equal bit for
