@@ -147,6 +147,7 @@ def load() -> C.CDLL:
         "abx_sim_sync": (i32, [vp]),
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32, i32]),
         "abx_sim_set_pipeline": (i32, [vp, i32]),
+        "abx_lattice_set_row_shift": (i32, [i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
         "abx_sim_reset": (i32, [vp]),

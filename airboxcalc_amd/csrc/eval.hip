@@ -25,12 +25,12 @@ struct EvalDev {
   const float* ns;  // [plane] of the variant or null
   const uint32_t* fluid_bits;
   int64_t plane;
-  int nx, ny, nz, nyp, nzp;
+  int nx, ny, nz, nyp, nzp, kofs;
   double div;
 };
 
 __device__ __forceinline__ int64_t dpos(const EvalDev& e, int64_t i, int64_t j, int64_t k) {
-  return ((i + 1) * e.nyp + (j + 1)) * e.nzp + (k + 1);
+  return ((i + 1) * e.nyp + (j + 1)) * e.nzp + (k + e.kofs);
 }
 // fluid of evaluate: g.fluid minus cells with ns > 0.5 (engine.js:634)
 __device__ __forceinline__ bool efluid(const EvalDev& e, int64_t d) {
@@ -391,6 +391,7 @@ EvalDev eval_dev(abx_sim* s, int b, double div) {
   e.nz = v.nz;
   e.nyp = v.nyp;
   e.nzp = v.nzp;
+  e.kofs = v.kofs;
   e.div = div;
   return e;
 }

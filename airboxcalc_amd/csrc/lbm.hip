@@ -353,11 +353,12 @@ struct ResampleArgs {
   float a[4], b[4];
 };
 
-__global__ void k_resample_rho_u(ResampleArgs r, const int32_t* cells, int64_t NF, int nyp, int nzp, float* rho_u) {
+__global__ void k_resample_rho_u(ResampleArgs r, const int32_t* cells, int64_t NF, int nyp, int nzp, int kofs,
+                                 float* rho_u) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= NF) return;
   const int d = cells[e];
-  const int k = d % nzp - 1, t = d / nzp, j = t % nyp - 1, i = t / nyp - 1;
+  const int k = d % nzp - kofs, t = d / nzp, j = t % nyp - 1, i = t / nyp - 1;
   const int n3[3] = {r.nxc, r.nyc, r.nzc};
   const int idx[3] = {i, j, k};
   int i0[3];
@@ -419,6 +420,8 @@ struct abx_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int prec = ABX_F32, B = 1, xcd_map = -1;  // default launch: one band of rows j per XCD
   int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
+  // position of cell k within its row: k + kofs (kofs - 1 halo positions in front)
+  int kofs = 1;
   int64_t ntot = 0, plane = 0, NF = 0;
   size_t ssz = 4;  // bytes per stored value
   void* f[2] = {nullptr, nullptr};
@@ -463,6 +466,45 @@ struct abx_sim {
 };
 
 namespace {
+
+// Row layout of new simulations (abx_lattice_set_row_shift).
+int g_row_shift = 1;
+
+// Offset kofs in [1, 32] of cell k = 0 within its lattice row that makes the
+// fewest 128-byte lines (32 positions) hold fluid: the stream-collide reads and
+// writes every such line whole, so at the ragged ends of the fluid runs a
+// line-aligned run start saves a partly used line.  Ties go to the shorter row.
+int best_row_offset(const uint8_t* fluid, int nx, int ny, int nz) {
+  int64_t cnt[32] = {0};
+  for (int64_t r = 0; r < (int64_t)nx * ny; ++r) {
+    const uint8_t* f = fluid + r * nz;
+    int last[32];
+    for (int t = 0; t < 32; ++t) last[t] = -1;
+    int k = 0;
+    while (k < nz) {  // fluid runs [k0, k1] in increasing k
+      if (!f[k]) {
+        ++k;
+        continue;
+      }
+      const int k0 = k;
+      while (k < nz && f[k]) ++k;
+      const int k1 = k - 1;
+      for (int t = 0; t < 32; ++t) {
+        int lo = (k0 + 1 + t) >> 5;
+        const int hi = (k1 + 1 + t) >> 5;
+        if (lo == last[t]) ++lo;
+        if (hi >= lo) cnt[t] += hi - lo + 1;
+        last[t] = hi;
+      }
+    }
+  }
+  int best = 0;
+  for (int t = 1; t < 32; ++t) {
+    const int64_t rt = (nz + t + 2 + 31) / 32, rb = (nz + best + 2 + 31) / 32;
+    if (cnt[t] < cnt[best] || (cnt[t] == cnt[best] && rt < rb)) best = t;
+  }
+  return 1 + best;
+}
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -911,7 +953,8 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->nz = nz;
   s->nxp = nx + 2;
   s->nyp = ny + 2;
-  s->nzp = ((nz + 2 + 31) / 32) * 32;
+  s->kofs = g_row_shift ? best_row_offset(g->fluid, nx, ny, nz) : 1;
+  s->nzp = ((nz + s->kofs + 1 + 31) / 32) * 32;
   s->ntot = (int64_t)s->nxp * s->nyp * s->nzp;
   s->plane = ((s->ntot + CHUNK - 1) / CHUNK) * CHUNK;
   if ((int64_t)s->plane * 19 * B > ((int64_t)1 << 40) || s->plane + (int64_t)(s->nyp + 2) * s->nzp > INT32_MAX) {
@@ -920,7 +963,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     return -1;
   }
   const int64_t N = (int64_t)nx * ny * nz;
-  auto D = [&](int64_t i, int64_t j, int64_t k) { return ((i + 1) * s->nyp + (j + 1)) * s->nzp + (k + 1); };
+  auto D = [&](int64_t i, int64_t j, int64_t k) { return ((i + 1) * s->nyp + (j + 1)) * s->nzp + (k + s->kofs); };
 
   // setupSim (engine.js:239-264): compact order, flags, Lp, Ain, layers, sigma
   const float* dc = g->dc;
@@ -1336,7 +1379,7 @@ extern "C" int abx_sim_init_eq_resample(abx_sim* s, const uint16_t* vol, int cst
   float* d = nullptr;
   ABX_HIP(hipMalloc((void**)&d, (size_t)(4 * NF) * sizeof(float)));
   hipLaunchKernelGGL(k_resample_rho_u, dim3(nblk(NF, 256)), dim3(256), 0, s->stream, r, s->cells, NF, s->nyp, s->nzp,
-                     d);
+                     s->kofs, d);
   const hipError_t le = hipGetLastError();
   int rc = 0;
   if (le != hipSuccess) {
@@ -1425,6 +1468,11 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk
   return 0;
 }
 
+extern "C" int abx_lattice_set_row_shift(int enable) {
+  g_row_shift = enable ? 1 : 0;
+  return 0;
+}
+
 extern "C" int abx_sim_set_pipeline(abx_sim* s, int planes) {
   if (!s || planes < 0 || planes > 4096) {
     set_error("abx_sim_set_pipeline: bad argument (planes 0..4096)");
@@ -1467,6 +1515,7 @@ SimView sim_view(abx_sim* s) {
   v.nz = s->nz;
   v.nyp = s->nyp;
   v.nzp = s->nzp;
+  v.kofs = s->kofs;
   v.plane = s->plane;
   v.ssz = s->ssz;
   v.acc = s->acc;

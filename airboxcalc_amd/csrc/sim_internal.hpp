@@ -12,6 +12,7 @@ struct SimView {
   hipStream_t stream;
   int B;
   int nx, ny, nz, nyp, nzp;
+  int kofs;                // position of cell k within its row: k + kofs
   int64_t plane;
   size_t ssz;              // bytes per stored value (4 or 8)
   const void* acc;         // [B][4][plane]

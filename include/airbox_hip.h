@@ -239,6 +239,11 @@ int abx_sim_sync(abx_sim* sim);
  * counts) and sweeps it plane by plane.  Every variant computes the same
  * values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
+/* Row layout of simulations created afterwards (process-wide; no reference
+ * counterpart): enable = 1 (the default) starts each lattice row at the
+ * offset that makes the fewest 128-byte lines hold fluid, 0 puts cell k = 0
+ * right after a one-position halo.  Results are the same bit for bit. */
+int abx_lattice_set_row_shift(int enable);
 /* Plane-pipelined AA pairs (kernel 4; no reference counterpart): planes > 0
  * issues each pair of steps as launches over bands of `planes` lattice
  * planes, the second step trailing the first by a little more than one

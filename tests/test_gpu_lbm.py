@@ -250,6 +250,42 @@ def test_plane_pipelined_pairs_equal_one_launch_per_step(precision):
             assert np.array_equal(got[b][1], want[b][1]), (pipe, graph, xm, b)
 
 
+@pytest.mark.parametrize("name", ["small_yaw", "c1_seed3"])
+def test_row_offset_layout_gives_the_same_results(name):
+    """Lattice rows placed at the line-saving offset (the default) and right
+    after a one-position halo: the same populations, averages, probe and
+    evaluate/field3d/streamlines results, bit for bit."""
+    fx = GU.load(name)
+    say("row offset layout", name)
+    g = gpu_geometry(fx)
+    r = fx["desc"]["run"]
+    lib = abx.load()
+    out = []
+    try:
+        for shift in (1, 0):
+            assert lib.abx_lattice_set_row_shift(shift) == 0
+            s = abx.Sim(dev(), g.grid(), r["K"], r["aniso"], r["u"], r["tau"], precision="f32")
+            rng = np.random.default_rng(12)
+            f = (np.repeat([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, s.NF) *
+                 rng.uniform(0.99, 1.01, 19 * s.NF)).astype(np.float32)
+            s.write_f(f)
+            s.steps(5)
+            s.steps(6, doacc=True)
+            ev = s.evaluate(g.eval_view(), divisor=6.0)
+            sl = s.streamlines(g.eval_view(), divisor=6.0)
+            out.append((s.read_f(), s.read_acc(), s.probe(11), ev, sl))
+            s.destroy()
+    finally:
+        lib.abx_lattice_set_row_shift(1)
+    (f1, a1, p1, e1, s1), (f0, a0, p0, e0, s0) = out
+    assert np.array_equal(f1, f0) and np.array_equal(a1, a0) and p1 == p0
+    assert e1["metrics"] == e0["metrics"]
+    for k in e1["vol"]:
+        assert np.array_equal(e1["vol"][k], e0["vol"][k], equal_nan=True), k
+    for k in s1:
+        assert np.array_equal(np.asarray(s1[k]), np.asarray(s0[k])), k
+
+
 @pytest.mark.parametrize("name", ["small", "small_yaw", "c1_seed3"])
 def test_gpu_evaluate_field3d_streamlines(name):
     """evaluate/field3d/streamlines on the GPU against engine.js's outputs on
