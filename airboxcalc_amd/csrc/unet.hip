@@ -429,28 +429,36 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
   if (do_bias && threadIdx.x < CO) partb[(int64_t)blockIdx.x * Co + co0 + threadIdx.x] = bsum;
 }
 
-// Weight gradient with transposed staging: the same partial sums as
-// k_conv_wgrad, but the dZ tile and the input brick are stored channel-major
-// in LDS (one row of voxels per channel), so every MFMA fragment (8
-// consecutive voxels of one channel) is one 16-byte read instead of eight
-// 16-bit reads.  A fragment shifted by one or two voxels in x (taps with
-// dx = 1, 2) is assembled from two aligned reads with v_alignbyte_b32 or a
-// dword shift.  Row and channel pitches are padded by 16 bytes so the 16 lanes
-// of a fragment read hit distinct banks.
+// Weight gradient on gfx950's transposing LDS reads: the partial sums of
+// k_conv_wgrad from the same voxel-major (NDHWC) staging, filled with 16-byte
+// copies.  An MFMA fragment holds 8 consecutive voxels (k) of one channel.
+// ds_read_b64_tr_b16 builds it from rows of the voxel-major tile: in each
+// group of 16 lanes, lane s points at 4 consecutive channels (columns
+// 4 (s & 3) .. +3) of row s / 4, and lane i receives column i of those 4 rows
+// (4 voxels of channel i).  Two reads give a lane its 8 voxels; the x shift of
+// a tap is a row offset.  Rows are not padded: padding them by 16 or 32 bytes
+// measured no faster.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+__device__ __forceinline__ u32x4 frag_tr(const u16* p, int pitch) {
+  const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(void*)p);
+  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(void*)(p + 4 * pitch));
+  const u32x2 l = __builtin_bit_cast(u32x2, lo), h = __builtin_bit_cast(u32x2, hi);
+  return (u32x4){l.x, l.y, h.x, h.y};
+}
+
 template <int TAPS, int NCT, int NCI>
-__global__ __launch_bounds__(256) void k_conv_wgrad_t(const u16* __restrict__ x, int xs, const u16* __restrict__ dz,
-                                                      int dzs, float* __restrict__ part, float* __restrict__ partb,
-                                                      int D, int H, int W, int Ci, int Co, int ntiles, int tg) {
+__global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x, int xs, const u16* __restrict__ dz,
+                                                       int dzs, float* __restrict__ part, float* __restrict__ partb,
+                                                       int D, int H, int W, int Ci, int Co, int ntiles, int tg) {
   using Bk = Brick<TAPS>;
   constexpr int CO = 16 * NCT, CI = 16 * NCI, MAXW = 8;
-  constexpr int ZP = TV + 8;                     // dZ row pitch (voxels of the tile)
-  constexpr int XR = Bk::X <= 16 ? 16 : 24;      // brick row pitch: room for a 16-voxel read at x = 8
-  constexpr int ROWS = Bk::Y * Bk::Z;
-  constexpr int CIP = ROWS * XR + 8;             // brick channel pitch
+  constexpr int PZ = CO, PX = CI;  // row pitches of the dZ tile and the brick
   constexpr int ZITEMS = TV * NCT * 2, XITEMS = Bk::V * NCI * 2;
   constexpr int ZSL = (ZITEMS + 255) / 256, XSL = (XITEMS + 255) / 256;
-  __shared__ __attribute__((aligned(16))) u16 szT[CO * ZP];
-  __shared__ __attribute__((aligned(16))) u16 sxT[CI * CIP];
+  __shared__ __attribute__((aligned(16))) u16 sz[TV * PZ];
+  __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * PX];
   const int nci = Ci / CI, ngr = (TAPS + tg - 1) / tg;
   int r = blockIdx.y;
   const int g = r % ngr;
@@ -463,16 +471,15 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_t(const u16* __restrict__ x,
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-  // this wave's (tap, channel tile) items: brick offset of the tap's row and
-  // input-channel tile, x shift, output-channel tile (wave-uniform)
-  int boff[MAXW], bdx[MAXW], bct[MAXW];
+  // this wave's (tap, channel tile) items: brick offset of the tap plus the
+  // input-channel tile's column, output-channel tile (wave-uniform)
+  int boff[MAXW], bct[MAXW];
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) {
     const int q = wave + 4 * u, qq = q < nq ? q : 0;
     const int tap = t0 + qq / (NCT * NCI), it = qq % NCI;
-    const int dzb = TAPS == 27 ? tap / 9 : 0, dyb = TAPS == 27 ? (tap / 3) % 3 : 0;
-    boff[u] = it * 16 * CIP + (dzb * Bk::Y + dyb) * XR;
-    bdx[u] = TAPS == 27 ? tap % 3 : 0;
+    const int dzb = TAPS == 27 ? tap / 9 : 0, dyb = TAPS == 27 ? (tap / 3) % 3 : 0, dxb = TAPS == 27 ? tap % 3 : 0;
+    boff[u] = ((dzb * Bk::Y + dyb) * Bk::X + dxb) * PX + it * 16;
     bct[u] = (qq / NCI) % NCT;
   }
   // this thread's staging slots (items e = thread + 256 m), decoded once
@@ -489,6 +496,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_t(const u16* __restrict__ x,
     const int bx = vb % Bk::X, row = vb / Bk::X, by = row % Bk::Y, bz = row / Bk::Y;
     xsl[m] = e < XITEMS ? (bx | by << 5 | bz << 8 | q << 11) : -1;
   }
+  // this lane's place in a transposing read: row lr / 4, columns 4 (lr % 4) ..
+  const int lrow = lr >> 2, lcol = 4 * (lr & 3);
   f32x4 acc[MAXW];
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) acc[u] = (f32x4){0.f, 0.f, 0.f, 0.f};
@@ -497,42 +506,30 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_t(const u16* __restrict__ x,
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tile_of(t, D, H, W);
     __syncthreads();
-    // dZ tile, channel-major: szT[co * ZP + v] (zero outside the grid)
+    // dZ tile: sz[v * PZ + co] (zero outside the grid)
 #pragma unroll
     for (int m = 0; m < ZSL; ++m) {
       if (zs[m] < 0) continue;
       const int v = zs[m] / (NCT * 2), q = zs[m] - v * (NCT * 2);
       const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
-      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (gx < W && gy < H && gz < D)
-        val = __builtin_bit_cast(u16x8, *(const u32x4*)(dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q));
-      u16* dst = szT + 8 * q * ZP + v;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) dst[c * ZP] = val[c];
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (gx < W && gy < H && gz < D) val = *(const u32x4*)(dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q);
+      *(u32x4*)(sz + v * PZ + 8 * q) = val;
     }
-    // input brick, channel-major: sxT[ci * CIP + row * XR + bx], row = bz * Y + by
+    // input brick: sx[brick voxel * PX + ci]
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
       if (xsl[m] < 0) continue;
       const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
       const int gx = tl.x0 + bx - Bk::HALO, gy = tl.y0 + by - Bk::HALO, gz = tl.z0 + bz - Bk::HALO;
-      u16x8 val = {0, 0, 0, 0, 0, 0, 0, 0};
+      u32x4 val = {0u, 0u, 0u, 0u};
       if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
-        val = __builtin_bit_cast(u16x8, *(const u32x4*)(x + vox(tl, D, H, W, gz, gy, gx) * xs + ci0 + 8 * q));
-      u16* dst = sxT + 8 * q * CIP + (bz * Bk::Y + by) * XR + bx;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) dst[c * CIP] = val[c];
+        val = *(const u32x4*)(x + vox(tl, D, H, W, gz, gy, gx) * xs + ci0 + 8 * q);
+      *(u32x4*)(sx + ((bz * Bk::Y + by) * Bk::X + bx) * PX + 8 * q) = val;
     }
     __syncthreads();
-    if (do_bias && threadIdx.x < CO) {
-      // voxels in order, as k_conv_wgrad sums them
-      const u32x4* row = (const u32x4*)(szT + threadIdx.x * ZP);
-      for (int v8 = 0; v8 < TV / 8; ++v8) {
-        const u16x8 h = __builtin_bit_cast(u16x8, row[v8]);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) bsum += bf2f(h[e]);
-      }
-    }
+    if (do_bias && threadIdx.x < CO)
+      for (int v = 0; v < TV; ++v) bsum += bf2f(sz[v * PZ + threadIdx.x]);
 #pragma unroll 1
     for (int s = 0; s < TV / 32; ++s) {
       // k = voxels 32s + 8lg + e (e = 0..7), consecutive in x
@@ -540,22 +537,12 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_t(const u16* __restrict__ x,
       const int tx0 = vb0 & 15, ty = (vb0 >> 4) & 3, tz = vb0 >> 6;
       u32x4 av[NCT];
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) av[ct] = *(const u32x4*)(szT + (ct * 16 + lr) * ZP + vb0);
-      const u16* lbase = sxT + lr * CIP + (tz * Bk::Y + ty) * XR + tx0;
+      for (int ct = 0; ct < NCT; ++ct) av[ct] = frag_tr(sz + (vb0 + lrow) * PZ + ct * 16 + lcol, PZ);
+      const u16* lbase = sx + (((tz * Bk::Y + ty) * Bk::X + tx0) + lrow) * PX + lcol;
 #pragma unroll
       for (int u = 0; u < MAXW; ++u) {
         if (wave + 4 * u >= nq) break;
-        const u16* bp = lbase + boff[u];
-        const u32x4 lo = *(const u32x4*)bp;
-        u32x4 b = lo;
-        if (bdx[u] != 0) {
-          const u32x4 hi = *(const u32x4*)(bp + 8);
-          if (bdx[u] == 1)
-            b = (u32x4){__builtin_amdgcn_alignbyte(lo.y, lo.x, 2), __builtin_amdgcn_alignbyte(lo.z, lo.y, 2),
-                        __builtin_amdgcn_alignbyte(lo.w, lo.z, 2), __builtin_amdgcn_alignbyte(hi.x, lo.w, 2)};
-          else
-            b = (u32x4){lo.y, lo.z, lo.w, hi.x};
-        }
+        const u32x4 b = frag_tr(lbase + boff[u], PX);
         u32x4 avc = av[0];
 #pragma unroll
         for (int c = 1; c < NCT; ++c)
@@ -789,8 +776,9 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 // workgroup, at every channel count (abx_conv3d_set_kernel switches back to
 // k_conv_fwd, for comparison).
 int g_fwd16 = 1;
-// Weight gradients use k_conv_wgrad_t (channel-major LDS staging); 0 = k_conv_wgrad.
-int g_wgrad_t = 1;
+// Weight gradients use k_conv_wgrad_tr (transposing LDS reads); 0 = the
+// gathering kernel k_conv_wgrad, for comparison.
+int g_wgrad_tr = 1;
 
 // Compute units of the current device (cached per device ordinal).
 int device_cus() {
@@ -908,7 +896,7 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
 
 extern "C" int abx_conv3d_set_kernel(int flags) {
   g_fwd16 = (flags & 1) ? 1 : 0;
-  g_wgrad_t = (flags & 2) ? 1 : 0;
+  g_wgrad_tr = (flags & 2) ? 1 : 0;
   return 0;
 }
 
@@ -941,34 +929,29 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   const int nt = (int)tiles_of(n, d, h, w);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sh.S, (unsigned)sh.blocks_y);
-#define ABX_WG_G(T_, A, B)                                                                                      \
-  hipLaunchKernelGGL((k_conv_wgrad<T_, A, B>), grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, \
-                     d, h, w, ci, co, nt, sh.tg)
-  // channel-major staging for 16 and 32 output channels per workgroup; with 64
-  // its LDS and registers leave too few waves, and gathering measured faster
-#define ABX_WG(T_, A, B)                                                                                          \
-  do {                                                                                                            \
-    if (g_wgrad_t)                                                                                                \
-      hipLaunchKernelGGL((k_conv_wgrad_t<T_, A, B>), grid, dim3(256), 0, s, x, xs, dz, dzs, part,                \
-                         db ? partb : nullptr, d, h, w, ci, co, nt, sh.tg);                                       \
-    else                                                                                                          \
-      ABX_WG_G(T_, A, B);                                                                                         \
-  } while (0)
-#define ABX_WG_T(T_)                          \
-  if (sh.nct == 4 && sh.nci == 2) ABX_WG_G(T_, 4, 2); \
-  else if (sh.nct == 4) ABX_WG_G(T_, 4, 1);   \
-  else if (sh.nct == 2 && sh.nci == 2) ABX_WG(T_, 2, 2); \
-  else if (sh.nct == 2) ABX_WG(T_, 2, 1);     \
-  else if (sh.nci == 2) ABX_WG(T_, 1, 2);     \
-  else ABX_WG(T_, 1, 1);
-  if (T == 27) {
-    ABX_WG_T(27)
+#define ABX_WG(K_, T_, A, B)                                                                                   \
+  hipLaunchKernelGGL((K_<T_, A, B>), grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, d, h, w, ci, \
+                     co, nt, sh.tg)
+#define ABX_WG_T(K_, T_)                                \
+  if (sh.nct == 4 && sh.nci == 2) ABX_WG(K_, T_, 4, 2); \
+  else if (sh.nct == 4) ABX_WG(K_, T_, 4, 1);           \
+  else if (sh.nct == 2 && sh.nci == 2) ABX_WG(K_, T_, 2, 2); \
+  else if (sh.nct == 2) ABX_WG(K_, T_, 2, 1);           \
+  else if (sh.nci == 2) ABX_WG(K_, T_, 1, 2);           \
+  else ABX_WG(K_, T_, 1, 1);
+  if (g_wgrad_tr) {
+    if (T == 27) {
+      ABX_WG_T(k_conv_wgrad_tr, 27)
+    } else {
+      ABX_WG_T(k_conv_wgrad_tr, 1)
+    }
+  } else if (T == 27) {
+    ABX_WG_T(k_conv_wgrad, 27)
   } else {
-    ABX_WG_T(1)
+    ABX_WG_T(k_conv_wgrad, 1)
   }
 #undef ABX_WG_T
 #undef ABX_WG
-#undef ABX_WG_G
   ABX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw, 64)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
   if (db)

@@ -202,9 +202,10 @@ def bench_unet(a, dev):
 
 
 def bench_conv(a, dev):
-    """The surrogate's 16-channel 3x3x3 forward convolutions (batch 8, 96 x 48
-    x 64): the 16-channel kernel against the general one, each shape timed
-    with HIP events over --steps calls."""
+    """The surrogate's 16-channel 3x3x3 convolutions (batch 8, 96 x 48 x 64):
+    forward, the 16-channel kernel against the general one, and the weight
+    gradient, transposing LDS reads against gathering; each timed with HIP
+    events over --steps calls."""
     import torch
     from airboxcalc_amd import abx
     L = abx.load()
@@ -236,16 +237,42 @@ def bench_conv(a, dev):
                 us[k] = e0.elapsed_time(e1) / calls * 1e3
         finally:
             L.abx_conv3d_set_kernel(3)
+        # weight gradient of the same shape: transposing LDS reads (bit 1) against gathering
+        dz = torch.randn((n, d, h, w, co), generator=g).to(torch.bfloat16).cuda()
+        wsb = L.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
+        ws = torch.empty(wsb // 4, dtype=torch.float32, device="cuda")
+        dw = torch.empty(co * 27 * ci, dtype=torch.float32, device="cuda")
+        db = torch.empty(co, dtype=torch.float32, device="cuda")
+        wg = {}
+        try:
+            for k in (1, 3):
+                L.abx_conv3d_set_kernel(k)
+                for _ in range(3):
+                    assert L.abx_conv3d_wgrad(s, x.data_ptr(), ci, dz.data_ptr(), co, n, d, h, w, ci, co, 3,
+                                              ws.data_ptr(), wsb, dw.data_ptr(), db.data_ptr(), 0) == 0
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(calls):
+                    L.abx_conv3d_wgrad(s, x.data_ptr(), ci, dz.data_ptr(), co, n, d, h, w, ci, co, 3, ws.data_ptr(),
+                                       wsb, dw.data_ptr(), db.data_ptr(), 0)
+                e1.record()
+                torch.cuda.synchronize()
+                wg[k] = e0.elapsed_time(e1) / calls * 1e3
+        finally:
+            L.abx_conv3d_set_kernel(3)
         fl = 2 * n * d * h * w * ci * co * 27
         rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[1], 1), general_us=round(us[0], 1),
-                         fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1)))
+                         fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1),
+                         wgrad_tr_us=round(wg[3], 1), wgrad_gather_us=round(wg[1], 1),
+                         wgrad_tr_tflops=round(fl / wg[3] / 1e6, 1)))
     r0 = rows[0]
     out = {
         "metric": "3x3x3 bf16 convolution TFLOP/s (16 -> 16 channels, 96x48x64, batch 8)",
         "value": r0["fwd16_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": calls, "warmup": 3,
         "ms_per_step": round(r0["fwd16_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random volumes)",
-        "config": {"workload": "the surrogate's 16-channel forward convolutions, k_conv_fwd16 against k_conv_fwd",
+        "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16 against k_conv_fwd) "
+                               "and weight gradient (k_conv_wgrad_tr against k_conv_wgrad)",
                    "shapes": rows},
         "roofline": {"bound": "mfma", "achieved": r0["fwd16_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(r0["fwd16_tflops"] / 2500.0, 4), "traffic": None, "kernel": "k_conv_fwd16"},

@@ -352,10 +352,9 @@ int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, in
 /* Process-wide choice of convolution kernels, for comparison and timing
  * (no reference counterpart).  flags bit 0: 3x3x3 forward convolutions use the
  * 16-channel-tile kernel (16x4x4 voxel tiles, weights held in registers)
- * instead of the general one; bit 1:
- * weight gradients with 16 or 32 output channels per workgroup stage their
- * tiles channel-major in LDS (16-byte fragment reads) instead of gathering
- * 16-bit values.  Default 3.  The choices compute the same sums. */
+ * instead of the general one; bit 1: weight gradients build their MFMA
+ * fragments with gfx950's transposing LDS reads (ds_read_b64_tr_b16) instead
+ * of gathering 16-bit values.  Default 3.  The choices compute the same sums. */
 int abx_conv3d_set_kernel(int flags);
 
 /* Bytes of FP32 workspace abx_conv3d_wgrad needs for this shape (-1 on a bad

@@ -138,15 +138,17 @@ def test_conv_forward16_equals_general_kernel(co):
 
 
 @pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 3), (1, 6, 5, 20, 32, 16, 3), (1, 4, 8, 16, 16, 32, 3),
-                                  (1, 3, 7, 33, 32, 32, 3), (2, 5, 4, 17, 16, 32, 1)],
+                                  (1, 3, 7, 33, 32, 32, 3), (2, 5, 4, 17, 16, 32, 1), (1, 3, 6, 18, 32, 64, 3),
+                                  (1, 3, 5, 16, 16, 64, 3), (1, 2, 3, 16, 32, 64, 1)],
                          ids=lambda c: "x".join(map(str, c)))
-def test_wgrad_transposed_staging_equals_gathered(case):
-    """The weight gradient with channel-major LDS staging (k_conv_wgrad_t,
-    the default) reads the same fragments as the 16-bit gathering kernel and
+def test_wgrad_transposed_reads_equal_gathered(case):
+    """The weight gradient on transposing LDS reads (k_conv_wgrad_tr, the
+    default) builds the same fragments as the 16-bit gathering kernel and
     issues the same MFMAs in the same order: equal bit for bit, weights and
-    bias, on grids that are not multiples of the tile."""
+    bias, on grids that are not multiples of the tile, for 16, 32 and 64
+    output channels per workgroup."""
     n, d, h, w, ci, co, k = case
-    say("wgrad transposed vs gathered", case)
+    say("wgrad transposed reads vs gathered", case)
     rng = np.random.default_rng(6)
     xt = bf(rng.standard_normal((n, d, h, w, ci)))
     dzt = bf(rng.standard_normal((n, d, h, w, co)))
