@@ -340,7 +340,30 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
 // output channels [co0, co0+16*NCT), input channels [ci0, ci0+16*NCI) and taps
 // [t0, t0+tg); its tg*NCT*NCI tiles of 16x16 are dealt to the 4 waves, at most
 // 8 each.  part: fp32 [S][Co][TAPS][Ci]; partb: fp32 [S][Co] (bias gradient,
-// written by the workgroups of ci chunk 0 and tap group 0).
+// written by the workgroups of ci chunk 0 and the last tap group).
+// Bias gradient on the matrix cores: in the workgroups of input-channel chunk
+// 0 and the last tap group (the group with the fewest taps), wave
+// 4 - NCT + c multiplies the dZ fragment of output-channel tile c by a
+// fragment of bf16 ones, so every column of its accumulator sums dZ over the
+// voxels.  Waves are dealt items w, w + 4, ..., so the last waves have the
+// fewest and the extra MFMA does not lengthen the longest wave.
+template <int NCT>
+__device__ __forceinline__ f32x4 bias_mfma(const u32x4 (&av)[NCT], int ct, f32x4 accb) {
+  const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
+  u32x4 a = av[0];
+#pragma unroll
+  for (int c = 1; c < NCT; ++c)
+    if (ct == c) a = av[c];
+  return mfma16(a, ones, accb);
+}
+
+// Column 0 of the bias accumulator: element i of lane group lg is output
+// channel 4 lg + i of the tile.
+__device__ __forceinline__ void bias_store(float* pb, f32x4 accb, int lg) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) pb[4 * lg + i] = accb[i];
+}
+
 template <int TAPS, int NCT, int NCI>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, int xs, const u16* __restrict__ dz,
                                                     int dzs, float* __restrict__ part, float* __restrict__ partb,
@@ -357,12 +380,15 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
   const int co0 = cb * CO, ci0 = cc * CI, t0 = g * tg;
   const int ntap = min(tg, TAPS - t0);
   const int nq = ntap * NCT * NCI;
-  const bool do_bias = partb && cc == 0 && g == 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  const bool do_bias = partb && cc == 0 && g == ngr - 1;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int bwt = wave - (4 - NCT);  // bias: output-channel tile of this wave
+  const bool bias_wave = do_bias && bwt >= 0;
   f32x4 acc[MAXW];
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) acc[u] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  float bsum = 0.f;
+  f32x4 accb = {0.f, 0.f, 0.f, 0.f};
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tl = tile_of(t, D, H, W);
@@ -377,8 +403,6 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
     }
     load_brick<TAPS>(sx, SXP, x, xs, tl, D, H, W, ci0, CI);
     __syncthreads();
-    if (do_bias && threadIdx.x < CO)
-      for (int v = 0; v < TV; ++v) bsum += bf2f(sz[v * SZP + threadIdx.x]);
 #pragma unroll 1
     for (int s = 0; s < TV / 32; ++s) {
       // k = voxels 32s + 8lg + e (e = 0..7), consecutive in x
@@ -409,6 +433,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
           if (ct == c) avc = av[c];
         acc[u] = mfma16(avc, __builtin_bit_cast(u32x4, b), acc[u]);
       }
+      // after the items, where the fragments have arrived (no wait at the branch)
+      if (bias_wave) accb = bias_mfma<NCT>(av, bwt, accb);
     }
   }
 
@@ -426,7 +452,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
       pp[((int64_t)co * TAPS + tap) * Ci + ci] = acc[u][i];
     }
   }
-  if (do_bias && threadIdx.x < CO) partb[(int64_t)blockIdx.x * Co + co0 + threadIdx.x] = bsum;
+  if (bias_wave && lr == 0) bias_store(partb + (int64_t)blockIdx.x * Co + co0 + bwt * 16, accb, lg);
 }
 
 // Weight gradient on gfx950's transposing LDS reads: the partial sums of
@@ -448,12 +474,22 @@ __device__ __forceinline__ u32x4 frag_tr(const u16* p, int pitch) {
   return (u32x4){l.x, l.y, h.x, h.y};
 }
 
+// Taps per workgroup of the weight gradient: at most 32 (tap, channel tile)
+// items, 8 per wave.
+__host__ __device__ constexpr int wgrad_tg(int taps, int nct, int nci) {
+  return taps < 32 / (nct * nci) ? taps : 32 / (nct * nci);
+}
+
 template <int TAPS, int NCT, int NCI>
 __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x, int xs, const u16* __restrict__ dz,
                                                        int dzs, float* __restrict__ part, float* __restrict__ partb,
                                                        int D, int H, int W, int Ci, int Co, int ntiles, int tg) {
   using Bk = Brick<TAPS>;
-  constexpr int CO = 16 * NCT, CI = 16 * NCI, MAXW = 8;
+  constexpr int CO = 16 * NCT, CI = 16 * NCI;
+  // items per wave, a compile-time count: the loop has no branches, so the
+  // LDS reads of later items are issued ahead of the MFMAs of earlier ones; a
+  // wave with fewer items computes the rest on item 0's operands and drops them
+  constexpr int MAXW = (wgrad_tg(TAPS, NCT, NCI) * NCT * NCI + 3) / 4;
   constexpr int PZ = CO, PX = CI;  // row pitches of the dZ tile and the brick
   constexpr int ZITEMS = TV * NCT * 2, XITEMS = Bk::V * NCI * 2;
   constexpr int ZSL = (ZITEMS + 255) / 256, XSL = (XITEMS + 255) / 256;
@@ -467,7 +503,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
   const int co0 = cb * CO, ci0 = cc * CI, t0 = g * tg;
   const int ntap = min(tg, TAPS - t0);
   const int nq = ntap * NCT * NCI;
-  const bool do_bias = partb && cc == 0 && g == 0;
+  const bool do_bias = partb && cc == 0 && g == ngr - 1;
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
@@ -482,54 +518,73 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
     boff[u] = ((dzb * Bk::Y + dyb) * Bk::X + dxb) * PX + it * 16;
     bct[u] = (qq / NCI) % NCT;
   }
-  // this thread's staging slots (items e = thread + 256 m), decoded once
+  // this thread's staging slots (items e = thread + 256 m), decoded once; a
+  // slot past the last item loads the last item and is not stored
   int zs[ZSL], xsl[XSL];
 #pragma unroll
-  for (int m = 0; m < ZSL; ++m) {
-    const int e = threadIdx.x + 256 * m;
-    zs[m] = e < ZITEMS ? e : -1;  // voxel e / (2 NCT), channel group e % (2 NCT)
-  }
+  for (int m = 0; m < ZSL; ++m) zs[m] = threadIdx.x + 256 * m;  // voxel e / (2 NCT), channel group e % (2 NCT)
 #pragma unroll
   for (int m = 0; m < XSL; ++m) {
-    const int e = threadIdx.x + 256 * m;
+    const int e = min((int)threadIdx.x + 256 * m, XITEMS - 1);
     const int vb = e / (NCI * 2), q = e - vb * (NCI * 2);
     const int bx = vb % Bk::X, row = vb / Bk::X, by = row % Bk::Y, bz = row / Bk::Y;
-    xsl[m] = e < XITEMS ? (bx | by << 5 | bz << 8 | q << 11) : -1;
+    xsl[m] = bx | by << 5 | bz << 8 | q << 11;
   }
+  const bool xlast = (int)threadIdx.x + 256 * (XSL - 1) < XITEMS;
+  // staged values of one tile and their in-grid bits (bit m)
+  u32x4 zv[ZSL], xv[XSL];
+  uint32_t zok = 0, xok = 0;
+  // every load from an address inside the grid (clamped); out-of-grid values
+  // are zeroed when they are stored, so no wait sits between the loads.
+  // (Loading the next tile while computing this one measured slower: its
+  // registers cost occupancy.)
+  auto fetch = [&](int t) {
+    const Tile tl = tile_of(t, D, H, W);
+    zok = 0;
+    xok = 0;
+#pragma unroll
+    for (int m = 0; m < ZSL; ++m) {
+      const int v = zs[m] / (NCT * 2), q = zs[m] - v * (NCT * 2);
+      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
+      zok |= (uint32_t)(gx < W && gy < H && gz < D) << m;
+      zv[m] = *(const u32x4*)(dz + vox(tl, D, H, W, min(gz, D - 1), min(gy, H - 1), min(gx, W - 1)) * dzs + co0 +
+                              8 * q);
+    }
+#pragma unroll
+    for (int m = 0; m < XSL; ++m) {
+      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
+      const int gx = tl.x0 + bx - Bk::HALO, gy = tl.y0 + by - Bk::HALO, gz = tl.z0 + bz - Bk::HALO;
+      xok |= (uint32_t)((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) << m;
+      xv[m] = *(const u32x4*)(x + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
+                                      min(max(gx, 0), W - 1)) * xs + ci0 + 8 * q);
+    }
+  };
+  const u32x4 zero4 = {0u, 0u, 0u, 0u};
   // this lane's place in a transposing read: row lr / 4, columns 4 (lr % 4) ..
   const int lrow = lr >> 2, lcol = 4 * (lr & 3);
+  const int bwt = wave - (4 - NCT);  // bias: output-channel tile of this wave
+  const bool bias_wave = do_bias && bwt >= 0;
   f32x4 acc[MAXW];
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) acc[u] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  float bsum = 0.f;
+  f32x4 accb = {0.f, 0.f, 0.f, 0.f};
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const Tile tl = tile_of(t, D, H, W);
+    fetch(t);
     __syncthreads();
-    // dZ tile: sz[v * PZ + co] (zero outside the grid)
+    // dZ tile sz[v * PZ + co] and input brick sx[brick voxel * PX + ci]
 #pragma unroll
     for (int m = 0; m < ZSL; ++m) {
-      if (zs[m] < 0) continue;
       const int v = zs[m] / (NCT * 2), q = zs[m] - v * (NCT * 2);
-      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
-      u32x4 val = {0u, 0u, 0u, 0u};
-      if (gx < W && gy < H && gz < D) val = *(const u32x4*)(dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q);
-      *(u32x4*)(sz + v * PZ + 8 * q) = val;
+      *(u32x4*)(sz + v * PZ + 8 * q) = ((zok >> m) & 1u) ? zv[m] : zero4;
     }
-    // input brick: sx[brick voxel * PX + ci]
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
-      if (xsl[m] < 0) continue;
+      if (m == XSL - 1 && !xlast) break;
       const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
-      const int gx = tl.x0 + bx - Bk::HALO, gy = tl.y0 + by - Bk::HALO, gz = tl.z0 + bz - Bk::HALO;
-      u32x4 val = {0u, 0u, 0u, 0u};
-      if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
-        val = *(const u32x4*)(x + vox(tl, D, H, W, gz, gy, gx) * xs + ci0 + 8 * q);
-      *(u32x4*)(sx + ((bz * Bk::Y + by) * Bk::X + bx) * PX + 8 * q) = val;
+      *(u32x4*)(sx + ((bz * Bk::Y + by) * Bk::X + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
     }
     __syncthreads();
-    if (do_bias && threadIdx.x < CO)
-      for (int v = 0; v < TV; ++v) bsum += bf2f(sz[v * PZ + threadIdx.x]);
 #pragma unroll 1
     for (int s = 0; s < TV / 32; ++s) {
       // k = voxels 32s + 8lg + e (e = 0..7), consecutive in x
@@ -541,7 +596,6 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
       const u16* lbase = sx + (((tz * Bk::Y + ty) * Bk::X + tx0) + lrow) * PX + lcol;
 #pragma unroll
       for (int u = 0; u < MAXW; ++u) {
-        if (wave + 4 * u >= nq) break;
         const u32x4 b = frag_tr(lbase + boff[u], PX);
         u32x4 avc = av[0];
 #pragma unroll
@@ -549,6 +603,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
           if (bct[u] == c) avc = av[c];
         acc[u] = mfma16(avc, b, acc[u]);
       }
+      if (bias_wave) accb = bias_mfma<NCT>(av, bwt, accb);
     }
   }
 
@@ -566,7 +621,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
       pp[((int64_t)co * TAPS + tap) * Ci + ci] = acc[u][i];
     }
   }
-  if (do_bias && threadIdx.x < CO) partb[(int64_t)blockIdx.x * Co + co0 + threadIdx.x] = bsum;
+  if (bias_wave && lr == 0) bias_store(partb + (int64_t)blockIdx.x * Co + co0 + bwt * 16, accb, lg);
 }
 
 // out[i] = (accum ? out[i] : 0) + sum_s part[s*n + i].  A workgroup owns 64
@@ -811,7 +866,7 @@ WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
   WgShape s;
   s.nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   s.nci = (ci % 32 == 0) ? 2 : 1;
-  s.tg = std::min(taps, 32 / (s.nct * s.nci));
+  s.tg = wgrad_tg(taps, s.nct, s.nci);
   const int ngr = (taps + s.tg - 1) / s.tg;
   s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci)) * ngr;
   const int64_t nt = tiles_of(n, d, h, w);
@@ -929,29 +984,29 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   const int nt = (int)tiles_of(n, d, h, w);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sh.S, (unsigned)sh.blocks_y);
-#define ABX_WG(K_, T_, A, B)                                                                                   \
-  hipLaunchKernelGGL((K_<T_, A, B>), grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, d, h, w, ci, \
-                     co, nt, sh.tg)
-#define ABX_WG_T(K_, T_)                                \
-  if (sh.nct == 4 && sh.nci == 2) ABX_WG(K_, T_, 4, 2); \
-  else if (sh.nct == 4) ABX_WG(K_, T_, 4, 1);           \
-  else if (sh.nct == 2 && sh.nci == 2) ABX_WG(K_, T_, 2, 2); \
-  else if (sh.nct == 2) ABX_WG(K_, T_, 2, 1);           \
-  else if (sh.nci == 2) ABX_WG(K_, T_, 1, 2);           \
-  else ABX_WG(K_, T_, 1, 1);
-  if (g_wgrad_tr) {
-    if (T == 27) {
-      ABX_WG_T(k_conv_wgrad_tr, 27)
-    } else {
-      ABX_WG_T(k_conv_wgrad_tr, 1)
-    }
-  } else if (T == 27) {
-    ABX_WG_T(k_conv_wgrad, 27)
+#define ABX_LAUNCH(KERNEL)                                                                                     \
+  hipLaunchKernelGGL(KERNEL, grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, d, h, w, ci, co, nt, \
+                     sh.tg)
+#define ABX_WG(T_, A, B)                                  \
+  do {                                                    \
+    if (g_wgrad_tr) ABX_LAUNCH((k_conv_wgrad_tr<T_, A, B>)); \
+    else ABX_LAUNCH((k_conv_wgrad<T_, A, B>));            \
+  } while (0)
+#define ABX_WG_T(T_)                                      \
+  if (sh.nct == 4 && sh.nci == 2) ABX_WG(T_, 4, 2);       \
+  else if (sh.nct == 4) ABX_WG(T_, 4, 1);                 \
+  else if (sh.nct == 2 && sh.nci == 2) ABX_WG(T_, 2, 2);  \
+  else if (sh.nct == 2) ABX_WG(T_, 2, 1);                 \
+  else if (sh.nci == 2) ABX_WG(T_, 1, 2);                 \
+  else ABX_WG(T_, 1, 1);
+  if (T == 27) {
+    ABX_WG_T(27)
   } else {
-    ABX_WG_T(k_conv_wgrad, 1)
+    ABX_WG_T(1)
   }
 #undef ABX_WG_T
 #undef ABX_WG
+#undef ABX_LAUNCH
   ABX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw, 64)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
   if (db)
