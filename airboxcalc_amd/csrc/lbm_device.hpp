@@ -449,12 +449,18 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
   const uint32_t* tw = a.type_bits + 3 * (d >> 5);
   const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
   M fi[19], fo[19];
+  // the fluid bits at positions p0 - 1 .. p0 + 1 of the nine neighbour rows,
+  // read from LDS before any population load (two words per row), so no wait
+  // for an LDS read sits between the 19 loads
+  uint64_t wb[9];
+  const int w0 = (p0 - 1) >> 5, sh0 = p0 - 32 * w0;
+#pragma unroll
+  for (int wi = 0; wi < 9; ++wi) wb[wi] = (uint64_t)win[wi][w0] | (uint64_t)win[wi][w0 + 1] << 32;
   uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const int p = p0 - CZ[q];
     const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const bool nbf = fluid && ((win[wi][p >> 5] >> (p & 31)) & 1u);
+    const bool nbf = fluid & (bool)((wb[wi] >> (sh0 - CZ[q])) & 1u);
     nbm |= (uint32_t)nbf << q;
     const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
     const S* src;
