@@ -171,6 +171,8 @@ def load() -> C.CDLL:
         "abx_surrogate_features": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, d, d, d, i32, i32, i32, vp, i64,
                                          vp, i32, i32]),
         "abx_conv3d_forward": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, i32, i32, i32, i32, vp, i32]),
+        "abx_conv3d_forward_gated": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, vp, i32, i32, i32, i32, vp, i32,
+                                           vp, i32]),
         "abx_conv3d_wgrad_workspace": (i64, [i32, i32, i32, i32, i32, i32, i32]),
         "abx_conv3d_wgrad": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp, i32]),
         "abx_maxpool2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),

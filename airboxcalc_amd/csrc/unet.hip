@@ -103,10 +103,20 @@ __device__ __forceinline__ void load_brick(u16* s, int sp, const u16* __restrict
 // wt: bf16 [Co][TAPS][Ci].  CK: channels per LDS chunk (divides Ci); the
 // brick takes (CK + 8) * 2 bytes per voxel, so small chunks leave room for
 // more workgroups per CU.
+// ReLU backward folded into a convolution's store: each of 4 packed bf16
+// values is zeroed where the gate value (the layer's ReLU output) is not
+// positive, as abx_relu_backward does.
+__device__ __forceinline__ u32x2 gate4(u32x2 o, u32x2 g) {
+  const uint32_t m0 = bf2f((u16)(g.x & 0xffffu)) > 0.f ? 0xffffu : 0u, m1 = bf2f((u16)(g.x >> 16)) > 0.f ? 0xffffu : 0u;
+  const uint32_t m2 = bf2f((u16)(g.y & 0xffffu)) > 0.f ? 0xffffu : 0u, m3 = bf2f((u16)(g.y >> 16)) > 0.f ? 0xffffu : 0u;
+  return (u32x2){o.x & (m0 | m1 << 16), o.y & (m2 | m3 << 16)};
+}
+
 template <int TAPS, int NCT, int CK>
 __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
-                                                  const float* __restrict__ bias, u16* __restrict__ y, int ys, int D,
-                                                  int H, int W, int Ci, int Co, int relu, int accum) {
+                                                  const float* __restrict__ bias, u16* __restrict__ y, int ys,
+                                                  const u16* __restrict__ gate, int gs, int D, int H, int W, int Ci,
+                                                  int Co, int relu, int accum) {
   using Bk = Brick<TAPS>;
   constexpr int sp = CK + 8, ck = CK;
   __shared__ __attribute__((aligned(16))) u16 sx[Bk::V * sp];
@@ -178,6 +188,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
       u32x2 o;
       o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
       o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+      if (gate) o = gate4(o, *(const u32x2*)(gate + vox(tl, D, H, W, gz, gy, gx) * gs + co));
       *(u32x2*)(yp + co) = o;
     }
   }
@@ -235,7 +246,8 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
 
 __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
                                                     const float* __restrict__ bias, u16* __restrict__ y, int ys,
-                                                    int D, int H, int W, int Ci, int ntiles, int relu, int accum) {
+                                                    const u16* __restrict__ gate, int gs, int D, int H, int W, int Ci,
+                                                    int ntiles, int relu, int accum) {
   __shared__ __attribute__((aligned(16))) u16 sx[F16BV * F16SP];
   const int co0 = blockIdx.y * 16, nch = Ci / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4, hi = lg >> 1;
@@ -329,6 +341,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
       u32x2 o;
       o.x = pack_bf16(v[0], v[1]);
       o.y = pack_bf16(v[2], v[3]);
+      if (gate) o = gate4(o, *(const u32x2*)(gate + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * gs + co));
       *(u32x2*)yp = o;
     }
   }
@@ -782,6 +795,40 @@ __global__ void k_relu_bwd(u16* __restrict__ dy, int dys, const u16* __restrict_
 
 // Masked, channel-weighted squared error and its gradient.  One thread per
 // voxel; loss partial sums per workgroup into loss[blockIdx.x].
+// k_mse for rows of whole 16-byte groups (ps, dps multiples of 8, C <= 8):
+// a voxel's prediction and gradient rows move as 16-byte vectors instead of
+// 2-byte accesses strided by the row; same sums in the same order.
+__global__ __launch_bounds__(256) void k_mse_v(const u16* __restrict__ pred, int ps, const u16* __restrict__ tgt,
+                                               int ts, const uint8_t* __restrict__ mask, int64_t nvox, int C,
+                                               const float* __restrict__ cw, float scale, float* __restrict__ loss,
+                                               u16* __restrict__ dpred, int dps) {
+  __shared__ float red[256];
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (v < nvox) {
+    const bool m = mask[v] != 0;
+    const u16x8 p = *(const u16x8*)(pred + v * ps);
+    u16x8 gr = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (m)
+      for (int c = 0; c < C; ++c) {
+        const float e = bf2f(p[c]) - bf2f(tgt[v * ts + c]);
+        l += cw[c] * e * e;
+        gr[c] = f2bf(2.f * scale * cw[c] * e);
+      }
+    u16* dp = dpred + v * dps;
+    *(u16x8*)dp = gr;
+    const u16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 8; c < dps; c += 8) *(u16x8*)(dp + c) = z;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = red[0] * scale;
+}
+
 __global__ __launch_bounds__(256) void k_mse(const u16* __restrict__ pred, int ps, const u16* __restrict__ tgt, int ts,
                                              const uint8_t* __restrict__ mask, int64_t nvox, int C,
                                              const float* __restrict__ cw, float scale, float* __restrict__ loss,
@@ -904,12 +951,17 @@ extern "C" int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, 
   return 0;
 }
 
-extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
-                                  const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
-                                  uint16_t* y, int ys) {
-  if (!conv_args_ok("abx_conv3d_forward", x, xs, n, d, h, w, ci, co, ksize)) return -1;
+namespace {
+int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                 const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate, uint16_t* y,
+                 int ys, const uint16_t* gate, int gs) {
+  if (!conv_args_ok(fn, x, xs, n, d, h, w, ci, co, ksize)) return -1;
   if (!wpack || !y || ys < co || ys % 8 || !aligned16(wpack) || !aligned16(y)) {
-    set_error("abx_conv3d_forward: bad output or weight argument");
+    set_errorf("%s: bad output or weight argument", fn);
+    return -1;
+  }
+  if (gate && (gs < co || gs % 8 || !aligned16(gate))) {
+    set_errorf("%s: gate stride must be >= co and a multiple of 8, the pointer 16-byte aligned", fn);
     return -1;
   }
   const int ck = fwd_chunk(ci);
@@ -921,14 +973,14 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
     // persistent workgroups: about four per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)4 * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h,
-                       w, ci, (int)tiles, relu, accumulate);
+    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(256), 0, s, x, xs, wpack, bias, y, ys, gate,
+                       gs, d, h, w, ci, (int)tiles, relu, accumulate);
     ABX_HIP(hipGetLastError());
     return 0;
   }
 #define ABX_FWD(T, NC, C)                                                                                          \
-  hipLaunchKernelGGL((k_conv_fwd<T, NC, C>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, d, h, w, ci, co, \
-                     relu, accumulate)
+  hipLaunchKernelGGL((k_conv_fwd<T, NC, C>), grid, dim3(256), 0, s, x, xs, wpack, bias, y, ys, gate, gs, d, h, w, \
+                     ci, co, relu, accumulate)
 #define ABX_FWD_CK(T, NC)          \
   if (ck == 64) ABX_FWD(T, NC, 64); \
   else if (ck == 48) ABX_FWD(T, NC, 48); \
@@ -947,6 +999,25 @@ extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n
 #undef ABX_FWD
   ABX_HIP(hipGetLastError());
   return 0;
+}
+}  // namespace
+
+extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                                  const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
+                                  uint16_t* y, int ys) {
+  return conv_forward("abx_conv3d_forward", stream, x, xs, n, d, h, w, ci, wpack, bias, co, ksize, relu, accumulate,
+                      y, ys, nullptr, 0);
+}
+
+extern "C" int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                                        const uint16_t* wpack, const float* bias, int co, int ksize, int relu,
+                                        int accumulate, uint16_t* y, int ys, const uint16_t* gate, int gs) {
+  if (!gate) {
+    set_error("abx_conv3d_forward_gated: gate is NULL");
+    return -1;
+  }
+  return conv_forward("abx_conv3d_forward_gated", stream, x, xs, n, d, h, w, ci, wpack, bias, co, ksize, relu,
+                      accumulate, y, ys, gate, gs);
 }
 
 extern "C" int abx_conv3d_set_kernel(int flags) {
@@ -1109,8 +1180,12 @@ extern "C" int abx_mse_loss(void* stream, const uint16_t* pred, int ps, const ui
     set_error("abx_mse_loss: bad argument");
     return -1;
   }
-  hipLaunchKernelGGL(k_mse, dim3(blocks_for(nvox)), dim3(256), 0, (hipStream_t)stream, pred, ps, target, ts, mask,
-                     nvox, c, chan_w, scale, loss_parts, dpred, dps);
+  if (ps % 8 == 0 && dps % 8 == 0 && c <= 8 && aligned16(pred) && aligned16(dpred))
+    hipLaunchKernelGGL(k_mse_v, dim3(blocks_for(nvox)), dim3(256), 0, (hipStream_t)stream, pred, ps, target, ts, mask,
+                       nvox, c, chan_w, scale, loss_parts, dpred, dps);
+  else
+    hipLaunchKernelGGL(k_mse, dim3(blocks_for(nvox)), dim3(256), 0, (hipStream_t)stream, pred, ps, target, ts, mask,
+                       nvox, c, chan_w, scale, loss_parts, dpred, dps);
   ABX_HIP(hipGetLastError());
   return 0;
 }

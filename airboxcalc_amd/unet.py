@@ -221,13 +221,22 @@ class UNet:
         _ck(abx.load().abx_conv3d_forward(self.stream, x.p, x.s, n, d, h, w, l.ci, self.packs[name][0].data_ptr(),
                                           self.bptr(l), l.co, l.ksize, int(l.relu), int(accumulate), y.p, y.s))
 
-    def _dgrad(self, name: str, dz: T, dx: T) -> None:
+    def _dgrad(self, name: str, dz: T, dx: T, gate: Optional[T] = None) -> None:
+        """dx = input gradient of layer name; with gate (the ReLU output that
+        is this layer's input) the ReLU's backward is applied in the store."""
         l = self.by[name]
         n = self.shape[0]
         d, h, w = self.dims[self._level(l)]
         assert dz.c == l.co and dx.c == l.ci, (name, dz.c, dx.c)
-        _ck(abx.load().abx_conv3d_forward(self.stream, dz.p, dz.s, n, d, h, w, l.co, self.packs[name][1].data_ptr(),
-                                          None, l.ci, l.ksize, 0, 0, dx.p, dx.s))
+        lib = abx.load()
+        if gate is None:
+            _ck(lib.abx_conv3d_forward(self.stream, dz.p, dz.s, n, d, h, w, l.co, self.packs[name][1].data_ptr(),
+                                       None, l.ci, l.ksize, 0, 0, dx.p, dx.s))
+        else:
+            assert gate.c == l.ci, (name, gate.c)
+            _ck(lib.abx_conv3d_forward_gated(self.stream, dz.p, dz.s, n, d, h, w, l.co,
+                                             self.packs[name][1].data_ptr(), None, l.ci, l.ksize, 0, 0, dx.p, dx.s,
+                                             gate.p, gate.s))
 
     def _wgrad(self, name: str, x: T, dz: T) -> None:
         l = self.by[name]
@@ -306,36 +315,31 @@ class UNet:
         g = lambda k: T(G[k])
         dO = T(dout)
         self._wgrad("head", a("d1b"), dO)
-        self._dgrad("head", dO, g("d1b"))
+        self._dgrad("head", dO, g("d1b"), gate=a("d1b"))
         # level 1 decoder
-        self._relu_bwd(g("d1b"), a("d1b"))
         self._wgrad("dec1b", a("d1a"), g("d1b"))
-        self._dgrad("dec1b", g("d1b"), g("d1a"))
-        self._relu_bwd(g("d1a"), a("d1a"))
+        self._dgrad("dec1b", g("d1b"), g("d1a"), gate=a("d1a"))
         self._wgrad("dec1a", a("cat1"), g("d1a"))
         self._dgrad("dec1a", g("d1a"), g("cat1"))
         self._up_bwd(g("cat1").sl(0, w2), g("d2b"), 1)
         # level 2 decoder
         self._relu_bwd(g("d2b"), a("d2b"))
         self._wgrad("dec2b", a("d2a"), g("d2b"))
-        self._dgrad("dec2b", g("d2b"), g("d2a"))
-        self._relu_bwd(g("d2a"), a("d2a"))
+        self._dgrad("dec2b", g("d2b"), g("d2a"), gate=a("d2a"))
         self._wgrad("dec2a", a("cat2"), g("d2a"))
         self._dgrad("dec2a", g("d2a"), g("cat2"))
         self._up_bwd(g("cat2").sl(0, w3), g("d3b"), 2)
         # level 3 decoder
         self._relu_bwd(g("d3b"), a("d3b"))
         self._wgrad("dec3b", a("d3a"), g("d3b"))
-        self._dgrad("dec3b", g("d3b"), g("d3a"))
-        self._relu_bwd(g("d3a"), a("d3a"))
+        self._dgrad("dec3b", g("d3b"), g("d3a"), gate=a("d3a"))
         self._wgrad("dec3a", a("cat3"), g("d3a"))
         self._dgrad("dec3a", g("d3a"), g("cat3"))
         self._up_bwd(g("cat3").sl(0, w4), g("b4"), 3)
         # bottleneck
         self._relu_bwd(g("b4"), a("b4"))
         self._wgrad("botb", a("a4"), g("b4"))
-        self._dgrad("botb", g("b4"), g("a4"))
-        self._relu_bwd(g("a4"), a("a4"))
+        self._dgrad("botb", g("b4"), g("a4"), gate=a("a4"))
         self._wgrad("bota", a("p3"), g("a4"))
         self._dgrad("bota", g("a4"), g("p3"))
         # level 3 encoder: skip gradient (from dec3a) + pooling gradient
@@ -343,8 +347,7 @@ class UNet:
         self._pool_bwd(e3, g("p3"), de3, 2)
         self._relu_bwd(de3, e3)
         self._wgrad("enc3b", a("a3"), de3)
-        self._dgrad("enc3b", de3, g("a3"))
-        self._relu_bwd(g("a3"), a("a3"))
+        self._dgrad("enc3b", de3, g("a3"), gate=a("a3"))
         self._wgrad("enc3a", a("p2"), g("a3"))
         self._dgrad("enc3a", g("a3"), g("p2"))
         # level 2 encoder
@@ -352,8 +355,7 @@ class UNet:
         self._pool_bwd(e2, g("p2"), de2, 1)
         self._relu_bwd(de2, e2)
         self._wgrad("enc2b", a("a2"), de2)
-        self._dgrad("enc2b", de2, g("a2"))
-        self._relu_bwd(g("a2"), a("a2"))
+        self._dgrad("enc2b", de2, g("a2"), gate=a("a2"))
         self._wgrad("enc2a", a("p1"), g("a2"))
         self._dgrad("enc2a", g("a2"), g("p1"))
         # level 1 encoder
@@ -361,8 +363,7 @@ class UNet:
         self._pool_bwd(e1, g("p1"), de1, 0)
         self._relu_bwd(de1, e1)
         self._wgrad("enc1b", a("a1"), de1)
-        self._dgrad("enc1b", de1, g("a1"))
-        self._relu_bwd(g("a1"), a("a1"))
+        self._dgrad("enc1b", de1, g("a1"), gate=a("a1"))
         self._wgrad("enc1a", T(self.x_in), g("a1"))
 
     def adam(self, lr: float = 1e-3, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:

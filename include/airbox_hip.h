@@ -349,6 +349,16 @@ int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, in
                        const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
                        uint16_t* y, int ys);
 
+/* abx_conv3d_forward, then every output value is set to zero where gate (a
+ * tensor of co channels at per-voxel stride gs, the output of a ReLU layer)
+ * is not positive: the input gradient of a convolution with the backward of
+ * the ReLU before it folded into the store, equal bit for bit to
+ * abx_conv3d_forward followed by abx_relu_backward(y, ys, gate, gs, ...).
+ * No reference counterpart. */
+int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
+                             const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
+                             uint16_t* y, int ys, const uint16_t* gate, int gs);
+
 /* Process-wide choice of convolution kernels, for comparison and timing
  * (no reference counterpart).  flags bit 0: 3x3x3 forward convolutions use the
  * 16-channel-tile kernel (16x4x4 voxel tiles, weights held in registers)

@@ -171,6 +171,66 @@ def test_wgrad_transposed_reads_equal_gathered(case):
     assert torch.equal(outs[0][1].view(torch.int32), outs[1][1].view(torch.int32))
 
 
+@pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 3, 3, 0), (1, 4, 6, 20, 32, 48, 3, 0, 1),
+                                  (1, 3, 5, 18, 16, 64, 3, 3, 1), (2, 5, 4, 17, 16, 16, 1, 3, 0)],
+                         ids=lambda c: "x".join(map(str, c)))
+def test_conv_forward_gated_equals_forward_then_relu_backward(case):
+    """abx_conv3d_forward_gated (the ReLU backward folded into the store)
+    equals abx_conv3d_forward followed by abx_relu_backward bit for bit, on
+    the 16-channel-tile kernel and the general one, with and without
+    accumulation, the gate a channel slice of a wider tensor."""
+    n, d, h, w, ci, co, k, flags, acc = case
+    say("conv forward gated", case)
+    rng = np.random.default_rng(11)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    W = (rng.standard_normal((co, k ** 3, ci)) / 20).astype(np.float32)
+    wp, _ = pack(W, k)
+    gate_wide = bf(rng.standard_normal((n, d, h, w, co + 16)))     # half of the values not positive
+    gate = gate_wide[..., 8:8 + co]
+    y0 = bf(rng.standard_normal((n, d, h, w, co)))
+    lib = abx.load()
+    try:
+        ck(lib.abx_conv3d_set_kernel(flags))
+        ya, yb = y0.clone(), y0.clone()
+        ck(lib.abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, k, 0, acc,
+                                  ya.data_ptr(), co))
+        ck(lib.abx_relu_backward(stream(), ya.data_ptr(), co, gate.data_ptr(), co + 16, n * d * h * w, co))
+        ck(lib.abx_conv3d_forward_gated(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, k, 0,
+                                        acc, yb.data_ptr(), co, gate.data_ptr(), co + 16))
+        torch.cuda.synchronize()
+    finally:
+        ck(lib.abx_conv3d_set_kernel(3))
+    assert torch.equal(ya.view(torch.int16), yb.view(torch.int16))
+    assert (f32(yb) == 0).mean() > 0.4
+
+
+def test_mse_vector_kernel_equals_generic():
+    """The MSE loss on 16-byte rows (k_mse_v) and the generic kernel (taken
+    when the gradient buffer is not 16-byte aligned) give the same loss
+    partials and gradients bit for bit; unused channels get zero gradient."""
+    say("mse vector vs generic")
+    rng = np.random.default_rng(12)
+    nvox, C, ps = 5000, 4, 16
+    pred = bf(rng.standard_normal((nvox, ps)))
+    tgt = bf(rng.standard_normal((nvox, C)))
+    mask = torch.as_tensor((rng.random(nvox) < 0.7).astype(np.uint8)).to(_dev())
+    cw = torch.as_tensor(np.array([1.0, 0.5, 2.0, 1.5], np.float32)).to(_dev())
+    lib = abx.load()
+    nparts = lib.abx_mse_partials(nvox)
+    outs = []
+    for off in (0, 1):                                # off 1: dpred 2 bytes past a 16-byte boundary
+        lp = torch.empty(nparts, dtype=torch.float32, device=_dev())
+        buf = torch.full((nvox * ps + 8,), 7.0, dtype=torch.bfloat16, device=_dev())
+        dp = buf[off:off + nvox * ps]
+        ck(lib.abx_mse_loss(stream(), pred.data_ptr(), ps, tgt.data_ptr(), C, mask.data_ptr(), nvox, C, cw.data_ptr(),
+                            1.0 / (nvox * C), lp.data_ptr(), dp.data_ptr(), ps))
+        torch.cuda.synchronize()
+        outs.append((lp.cpu(), dp.cpu().view(torch.int16).reshape(nvox, ps)))
+    assert torch.equal(outs[0][0].view(torch.int32), outs[1][0].view(torch.int32))
+    assert torch.equal(outs[0][1], outs[1][1])
+    assert (outs[0][1][:, C:] == 0).all()
+
+
 def test_conv_forward_slices_and_accumulate():
     """Input and output are channel slices of wider tensors; accumulate adds."""
     n, d, h, w, ci, co = 1, 4, 4, 16, 32, 16
