@@ -301,16 +301,29 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
         }
       __syncthreads();
       if (nch > 1) load_weights(c);
+      // the fragments of tap pair s + 1 are read while the MFMAs of pair s
+      // run (registers are free up to the 2-wave limit), so LDS latency is
+      // not waited out once per pair of MFMAs
+      u32x4 bf[2][F16Y];
+#pragma unroll
+      for (int r = 0; r < F16Y; ++r)
+        bf[0][r] = *(const u32x4*)(sx + lbase + (hi ? f16_tapoff(1) : f16_tapoff(0)) * F16SP + r * F16BX * F16SP);
 #pragma unroll
       for (int s = 0; s < 14; ++s) {
-        const int off = (hi ? f16_tapoff(2 * s + 1) : f16_tapoff(2 * s)) * F16SP;
+        if (s + 1 < 14) {
+          const int offn = (hi ? f16_tapoff(2 * s + 3) : f16_tapoff(2 * s + 2)) * F16SP;
+#pragma unroll
+          for (int r = 0; r < F16Y; ++r) bf[(s + 1) & 1][r] = *(const u32x4*)(sx + lbase + offn + r * F16BX * F16SP);
+        }
+        // keep the scheduler from sinking those reads below this pair's MFMAs
+        __builtin_amdgcn_sched_barrier(0);
         const bool none = s == 13 && hi;  // tap 27 does not exist
 #pragma unroll
         for (int r = 0; r < F16Y; ++r) {
-          u32x4 b = *(const u32x4*)(sx + lbase + off + r * F16BX * F16SP);
-          if (none) b = (u32x4){0u, 0u, 0u, 0u};
+          const u32x4 b = none ? (u32x4){0u, 0u, 0u, 0u} : bf[s & 1][r];
           acc[r] = mfma16(wf[s], b, acc[r]);
         }
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
 
