@@ -117,14 +117,18 @@ def fp64_errors(dev: abx.Device, g: abx.Geometry, p: dict, precision: str = "f32
 
 
 def fixed_run(dev: abx.Device, g: abx.Geometry, p: dict, steps: int = 6000, avg: int = 2000, precision: str = "f32",
-              init_rho_u: Optional[np.ndarray] = None) -> SteadyResult:
+              init_rho_u: Optional[np.ndarray] = None,
+              init_fn: Optional[Callable[[abx.Sim], None]] = None) -> SteadyResult:
     """The reference's protocol (simulate, engine.js:341-358 / 613-630): a
     fixed number of steps from the start state, averaging the last `avg`;
-    evaluated on the GPU.  converged is True by definition."""
+    evaluated on the GPU.  converged is True by definition.  init_fn: any
+    start set on the fresh handle (its time is not part of wall_s)."""
     sim = abx.Sim(dev, g.grid(), p["K"], p["aniso"], p["u"], p["tau"], precision=precision)
     try:
         if init_rho_u is not None:
             sim.init_eq(init_rho_u)
+        if init_fn is not None:
+            init_fn(sim)
         sim.sync()
         t0 = time.perf_counter()
         sim.steps(steps - avg, False)

@@ -397,17 +397,23 @@ STUDY_SUBDIV = 57        # quads split 57 x 57: about 200 k triangles per box
 
 def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box], dx: float = 2.0,
                      window: int = 1000, max_steps: int = 60000, fixed_steps: int = 6000, fixed_avg: int = 2000,
-                     fp64_every: int = 4, subdiv: int = STUDY_SUBDIV,
+                     fp64_every: int = 4, subdiv: int = STUDY_SUBDIV, warm_steps: int = 2000, warm_avg: int = 2000,
                      log: Optional[Callable[[str], None]] = None) -> dict:
     """Cold against surrogate-warm time to steady for held-out boxes.
 
     Each box is written as a binary STL of about 200 k triangles, parsed and
     prepared on the GPU at dx.  Runs per box:
-      fixed   from rest, the reference's fixed-step protocol (fixed_steps,
-              averaging the last fixed_avg): the result the others are judged by;
-      cold    from rest to the steady criterion (steady.solve_to_steady);
-      warm    from the surrogate's equilibrium to the same criterion;
-      fp64    (every fp64_every-th box) the fixed protocol in FP64.
+      fixed       from rest, the reference's fixed-step protocol (fixed_steps,
+                  averaging the last fixed_avg): the result the others are
+                  judged by;
+      cold        from rest to the steady criterion (steady.solve_to_steady);
+      warm        from the surrogate's equilibrium to the same criterion;
+      warm_fixed  from the surrogate's equilibrium, the fixed protocol
+                  shortened to warm_steps averaging the last warm_avg (a warm
+                  start needs no transient to discard);
+      cold_short  the same shortened protocol from rest (what the warm start
+                  is worth at that length);
+      fp64        (every fp64_every-th box) the fixed protocol in FP64.
     Velocity-field errors are relative L2 norms over the fluid cells."""
     from . import steady
     rows = []
@@ -436,6 +442,9 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                 warm = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, init_fn=init,
                                               keep_acc=True)
                 rho_u = timing["rho_u"]
+                warm_fixed = steady.fixed_run(dev, g, p, warm_steps, warm_avg,
+                                              init_fn=lambda sim: warm_start_init(net, norm, dev, box, g, sim, tris))
+                cold_short = steady.fixed_run(dev, g, p, warm_steps, warm_avg)
                 f64 = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg, precision="f64") \
                     if fp64_every and bi % fp64_every == 0 else None
             except abx.AbxError as e:       # e.g. the reference's instability error
@@ -467,6 +476,8 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                    aniso=box.aniso, stl_s=t1 - t0, prepare_s=t2 - t1, predict_s=timing["predict_s"],
                    fixed=run(fixed), cold=run(cold), warm=run(warm),
                    cold_vs_fixed=judge(cold, fixed), warm_vs_fixed=judge(warm, fixed),
+                   warm_fixed=run(warm_fixed), warm_fixed_vs_fixed=judge(warm_fixed, fixed),
+                   cold_short_vs_fixed=judge(cold_short, fixed),
                    prediction_u_err_vs_fixed=uerr(init, fixed.acc))
         if f64 is not None:
             row["fp64"] = run(f64)
@@ -480,7 +491,7 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                 f"dzeta {row['cold_vs_fixed']['zeta_rel']:.3f}, warm dgamma {row['warm_vs_fixed']['gammaCore_abs']:.4f} "
                 f"dzeta {row['warm_vs_fixed']['zeta_rel']:.3f}; prediction u err {row['prediction_u_err_vs_fixed']:.3f}")
     return dict(rows=rows, summary=summarize(rows), dx=dx, window=window, fixed_steps=fixed_steps,
-                fixed_avg=fixed_avg)
+                fixed_avg=fixed_avg, warm_steps=warm_steps, warm_avg=warm_avg)
 
 
 def summarize(rows: List[dict]) -> dict:
@@ -505,6 +516,19 @@ def summarize(rows: List[dict]) -> dict:
                warm_u_err_vs_fixed_median=float(np.median(col("warm_vs_fixed", "u_err"))),
                cold_u_err_vs_fixed_median=float(np.median(col("cold_vs_fixed", "u_err"))),
                prediction_u_err_vs_fixed_median=float(np.median([r["prediction_u_err_vs_fixed"] for r in ok])))
+    wf = [r for r in ok if "warm_fixed" in r]
+    if wf:
+        fw = np.array([r["fixed"]["wall_s"] for r in wf])
+        ww = np.array([r["warm_fixed"]["wall_s"] + r["predict_s"] for r in wf])
+        out.update(warm_fixed_step_ratio=float(wf[0]["fixed"]["steps"] / wf[0]["warm_fixed"]["steps"]),
+                   warm_fixed_wall_speedup_total_incl_predict=float(fw.sum() / ww.sum()),
+                   warm_fixed_within_tol_of_fixed=float(np.mean([r["warm_fixed_vs_fixed"]["within"] for r in wf])),
+                   cold_short_within_tol_of_fixed=float(np.mean([r["cold_short_vs_fixed"]["within"] for r in wf])),
+                   warm_fixed_dgamma_vs_fixed_median=float(np.median([r["warm_fixed_vs_fixed"]["gammaCore_abs"]
+                                                                      for r in wf])),
+                   warm_fixed_dzeta_vs_fixed_median=float(np.median([r["warm_fixed_vs_fixed"]["zeta_rel"] for r in wf])),
+                   cold_short_dzeta_vs_fixed_median=float(np.median([r["cold_short_vs_fixed"]["zeta_rel"] for r in wf])),
+                   warm_fixed_u_err_vs_fixed_median=float(np.median([r["warm_fixed_vs_fixed"]["u_err"] for r in wf])))
     f = [r for r in ok if "fp64" in r]
     if f:
         out.update(fp64_boxes=len(f),
@@ -545,6 +569,8 @@ def main(argv=None) -> int:
     st.add_argument("--window", type=int, default=1000)
     st.add_argument("--fp64-every", type=int, default=4)
     st.add_argument("--subdiv", type=int, default=STUDY_SUBDIV)
+    st.add_argument("--warm-steps", type=int, default=2000, help="steps of the shortened fixed protocol")
+    st.add_argument("--warm-avg", type=int, default=2000, help="steps it averages")
     st.add_argument("--report", help="JSON report")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -568,7 +594,7 @@ def main(argv=None) -> int:
         norm, meta = load(a.weights, net)
         boxes = family(range(a.first_seed, a.first_seed + a.boxes))
         rep = warm_start_study(dev, net, norm, boxes, dx=a.dx, window=a.window, fp64_every=a.fp64_every,
-                               subdiv=a.subdiv, log=log)
+                               subdiv=a.subdiv, warm_steps=a.warm_steps, warm_avg=a.warm_avg, log=log)
         rep["weights_meta"] = meta
         rep["device"] = dev.name
         log(json.dumps(rep["summary"]))
