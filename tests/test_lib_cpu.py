@@ -54,3 +54,18 @@ def test_device_open_fails_loudly_without_gpu():
         return
     d.close()
     pytest.skip("a GPU is present")
+
+
+def test_gated_convolution_rejects_bad_gate_before_any_launch():
+    """abx_conv3d_forward_gated validates its gate before touching a device:
+    NULL, a stride below the channel count and an unaligned pointer are
+    errors with a message (the tensors here are never dereferenced)."""
+    L = abx.load()
+    p = ctypes.c_void_p(1 << 20)                      # 16-byte aligned, never dereferenced
+    args = (None, p, 16, 1, 2, 4, 16, 16, p, None, 16, 3, 0, 0, p, 16)
+    assert L.abx_conv3d_forward_gated(*args, None, 16) == -1
+    assert b"gate is NULL" in L.abx_last_error()
+    assert L.abx_conv3d_forward_gated(*args, p, 8) == -1
+    assert b"gate stride" in L.abx_last_error()
+    assert L.abx_conv3d_forward_gated(*args, ctypes.c_void_p((1 << 20) + 8), 16) == -1
+    assert b"gate stride" in L.abx_last_error()
