@@ -281,9 +281,15 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
 
 
 def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3, val_frac: float = 0.05,
-          seed: int = 0, log: Optional[Callable[[str], None]] = None, max_seconds: float = 1e30) -> dict:
+          seed: int = 0, log: Optional[Callable[[str], None]] = None, max_seconds: float = 1e30,
+          schedule: bool = False, keep_best: bool = False) -> dict:
     """Adam on the masked MSE of the normalised targets; the last val_frac of
-    the samples are held out.  Returns the per-epoch losses and timings."""
+    the samples are held out.  Returns the per-epoch losses and timings.
+    schedule: the learning rate follows a cosine from lr to lr / 10 over the
+    run (its progress is the larger of the epoch and the time fraction);
+    keep_best: the weights of the epoch with the lowest held-out loss are
+    restored at the end."""
+    import math
     import torch
     X, Y, M = data["X"], data["Y"], data["M"]
     n = X.shape[0]
@@ -294,7 +300,10 @@ def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3
     hist = []
     t_start = time.perf_counter()
     step_times = []
+    best = (float("inf"), None, 0)
     for ep in range(epochs):
+        progress = max(ep / epochs, (time.perf_counter() - t_start) / max_seconds)
+        lr_ep = lr * (0.1 + 0.45 * (1.0 + math.cos(math.pi * min(progress, 1.0)))) if schedule else lr
         order = rng.permutation(ntr)
         tot, cnt = 0.0, 0
         losses = []
@@ -303,20 +312,26 @@ def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3
             xb, yb, mb = X[idx].contiguous(), Y[idx].contiguous(), M[idx].contiguous()
             torch.cuda.synchronize()
             ts = time.perf_counter()
-            loss = net.train_step(xb, yb, mb, cw, lr=lr)
+            loss = net.train_step(xb, yb, mb, cw, lr=lr_ep)
             losses.append(loss)
             torch.cuda.synchronize()
             step_times.append(time.perf_counter() - ts)
         tr = float(torch.stack(losses).mean().item()) if losses else float("nan")
         va = evaluate_loss(net, data, range(ntr, n), batch) if nval else float("nan")
-        hist.append(dict(epoch=ep + 1, train=tr, val=va, wall_s=time.perf_counter() - t_start))
+        hist.append(dict(epoch=ep + 1, train=tr, val=va, lr=lr_ep, wall_s=time.perf_counter() - t_start))
+        if keep_best and va < best[0]:
+            best = (va, net.w.clone(), ep + 1)
         if log:
-            log(f"epoch {ep + 1}: train {tr:.4f}  val {va:.4f}  ({time.perf_counter() - t_start:.0f} s, "
+            log(f"epoch {ep + 1}: train {tr:.4f}  val {va:.4f}  lr {lr_ep:.2e}  ({time.perf_counter() - t_start:.0f} s, "
                 f"{1e3 * np.median(step_times):.1f} ms/step)")
         if time.perf_counter() - t_start > max_seconds:
             break
+    if keep_best and best[1] is not None:
+        net.w.copy_(best[1])
+        net.repack()
     return dict(history=hist, ms_per_step=1e3 * float(np.median(step_times)) if step_times else None,
-                n_train=ntr, n_val=nval)
+                n_train=ntr, n_val=nval, best_epoch=best[2] if keep_best else None,
+                best_val=best[0] if keep_best else None)
 
 
 def evaluate_loss(net: U.UNet, data: dict, indices, batch: int = 8) -> float:
@@ -559,6 +574,8 @@ def main(argv=None) -> int:
     tr.add_argument("--max-train-seconds", type=float, default=600)
     tr.add_argument("--target-dx", type=float, default=None,
                     help="solve the boxes at this cell size and restrict the fields to the 96x48x64 inputs")
+    tr.add_argument("--cosine", action="store_true", help="cosine learning-rate decay to lr / 10")
+    tr.add_argument("--keep-best", action="store_true", help="keep the weights of the best held-out epoch")
     tr.add_argument("--out", required=True, help="weights .npz")
     tr.add_argument("--report", help="JSON report")
     st = sub.add_parser("study", help="cold vs warm time to steady on held-out boxes")
@@ -581,9 +598,11 @@ def main(argv=None) -> int:
         data = make_dataset(dev, boxes, a.steps, a.avg, log=log, target_dx=a.target_dx)
         t_data = time.perf_counter() - t0
         net = U.UNet(seed=0)
-        res = train(net, data, a.epochs, a.batch, a.lr, log=log, max_seconds=a.max_train_seconds)
+        res = train(net, data, a.epochs, a.batch, a.lr, log=log, max_seconds=a.max_train_seconds,
+                    schedule=a.cosine, keep_best=a.keep_best)
         meta = dict(boxes=a.boxes, first_seed=a.first_seed, steps=a.steps, avg=a.avg, dx=FAMILY_DX, grid=GRID,
-                    target_dx=a.target_dx, epochs_run=len(res["history"]), batch=a.batch, lr=a.lr)
+                    target_dx=a.target_dx, epochs_run=len(res["history"]), batch=a.batch, lr=a.lr,
+                    cosine=a.cosine, keep_best=a.keep_best, best_epoch=res.get("best_epoch"))
         save(a.out, net, data["norm"], meta)
         rep = dict(meta=meta, dataset_s=t_data, dataset_stats=data["stats"], norm={k: v.tolist() for k, v in
                                                                                    data["norm"].items()},
