@@ -200,12 +200,15 @@ def restrict_targets(acc: np.ndarray, gf: abx.Geometry, gc: abx.Geometry, shape=
 
 
 def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk: int = 64,
-                 log: Optional[Callable[[str], None]] = None, target_dx: Optional[float] = None) -> dict:
+                 log: Optional[Callable[[str], None]] = None, target_dx: Optional[float] = None,
+                 norm: Optional[dict] = None) -> dict:
     """Solve every box (batched, one launch per step for `chunk` boxes, the
     last `avg` of `steps` steps averaged) and build the training volumes on
     the device: X (n, 96, 48, 64, IN_CH) bf16, Y (n, ..., 4) bf16, M uint8.
     With target_dx the solves run at that (finer) cell size and their fields
-    are restricted to the FAMILY_DX lattice of the inputs."""
+    are restricted to the FAMILY_DX lattice of the inputs.  The targets are
+    standardised per channel over the fluid cells, with this dataset's mean
+    and deviation or with a given norm (to continue training a network)."""
     import torch
     n = len(boxes)
     D, H, W = GRID
@@ -272,6 +275,9 @@ def make_dataset(dev: abx.Device, boxes: List[Box], steps: int, avg: int, chunk:
         cnt += v.shape[0]
     mean = (s1 / max(cnt, 1)).float()
     std = torch.sqrt(torch.clamp(s2 / max(cnt, 1) - (s1 / max(cnt, 1)) ** 2, min=1e-12)).float()
+    if norm is not None:
+        mean = torch.as_tensor(np.asarray(norm["mean"], np.float32), device=cuda)
+        std = torch.as_tensor(np.asarray(norm["std"], np.float32), device=cuda)
     for c0 in range(0, n, chunk):
         y = (Y[c0:c0 + chunk].float() - mean) / std
         Y[c0:c0 + chunk] = (y * M[c0:c0 + chunk, ..., None]).to(torch.bfloat16)
@@ -574,6 +580,7 @@ def main(argv=None) -> int:
     tr.add_argument("--max-train-seconds", type=float, default=600)
     tr.add_argument("--target-dx", type=float, default=None,
                     help="solve the boxes at this cell size and restrict the fields to the 96x48x64 inputs")
+    tr.add_argument("--init-weights", help="continue training these weights (their standardisation is kept)")
     tr.add_argument("--cosine", action="store_true", help="cosine learning-rate decay to lr / 10")
     tr.add_argument("--keep-best", action="store_true", help="keep the weights of the best held-out epoch")
     tr.add_argument("--out", required=True, help="weights .npz")
@@ -594,15 +601,18 @@ def main(argv=None) -> int:
     dev = abx.device(0)
     if a.cmd == "train":
         boxes = family(range(a.first_seed, a.first_seed + a.boxes))
-        t0 = time.perf_counter()
-        data = make_dataset(dev, boxes, a.steps, a.avg, log=log, target_dx=a.target_dx)
-        t_data = time.perf_counter() - t0
         net = U.UNet(seed=0)
+        init_norm, init_meta = load(a.init_weights, net) if a.init_weights else (None, None)
+        t0 = time.perf_counter()
+        data = make_dataset(dev, boxes, a.steps, a.avg, log=log, target_dx=a.target_dx, norm=init_norm)
+        t_data = time.perf_counter() - t0
         res = train(net, data, a.epochs, a.batch, a.lr, log=log, max_seconds=a.max_train_seconds,
                     schedule=a.cosine, keep_best=a.keep_best)
         meta = dict(boxes=a.boxes, first_seed=a.first_seed, steps=a.steps, avg=a.avg, dx=FAMILY_DX, grid=GRID,
                     target_dx=a.target_dx, epochs_run=len(res["history"]), batch=a.batch, lr=a.lr,
                     cosine=a.cosine, keep_best=a.keep_best, best_epoch=res.get("best_epoch"))
+        if init_meta is not None:
+            meta["continued_from"] = init_meta
         save(a.out, net, data["norm"], meta)
         rep = dict(meta=meta, dataset_s=t_data, dataset_stats=data["stats"], norm={k: v.tolist() for k, v in
                                                                                    data["norm"].items()},
