@@ -373,7 +373,10 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
     fluid cells."""
     import torch
     tris = G.airbox_triangles(box.spec) if tris is None else tris
-    gc = abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
+    # a solve at the network's own cell size already holds the same prepared
+    # geometry (prepare is deterministic): reuse it instead of preparing twice
+    same = abs(g_fine.h - FAMILY_DX) < 1e-9
+    gc = g_fine if same else abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
     try:
         grid = gc.grid()
         x = torch.empty((1,) + GRID + (U.IN_CH,), dtype=torch.bfloat16, device=net.dev)
@@ -387,7 +390,8 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
         sim.init_eq_resample(out.data_ptr(), U.HEAD_CH, GRID, (gc.nx, gc.ny, gc.nz), fl.data_ptr(), gc.origin, gc.h,
                              g_fine.origin, g_fine.h, affine)
     finally:
-        gc.free()
+        if not same:
+            gc.free()
 
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_unet_96x48x64.npz")
