@@ -365,7 +365,7 @@ def predict(net: U.UNet, x: np.ndarray, norm: dict) -> np.ndarray:
 
 
 def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: abx.Geometry, sim: abx.Sim,
-                    tris: Optional[np.ndarray] = None) -> None:
+                    tris: Optional[np.ndarray] = None, reuse: bool = True) -> None:
     """Seed sim (on g_fine's lattice) at the equilibrium of the surrogate's
     prediction without leaving the GPU: the box is prepared at FAMILY_DX, its
     features go through the network, and abx_sim_init_eq_resample maps the
@@ -375,7 +375,7 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
     tris = G.airbox_triangles(box.spec) if tris is None else tris
     # a solve at the network's own cell size already holds the same prepared
     # geometry (prepare is deterministic): reuse it instead of preparing twice
-    same = abs(g_fine.h - FAMILY_DX) < 1e-9
+    same = reuse and abs(g_fine.h - FAMILY_DX) < 1e-9
     gc = g_fine if same else abx.Geometry(dev, tris, G.prepare_params(box.spec, FAMILY_DX), max_cells=0)
     try:
         grid = gc.grid()

@@ -108,3 +108,28 @@ def test_warm_started_solve_reaches_steady():
         assert abs(warm.metrics["gammaCore"] - cold.metrics["gammaCore"]) < 0.01
     finally:
         g.free()
+
+
+def test_warm_start_reuses_the_solve_geometry_bit_for_bit():
+    """At the network's own cell size the prediction takes its input volume
+    from the solve's prepared geometry instead of preparing the box again:
+    the seeded populations are identical bit for bit."""
+    say("warm start: geometry reuse at the family cell size")
+    dev = abx.device(0)
+    net, norm = _net()
+    box = S.family([1000007])[0]
+    tris = G.airbox_triangles(box.spec)
+    g = abx.Geometry(dev, tris, G.prepare_params(box.spec, S.FAMILY_DX), max_cells=0)
+    try:
+        fs = []
+        for reuse in (True, False):
+            sim = abx.Sim(dev, g.grid(), box.K, box.aniso, S.U0, S.TAU)
+            try:
+                S.warm_start_init(net, norm, dev, box, g, sim, tris, reuse=reuse)
+                fs.append(sim.read_f())
+            finally:
+                sim.destroy()
+        assert np.array_equal(fs[0].view(np.uint32), fs[1].view(np.uint32))
+        assert np.abs(fs[0] - np.asarray(fs[0]).mean()).max() > 0      # a non-trivial start
+    finally:
+        g.free()
