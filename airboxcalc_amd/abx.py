@@ -177,8 +177,10 @@ def load() -> C.CDLL:
         "abx_conv3d_wgrad": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp, i32]),
         "abx_maxpool2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
         "abx_maxpool2_backward": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),
+        "abx_maxpool2_backward_gated": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),
         "abx_upsample2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
         "abx_upsample2_backward": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
+        "abx_upsample2_backward_gated": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, i32]),
         "abx_relu_backward": (i32, [vp, vp, i32, vp, i32, i64, i32]),
         "abx_mse_partials": (i64, [i64]),
         "abx_mse_loss": (i32, [vp, vp, i32, vp, i32, vp, i64, i32, vp, C.c_float, vp, vp, i32]),

@@ -718,7 +718,7 @@ __global__ void k_maxpool2(const u16* __restrict__ x, int xs, int N, int D, int 
 
 // Gradient of k_maxpool2: dy goes to the first maximum (fine order z, y, x).
 __global__ void k_maxpool2_bwd(const u16* __restrict__ x, int xs, const u16* __restrict__ dy, int dys, int N, int D,
-                               int H, int W, int C, u16* __restrict__ dx, int dxs, int accum) {
+                               int H, int W, int C, u16* __restrict__ dx, int dxs, int accum, int gate) {
   const int G = C >> 3, d = D >> 1, h = H >> 1, w = W >> 1;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)N * d * h * w * G) return;
@@ -746,11 +746,13 @@ __global__ void k_maxpool2_bwd(const u16* __restrict__ x, int xs, const u16* __r
   for (int o = 0; o < 8; ++o) {
     u16* p = dx + fv[o] * dxs + 8 * g;
     u16x8 cur = accum ? *(const u16x8*)p : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+    // gate: x is the output of a ReLU; its backward zeroes dx where x <= 0
+    const u16x8 xo = gate ? *(const u16x8*)(x + fv[o] * xs + 8 * g) : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
     u16x8 out;
     for (int c = 0; c < 8; ++c) {
       float val = accum ? bf2f(cur[c]) : 0.f;
       if (am[c] == o) val += bf2f(gy[c]);
-      out[c] = f2bf(val);
+      out[c] = gate && !(bf2f(xo[c]) > 0.f) ? (u16)0 : f2bf(val);
     }
     *(u16x8*)p = out;
   }
@@ -773,7 +775,7 @@ __global__ void k_upsample2(const u16* __restrict__ x, int xs, int N, int D, int
 
 // Gradient of k_upsample2: dx[coarse] = sum of the 8 fine dy (FP32 sum).
 __global__ void k_upsample2_bwd(const u16* __restrict__ dy, int dys, int N, int D, int H, int W, int C,
-                                u16* __restrict__ dx, int dxs) {
+                                u16* __restrict__ dx, int dxs, const u16* __restrict__ gate, int gs) {
   const int G = C >> 3, Hf = 2 * H, Wf = 2 * W;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)N * D * H * W * G) return;
@@ -787,8 +789,10 @@ __global__ void k_upsample2_bwd(const u16* __restrict__ dy, int dys, int N, int 
     const u16x8 a = *(const u16x8*)(dy + fv * dys + 8 * g);
     for (int c = 0; c < 8; ++c) s[c] += bf2f(a[c]);
   }
+  // gate: the output of the ReLU before dx; its backward zeroes dx where gate <= 0
+  const u16x8 gv = gate ? *(const u16x8*)(gate + v * gs + 8 * g) : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
   u16x8 o8;
-  for (int c = 0; c < 8; ++c) o8[c] = f2bf(s[c]);
+  for (int c = 0; c < 8; ++c) o8[c] = gate && !(bf2f(gv[c]) > 0.f) ? (u16)0 : f2bf(s[c]);
   *(u16x8*)(dx + v * dxs + 8 * g) = o8;
 }
 
@@ -1133,18 +1137,45 @@ extern "C" int abx_maxpool2(void* stream, const uint16_t* x, int xs, int n, int 
   return 0;
 }
 
-extern "C" int abx_maxpool2_backward(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d,
-                                     int h, int w, int c, uint16_t* dx, int dxs, int accumulate) {
-  if (!ew_ok("abx_maxpool2_backward", {x, dy, dx}, {xs, dys, dxs}, c)) return -1;
+namespace {
+int maxpool2_backward(const char* fn, void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n,
+                      int d, int h, int w, int c, uint16_t* dx, int dxs, int accumulate, int gate) {
+  if (!ew_ok(fn, {x, dy, dx}, {xs, dys, dxs}, c)) return -1;
   if (n < 1 || d < 2 || h < 2 || w < 2 || d % 2 || h % 2 || w % 2) {
-    set_error("abx_maxpool2_backward: the grid must have even, positive sizes");
+    set_errorf("%s: the grid must have even, positive sizes", fn);
     return -1;
   }
   const int64_t tot = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * (c / 8);
   hipLaunchKernelGGL(k_maxpool2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, dy, dys, n, d,
-                     h, w, c, dx, dxs, accumulate);
+                     h, w, c, dx, dxs, accumulate, gate);
   ABX_HIP(hipGetLastError());
   return 0;
+}
+
+int upsample2_backward(const char* fn, void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c,
+                       uint16_t* dx, int dxs, const uint16_t* gate, int gs) {
+  if (!ew_ok(fn, {dy, dx}, {dys, dxs}, c)) return -1;
+  if (n < 1 || d < 1 || h < 1 || w < 1) {
+    set_errorf("%s: bad grid", fn);
+    return -1;
+  }
+  const int64_t tot = (int64_t)n * d * h * w * (c / 8);
+  hipLaunchKernelGGL(k_upsample2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d, h, w,
+                     c, dx, dxs, gate, gs);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+extern "C" int abx_maxpool2_backward(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d,
+                                     int h, int w, int c, uint16_t* dx, int dxs, int accumulate) {
+  return maxpool2_backward("abx_maxpool2_backward", stream, x, xs, dy, dys, n, d, h, w, c, dx, dxs, accumulate, 0);
+}
+
+extern "C" int abx_maxpool2_backward_gated(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n,
+                                           int d, int h, int w, int c, uint16_t* dx, int dxs, int accumulate) {
+  return maxpool2_backward("abx_maxpool2_backward_gated", stream, x, xs, dy, dys, n, d, h, w, c, dx, dxs, accumulate,
+                           1);
 }
 
 extern "C" int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y,
@@ -1163,16 +1194,16 @@ extern "C" int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int
 
 extern "C" int abx_upsample2_backward(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c,
                                       uint16_t* dx, int dxs) {
-  if (!ew_ok("abx_upsample2_backward", {dy, dx}, {dys, dxs}, c)) return -1;
-  if (n < 1 || d < 1 || h < 1 || w < 1) {
-    set_error("abx_upsample2_backward: bad grid");
+  return upsample2_backward("abx_upsample2_backward", stream, dy, dys, n, d, h, w, c, dx, dxs, nullptr, 0);
+}
+
+extern "C" int abx_upsample2_backward_gated(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w,
+                                            int c, uint16_t* dx, int dxs, const uint16_t* gate, int gs) {
+  if (!gate || gs < c || gs % 8 || !aligned16(gate)) {
+    set_error("abx_upsample2_backward_gated: gate must be 16-byte aligned with a stride >= c, a multiple of 8");
     return -1;
   }
-  const int64_t tot = (int64_t)n * d * h * w * (c / 8);
-  hipLaunchKernelGGL(k_upsample2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d, h, w,
-                     c, dx, dxs);
-  ABX_HIP(hipGetLastError());
-  return 0;
+  return upsample2_backward("abx_upsample2_backward_gated", stream, dy, dys, n, d, h, w, c, dx, dxs, gate, gs);
 }
 
 extern "C" int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint16_t* y, int ys, int64_t nvox, int c) {

@@ -11,7 +11,8 @@ encoder output concatenated as a skip, a 1x1x1 head.  Weights and activations
 are bf16, accumulation FP32 (MFMA), master weights FP32 with Adam.
 
 Every compute step is a kernel of libairbox_hip.so (abx_conv3d_*,
-abx_maxpool2*, abx_upsample2*, abx_relu_backward, abx_mse_loss, abx_adam);
+abx_maxpool2*, abx_upsample2*, abx_mse_loss, abx_adam; each ReLU's backward
+is applied in the store of the gradient kernel before it);
 torch only allocates device memory and provides the stream.  Forward and
 backward are written out by hand; the skip concatenations are channel slices
 of one buffer per level (decoder input = [upsampled | skip]), so no copy is
@@ -246,29 +247,30 @@ class UNet:
                                         self.ws.data_ptr(), self.ws_bytes, self.wptr(l, self.g),
                                         self.bptr(l, self.g), 0))
 
-    def _relu_bwd(self, dy: T, y: T) -> None:
-        nvox = y.t.numel() // y.s
-        _ck(abx.load().abx_relu_backward(self.stream, dy.p, dy.s, y.p, y.s, nvox, y.c))
-
     def _pool(self, x: T, y: T, lvl: int) -> None:
         n = self.shape[0]
         d, h, w = self.dims[lvl]
         _ck(abx.load().abx_maxpool2(self.stream, x.p, x.s, n, d, h, w, x.c, y.p, y.s))
 
     def _pool_bwd(self, x: T, dy: T, dx: T, lvl: int) -> None:
+        """dx += pooling gradient, then the backward of the ReLU whose output
+        is x (the pooled encoder output), in one pass."""
         n = self.shape[0]
         d, h, w = self.dims[lvl]
-        _ck(abx.load().abx_maxpool2_backward(self.stream, x.p, x.s, dy.p, dy.s, n, d, h, w, x.c, dx.p, dx.s, 1))
+        _ck(abx.load().abx_maxpool2_backward_gated(self.stream, x.p, x.s, dy.p, dy.s, n, d, h, w, x.c, dx.p, dx.s, 1))
 
     def _up(self, x: T, y: T, lvl: int) -> None:
         n = self.shape[0]
         d, h, w = self.dims[lvl]     # coarse level
         _ck(abx.load().abx_upsample2(self.stream, x.p, x.s, n, d, h, w, x.c, y.p, y.s))
 
-    def _up_bwd(self, dy: T, dx: T, lvl: int) -> None:
+    def _up_bwd(self, dy: T, dx: T, lvl: int, gate: T) -> None:
+        """dx = upsampling gradient, masked by the backward of the ReLU whose
+        output (gate) was upsampled."""
         n = self.shape[0]
         d, h, w = self.dims[lvl]
-        _ck(abx.load().abx_upsample2_backward(self.stream, dy.p, dy.s, n, d, h, w, dx.c, dx.p, dx.s))
+        _ck(abx.load().abx_upsample2_backward_gated(self.stream, dy.p, dy.s, n, d, h, w, dx.c, dx.p, dx.s, gate.p,
+                                                    gate.s))
 
     # -- network ----------------------------------------------------------------
     def forward(self, x) -> "object":
@@ -321,23 +323,20 @@ class UNet:
         self._dgrad("dec1b", g("d1b"), g("d1a"), gate=a("d1a"))
         self._wgrad("dec1a", a("cat1"), g("d1a"))
         self._dgrad("dec1a", g("d1a"), g("cat1"))
-        self._up_bwd(g("cat1").sl(0, w2), g("d2b"), 1)
+        self._up_bwd(g("cat1").sl(0, w2), g("d2b"), 1, gate=a("d2b"))
         # level 2 decoder
-        self._relu_bwd(g("d2b"), a("d2b"))
         self._wgrad("dec2b", a("d2a"), g("d2b"))
         self._dgrad("dec2b", g("d2b"), g("d2a"), gate=a("d2a"))
         self._wgrad("dec2a", a("cat2"), g("d2a"))
         self._dgrad("dec2a", g("d2a"), g("cat2"))
-        self._up_bwd(g("cat2").sl(0, w3), g("d3b"), 2)
+        self._up_bwd(g("cat2").sl(0, w3), g("d3b"), 2, gate=a("d3b"))
         # level 3 decoder
-        self._relu_bwd(g("d3b"), a("d3b"))
         self._wgrad("dec3b", a("d3a"), g("d3b"))
         self._dgrad("dec3b", g("d3b"), g("d3a"), gate=a("d3a"))
         self._wgrad("dec3a", a("cat3"), g("d3a"))
         self._dgrad("dec3a", g("d3a"), g("cat3"))
-        self._up_bwd(g("cat3").sl(0, w4), g("b4"), 3)
+        self._up_bwd(g("cat3").sl(0, w4), g("b4"), 3, gate=a("b4"))
         # bottleneck
-        self._relu_bwd(g("b4"), a("b4"))
         self._wgrad("botb", a("a4"), g("b4"))
         self._dgrad("botb", g("b4"), g("a4"), gate=a("a4"))
         self._wgrad("bota", a("p3"), g("a4"))
@@ -345,7 +344,6 @@ class UNet:
         # level 3 encoder: skip gradient (from dec3a) + pooling gradient
         e3, de3 = a("cat3").sl(w4, w3), g("cat3").sl(w4, w3)
         self._pool_bwd(e3, g("p3"), de3, 2)
-        self._relu_bwd(de3, e3)
         self._wgrad("enc3b", a("a3"), de3)
         self._dgrad("enc3b", de3, g("a3"), gate=a("a3"))
         self._wgrad("enc3a", a("p2"), g("a3"))
@@ -353,7 +351,6 @@ class UNet:
         # level 2 encoder
         e2, de2 = a("cat2").sl(w3, w2), g("cat2").sl(w3, w2)
         self._pool_bwd(e2, g("p2"), de2, 1)
-        self._relu_bwd(de2, e2)
         self._wgrad("enc2b", a("a2"), de2)
         self._dgrad("enc2b", de2, g("a2"), gate=a("a2"))
         self._wgrad("enc2a", a("p1"), g("a2"))
@@ -361,7 +358,6 @@ class UNet:
         # level 1 encoder
         e1, de1 = a("cat1").sl(w2, w1), g("cat1").sl(w2, w1)
         self._pool_bwd(e1, g("p1"), de1, 0)
-        self._relu_bwd(de1, e1)
         self._wgrad("enc1b", a("a1"), de1)
         self._dgrad("enc1b", de1, g("a1"), gate=a("a1"))
         self._wgrad("enc1a", T(self.x_in), g("a1"))

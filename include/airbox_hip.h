@@ -385,12 +385,22 @@ int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz
 int abx_maxpool2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y, int ys);
 int abx_maxpool2_backward(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d, int h,
                           int w, int c, uint16_t* dx, int dxs, int accumulate);
+/* abx_maxpool2_backward where x is the output of a ReLU: dx is also zeroed
+ * wherever x is not positive (the ReLU's backward, as abx_relu_backward(dx,
+ * dxs, x, xs, ...) after it, bit for bit).  No reference counterpart. */
+int abx_maxpool2_backward_gated(void* stream, const uint16_t* x, int xs, const uint16_t* dy, int dys, int n, int d,
+                                int h, int w, int c, uint16_t* dx, int dxs, int accumulate);
 
 /* Nearest-neighbour 2x upsampling of an n x d x h x w grid and its gradient
  * (sum of the 8 fine values, FP32). */
 int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int c, uint16_t* y, int ys);
 int abx_upsample2_backward(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c, uint16_t* dx,
                            int dxs);
+/* abx_upsample2_backward, then dx = 0 wherever gate (the coarse output of a
+ * ReLU, c channels at stride gs) is not positive, bit for bit as
+ * abx_relu_backward after it.  No reference counterpart. */
+int abx_upsample2_backward_gated(void* stream, const uint16_t* dy, int dys, int n, int d, int h, int w, int c,
+                                 uint16_t* dx, int dxs, const uint16_t* gate, int gs);
 
 /* dy *= (y > 0) in place over nvox voxels of c channels. */
 int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint16_t* y, int ys, int64_t nvox, int c);

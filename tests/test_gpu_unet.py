@@ -204,6 +204,38 @@ def test_conv_forward_gated_equals_forward_then_relu_backward(case):
     assert (f32(yb) == 0).mean() > 0.4
 
 
+def test_pool_and_upsample_gated_backward_equal_backward_then_relu():
+    """abx_maxpool2_backward_gated (x is the ReLU output) and
+    abx_upsample2_backward_gated equal the plain gradient kernel followed by
+    abx_relu_backward, bit for bit, on channel slices of wider tensors."""
+    say("pool / upsample gated backward")
+    rng = np.random.default_rng(13)
+    n, d, h, w, c = 2, 4, 6, 8, 16
+    lib = abx.load()
+    xw = bf(rng.standard_normal((n, d, h, w, c + 8)))
+    x = xw[..., 8:]
+    dy = bf(rng.standard_normal((n, d // 2, h // 2, w // 2, c)))
+    dx0 = bf(rng.standard_normal((n, d, h, w, c)))
+    da, db = dx0.clone(), dx0.clone()
+    ck(lib.abx_maxpool2_backward(stream(), x.data_ptr(), c + 8, dy.data_ptr(), c, n, d, h, w, c, da.data_ptr(), c, 1))
+    ck(lib.abx_relu_backward(stream(), da.data_ptr(), c, x.data_ptr(), c + 8, n * d * h * w, c))
+    ck(lib.abx_maxpool2_backward_gated(stream(), x.data_ptr(), c + 8, dy.data_ptr(), c, n, d, h, w, c, db.data_ptr(),
+                                       c, 1))
+    gw = bf(rng.standard_normal((n, d, h, w, c + 8)))
+    gate = gw[..., :c]
+    fine = bf(rng.standard_normal((n, 2 * d, 2 * h, 2 * w, c)))
+    ua = torch.empty((n, d, h, w, c), dtype=torch.bfloat16, device=_dev())
+    ub = torch.empty_like(ua)
+    ck(lib.abx_upsample2_backward(stream(), fine.data_ptr(), c, n, d, h, w, c, ua.data_ptr(), c))
+    ck(lib.abx_relu_backward(stream(), ua.data_ptr(), c, gate.data_ptr(), c + 8, n * d * h * w, c))
+    ck(lib.abx_upsample2_backward_gated(stream(), fine.data_ptr(), c, n, d, h, w, c, ub.data_ptr(), c,
+                                        gate.data_ptr(), c + 8))
+    torch.cuda.synchronize()
+    assert torch.equal(da.view(torch.int16), db.view(torch.int16))
+    assert torch.equal(ua.view(torch.int16), ub.view(torch.int16))
+    assert (f32(db) == 0).mean() > 0.3 and (f32(ub) == 0).mean() > 0.3
+
+
 def test_mse_vector_kernel_equals_generic():
     """The MSE loss on 16-byte rows (k_mse_v) and the generic kernel (taken
     when the gradient buffer is not 16-byte aligned) give the same loss
