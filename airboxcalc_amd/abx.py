@@ -54,6 +54,12 @@ class GeomInfo(C.Structure):
                 ("pleat", C.c_double)]
 
 
+class PackDesc(C.Structure):
+    """abx_pack_desc: one layer of abx_conv3d_pack_all."""
+    _fields_ = [("w", C.c_void_p), ("fwd", C.c_void_p), ("bwd", C.c_void_p), ("co", C.c_int), ("taps", C.c_int),
+                ("ci", C.c_int)]
+
+
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("fluid", C.c_void_p), ("opening", C.c_void_p),
                 ("outlet", C.c_void_p), ("dc", C.c_void_p), ("top", C.c_int), ("jm", C.c_int),
@@ -166,6 +172,7 @@ def load() -> C.CDLL:
                                   C.POINTER(i64), C.POINTER(i64)]),
         # flow surrogate kernels (stream first; tensors as device addresses)
         "abx_conv3d_pack": (i32, [vp, vp, i32, i32, i32, i32, vp]),
+        "abx_conv3d_pack_all": (i32, [vp, i32, vp]),
         "abx_conv3d_set_kernel": (i32, [i32]),
         "abx_surrogate_features_workspace": (i64, [i32, i32, i32]),
         "abx_surrogate_features": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, d, d, d, i32, i32, i32, vp, i64,

@@ -954,6 +954,53 @@ bool conv_args_ok(const char* fn, const void* x, int xs, int n, int d, int h, in
 
 using namespace abx;
 
+namespace {
+// Every layer's two packed layouts in one launch: thread i finds its layer by
+// the running element offsets (at most ABX_PACK_MAX layers) and writes the
+// forward and the input-gradient copy of its weight, as k_pack_w does.
+struct PackTable {
+  abx_pack_desc d[ABX_PACK_MAX];
+  int64_t start[ABX_PACK_MAX + 1];
+  int n;
+};
+
+__global__ void k_pack_all(PackTable tab) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= tab.start[tab.n]) return;
+  int l = 0;
+  while (l + 1 < tab.n && i >= tab.start[l + 1]) ++l;
+  const abx_pack_desc& d = tab.d[l];
+  const int64_t j = i - tab.start[l];
+  const int T = d.taps, Ci = d.ci, Co = d.co;
+  const int ci = (int)(j % Ci), t = (int)((j / Ci) % T), co = (int)(j / ((int64_t)Ci * T));
+  const u16 v = f2bf(d.w[j]);
+  d.fwd[j] = v;
+  d.bwd[((int64_t)ci * T + (T - 1 - t)) * Co + co] = v;
+}
+}  // namespace
+
+extern "C" int abx_conv3d_pack_all(void* stream, int n, const abx_pack_desc* desc) {
+  if (n < 1 || n > ABX_PACK_MAX || !desc) {
+    set_errorf("abx_conv3d_pack_all: between 1 and %d layers", ABX_PACK_MAX);
+    return -1;
+  }
+  PackTable tab{};
+  tab.n = n;
+  tab.start[0] = 0;
+  for (int l = 0; l < n; ++l) {
+    const abx_pack_desc& d = desc[l];
+    if (!d.w || !d.fwd || !d.bwd || d.co < 1 || d.ci < 1 || (d.taps != 1 && d.taps != 27)) {
+      set_errorf("abx_conv3d_pack_all: bad layer %d", l);
+      return -1;
+    }
+    tab.d[l] = d;
+    tab.start[l + 1] = tab.start[l] + (int64_t)d.co * d.taps * d.ci;
+  }
+  hipLaunchKernelGGL(k_pack_all, dim3(blocks_for(tab.start[n])), dim3(256), 0, (hipStream_t)stream, tab);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
 extern "C" int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, int ci, int for_input_grad,
                                uint16_t* out) {
   if (!w || !out || co < 1 || ci < 1 || (ksize != 1 && ksize != 3)) {

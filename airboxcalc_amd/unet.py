@@ -155,11 +155,14 @@ class UNet:
         return t.data_ptr() + 4 * l.b_off
 
     def repack(self) -> None:
-        lib = abx.load()
-        for l in self.L:
-            fw, bw = self.packs[l.name]
-            _ck(lib.abx_conv3d_pack(self.stream, self.wptr(l), l.co, l.ksize, l.ci, 0, fw.data_ptr()))
-            _ck(lib.abx_conv3d_pack(self.stream, self.wptr(l), l.co, l.ksize, l.ci, 1, bw.data_ptr()))
+        """Both bf16 layouts of every layer's weights, in one launch."""
+        if getattr(self, "_pack_table", None) is None:
+            descs = (abx.PackDesc * len(self.L))()
+            for i, l in enumerate(self.L):
+                fw, bw = self.packs[l.name]
+                descs[i] = abx.PackDesc(self.wptr(l), fw.data_ptr(), bw.data_ptr(), l.co, l.ksize ** 3, l.ci)
+            self._pack_table = descs
+        _ck(abx.load().abx_conv3d_pack_all(self.stream, len(self.L), self._pack_table))
 
     def state(self) -> dict:
         return {"w": self.w.cpu().numpy(), "m": self.m.cpu().numpy(), "v": self.v.cpu().numpy(),

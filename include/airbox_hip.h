@@ -343,6 +343,18 @@ int abx_streamlines(abx_sim* sim, const abx_eval_geom* g, int b, double divisor,
  * which makes abx_conv3d_forward over dZ compute the input gradient. */
 int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, int ci, int for_input_grad, uint16_t* out);
 
+/* abx_conv3d_pack of several layers in one launch, both layouts of each: fwd
+ * as for_input_grad = 0, bwd as for_input_grad = 1 (taps 1 or 27).  At most
+ * ABX_PACK_MAX layers.  No reference counterpart. */
+#define ABX_PACK_MAX 32
+typedef struct {
+  const float* w;
+  uint16_t* fwd;
+  uint16_t* bwd;
+  int co, taps, ci;
+} abx_pack_desc;
+int abx_conv3d_pack_all(void* stream, int n, const abx_pack_desc* desc);
+
 /* y = act(conv(x, w) + bias) [+ y if accumulate]; act = ReLU if relu.  bias
  * may be NULL.  MFMA implicit GEMM, FP32 accumulation, bf16 result. */
 int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,

@@ -236,6 +236,32 @@ def test_pool_and_upsample_gated_backward_equal_backward_then_relu():
     assert (f32(db) == 0).mean() > 0.3 and (f32(ub) == 0).mean() > 0.3
 
 
+def test_pack_all_equals_per_layer_pack():
+    """abx_conv3d_pack_all (every layer, both layouts, one launch) writes what
+    abx_conv3d_pack writes layer by layer, bit for bit."""
+    say("pack all layers in one launch")
+    rng = np.random.default_rng(14)
+    lib = abx.load()
+    shapes = [(16, 3, 16), (32, 3, 48), (64, 1, 16), (16, 3, 32)]
+    ws, ref, got = [], [], []
+    descs = (abx.PackDesc * len(shapes))()
+    for i, (co, k, ci) in enumerate(shapes):
+        w = torch.as_tensor((rng.standard_normal(co * k ** 3 * ci) / 10).astype(np.float32)).to(_dev())
+        ws.append(w)
+        pair_ref = [torch.empty(co * k ** 3 * ci, dtype=torch.bfloat16, device=_dev()) for _ in range(2)]
+        for flip in (0, 1):
+            ck(lib.abx_conv3d_pack(stream(), w.data_ptr(), co, k, ci, flip, pair_ref[flip].data_ptr()))
+        pair = [torch.empty_like(pair_ref[0]) for _ in range(2)]
+        descs[i] = abx.PackDesc(w.data_ptr(), pair[0].data_ptr(), pair[1].data_ptr(), co, k ** 3, ci)
+        ref.append(pair_ref)
+        got.append(pair)
+    ck(lib.abx_conv3d_pack_all(stream(), len(shapes), descs))
+    torch.cuda.synchronize()
+    for r, g in zip(ref, got):
+        for a, b in zip(r, g):
+            assert torch.equal(a.view(torch.int16), b.view(torch.int16))
+
+
 def test_mse_vector_kernel_equals_generic():
     """The MSE loss on 16-byte rows (k_mse_v) and the generic kernel (taken
     when the gradient buffer is not 16-byte aligned) give the same loss
