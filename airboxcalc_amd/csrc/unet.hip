@@ -365,8 +365,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
 // grid (S, co blocks * ci chunks * tap groups), 256 threads.  A workgroup owns
 // output channels [co0, co0+16*NCT), input channels [ci0, ci0+16*NCI) and taps
 // [t0, t0+tg); its tg*NCT*NCI tiles of 16x16 are dealt to the 4 waves, at most
-// 8 each.  part: fp32 [S][Co][TAPS][Ci]; partb: fp32 [S][Co] (bias gradient,
-// written by the workgroups of ci chunk 0 and the last tap group).
+// 8 each.  part: fp32 rows [S][Co TAPS Ci + Co]: the weight partials
+// [Co][TAPS][Ci], then the bias partials [Co] (written by the workgroups of ci
+// chunk 0 and the last tap group when partb, a flag, is not null).
 // Bias gradient on the matrix cores: in the workgroups of input-channel chunk
 // 0 and the last tap group (the group with the fewest taps), wave
 // 4 - NCT + c multiplies the dZ fragment of output-channel tile c by a
@@ -465,7 +466,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
   }
 
   // C element i of lane l: output channel 4*lg + i, input channel lr of the tile
-  float* pp = part + (int64_t)blockIdx.x * Co * TAPS * Ci;
+  const int64_t nw = (int64_t)Co * TAPS * Ci;
+  float* pp = part + (int64_t)blockIdx.x * (nw + Co);
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) {
     const int q = wave + 4 * u;
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
       pp[((int64_t)co * TAPS + tap) * Ci + ci] = acc[u][i];
     }
   }
-  if (bias_wave && lr == 0) bias_store(partb + (int64_t)blockIdx.x * Co + co0 + bwt * 16, accb, lg);
+  if (bias_wave && lr == 0) bias_store(pp + nw + co0 + bwt * 16, accb, lg);
 }
 
 // Weight gradient on gfx950's transposing LDS reads: the partial sums of
@@ -634,7 +636,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
   }
 
   // C element i of lane l: output channel 4*lg + i, input channel lr of the tile
-  float* pp = part + (int64_t)blockIdx.x * Co * TAPS * Ci;
+  const int64_t nw = (int64_t)Co * TAPS * Ci;
+  float* pp = part + (int64_t)blockIdx.x * (nw + Co);
 #pragma unroll
   for (int u = 0; u < MAXW; ++u) {
     const int q = wave + 4 * u;
@@ -647,15 +650,17 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
       pp[((int64_t)co * TAPS + tap) * Ci + ci] = acc[u][i];
     }
   }
-  if (bias_wave && lr == 0) bias_store(partb + (int64_t)blockIdx.x * Co + co0 + bwt * 16, accb, lg);
+  if (bias_wave && lr == 0) bias_store(pp + nw + co0 + bwt * 16, accb, lg);
 }
 
-// out[i] = (accum ? out[i] : 0) + sum_s part[s*n + i].  A workgroup owns 64
+// r[i] = (accum ? r[i] : 0) + sum_s part[s*stride + i] for i < n, where r is
+// out for i < n1 and out2 (from index n1 on) otherwise.  A workgroup owns 64
 // consecutive i; its 4 waves sum the partials s = g, g+4, g+8, ... (8
 // independent loads in flight per lane), then wave 0 adds the four sums in a
 // fixed order, so the result does not depend on scheduling.
 __global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, int S, int64_t n,
-                                                      float* __restrict__ out, int accum) {
+                                                      int64_t stride, int64_t n1, float* __restrict__ out,
+                                                      float* __restrict__ out2, int accum) {
   __shared__ float red[4][64];
   const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
   const int64_t i = (int64_t)blockIdx.x * 64 + lane;
@@ -665,17 +670,18 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ 
     for (; k + 28 < S; k += 32) {
       float v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = part[(int64_t)(k + 4 * u) * n + i];
+      for (int u = 0; u < 8; ++u) v[u] = part[(int64_t)(k + 4 * u) * stride + i];
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc += v[u];
     }
-    for (; k < S; k += 4) acc += part[(int64_t)k * n + i];
+    for (; k < S; k += 4) acc += part[(int64_t)k * stride + i];
   }
   red[g][lane] = acc;
   __syncthreads();
   if (g == 0 && i < n) {
     const float s = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
-    out[i] = accum ? out[i] + s : s;
+    float* o = i < n1 ? out + i : out2 + (i - n1);
+    *o = accum ? *o + s : s;
   }
 }
 
@@ -1114,13 +1120,13 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
                (long long)((int64_t)sh.S * (nw + co) * 4));
     return -1;
   }
-  float* part = workspace;
-  float* partb = workspace + (int64_t)sh.S * nw;
+  float* part = workspace;              // rows of nw weight partials, then co bias partials
+  float* partb = db ? workspace : nullptr;  // a flag: the bias partials are written
   const int nt = (int)tiles_of(n, d, h, w);
   hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)sh.S, (unsigned)sh.blocks_y);
 #define ABX_LAUNCH(KERNEL)                                                                                     \
-  hipLaunchKernelGGL(KERNEL, grid, dim3(256), 0, s, x, xs, dz, dzs, part, db ? partb : nullptr, d, h, w, ci, co, nt, \
+  hipLaunchKernelGGL(KERNEL, grid, dim3(256), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt, \
                      sh.tg)
 #define ABX_WG(T_, A, B)                                  \
   do {                                                    \
@@ -1143,10 +1149,10 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
 #undef ABX_WG
 #undef ABX_LAUNCH
   ABX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nw, 64)), dim3(256), 0, s, part, sh.S, nw, dw, accumulate);
-  if (db)
-    hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(co, 64)), dim3(256), 0, s, partb, sh.S, (int64_t)co, db,
-                       accumulate);
+  // weights and bias in one reduction (rows of nw + co partials)
+  const int64_t nred = nw + (db ? co : 0);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nred, 64)), dim3(256), 0, s, part, sh.S, nred, nw + co, nw, dw,
+                     db, accumulate);
   ABX_HIP(hipGetLastError());
   return 0;
 }
