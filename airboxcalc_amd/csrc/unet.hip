@@ -735,30 +735,32 @@ __global__ void k_maxpool2_bwd(const u16* __restrict__ x, int xs, const u16* __r
   int64_t fv[8];
   float m[8];
   int am[8];
+  u16x8 xa[8];  // the window's inputs, kept for the gate
   for (int c = 0; c < 8; ++c) {
     m[c] = -__builtin_inff();
     am[c] = 0;
   }
+#pragma unroll
   for (int o = 0; o < 8; ++o) {
     fv[o] = (((int64_t)n * D + 2 * cz + (o >> 2)) * H + 2 * cy + ((o >> 1) & 1)) * W + 2 * cx + (o & 1);
-    const u16x8 a = *(const u16x8*)(x + fv[o] * xs + 8 * g);
+    xa[o] = *(const u16x8*)(x + fv[o] * xs + 8 * g);
     for (int c = 0; c < 8; ++c)
-      if (bf2f(a[c]) > m[c]) {
-        m[c] = bf2f(a[c]);
+      if (bf2f(xa[o][c]) > m[c]) {
+        m[c] = bf2f(xa[o][c]);
         am[c] = o;
       }
   }
   const u16x8 gy = *(const u16x8*)(dy + v * dys + 8 * g);
+#pragma unroll
   for (int o = 0; o < 8; ++o) {
     u16* p = dx + fv[o] * dxs + 8 * g;
     u16x8 cur = accum ? *(const u16x8*)p : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
-    // gate: x is the output of a ReLU; its backward zeroes dx where x <= 0
-    const u16x8 xo = gate ? *(const u16x8*)(x + fv[o] * xs + 8 * g) : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
     u16x8 out;
     for (int c = 0; c < 8; ++c) {
       float val = accum ? bf2f(cur[c]) : 0.f;
       if (am[c] == o) val += bf2f(gy[c]);
-      out[c] = gate && !(bf2f(xo[c]) > 0.f) ? (u16)0 : f2bf(val);
+      // gate: x is the output of a ReLU; its backward zeroes dx where x <= 0
+      out[c] = gate && !(bf2f(xa[o][c]) > 0.f) ? (u16)0 : f2bf(val);
     }
     *(u16x8*)p = out;
   }
