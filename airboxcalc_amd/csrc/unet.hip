@@ -962,6 +962,7 @@ bool conv_args_ok(const char* fn, const void* x, int xs, int n, int d, int h, in
 
 using namespace abx;
 
+namespace abx {
 namespace {
 // Every layer's two packed layouts in one launch: thread i finds its layer by
 // the running element offsets (at most ABX_PACK_MAX layers) and writes the
@@ -986,6 +987,7 @@ __global__ void k_pack_all(PackTable tab) {
   d.bwd[((int64_t)ci * T + (T - 1 - t)) * Co + co] = v;
 }
 }  // namespace
+}  // namespace abx
 
 extern "C" int abx_conv3d_pack_all(void* stream, int n, const abx_pack_desc* desc) {
   if (n < 1 || n > ABX_PACK_MAX || !desc) {
