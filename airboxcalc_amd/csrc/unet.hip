@@ -208,8 +208,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 //
 // Workgroups are persistent over tiles.  With 16 input channels the weights
 // are loaded once per workgroup; each thread owns fixed slots of the brick
-// (decoded once), and the global loads of the next (tile, chunk) brick are
-// issued before the MFMAs of the current one, so their latency is hidden.
+// (decoded once).  The bias is read once per workgroup, not per tile, so the
+// epilogue issues no load that it has to wait for (unless it accumulates or
+// is gated).
 constexpr int F16X = 16, F16Y = 4, F16Z = 4;
 constexpr int F16BX = F16X + 2, F16BY = F16Y + 2, F16BZ = F16Z + 2, F16BV = F16BX * F16BY * F16BZ;
 constexpr int F16SP = 24;  // u16 per brick voxel: 16 channels + 8 pad (48-byte stride: conflict-free 16-byte reads)
@@ -286,6 +287,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
   };
 
   if (nch == 1) load_weights(0);
+  // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
+  const int co = co0 + 4 * lg;
+  float bv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[co + i] : 0.f;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     f32x4 acc[F16Y];
 #pragma unroll
@@ -327,12 +333,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
       }
     }
 
-    // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
     const F16Tile tl = f16_tile(t, D, H, W);
-    const int gx = tl.x0 + lr, gz = tl.z0 + wave, co = co0 + 4 * lg;
-    float bv[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[co + i] : 0.f;
+    const int gx = tl.x0 + lr, gz = tl.z0 + wave;
 #pragma unroll
     for (int r = 0; r < F16Y; ++r) {
       const int gy = tl.y0 + r;
