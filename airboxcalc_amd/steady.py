@@ -2,7 +2,7 @@
 
 The reference has no convergence criterion: it always runs a fixed number of
 steps and averages the last `avg` of them (engine.js:341-358, 613-630).  This
-module defines the criterion the warm-start study is judged by:
+module defines the criterion the warm-start study measures against:
 
   run in windows of `window` steps, averaging each window on the GPU, and
   evaluate it (gamma-core and zeta, engine.js:632-691); the solve is steady

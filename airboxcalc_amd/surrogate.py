@@ -430,7 +430,7 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
     prepared on the GPU at dx.  Runs per box:
       fixed       from rest, the reference's fixed-step protocol (fixed_steps,
                   averaging the last fixed_avg): the result the others are
-                  judged by;
+                  measured against;
       cold        from rest to the steady criterion (steady.solve_to_steady);
       warm        from the surrogate's equilibrium to the same criterion;
       warm_fixed  from the surrogate's equilibrium, the fixed protocol
@@ -485,7 +485,7 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
             b = np.asarray(b, np.float64).reshape(4, -1)[:3]
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-        def judge(r, ref):
+        def compare(r, ref):
             dg = abs(r.metrics["gammaCore"] - ref.metrics["gammaCore"])
             dz = abs(r.metrics["zeta"] - ref.metrics["zeta"]) / abs(ref.metrics["zeta"])
             return dict(gammaCore_abs=dg, zeta_rel=dz, within=bool(dg <= 0.005 and dz <= 0.02),
@@ -500,14 +500,14 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
         row = dict(seed=box.seed, ntri=int(tris.size // 9), grid=[g.nx, g.ny, g.nz], NF=int(NF), K=box.K,
                    aniso=box.aniso, stl_s=t1 - t0, prepare_s=t2 - t1, predict_s=timing["predict_s"],
                    fixed=run(fixed), cold=run(cold), warm=run(warm),
-                   cold_vs_fixed=judge(cold, fixed), warm_vs_fixed=judge(warm, fixed),
-                   warm_fixed=run(warm_fixed), warm_fixed_vs_fixed=judge(warm_fixed, fixed),
-                   cold_short_vs_fixed=judge(cold_short, fixed),
+                   cold_vs_fixed=compare(cold, fixed), warm_vs_fixed=compare(warm, fixed),
+                   warm_fixed=run(warm_fixed), warm_fixed_vs_fixed=compare(warm_fixed, fixed),
+                   cold_short_vs_fixed=compare(cold_short, fixed),
                    prediction_u_err_vs_fixed=uerr(init, fixed.acc))
         if f64 is not None:
             row["fp64"] = run(f64)
-            row["fixed_vs_fp64"] = judge(fixed, f64)
-            row["warm_vs_fp64"] = judge(warm, f64)
+            row["fixed_vs_fp64"] = compare(fixed, f64)
+            row["warm_vs_fp64"] = compare(warm, f64)
         rows.append(row)
         if log:
             log(f"box {bi + 1}/{len(boxes)} seed {box.seed}: {row['ntri']} tris, grid {row['grid']}, "

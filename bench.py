@@ -344,7 +344,7 @@ def main():
                     help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this build "
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
               "(the single-grid solve runs on one MI355X)", file=sys.stderr)
         sys.exit(2)
 
