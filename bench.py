@@ -109,8 +109,8 @@ def bench_warmstart(a, dev):
     (the 1500-box 2 mm-target network at 2 mm, the 4.8 mm network at 4.8 mm).
     Reports the wall speedup to the steady criterion (windows of 1000 steps,
     gamma-core within 0.005 and zeta within 2 % between windows) and of the
-    reference's fixed protocol without its transient (2000 steps averaging
-    2000 from the prediction against 6000 averaging 2000 from rest)."""
+    reference's fixed protocol without its transient (1500 steps averaging
+    1500 from the prediction against 6000 averaging 2000 from rest)."""
     from airboxcalc_amd import surrogate as S, unet as U
     dx = a.dx if a.dx != 1.0 else 2.0
     n = a.variants if a.variants != 64 else 16
@@ -120,7 +120,8 @@ def bench_warmstart(a, dev):
     norm, meta = S.load(weights, net)
     boxes = S.family(range(1_000_000, 1_000_000 + n))
     t = time.perf_counter()
-    rep = S.warm_start_study(dev, net, norm, boxes, dx=dx, fp64_every=0, log=lambda m: log(m))
+    rep = S.warm_start_study(dev, net, norm, boxes, dx=dx, fp64_every=0, warm_steps=1500, warm_avg=1500,
+                             log=lambda m: log(m))
     wall = time.perf_counter() - t
     sm = rep["summary"]
     ok = [r for r in rep["rows"] if "error" not in r]
