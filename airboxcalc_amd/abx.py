@@ -74,7 +74,8 @@ class SimInfo(C.Structure):
     _fields_ = [("NF", C.c_int64), ("Ain", C.c_int), ("Af", C.c_int), ("Lp", C.c_int), ("nM", C.c_int),
                 ("nT", C.c_int), ("u0", C.c_double), ("tau0", C.c_double), ("sig", C.c_double),
                 ("aniso", C.c_double), ("cs2s", C.c_double), ("precision", C.c_int), ("B", C.c_int),
-                ("device_bytes", C.c_int64)]
+                ("device_bytes", C.c_int64), ("plane", C.c_int64), ("nyp", C.c_int), ("nzp", C.c_int),
+                ("kofs", C.c_int), ("nseg", C.c_int)]
 
 
 class EvalGeom(C.Structure):
@@ -151,8 +152,7 @@ def load() -> C.CDLL:
         "abx_sim_clear_acc": (i32, [vp]),
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
-        "abx_sim_set_launch": (i32, [vp, i32, i32, i32, i32]),
-        "abx_sim_set_pipeline": (i32, [vp, i32]),
+        "abx_sim_set_launch": (i32, [vp, i32, i32, i32]),
         "abx_lattice_set_row_shift": (i32, [i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
@@ -509,19 +509,14 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, kernel: int = 4, xcd_map: int = -1, chunk: int = 256, nt: int = 0) -> None:
-        """kernel 4 = AA pattern (default); 0 wide, 1 narrow, 2 segment, 3
-        persistent segment (two lattices); 5 AA with per-cell geometry words;
-        6 AA pattern out of place (two lattices).
-        xcd_map: block order across XCDs (0 round robin, 1 contiguous, G > 1
-        groups of G blocks, -1 one band of rows j per XCD: the library's
-        default)."""
-        _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, chunk, nt))
-
-    def set_pipeline(self, planes: int) -> None:
-        """AA pairs issued plane-pipelined over bands of `planes` lattice
-        planes (kernel 4); 0 = one launch per step.  Same values bit for bit."""
-        _check(load().abx_sim_set_pipeline(self.h, planes))
+    def set_launch(self, kernel: int = 4, xcd_map: int = -1, nt: int = 0) -> None:
+        """kernel 4 = AA pattern in one lattice (default), 2 = two lattices
+        with pull streaming.  xcd_map: block order across XCDs (0 round
+        robin, 1 contiguous, G > 1 groups of G blocks, -1 one band of rows j
+        per XCD: the library's default).  nt: non-temporal population
+        accesses of the AA steps (bits 0/1 even-step loads/stores, bits 2/3
+        odd-step loads/stores)."""
+        _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, nt))
 
     def set_graph(self, enable: bool = True) -> None:
         _check(load().abx_sim_set_graph(self.h, 1 if enable else 0))

@@ -165,7 +165,7 @@ extern "C" int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_b
       return -1;
     }
     // every member on the AA pattern, in the same (swapped) form
-    if (abx_sim_set_launch(s, 4, 0, 256, 0) || sim_normalize(s)) {
+    if (abx_sim_set_launch(s, 4, 0, 0) || sim_normalize(s)) {
       delete bt;
       return -1;
     }

@@ -607,5 +607,10 @@ extern "C" int abx_streamlines(abx_sim* s, const abx_eval_geom* g, int b, double
   }
   *npts_out = npts;
   *nlines_out = nlines;
+  if ((pts && 3 * npts > pts_cap) || (spd && npts > spd_cap) || (off && nlines + 1 > off_cap)) {
+    set_errorf("abx_streamlines: output buffers too small for %lld points on %lld lines", (long long)npts,
+               (long long)nlines);
+    return -1;
+  }
   return 0;
 }

@@ -5,7 +5,8 @@
 // 19-entry neighbour table (engine.js:241-251), 76 B of indices per cell that
 // every step re-reads.  Here the lattice is dense: the nx*ny*nz grid with a
 // one-cell halo on every side and the k extent padded to a multiple of 32,
-//     D(i,j,k) = ((i+1)*nyp + (j+1))*nzp + (k+1),  nyp = ny+2, nzp = 32*ceil((nz+2)/32),
+//     D(i,j,k) = ((i+1)*nyp + (j+1))*nzp + (k+kofs),  nyp = ny+2,
+//     nzp = 32*ceil((nz+kofs+1)/32), kofs in [1, 32] (best_row_offset),
 // one plane of `plane` elements per population q (structure of arrays,
 // f[q*plane + D]).  Streaming pulls f_q(x - c_q) at the constant offset
 // off_q = (cx*nyp + cy)*nzp + cz; a neighbour that is not fluid (a wall, the
@@ -16,10 +17,10 @@
 //
 // Per cell, the only geometry data are bits: a fluid bit (bit-packed, 32
 // cells per word, staged per workgroup in LDS for the 18 neighbour tests) and
-// three type bits (inlet / outlet / porous, engine.js:252-254).  A workgroup
-// owns 256 consecutive lattice positions; only workgroups that hold fluid are
-// launched (the chunk list).  Algorithmic traffic per fluid cell update:
-// 19 loads + 19 stores of the storage type (152 B in float).
+// three type bits (inlet / outlet / porous, engine.js:252-254).  A wavefront
+// owns 64 consecutive lattice positions (a segment); only segments that hold
+// fluid are launched (the segment list).  Algorithmic traffic per fluid cell
+// update: 19 loads + 19 stores of the storage type (152 B in float).
 #include <math.h>
 
 #include <algorithm>
@@ -35,64 +36,9 @@
 
 namespace abx {
 
-// Narrow variant: one lattice position per lane, CH positions per
-// workgroup (64 or 256); NTL / NTS = non-temporal population loads / stores.
-template <typename S, typename M, bool ACC, bool NS, int CH, bool NTL, bool NTS>
-__global__ __launch_bounds__(CH) void k_stream_collide(StepArgs a) {
-  constexpr int WW = CH / 32 + 2;  // words per neighbour-row window
-  __shared__ uint32_t win[9][WW];
-  const int w = blockIdx.x;
-  const int ci = a.xcd_map ? (w & 7) * a.per_xcd + (w >> 3) : w;
-  if (ci >= a.nchunks) return;  // whole workgroup leaves together
-  const int b = blockIdx.y;
-  const int d0 = a.chunks[ci];
-  const int t = threadIdx.x;
-  // Fluid bits of the nine (di, dj) neighbour rows of the chunk, each window
-  // covering positions [d0 - (di*nyp + dj)*nzp - 32, ... + CH + 64).
-  for (int u = t; u < 9 * WW; u += CH) {
-    const int wi = u / WW, wo = u - wi * WW;
-    const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-    const int64_t base = (int64_t)d0 - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-    win[wi][wo] = a.fluid_bits[(base >> 5) + wo];
-  }
-  __syncthreads();
-  if (!((win[4][(t + 32) >> 5] >> ((t + 32) & 31)) & 1u)) return;
-  const int d = d0 + t;
-  const uint32_t tb = (uint32_t)d & 31u;
-  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
-  const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
-  M fi[19], fo[19];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const int p = t + 32 - CZ[q];
-    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const bool nbf = (win[wi][p >> 5] >> (p & 31)) & 1u;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-    const S* src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
-    fi[q] = (M)ld(src, NTL);
-  }
-  M s = 0;
-  if (NS) s = (M)a.ns[(int64_t)b * P + d];
-  M ux, uy, uz, rho;
-  collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
-#pragma unroll
-  for (int q = 0; q < 19; ++q) st(fout + q * P + d, (S)fo[q], NTS);
-  if (ACC) {
-    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-    ac[d] = (S)((M)ac[d] + ux);
-    ac[P + d] = (S)((M)ac[P + d] + uy);
-    ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
-    ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
-  }
-}
-
-// Segment variant: one position per lane, a workgroup of 256 lanes takes
+// Segment kernel: one position per lane, a workgroup of 256 lanes takes
 // four independent active 64-position segments (one per wavefront), so only
-// segments holding fluid occupy lanes.  PERSIST: a fixed grid whose
-// workgroups stride over the segment list.
+// segments holding fluid occupy lanes.
 //
 // MODE selects the propagation scheme:
 //  0  two lattices, pull: read fin[q](x - c_q) (or fin[opp q](x) behind a
@@ -104,8 +50,9 @@ __global__ __launch_bounds__(CH) void k_stream_collide(StepArgs a) {
 // Both schemes apply the same halfway bounce-back and give identical values;
 // the AA steps touch each population slot once for reading and once for
 // writing in place, which halves the lattice footprint and streams far more
-// efficiently (even steps are fully aligned).
-template <typename S, typename M, bool ACC, bool NS, bool PERSIST, int MODE>
+// efficiently (even steps are fully aligned).  NT: non-temporal population
+// loads (bit 0) / stores (bit 1).
+template <typename S, typename M, bool ACC, bool NS, int MODE, int NT>
 __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   __shared__ uint32_t win[4][9][4];
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
@@ -113,119 +60,25 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   const int nblk = (a.nchunks + 3) >> 2;
   // Block order across the 8 XCDs (workgroup x runs on XCD x % 8 as the
   // x-th of that XCD's workgroups, s = x / 8):
-  //   xcd_map 0: block x (round robin);
+  //   xcd_map 0: block x (round robin; the band order of the segment list
+  //   then gives each XCD one band of rows j);
   //   xcd_map 1: XCD x % 8 owns the contiguous range [(x % 8) per_xcd, ...);
   //   xcd_map G > 1: groups of G consecutive blocks per XCD, all XCDs moving
-  //   through the lattice together (block (s / G) 8G + (x % 8) G + s % G), so
-  //   neighbouring rows share an L2 while the active region stays narrow.
-  int first = (int)blockIdx.x;
-  if (!PERSIST && a.xcd_map == 1) first = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
-  if (!PERSIST && a.xcd_map > 1) {
-    const int sq = (int)(blockIdx.x >> 3), G = a.xcd_map;
-    first = (sq / G) * 8 * G + (int)(blockIdx.x & 7) * G + sq % G;
-  }
-  for (int blk = first; blk < nblk; blk += PERSIST ? gridDim.x : nblk) {
-    if (t < 4 * 36) {
-      const int ss = t / 36, sg = 4 * blk + ss;
-      (&win[ss][0][0])[t - 36 * ss] = sg < a.nchunks ? seg_window_word(a, a.chunks[sg], t - 36 * ss) : 0u;
-    }
-    __syncthreads();
-    const int sg = 4 * blk + wv;
-    if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE>(a, win[wv], a.chunks[sg], l, b);
-    if (!PERSIST) break;
-    __syncthreads();
-  }
-}
-
-// Plane-pipelined AA pair: the two steps of an (A, B) pair of AA steps are
-// issued as a sequence of launches that each run step A over one band of
-// lattice planes and step B over an earlier band that trails it by more than
-// one plane, so step B re-reads lines that step A wrote shortly before (they
-// are still in the last-level cache) instead of a whole sweep later.
-// Segments [a0, a0 + nA) run mode MA, segments [b0, b0 + nB) mode 3 - MA; the
-// host picks the bands so that no position touched by one range is touched by
-// the other and every B position's A-step inputs were written by earlier
-// launches.  Block order across XCDs as k_stream_collide_seg's xcd_map G > 1.
-template <typename S, typename M, bool ACC, bool NS, int MA>
-__global__ __launch_bounds__(256) void k_stream_collide_pipe(StepArgs a, int a0, int nA, int b0, int nB) {
-  __shared__ uint32_t win[4][9][4];
-  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int b = blockIdx.y;
-  const int nbA = (nA + 3) >> 2, nbB = (nB + 3) >> 2;
+  //   through the lattice together (block (s / G) 8G + (x % 8) G + s % G).
   int blk = (int)blockIdx.x;
+  if (a.xcd_map == 1) blk = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
   if (a.xcd_map > 1) {
-    const int sq = blk >> 3, G = a.xcd_map;
-    blk = (sq / G) * 8 * G + (blk & 7) * G + sq % G;
+    const int sq = (int)(blockIdx.x >> 3), G = a.xcd_map;
+    blk = (sq / G) * 8 * G + (int)(blockIdx.x & 7) * G + sq % G;
   }
-  if (blk >= nbA + nbB) return;  // whole workgroup leaves together
-  const bool first = blk < nbA;
-  const int base = first ? a0 : b0, n = first ? nA : nB, bb = first ? blk : blk - nbA;
+  if (blk >= nblk) return;  // whole workgroup leaves together
   if (t < 4 * 36) {
-    const int ss = t / 36, sg = 4 * bb + ss;
-    (&win[ss][0][0])[t - 36 * ss] = sg < n ? seg_window_word(a, a.chunks[base + sg], t - 36 * ss) : 0u;
+    const int ss = t / 36, sg = 4 * blk + ss;
+    (&win[ss][0][0])[t - 36 * ss] = sg < a.nchunks ? seg_window_word(a, a.chunks[sg], t - 36 * ss) : 0u;
   }
   __syncthreads();
-  const int sg = 4 * bb + wv;
-  if (sg >= n) return;
-  if (first)
-    seg_cell<S, M, ACC, NS, MA>(a, win[wv], a.chunks[base + sg], l, b);
-  else
-    seg_cell<S, M, ACC, NS, 3 - MA>(a, win[wv], a.chunks[base + sg], l, b);
-}
-
-// Per-cell geometry word of the metadata variant: bit q (q < 19) = the cell
-// x - c_q is fluid, bits 19/20/21 = inlet/outlet/porous, bit 31 = fluid.
-constexpr uint32_t META_INLET = 1u << 19, META_OUTLET = 1u << 20, META_POROUS = 1u << 21, META_FLUID = 1u << 31;
-
-// AA-pattern step reading one 32-bit geometry word per position instead of
-// staging bit windows in LDS: no barrier, no dependent load before the
-// population loads.  MODE 1 / 2 = even / odd AA step (as k_stream_collide_seg).
-template <typename S, typename M, bool ACC, bool NS, int MODE>
-__global__ __launch_bounds__(256) void k_stream_collide_meta(StepArgs a, const uint32_t* __restrict__ meta) {
-  const int t = threadIdx.x;
-  const int sg = 4 * blockIdx.x + (t >> 6);
-  if (sg >= a.nchunks) return;
-  const int d = a.chunks[sg] + (t & 63);
-  const uint32_t m = meta[d];
-  if (!(m & META_FLUID)) return;
-  const int b = blockIdx.y;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
-  M fi[19], fo[19];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const bool nbf = (m >> q) & 1u;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-    const S* src;
-    if (MODE == 1)
-      src = fin + q * P + d;
-    else
-      src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
-    fi[q] = (M)*src;
-  }
-  M s = 0;
-  if (NS) s = (M)a.ns[(int64_t)b * P + d];
-  M ux, uy, uz, rho;
-  collide_any<M, NS>(fi, m & META_INLET, m & META_OUTLET, m & META_POROUS, s, a, fo, ux, uy, uz, rho);
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    S* dst;
-    if (MODE == 1) {
-      dst = fout + OPP[q] * P + d;
-    } else {
-      const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-      dst = ((m >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
-    }
-    *dst = (S)fo[q];
-  }
-  if (ACC) {
-    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-    ac[d] = (S)((M)ac[d] + ux);
-    ac[P + d] = (S)((M)ac[P + d] + uy);
-    ac[2 * P + d] = (S)((M)ac[2 * P + d] + uz);
-    ac[3 * P + d] = (S)((M)ac[3 * P + d] + rho);
-  }
+  const int sg = 4 * blk + wv;
+  if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE, NT>(a, win[wv], a.chunks[sg], l, b);
 }
 
 template <typename S, typename M>
@@ -431,10 +284,7 @@ struct abx_sim {
   uint32_t* fluid_alloc = nullptr;
   const uint32_t* fluid_bits = nullptr;
   uint32_t* type_bits = nullptr;
-  int32_t* chunks = nullptr;      // active 256-position chunks
-  int nchunks = 0;
-  int32_t* chunks64 = nullptr;    // active 64-position chunks
-  uint32_t* meta = nullptr;       // per-position geometry words (kernel 5)
+  int32_t* chunks64 = nullptr;    // active 64-position segments
   int nchunks64 = 0;
   std::vector<int32_t> seg_host;  // host copy of chunks64 (ascending positions)
   // xcd_map -1: the segments reordered so that XCD x (workgroups x, x + 8, ...)
@@ -442,13 +292,10 @@ struct abx_sim {
   // sweeps it plane by plane; padded with position 0 (never fluid)
   int32_t* chunks_slab = nullptr;
   int nslab = 0;
-  // AA pairs issued plane-pipelined over bands of `pipe` lattice planes
-  // (kernel 4 only; 0 = one launch per step)
-  int pipe = 0;
-  // kernel 0 wide, 1 narrow, 2 segment, 3 segment persistent (two lattices),
-  // 6 segment AA pattern out of place (reads f[cur], writes f[cur ^ 1]),
-  // 4 segment AA pattern (one lattice), 5 AA pattern with per-cell geometry words
-  int kernel = 4, chunk = 256, nt = 0;
+  // kernel 4: AA pattern in one lattice (the default); 2: two lattices, pull.
+  // nt: non-temporal population accesses of the AA steps, bits 0/1 = loads /
+  // stores of the even step, bits 2/3 = loads / stores of the odd step.
+  int kernel = 4, nt = 0;
   // stored form of the populations: 0 plain (f[cur][q](x) = f_q), 1 AA
   // swapped (F[opp q](x) = f_q), 2 AA streamed (after an odd AA step)
   int form = 1;
@@ -522,22 +369,20 @@ int dalloc(abx_sim* s, T** p, size_t count) {
 }
 
 int ensure_second_lattice(abx_sim* s);
-int ensure_meta(abx_sim* s);
 
 StepArgs step_args(abx_sim* s, int doacc) {
   StepArgs a;
   a.fin = s->f[s->cur];
-  a.fout = (s->kernel == 4 || s->kernel == 5) ? s->f[s->cur] : s->f[s->cur ^ 1];
+  a.fout = s->kernel == 4 ? s->f[s->cur] : s->f[s->cur ^ 1];
   a.acc = doacc ? s->acc : nullptr;
   a.ns = s->ns;
   a.fluid_bits = s->fluid_bits;
   a.type_bits = s->type_bits;
-  const bool seg64 = s->kernel != 1 || s->chunk == 64;
-  a.chunks = seg64 ? s->chunks64 : s->chunks;
-  a.nchunks = seg64 ? s->nchunks64 : s->nchunks;
+  a.chunks = s->chunks64;
+  a.nchunks = s->nchunks64;
   a.xcd_map = s->xcd_map;
   if (s->xcd_map < 0) {
-    if (s->chunks_slab && (s->kernel == 2 || s->kernel == 4 || s->kernel == 6)) {
+    if (s->chunks_slab) {
       a.chunks = s->chunks_slab;
       a.nchunks = s->nslab;
     }
@@ -555,129 +400,62 @@ StepArgs step_args(abx_sim* s, int doacc) {
   return a;
 }
 
-template <typename S, typename M, int CH, bool NTL, bool NTS>
-void launch_step_c(abx_sim* s, const StepArgs& a, int doacc) {
-  const unsigned gx = a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)a.nchunks;
-  dim3 grid(gx, (unsigned)s->B), block(CH);
-  const bool ns = a.ns != nullptr;
-  if (doacc) {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide<S, M, true, true, CH, NTL, NTS>), grid, block, 0, s->stream, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide<S, M, true, false, CH, NTL, NTS>), grid, block, 0, s->stream, a);
-  } else {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide<S, M, false, true, CH, NTL, NTS>), grid, block, 0, s->stream, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide<S, M, false, false, CH, NTL, NTS>), grid, block, 0, s->stream, a);
-  }
-}
-
-template <typename S, typename M>
-void launch_wide(abx_sim* s, StepArgs a, int doacc) {
-  constexpr int SPB = 256 / (64 / (16 / (int)sizeof(S)));
-  const int nblk = (a.nchunks + SPB - 1) / SPB;
-  a.per_xcd = (nblk + 7) / 8;
-  const unsigned gx = a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk;
-  dim3 grid(gx, (unsigned)s->B), block(256);
-  const bool ns = a.ns != nullptr;
-  if (doacc) {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_wide<S, M, true, true>), grid, block, 0, s->stream, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide_wide<S, M, true, false>), grid, block, 0, s->stream, a);
-  } else {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_wide<S, M, false, true>), grid, block, 0, s->stream, a);
-    else
-      hipLaunchKernelGGL((k_stream_collide_wide<S, M, false, false>), grid, block, 0, s->stream, a);
-  }
-}
-
-template <typename S, typename M, bool PERSIST, int MODE>
+template <typename S, typename M, int MODE, int NT>
 void launch_seg(abx_sim* s, StepArgs a, int doacc) {
   const int nblk = (a.nchunks + 3) / 4;
   a.per_xcd = (nblk + 7) / 8;
-  unsigned gx = PERSIST ? (unsigned)std::min(nblk, s->dev->ncu * 8)
-                        : (a.xcd_map ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk);
-  if (!PERSIST && a.xcd_map > 1) gx = (unsigned)((nblk + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
+  unsigned gx = a.xcd_map == 1 ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk;
+  if (a.xcd_map > 1) gx = (unsigned)((nblk + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
   dim3 grid(gx, (unsigned)s->B), block(256);
   const bool ns = a.ns != nullptr;
   if (doacc) {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, PERSIST, MODE>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, MODE, NT>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, PERSIST, MODE>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, MODE, NT>), grid, block, 0, s->stream, a);
   } else {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, PERSIST, MODE>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, MODE, NT>), grid, block, 0, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, PERSIST, MODE>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, MODE, NT>), grid, block, 0, s->stream, a);
   }
 }
 
+// AA step of mode MODE with the non-temporal flags nt (float storage only;
+// double storage always uses cached accesses).
 template <typename S, typename M, int MODE>
-void launch_meta(abx_sim* s, const StepArgs& a, int doacc) {
-  dim3 grid((unsigned)((a.nchunks + 3) / 4), (unsigned)s->B), block(256);
-  const bool ns = a.ns != nullptr;
-  if (doacc) {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_meta<S, M, true, true, MODE>), grid, block, 0, s->stream, a, s->meta);
-    else
-      hipLaunchKernelGGL((k_stream_collide_meta<S, M, true, false, MODE>), grid, block, 0, s->stream, a, s->meta);
-  } else {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_meta<S, M, false, true, MODE>), grid, block, 0, s->stream, a, s->meta);
-    else
-      hipLaunchKernelGGL((k_stream_collide_meta<S, M, false, false, MODE>), grid, block, 0, s->stream, a, s->meta);
+void launch_aa(abx_sim* s, const StepArgs& a, int doacc, int nt) {
+  if constexpr (sizeof(S) == 4) {
+    switch (nt & 3) {
+      case 1:
+        return launch_seg<S, M, MODE, 1>(s, a, doacc);
+      case 2:
+        return launch_seg<S, M, MODE, 2>(s, a, doacc);
+      case 3:
+        return launch_seg<S, M, MODE, 3>(s, a, doacc);
+      default:
+        break;
+    }
   }
+  launch_seg<S, M, MODE, 0>(s, a, doacc);
 }
 
 template <typename S, typename M>
 void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
-  const bool ntl = s->nt & 1, nts = (s->nt >> 1) & 1;
-  if (s->kernel == 5) {
-    if (s->form == 2)
-      launch_meta<S, M, 1>(s, a, doacc);
-    else
-      launch_meta<S, M, 2>(s, a, doacc);
-  } else if (s->kernel == 4 || s->kernel == 6) {
+  if (s->kernel == 4) {
     // AA pattern: an odd step follows the swapped form, an even one the streamed form
     if (s->form == 2)
-      launch_seg<S, M, false, 1>(s, a, doacc);
+      launch_aa<S, M, 1>(s, a, doacc, s->nt);
     else
-      launch_seg<S, M, false, 2>(s, a, doacc);
-  } else if (s->kernel == 0) {
-    launch_wide<S, M>(s, a, doacc);
-  } else if (s->kernel == 2) {
-    launch_seg<S, M, false, 0>(s, a, doacc);
-  } else if (s->kernel == 3) {
-    launch_seg<S, M, true, 0>(s, a, doacc);
-  } else if (s->chunk == 64) {
-    if (ntl && nts)
-      launch_step_c<S, M, 64, true, true>(s, a, doacc);
-    else if (ntl)
-      launch_step_c<S, M, 64, true, false>(s, a, doacc);
-    else if (nts)
-      launch_step_c<S, M, 64, false, true>(s, a, doacc);
-    else
-      launch_step_c<S, M, 64, false, false>(s, a, doacc);
+      launch_aa<S, M, 2>(s, a, doacc, s->nt >> 2);
   } else {
-    if (ntl && nts)
-      launch_step_c<S, M, 256, true, true>(s, a, doacc);
-    else if (ntl)
-      launch_step_c<S, M, 256, true, false>(s, a, doacc);
-    else if (nts)
-      launch_step_c<S, M, 256, false, true>(s, a, doacc);
-    else
-      launch_step_c<S, M, 256, false, false>(s, a, doacc);
+    launch_seg<S, M, 0, 0>(s, a, doacc);
   }
 }
 
 int launch_step(abx_sim* s, int doacc) {
-  if ((s->kernel != 1 || s->chunk == 64 ? s->nchunks64 : s->nchunks) == 0) return 0;
-  if ((s->kernel < 4 || s->kernel == 6) && ensure_second_lattice(s)) return -1;
-  if (s->kernel == 5 && ensure_meta(s)) return -1;
+  if (s->nchunks64 == 0) return 0;
+  if (s->kernel != 4 && ensure_second_lattice(s)) return -1;
   StepArgs a = step_args(s, doacc);
   switch (s->prec) {
     case ABX_F32:
@@ -690,96 +468,14 @@ int launch_step(abx_sim* s, int doacc) {
       launch_step_t<double, double>(s, a, doacc);
   }
   ABX_HIP(hipGetLastError());
-  if (s->kernel >= 4) s->form = s->form == 2 ? 1 : 2;
-  if (s->kernel < 4 || s->kernel == 6) s->cur ^= 1;
-  return 0;
-}
-
-template <typename S, typename M, int MA>
-void launch_pipe_t(abx_sim* s, const StepArgs& a, int doacc, int a0, int nA, int b0, int nB) {
-  const int nb = (nA + 3) / 4 + (nB + 3) / 4;
-  if (nb == 0) return;
-  unsigned gx = (unsigned)nb;
-  if (a.xcd_map > 1) gx = (unsigned)((nb + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
-  dim3 grid(gx, (unsigned)s->B), block(256);
-  const bool ns = a.ns != nullptr;
-  if (doacc) {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, true, true, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
-    else
-      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, true, false, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
-  } else {
-    if (ns)
-      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, false, true, MA>), grid, block, 0, s->stream, a, a0, nA, b0, nB);
-    else
-      hipLaunchKernelGGL((k_stream_collide_pipe<S, M, false, false, MA>), grid, block, 0, s->stream, a, a0, nA, b0,
-                         nB);
-  }
-}
-
-template <typename S, typename M>
-void launch_pipe_m(abx_sim* s, const StepArgs& a, int doacc, int ma, int a0, int nA, int b0, int nB) {
-  if (ma == 1)
-    launch_pipe_t<S, M, 1>(s, a, doacc, a0, nA, b0, nB);
+  if (s->kernel == 4)
+    s->form = s->form == 2 ? 1 : 2;
   else
-    launch_pipe_t<S, M, 2>(s, a, doacc, a0, nA, b0, nB);
-}
-
-// One pair of AA steps issued plane-pipelined (k_stream_collide_pipe).  Band c
-// of step A holds the segments starting in [c W, (c + 1) W), W = pipe lattice
-// planes; launch c runs it together with the segments of step B that start in
-// [c' W - GAP, c W - GAP) (c' = c - 1).  GAP exceeds the largest neighbour
-// offset |off_q| = nyp nzp + nzp + 1 by more than a segment, so (i) no
-// position touched by step B in launch c is touched by step A in launch c,
-// and (ii) every input of step B comes from step A launches issued before.
-// The stored form is the same before and after the pair.
-int launch_pair_pipelined(abx_sim* s, int doacc) {
-  const std::vector<int32_t>& sg = s->seg_host;
-  const int n = (int)sg.size();
-  if (n == 0) return 0;
-  StepArgs a = step_args(s, doacc);
-  a.chunks = s->chunks64;  // bands index the position-ordered list
-  a.nchunks = s->nchunks64;
-  const int64_t ipl = (int64_t)s->nyp * s->nzp;
-  const int64_t W = (int64_t)s->pipe * ipl;
-  const int64_t GAP = ipl + s->nzp + 128;
-  const int ma = s->form == 2 ? 1 : 2;  // streamed form: even step first
-  auto lb = [&](int64_t x) -> int {
-    if (x <= 0) return 0;
-    return (int)(std::lower_bound(sg.begin(), sg.end(), x, [](int32_t v, int64_t y) { return (int64_t)v < y; }) -
-                 sg.begin());
-  };
-  const int nc = (int)((int64_t)sg.back() / W) + 1;
-  int b0 = 0;
-  for (int c = 0; c <= nc; ++c) {
-    int a0 = 0, nA = 0;
-    if (c < nc) {
-      a0 = lb((int64_t)c * W);
-      nA = lb((int64_t)(c + 1) * W) - a0;
-    }
-    const int b1 = c == nc ? n : (c == 0 ? 0 : std::max(b0, lb((int64_t)c * W - GAP)));
-    const int nB = b1 - b0;
-    switch (s->prec) {
-      case ABX_F32:
-        launch_pipe_m<float, float>(s, a, doacc, ma, a0, nA, b0, nB);
-        break;
-      case ABX_F32_F64:
-        launch_pipe_m<float, double>(s, a, doacc, ma, a0, nA, b0, nB);
-        break;
-      default:
-        launch_pipe_m<double, double>(s, a, doacc, ma, a0, nA, b0, nB);
-    }
-    b0 = b1;
-  }
-  ABX_HIP(hipGetLastError());
+    s->cur ^= 1;
   return 0;
 }
 
-// n steps: AA pairs plane-pipelined when enabled, the rest one launch each.
 int launch_steps(abx_sim* s, int n, int doacc) {
-  if (s->kernel == 4 && s->pipe > 0 && s->nchunks64 > 0)
-    for (; n >= 2; n -= 2)
-      if (launch_pair_pipelined(s, doacc)) return -1;
   for (int i = 0; i < n; ++i)
     if (launch_step(s, doacc)) return -1;
   return 0;
@@ -798,36 +494,8 @@ int fill_cold(abx_sim* s) {
   }
   ABX_HIP(hipGetLastError());
   // WT is symmetric under q -> opp q, so the fill is valid in either form
-  s->form = s->kernel >= 4 ? 1 : 0;
+  s->form = s->kernel == 4 ? 1 : 0;
   s->cur = 0;
-  return 0;
-}
-
-// Per-position geometry words for kernel 5, built once from the fluid and
-// type bits.
-__global__ void k_build_meta(const uint32_t* fb, const uint32_t* tb, int64_t plane, int nyp, int nzp,
-                             uint32_t* meta) {
-  const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= plane) return;
-  auto bit = [&](int64_t x) { return (fb[x >> 5] >> (x & 31)) & 1u; };
-  uint32_t m = 0;
-  if (bit(d)) {
-    m = META_FLUID;
-#pragma unroll
-    for (int q = 0; q < 19; ++q) m |= bit(d - ((int64_t)(CX[q] * nyp + CY[q]) * nzp + CZ[q])) << q;
-    const uint32_t* w = tb + 3 * (d >> 5);
-    const uint32_t sh = (uint32_t)(d & 31);
-    m |= ((w[0] >> sh) & 1u) << 19 | ((w[1] >> sh) & 1u) << 20 | ((w[2] >> sh) & 1u) << 21;
-  }
-  meta[d] = m;
-}
-
-int ensure_meta(abx_sim* s) {
-  if (s->meta) return 0;
-  if (dalloc(s, &s->meta, (size_t)s->plane)) return -1;
-  hipLaunchKernelGGL(k_build_meta, dim3(nblk(s->plane, 256)), dim3(256), 0, s->stream, s->fluid_bits, s->type_bits,
-                     s->plane, s->nyp, s->nzp, s->meta);
-  ABX_HIP(hipGetLastError());
   return 0;
 }
 
@@ -905,8 +573,8 @@ void sim_free(abx_sim* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   sim_drop_graphs(s);
-  void* ptrs[] = {s->f[0],   s->f[1],  s->acc,   s->ns,    s->fluid_alloc, s->type_bits, s->chunks,
-                  s->chunks64, s->meta, s->cells, s->plist, s->vy,    s->red,   s->chunks_slab};
+  void* ptrs[] = {s->f[0],   s->f[1], s->acc,   s->ns, s->fluid_alloc, s->type_bits,
+                  s->chunks64, s->cells, s->plist, s->vy, s->red,         s->chunks_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->red_host) (void)hipHostFree(s->red_host);
@@ -1017,15 +685,9 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->sig = rp->K * 0.5 * s->u0 * s->u0 / (uface * std::max(Lp, 1));
   s->cs2s = 18 * 1.4142135623730951 * 0.17 * 0.17;
 
-  std::vector<int32_t> chunks, chunks64;
-  for (int64_t c0 = 0; c0 < s->plane; c0 += CHUNK) {
-    uint32_t any = 0;
-    for (int w = 0; w < CHUNK / 32; ++w) any |= fbits[(c0 >> 5) + w];
-    if (any) chunks.push_back((int32_t)c0);
-  }
+  std::vector<int32_t> chunks64;
   for (int64_t c0 = 0; c0 < s->plane; c0 += 64)
     if (fbits[c0 >> 5] | fbits[(c0 >> 5) + 1]) chunks64.push_back((int32_t)c0);
-  s->nchunks = (int)chunks.size();
   s->nchunks64 = (int)chunks64.size();
   s->seg_host = chunks64;
   std::vector<int32_t> plist((size_t)(s->nM + s->nT));
@@ -1049,7 +711,6 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   }
   rc |= rc ? 0 : dalloc(s, &s->fluid_alloc, fb.size());
   rc |= rc ? 0 : dalloc(s, &s->type_bits, typ.size());
-  rc |= rc ? 0 : dalloc(s, &s->chunks, std::max<size_t>(chunks.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->chunks64, std::max<size_t>(chunks64.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->cells, std::max<size_t>(cells.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->plist, std::max<size_t>(plist.size(), 1));
@@ -1069,7 +730,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
   if (up(s->fluid_alloc, fb.data(), fb.size() * 4) != hipSuccess || up(s->type_bits, typ.data(), typ.size() * 4) ||
-      up(s->chunks, chunks.data(), chunks.size() * 4) || up(s->chunks64, chunks64.data(), chunks64.size() * 4) || up(s->cells, cells.data(), cells.size() * 4) ||
+      up(s->chunks64, chunks64.data(), chunks64.size() * 4) || up(s->cells, cells.data(), cells.size() * 4) ||
       up(s->plist, plist.data(), plist.size() * 4)) {
     set_error("abx_sim_create: upload failed");
     sim_free(s);
@@ -1107,6 +768,11 @@ extern "C" int abx_sim_info_get(const abx_sim* s, abx_sim_info* o) {
   o->precision = s->prec;
   o->B = s->B;
   o->device_bytes = s->device_bytes;
+  o->plane = s->plane;
+  o->nyp = s->nyp;
+  o->nzp = s->nzp;
+  o->kofs = s->kofs;
+  o->nseg = s->nchunks64;
   return 0;
 }
 
@@ -1120,11 +786,9 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
   if (!s->use_graph || n < 2) return launch_steps(s, n, doacc);
   // a chunk of n steps depends only on (n, doacc, stored form, current
   // lattice, kernel): capture it once, replay it afterwards
-  if ((s->kernel < 4 || s->kernel == 6) && ensure_second_lattice(s)) return -1;
-  if (s->kernel == 5 && ensure_meta(s)) return -1;
+  if (s->kernel != 4 && ensure_second_lattice(s)) return -1;
   const int form0 = s->form, cur0 = s->cur;
-  const auto key = std::make_tuple(n, doacc, form0, cur0,
-                                   s->kernel * 1000 + s->chunk + 10 * s->nt + 7 * s->xcd_map + 100000 * s->pipe);
+  const auto key = std::make_tuple(n, doacc, form0, cur0, s->kernel + 8 * s->nt + 256 * (s->xcd_map + 1));
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
     hipGraph_t g;
@@ -1148,8 +812,10 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
   }
   ABX_HIP(hipGraphLaunch(it->second, s->stream));
   if (n % 2) {
-    if (s->kernel >= 4) s->form = form0 == 2 ? 1 : 2;
-    if (s->kernel < 4 || s->kernel == 6) s->cur = cur0 ^ 1;
+    if (s->kernel == 4)
+      s->form = form0 == 2 ? 1 : 2;
+    else
+      s->cur = cur0 ^ 1;
   }
   return 0;
 }
@@ -1438,48 +1104,34 @@ extern "C" int abx_sim_sync(abx_sim* s) {
   return 0;
 }
 
-extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int chunk, int nt) {
-  if (!s || kernel < 0 || kernel > 6 || (chunk != 64 && chunk != 256) || nt < 0 || nt > 3) {
-    set_error("abx_sim_set_launch: bad argument (kernel 0..6, chunk 64 or 256, nt 0..3)");
+extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
+  if (!s || (kernel != 2 && kernel != 4) || nt < 0 || nt > 15) {
+    set_error("abx_sim_set_launch: bad argument (kernel 2 or 4, nt 0..15)");
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   sim_drop_graphs(s);
-  const bool was_aa = s->kernel >= 4, to_aa = kernel >= 4;
-  if ((!to_aa || kernel == 6) && ensure_second_lattice(s)) return -1;
-  if (kernel == 5 && ensure_meta(s)) return -1;
+  if (kernel == 2 && ensure_second_lattice(s)) return -1;
   if (xcd_map < 0 && ensure_slab_order(s)) return -1;
-  if (was_aa != to_aa) {
+  if (kernel != s->kernel) {
     // carry the state over: read it in the old form, store it in the new one
     const size_t n = (size_t)(19 * s->NF);
     std::vector<char> host(std::max<size_t>(n * s->ssz, 1) * (size_t)s->B);
     for (int b = 0; b < s->B; ++b)
       if (abx_sim_read_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
     s->kernel = kernel;
-    s->form = to_aa ? 1 : 0;
+    s->form = kernel == 4 ? 1 : 0;
     s->cur = 0;
     for (int b = 0; b < s->B; ++b)
       if (abx_sim_write_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
   }
-  s->kernel = kernel;
   s->xcd_map = xcd_map < 0 ? -1 : std::min(xcd_map, 4096);
-  s->chunk = chunk;
   s->nt = nt;
   return 0;
 }
 
 extern "C" int abx_lattice_set_row_shift(int enable) {
   g_row_shift = enable ? 1 : 0;
-  return 0;
-}
-
-extern "C" int abx_sim_set_pipeline(abx_sim* s, int planes) {
-  if (!s || planes < 0 || planes > 4096) {
-    set_error("abx_sim_set_pipeline: bad argument (planes 0..4096)");
-    return -1;
-  }
-  sim_drop_graphs(s);
-  s->pipe = planes;
   return 0;
 }
 
@@ -1490,7 +1142,7 @@ StepArgs sim_step_args(abx_sim* s, int doacc) { return step_args(s, doacc); }
 int sim_form(abx_sim* s) { return s->form; }
 void sim_set_form(abx_sim* s, int form) { s->form = form; }
 int sim_normalize(abx_sim* s) {
-  if (s->kernel < 4 || s->form == 1) return 0;
+  if (s->kernel != 4 || s->form == 1) return 0;
   const size_t n = (size_t)(19 * s->NF) * s->ssz;
   std::vector<char> host(std::max<size_t>(n, 1) * (size_t)s->B);
   for (int b = 0; b < s->B; ++b)

@@ -298,123 +298,6 @@ __device__ __forceinline__ S post_f(const S* F, int64_t P, const uint32_t* fb, i
   return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? F[q * P + dn] : F[OPP[q] * P + d];
 }
 
-// V consecutive storage values (16 bytes) at an address aligned to the
-// element only: one global_load_dwordx4 / global_store_dwordx4.
-template <typename S, int V>
-__device__ __forceinline__ void vload(const S* p, S (&v)[V]) {
-  __builtin_memcpy(v, p, sizeof(S) * V);
-}
-template <typename S, int V>
-__device__ __forceinline__ void vstore(S* p, const S (&v)[V]) {
-  __builtin_memcpy(p, v, sizeof(S) * V);
-}
-
-// Wide variant (the default): every lane owns V = 16 / sizeof(S)
-// consecutive positions along k and moves each population as one 16-byte
-// access, so a wavefront reads and writes 1 KiB per population.  A
-// workgroup (256 lanes) takes 256*V/64 active 64-position segments from the
-// segment list, each staging its nine neighbour-row bit windows in LDS.
-// Directions with cz = +-1 read 4-byte-aligned (not 16-byte-aligned)
-// vectors.  A population whose pulled source is a wall for some of the lane's
-// cells is read both ways and selected per cell (halfway bounce-back).
-template <typename S, typename M, bool ACC, bool NS>
-__global__ __launch_bounds__(256) void k_stream_collide_wide(StepArgs a) {
-  constexpr int V = 16 / (int)sizeof(S);  // positions per lane
-  constexpr int LPS = 64 / V;             // lanes per 64-position segment
-  constexpr int SPB = 256 / LPS;          // segments per workgroup
-  constexpr int WW = 5;                   // window words: positions seg-32 .. seg+128
-  constexpr uint32_t FULL = (1u << V) - 1;
-  __shared__ uint32_t win[SPB][9][WW];
-  const int t = threadIdx.x, sl = t / LPS, l = t - sl * LPS;
-  const int blk = a.xcd_map ? (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3) : (int)blockIdx.x;
-  const int seg0 = blk * SPB;
-  if (seg0 >= a.nchunks) return;  // whole workgroup leaves together
-  for (int u = t; u < SPB * 9 * WW; u += 256) {
-    const int ss = u / (9 * WW), r = u - ss * (9 * WW), wi = r / WW, wo = r - wi * WW;
-    uint32_t val = 0;
-    if (seg0 + ss < a.nchunks) {
-      const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-      const int64_t base = (int64_t)a.chunks[seg0 + ss] - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-      val = a.fluid_bits[(base >> 5) + wo];
-    }
-    win[ss][wi][wo] = val;
-  }
-  __syncthreads();
-  if (seg0 + sl >= a.nchunks) return;
-  // V neighbour bits starting at window position p
-  auto bits = [&](int wi, int p) -> uint32_t {
-    const uint64_t w = ((uint64_t)win[sl][wi][(p >> 5) + 1] << 32) | win[sl][wi][p >> 5];
-    return (uint32_t)(w >> (p & 31)) & FULL;
-  };
-  const int pl = V * l + 32;
-  const uint32_t own = bits(4, pl);
-  if (!own) return;
-  const int b = blockIdx.y;
-  const int d = a.chunks[seg0 + sl] + V * l;
-  const uint32_t tb = (uint32_t)d & 31u;
-  const uint32_t* tw = a.type_bits + 3 * (d >> 5);
-  const uint32_t inl = (tw[0] >> tb) & FULL, outl = (tw[1] >> tb) & FULL, por = (tw[2] >> tb) & FULL;
-  const int64_t P = a.plane;
-  const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
-  S* fout = (S*)a.fout + (int64_t)b * 19 * P;
-
-  S f[19][V];
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
-    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const uint32_t nb = bits(wi, pl - CZ[q]) & own;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-    if (nb == own) {
-      vload<S, V>(fin + q * P + (d - off), f[q]);
-    } else if (nb == 0) {
-      vload<S, V>(fin + OPP[q] * P + d, f[q]);
-    } else {
-      S bb[V];
-      vload<S, V>(fin + q * P + (d - off), f[q]);
-      vload<S, V>(fin + OPP[q] * P + d, bb);
-#pragma unroll
-      for (int c = 0; c < V; ++c)
-        if (!((nb >> c) & 1u)) f[q][c] = bb[c];
-    }
-  }
-  float nsv[V];
-  if (NS) vload<float, V>(a.ns + (int64_t)b * P + d, nsv);
-  S acc_u[4][V];
-  if (ACC) {
-    const S* ac = (const S*)a.acc + (int64_t)b * 4 * P;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) vload<S, V>(ac + m * P + d, acc_u[m]);
-  }
-#pragma unroll
-  for (int c = 0; c < V; ++c) {
-    if (!((own >> c) & 1u)) continue;
-    M fi[19], fo[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) fi[q] = (M)f[q][c];
-    M ux, uy, uz, rho;
-    collide_any<M, NS>(fi, (inl >> c) & 1u, (outl >> c) & 1u, (por >> c) & 1u, NS ? (M)nsv[c] : (M)0, a, fo, ux,
-                       uy, uz, rho);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) f[q][c] = (S)fo[q];
-    if (ACC) {
-      acc_u[0][c] = (S)((M)acc_u[0][c] + ux);
-      acc_u[1][c] = (S)((M)acc_u[1][c] + uy);
-      acc_u[2][c] = (S)((M)acc_u[2][c] + uz);
-      acc_u[3][c] = (S)((M)acc_u[3][c] + rho);
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 19; ++q) vstore<S, V>(fout + q * P + d, f[q]);
-  if (ACC) {
-    S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) vstore<S, V>(ac + m * P + d, acc_u[m]);
-  }
-}
-
-// u_y = sum c_y f / sum f on the probe cells (engine.js:324-327, WGSL probe
-// engine.js:455-461); the value is rounded to float as the reference's
-// Float32Array holds it, except for double storage.
 // Slot that holds post-collision population q of the cell at d in a given
 // form (the inverse of post_f).
 __device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
@@ -428,7 +311,9 @@ __device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int 
 // One lane's cell of a 64-position segment starting at seg: the fused
 // stream-collide of the scheme MODE (see k_stream_collide_seg).  win holds
 // the segment's nine neighbour-row bit windows (positions seg-32 .. seg+96).
-template <typename S, typename M, bool ACC, bool NS, int MODE>
+// NT bit 0 / bit 1: population loads / stores are non-temporal (streamed
+// through the caches without keeping the lines).
+template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0>
 __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win)[9][4], int seg, int l, int b) {
   const int p0 = l + 32;
   // AA steps: a non-fluid position (solid or padding) whose 32-position half
@@ -470,7 +355,7 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
       src = fin + q * P + d;
     else
       src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
-    fi[q] = (M)*src;
+    fi[q] = (M)ld(src, NT & 1);
   }
   M s = 0;
   if (NS) s = (M)a.ns[(int64_t)b * P + d];
@@ -487,7 +372,7 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
       const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
       dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
     }
-    *dst = fluid ? (S)fo[q] : (S)fi[q];
+    st(dst, fluid ? (S)fo[q] : (S)fi[q], NT & 2);
   }
   if (ACC) {
     S* ac = (S*)a.acc + (int64_t)b * 4 * P;

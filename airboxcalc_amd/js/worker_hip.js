@@ -29,13 +29,16 @@ const getDev = () => {
 };
 const post = (m, tr) => parentPort.postMessage(m, tr || []);
 
+/* The run parameters are the reference's (engine.js:765-779); a caller may
+   add gateTolerance to tighten the self-test gate (default 1e-3). */
 async function runSim(p) {
   const prog = (info) => post({ type: "progress", info });
   const d = getDev();
   if (!d) throw new Error("Keine MI355X-Grafikkarte (gfx950) verfügbar.");
   post({ type: "status", text: "Grafikkarte wird geprüft …" });
+  const tol = p.gateTolerance !== undefined ? p.gateTolerance : 1e-3;
   const md = H.gpuSelfTestHIP(d, geo, p);
-  if (!(md < 1e-3)) throw new Error("Abweichung GPU/CPU " + md.toExponential(1));
+  if (!(md < tol)) throw new Error("Abweichung GPU/CPU " + md.toExponential(1));
   post({ type: "status", text: "Rechnet auf der Grafikkarte (" + d.name + ")" });
   const res = await H.simulateHIP(d, geo, p, prog, null);
   res.device = "GPU · " + d.name;

@@ -39,24 +39,35 @@ napi_value throw_msg(napi_env env, const std::string& m) {
 
 // ---- handles -----------------------------------------------------------------
 enum Kind { K_DEV = 1, K_GEOM = 2, K_SIM = 3 };
+// busy: a simChunk of this simulation handle is running on a pool thread.
+// While it runs, synchronous calls on the handle throw and destroy() is
+// deferred to the chunk's completion (the handle's stream and state belong to
+// the running chunk).
 struct Handle {
   int kind;
   void* p;
+  int busy;
+  bool destroy_pending;
 };
 
-void finalize_handle(napi_env, void* data, void*) {
-  Handle* h = (Handle*)data;
+void release(Handle* h) {
   if (h->p) {
     if (h->kind == K_SIM) abx_sim_destroy((abx_sim*)h->p);
     if (h->kind == K_GEOM) abx_geom_free((abx_geom*)h->p);
     if (h->kind == K_DEV) abx_device_close((abx_dev*)h->p);
+    h->p = nullptr;
   }
+}
+
+void finalize_handle(napi_env, void* data, void*) {
+  Handle* h = (Handle*)data;
+  release(h);  // a running chunk holds a reference, so busy is 0 here
   delete h;
 }
 
 napi_value wrap(napi_env env, int kind, void* p) {
   napi_value v;
-  Handle* h = new Handle{kind, p};
+  Handle* h = new Handle{kind, p, 0, false};
   if (napi_create_external(env, h, finalize_handle, nullptr, &v) != napi_ok) {
     delete h;
     return nullptr;
@@ -72,8 +83,9 @@ Handle* unwrap(napi_env env, napi_value v, int kind) {
     return nullptr;
   }
   Handle* h = (Handle*)data;
-  if (h->kind != kind || !h->p) {
-    napi_throw_type_error(env, nullptr, h->p ? "wrong kind of native handle" : "native handle already destroyed");
+  if (h->kind != kind || !h->p || h->destroy_pending) {
+    napi_throw_type_error(env, nullptr,
+                          h->p && !h->destroy_pending ? "wrong kind of native handle" : "native handle already destroyed");
     return nullptr;
   }
   return h;
@@ -92,12 +104,35 @@ bool get_args(napi_env env, napi_callback_info info, size_t need, std::vector<na
   return true;
 }
 
+// An integer argument in [lo, hi]; throws a RangeError otherwise (NaN, a
+// fraction or a value outside the range).
+bool int_arg(napi_env env, napi_value v, int lo, int hi, const char* what, int* out) {
+  napi_valuetype t;
+  double d = NAN;
+  if (v && napi_typeof(env, v, &t) == napi_ok && t == napi_number) napi_get_value_double(env, v, &d);
+  if (!(d >= lo && d <= hi) || d != std::floor(d)) {
+    const std::string m = std::string(what) + ": expected an integer in [" + std::to_string(lo) + ", " +
+                          std::to_string(hi) + "]";
+    napi_throw_range_error(env, nullptr, m.c_str());
+    return false;
+  }
+  *out = (int)d;
+  return true;
+}
+
 double num(napi_env env, napi_value v, double dflt = NAN) {
   napi_valuetype t;
   if (!v || napi_typeof(env, v, &t) != napi_ok || t != napi_number) return dflt;
   double d = dflt;
   napi_get_value_double(env, v, &d);
   return d;
+}
+
+// num() as an int: dflt unless v is a number in the int range (so no NaN or
+// out-of-range value reaches a cast); fractions truncate toward zero.
+int inum(napi_env env, napi_value v, int dflt) {
+  const double d = num(env, v, NAN);
+  return d >= (double)INT32_MIN && d <= (double)INT32_MAX ? (int)d : dflt;
 }
 
 napi_value prop(napi_env env, napi_value obj, const char* key) {
@@ -191,7 +226,7 @@ napi_value Version(napi_env env, napi_callback_info) {
 napi_value DeviceOpen(napi_env env, napi_callback_info info) {
   std::vector<napi_value> a;
   get_args(env, info, 0, a);
-  const int ord = a.empty() ? 0 : (int)num(env, a[0], 0);
+  const int ord = a.empty() ? 0 : inum(env, a[0], 0);
   abx_dev* d = nullptr;
   char name[256];
   if (abx_device_open(ord, &d, name, sizeof name)) return throw_lib(env);
@@ -230,7 +265,7 @@ napi_value Voxelize(napi_env env, napi_callback_info info) {
     napi_get_element(env, a[2], k, &e);
     origin[k] = num(env, e);
   }
-  const int nx = (int)num(env, a[3]), ny = (int)num(env, a[4]), nz = (int)num(env, a[5]);
+  const int nx = inum(env, a[3], 0), ny = inum(env, a[4], 0), nz = inum(env, a[5], 0);
   const double h = num(env, a[6]);
   uint8_t* out = nullptr;
   napi_value r = typed<uint8_t>(env, napi_uint8_array, nullptr, (size_t)nx * ny * nz, &out);
@@ -317,11 +352,11 @@ napi_value Prepare(napi_env env, napi_callback_info info) {
   abx_prep_params P;
   memset(&P, 0, sizeof P);
   P.dx = num(env, prop(env, p, "dx"));
-  P.sealAxis = (int)num(env, prop(env, p, "sealAxis"), 0);
-  P.sealSign = (int)num(env, prop(env, p, "sealSign"), 1);
+  P.sealAxis = inum(env, prop(env, p, "sealAxis"), 0);
+  P.sealSign = inum(env, prop(env, p, "sealSign"), 1);
   P.seal = num(env, prop(env, p, "seal"), 0);
-  P.inletAxis = (int)num(env, prop(env, p, "inletAxis"), 0);
-  P.inletSign = (int)num(env, prop(env, p, "inletSign"), 1);
+  P.inletAxis = inum(env, prop(env, p, "inletAxis"), 0);
+  P.inletSign = inum(env, prop(env, p, "inletSign"), 1);
   P.inletYaw = num(env, prop(env, p, "inletYaw"), 0);
   P.cy = num(env, prop(env, p, "cy"));
   P.sy = num(env, prop(env, p, "sy"));
@@ -391,8 +426,8 @@ napi_value SimCreate(napi_env env, napi_callback_info info) {
   rp.aniso = num(env, prop(env, a[2], "aniso"));
   rp.u = num(env, prop(env, a[2], "u"));
   rp.tau = num(env, prop(env, a[2], "tau"));
-  const int prec = a.size() > 3 ? (int)num(env, a[3], 0) : 0;
-  const int B = a.size() > 4 ? (int)num(env, a[4], 1) : 1;
+  const int prec = a.size() > 3 ? inum(env, a[3], 0) : 0;
+  const int B = a.size() > 4 ? inum(env, a[4], 1) : 1;
   abx_grid grid;
   abx_geom_grid((abx_geom*)g->p, &grid);
   const float* ns = nullptr;
@@ -409,9 +444,15 @@ napi_value SimCreate(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// The simulation of a handle that no chunk is running on (synchronous calls).
 abx_sim* sim_arg(napi_env env, napi_value v) {
   Handle* h = unwrap(env, v, K_SIM);
-  return h ? (abx_sim*)h->p : nullptr;
+  if (!h) return nullptr;
+  if (h->busy) {
+    napi_throw_error(env, nullptr, "simulation handle busy: a simChunk is still running on it");
+    return nullptr;
+  }
+  return (abx_sim*)h->p;
 }
 
 napi_value SimSteps(napi_env env, napi_callback_info info) {
@@ -419,9 +460,25 @@ napi_value SimSteps(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 3, a)) return nullptr;
   abx_sim* s = sim_arg(env, a[0]);
   if (!s) return nullptr;
+  int n;
+  if (!int_arg(env, a[1], 0, 1 << 30, "simSteps n", &n)) return nullptr;
   bool doacc = false;
   napi_get_value_bool(env, a[2], &doacc);
-  if (abx_sim_steps(s, (int)num(env, a[1]), doacc ? 1 : 0) || abx_sim_sync(s)) return throw_lib(env);
+  if (abx_sim_steps(s, n, doacc ? 1 : 0) || abx_sim_sync(s)) return throw_lib(env);
+  return nullptr;
+}
+
+// simSetLaunch(sim, kernel, xcdMap, nt): abx_sim_set_launch.
+napi_value SimSetLaunch(napi_env env, napi_callback_info info) {
+  std::vector<napi_value> a;
+  if (!get_args(env, info, 2, a)) return nullptr;
+  abx_sim* s = sim_arg(env, a[0]);
+  if (!s) return nullptr;
+  int kernel, xm = -1, nt = 0;
+  if (!int_arg(env, a[1], 0, 16, "simSetLaunch kernel", &kernel)) return nullptr;
+  if (a.size() > 2 && !int_arg(env, a[2], -1, 4096, "simSetLaunch xcdMap", &xm)) return nullptr;
+  if (a.size() > 3 && !int_arg(env, a[3], 0, 15, "simSetLaunch nt", &nt)) return nullptr;
+  if (abx_sim_set_launch(s, kernel, xm, nt)) return throw_lib(env);
   return nullptr;
 }
 
@@ -438,8 +495,10 @@ napi_value SimProbe(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 2, a)) return nullptr;
   abx_sim* s = sim_arg(env, a[0]);
   if (!s) return nullptr;
+  int step;
+  if (!int_arg(env, a[1], 0, INT32_MAX, "simProbe step", &step)) return nullptr;
   double out[3];
-  if (abx_sim_probe(s, (int)num(env, a[1]), out)) return throw_lib(env);
+  if (abx_sim_probe(s, step, out)) return throw_lib(env);
   return probe_object(env, out);
 }
 
@@ -449,6 +508,7 @@ struct ChunkWork {
   napi_async_work work;
   napi_deferred deferred;
   napi_ref simref;
+  Handle* h;
   abx_sim* sim;
   int n, doacc, step, probe, rc;
   double out[3];
@@ -476,6 +536,7 @@ void chunk_done(napi_env env, napi_status, void* data) {
     napi_get_undefined(env, &u);
     napi_resolve_deferred(env, w->deferred, u);
   }
+  if (--w->h->busy == 0 && w->h->destroy_pending) release(w->h);
   napi_delete_reference(env, w->simref);
   napi_delete_async_work(env, w->work);
   delete w;
@@ -486,13 +547,19 @@ napi_value SimChunk(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 4, a)) return nullptr;
   abx_sim* s = sim_arg(env, a[0]);
   if (!s) return nullptr;
+  int n, step;
+  if (!int_arg(env, a[1], 0, 1 << 30, "simChunk n", &n) ||
+      !int_arg(env, a[3], 0, INT32_MAX, "simChunk step", &step))
+    return nullptr;
   ChunkWork* w = new ChunkWork();
+  w->h = unwrap(env, a[0], K_SIM);
+  w->h->busy++;
   w->sim = s;
-  w->n = (int)num(env, a[1]);
+  w->n = n;
   bool doacc = false;
   napi_get_value_bool(env, a[2], &doacc);
   w->doacc = doacc ? 1 : 0;
-  w->step = (int)num(env, a[3]);
+  w->step = step;
   bool probe = true;
   if (a.size() > 4) napi_get_value_bool(env, a[4], &probe);
   w->probe = probe ? 1 : 0;
@@ -514,7 +581,7 @@ napi_value SimReadAcc(napi_env env, napi_callback_info info) {
   abx_sim_info_get(s, &I);
   float* out = nullptr;
   napi_value r = typed<float>(env, napi_float32_array, nullptr, (size_t)(4 * I.NF), &out);
-  if (abx_sim_read_acc(s, (int)num(env, a[1]), num(env, a[2], 1.0), out)) return throw_lib(env);
+  if (abx_sim_read_acc(s, inum(env, a[1], -1), num(env, a[2], 1.0), out)) return throw_lib(env);
   return r;
 }
 
@@ -532,7 +599,7 @@ napi_value SimReadF(napi_env env, napi_callback_info info) {
     r = typed<double>(env, napi_float64_array, nullptr, n, (double**)&out);
   else
     r = typed<float>(env, napi_float32_array, nullptr, n, (float**)&out);
-  if (abx_sim_read_f(s, (int)num(env, a[1]), out)) return throw_lib(env);
+  if (abx_sim_read_f(s, inum(env, a[1], -1), out)) return throw_lib(env);
   return r;
 }
 
@@ -547,7 +614,7 @@ napi_value SimWriteF(napi_env env, napi_callback_info info) {
   if (!want(env, v, I.precision == ABX_F64 ? napi_float64_array : napi_float32_array, (size_t)(19 * I.NF),
             "simWriteF"))
     return nullptr;
-  if (abx_sim_write_f(s, (int)num(env, a[1]), v.data)) return throw_lib(env);
+  if (abx_sim_write_f(s, inum(env, a[1], -2), v.data)) return throw_lib(env);
   return nullptr;
 }
 
@@ -605,7 +672,7 @@ napi_value Evaluate(napi_env env, napi_callback_info info) {
   if (!s) return nullptr;
   Handle* g = unwrap(env, a[1], K_GEOM);
   if (!g) return nullptr;
-  const int b = (int)num(env, a[2], 0);
+  const int b = inum(env, a[2], 0);
   const double div = num(env, a[3], 1.0);
   bool f3 = true;
   if (a.size() > 4) napi_get_value_bool(env, a[4], &f3);
@@ -653,14 +720,17 @@ napi_value Streamlines(napi_env env, napi_callback_info info) {
   if (!s) return nullptr;
   Handle* g = unwrap(env, a[1], K_GEOM);
   if (!g) return nullptr;
-  const int b = (int)num(env, a[2], 0);
+  const int b = inum(env, a[2], 0);
   const double div = num(env, a[3], 1.0);
-  const int maxSeeds = a.size() > 4 ? (int)num(env, a[4], 220) : 220;
+  const int maxSeeds = a.size() > 4 ? inum(env, a[4], 220) : 220;
   abx_eval_geom eg;
   abx_geom_eval_view((abx_geom*)g->p, &eg);
-  const int64_t cap = 2002 * (int64_t)maxSeeds + 16;
+  if (maxSeeds < 1 || maxSeeds > 100000) return throw_msg(env, "streamlines: maxSeeds out of range");
+  // at most maxSeeds + 1 lines (the fractional seed stride) of at most 2001
+  // points (every 3rd of 6000 RK2 steps plus the last one), engine.js:738-748
+  const int64_t cap = 2002 * ((int64_t)maxSeeds + 1) + 16;
   std::vector<float> pts((size_t)(3 * cap)), spd((size_t)cap);
-  std::vector<uint32_t> off((size_t)maxSeeds + 2);
+  std::vector<uint32_t> off((size_t)maxSeeds + 3);
   int64_t npts = 0, nl = 0;
   if (abx_streamlines(s, &eg, b, div, maxSeeds, pts.data(), (int64_t)pts.size(), spd.data(), (int64_t)spd.size(),
                       off.data(), (int64_t)off.size(), &npts, &nl))
@@ -678,12 +748,10 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   void* data = nullptr;
   if (napi_get_value_external(env, a[0], &data) != napi_ok || !data) return nullptr;
   Handle* h = (Handle*)data;
-  if (h->p) {
-    if (h->kind == K_SIM) abx_sim_destroy((abx_sim*)h->p);
-    if (h->kind == K_GEOM) abx_geom_free((abx_geom*)h->p);
-    if (h->kind == K_DEV) abx_device_close((abx_dev*)h->p);
-    h->p = nullptr;
-  }
+  if (h->busy)
+    h->destroy_pending = true;  // released when the running chunk completes
+  else
+    release(h);
   return nullptr;
 }
 
@@ -697,7 +765,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"simProbe", SimProbe},     {"simReadAcc", SimReadAcc}, {"simReadF", SimReadF},
              {"simWriteF", SimWriteF},   {"simInitEq", SimInitEq},   {"simSetNS", SimSetNS},
              {"simClearAcc", SimClearAcc}, {"evaluate", Evaluate},   {"streamlines", Streamlines},
-             {"destroy", Destroy}};
+             {"simSetLaunch", SimSetLaunch}, {"destroy", Destroy}};
   for (auto& f : fns) {
     napi_value fn;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);

@@ -4,11 +4,15 @@ Workload: the roofline airbox (300 x 150 x 200 mm, 6 mm walls, 70 mm inlet on
 the Z-min face, seal plane at Y=0 with a 40 mm pleat layer K=3 aniso=10, lid
 outlet 60 mm, 50 mm lid plenum) at dx = 1 mm, seeded internal baffles and an
 oblique inlet (seed 1), FP32, u = 0.06, tau = 0.51.  One step is one fused
-stream-collide pass over every fluid cell.  The timed region is the
-reference's time loop (simulateGPU, engine.js:613-630) over --steps steps:
-chunks of 100 steps, a gamma/mass-ratio probe after each chunk, the last third
-of the steps accumulating the averaged fields, and the final read-back of the
-averaged fields.  Inputs are resident in HBM before timing starts.
+stream-collide pass over every fluid cell.  The flow first develops from rest
+for --spinup untimed steps (default 3000), then --warmup steps run untimed.
+The timed region is the reference's time loop (simulateGPU,
+engine.js:613-630) over --steps steps: chunks of 100 steps, a
+gamma/mass-ratio probe after each chunk and at the end, the last third of the
+steps accumulating the averaged fields.  It ends at the last probe, where the
+reference takes its MLUPS (engine.js:338); the read-back of the averaged
+fields is reported apart (readback_ms).  Inputs are resident in HBM before
+timing starts.
 
 Prints one JSON line.  MLUPS counts fluid cells only (engine.js:338).
 """
@@ -40,10 +44,38 @@ def workload(dx: float, seed: int):
     return spec, G.airbox_triangles(spec), p
 
 
-def dense_positions(g) -> int:
-    """Positions of the padded dense lattice the stream-collide runs on:
-    (nx+2)(ny+2) rows of nz+2 cells rounded up to 32."""
-    return (g.nx + 2) * (g.ny + 2) * (32 * ((g.nz + 2 + 31) // 32))
+def host_cpu() -> dict:
+    """Host CPU of this machine: logical CPUs, the ones this process may use,
+    and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
+def engine_js_rate() -> dict:
+    """engine.js's own CPU backend on the 4 mm airbox (161,144 fluid cells,
+    2000 steps): the wall time its simulate() recorded when the golden
+    fixture tests/golden/c1.json was written (Node v12, one thread, 8 x Xeon
+    @ 2.70 GHz build host; not this machine)."""
+    with open(os.path.join(ROOT, "tests", "golden", "c1.json")) as fh:
+        sim = json.load(fh)["sim"]["sim"]
+    steps = 2000
+    return {"value": round(sim["NF"] * steps / (sim["ms"] / 1e3) / 1e6, 3), "unit": "MLUPS", "cores": 1,
+            "kind": "reference",
+            "sample": f"engine.js simulate() of the dx=4 mm airbox, {sim['NF']} fluid cells x {steps} steps in "
+                      f"{sim['ms'] / 1e3:.1f} s on one thread (Node v12, 8 x Xeon @ 2.70 GHz build host), "
+                      f"from tests/golden/c1.json"}
 
 
 def cpu_baseline(tris, p, run, seconds: float, threads: int):
@@ -72,7 +104,7 @@ def cpu_baseline(tris, p, run, seconds: float, threads: int):
     nt, elt = timed(threads, 2 * seconds / 3)
     one = S.NF * n1 / el1 / 1e6
     return dict(value=S.NF * nt / elt / 1e6, unit="MLUPS", cores=threads, kind="port",
-                one_core_mlups=round(one, 2),
+                one_core_mlups=round(one, 2), host=host_cpu(), reference_engine_js=engine_js_rate(),
                 sample=f"{nt} cpuStep steps (f32 storage, f64 arithmetic) of the same {S.NF}-cell lattice on "
                        f"{threads} OpenMP threads in {elt:.1f} s, and {n1} steps on one thread in {el1:.1f} s "
                        f"({one:.2f} MLUPS); oracle/engine_oracle.c")
@@ -332,8 +364,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host threads of the CPU baseline (default: up to 16, the host share of one GPU)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel", type=int, default=None, help="stream-collide launch variant (default: library's)")
+    ap.add_argument("--kernel", type=int, default=None, help="stream-collide scheme: 4 AA pattern (default), 2 two lattices")
     ap.add_argument("--xcd-map", type=int, default=None, help="block order across XCDs (abx_sim_set_launch)")
+    ap.add_argument("--nt", type=int, default=None, help="non-temporal access bits (abx_sim_set_launch)")
+    ap.add_argument("--spinup", type=int, default=3000,
+                    help="untimed steps from rest before the warmup, so the timed probes read a developed flow")
     ap.add_argument("--workload", default="roofline",
                     choices=["roofline", "batch", "fp64", "unet", "conv", "warmstart"],
                     help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch; "
@@ -373,22 +408,34 @@ def main():
     g = abx.Geometry(dev, tris, p, max_cells=0)
     log(f"prepare {time.perf_counter() - t:.2f}s grid {g.nx}x{g.ny}x{g.nz} nf={g.nf} ntri={tris.size // 9}")
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
-    if a.kernel is not None or a.xcd_map is not None:
-        sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=-1 if a.xcd_map is None else a.xcd_map)
+    if a.kernel is not None or a.xcd_map is not None or a.nt is not None:
+        sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=-1 if a.xcd_map is None else a.xcd_map,
+                       nt=0 if a.nt is None else a.nt)
     NF = sim.NF
-    log(f"sim NF={NF} device bytes {sim.info.device_bytes / 2**30:.2f} GiB")
+    I = sim.info
+    log(f"sim NF={NF} device bytes {I.device_bytes / 2**30:.2f} GiB, lattice plane {I.plane} positions "
+        f"(rows of {I.nzp}, cell k=0 at {I.kofs}), {I.nseg} active segments")
 
+    # Spin-up (untimed): the flow develops from rest until it has crossed the
+    # filter layer, so that the probes of the timed steps read a real flow.
+    t = time.perf_counter()
+    sim.steps(a.spinup)
     sim.steps(a.warmup)
     sim.sync()
     sim.clear_acc()
     sim.sync()
+    log(f"spin-up {a.spinup} + warmup {a.warmup} steps in {time.perf_counter() - t:.2f}s")
 
+    # Timed region: the reference's time loop (simulateGPU, engine.js:613-630),
+    # ending at the last probe, where the reference takes its MLUPS
+    # (engine.js:338).  The read-back of the averaged fields is timed apart.
     steps, every = a.steps, a.every
     avg = steps // 3
     start_acc = steps - avg
     ev_ms = {0: 0.0, 1: 0.0}
     ev_n = {0: 0, 1: 0}
     hist = []
+    sim.sync()
     t0 = time.perf_counter()
     step = nacc = 0
     while step < steps:
@@ -402,26 +449,33 @@ def main():
         if doacc:
             nacc += n
         if step % every == 0 or step == steps:
-            gam, mr, _ = sim.probe(step)
+            gam, mr, _ = sim.probe(a.spinup + a.warmup + step)
             hist.append((step, gam, mr))
+    sim.sync()
+    t_solve = time.perf_counter() - t0
+    t1 = time.perf_counter()
     acc = sim.read_acc(max(nacc, 1))
-    wall = time.perf_counter() - t0
-    log(f"timed {steps} steps in {wall:.3f}s; last gamma {hist[-1][1]:.4f} massRatio {hist[-1][2]:.4f}")
+    t_read = time.perf_counter() - t1
+    log(f"timed {steps} steps in {t_solve:.3f}s (+ read-back {t_read * 1e3:.1f} ms); last gamma {hist[-1][1]:.4f} "
+        f"massRatio {hist[-1][2]:.4f}")
     assert np.isfinite(acc).all()
 
     copy_gbps, _ = abx.copy_bandwidth(dev, copy_bytes, 10)
     # the stream-collide's own access pattern over the dense lattice (all positions active)
-    aa_gbps, _ = abx.aa_pattern_bandwidth(dev, dense_positions(g), 10)
+    aa_gbps, _ = abx.aa_pattern_bandwidth(dev, int(I.plane), 10)
     bpu = BYTES_PER_UPDATE[a.precision]
-    mlups = NF * steps / wall / 1e6
+    mlups = NF * steps / t_solve / 1e6
     kern_ms = ev_ms[0] / max(ev_n[0], 1)
     achieved = NF * bpu / (kern_ms * 1e-3) / 1e9
     kern_acc_ms = ev_ms[1] / max(ev_n[1], 1)
-    traffic = None
+    traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", f"pmc_stream_collide_dx{a.dx:g}_{a.precision}.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and a.nt is None and a.kernel is None and a.xcd_map is None:
         with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            pm = json.load(fh)
+        traffic = pm.get("hbm_bytes_per_launch")
+        traffic_src = (f"{os.path.relpath(tf, ROOT)} (rocprofv3 FETCH_SIZE/WRITE_SIZE runs of this kernel on this "
+                       f"workload, scaled by the copy-kernel calibration; not measured in this run)")
     out = {
         "metric": "MLUPS + HBM GB/s (D3Q19 stream-collide, fluid-cell updates)",
         "value": round(mlups, 1),
@@ -429,7 +483,7 @@ def main():
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": round(wall * 1e3 / steps, 4),
+        "ms_per_step": round(t_solve * 1e3 / steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -437,11 +491,17 @@ def main():
         "data": "synthetic (procedural airbox STL, seeded)",
         "config": {"workload": f"airbox 300x150x200 mm, dx={a.dx:g} mm, seed {a.seed} "
                                f"(baffles {len(spec.baffles)}, inlet yaw {spec.yaw_deg} deg), "
-                               f"{steps} steps averaging {avg}, probe every {every}",
-                   "grid": [g.nx, g.ny, g.nz], "fluid_cells": NF, "triangles": int(tris.size // 9)},
-        "hbm_gbps": round(NF * steps * bpu / wall / 1e9, 1),
+                               f"{steps} steps averaging {avg}, probe every {every}, after {a.spinup} untimed "
+                               f"spin-up steps from rest",
+                   "grid": [g.nx, g.ny, g.nz], "fluid_cells": NF, "triangles": int(tris.size // 9),
+                   "lattice_positions": int(I.plane), "active_segments": int(I.nseg)},
+        "value_definition": "fluid cells x steps / time from the start of the loop to its last probe (engine.js:338)",
+        "readback_ms": round(t_read * 1e3, 2),
+        "wall_s_total": round(t_solve + t_read, 4),
+        "last_probe": {"step": a.spinup + a.warmup + hist[-1][0], "gamma": hist[-1][1], "massRatio": hist[-1][2]},
+        "hbm_gbps": round(NF * steps * bpu / t_solve / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": "k_stream_collide_seg, AA pattern (steps without averaging)",
                      "avg_launch_ms": round(kern_ms, 5), "bytes_per_launch": NF * bpu,
                      "avg_launch_ms_with_acc": round(kern_acc_ms, 5),

@@ -160,6 +160,11 @@ typedef struct {
   double u0, tau0, sig, aniso, cs2s;
   int precision, B;
   int64_t device_bytes;
+  /* the device lattice (no reference counterpart): positions per population
+   * plane, padded row and column extents, position of cell k = 0 within its
+   * row, and the number of 64-position segments that hold fluid */
+  int64_t plane;
+  int nyp, nzp, kofs, nseg;
 } abx_sim_info;
 
 /* Replaces setupSim + new GpuSim(dev, S, B, ns) (engine.js:239-264,
@@ -223,34 +228,23 @@ int abx_sim_set_ns(abx_sim* sim, const float* ns);
 /* Wait for the handle's stream. */
 int abx_sim_sync(abx_sim* sim);
 /* Tuning knobs of the stream-collide launch (no reference counterpart).
- * kernel: 0 = wide (16-byte accesses, several positions per lane),
- *         1 = narrow (one position per lane, chunk = 64 or 256 positions per
- *             workgroup, nt bit 0 / bit 1 = non-temporal loads / stores),
- *         2 = segment (one position per lane, four active 64-position
- *             segments per workgroup), two lattices,
- *         3 = segment, persistent grid, two lattices,
- *         4 = segment, AA pattern in one lattice (the default),
- *         5 = AA pattern with a per-position geometry word,
- *         6 = segment, AA pattern out of place (two lattices).
- * xcd_map 1 = each XCD sweeps one contiguous part of the lattice; xcd_map
- * G > 1 (segment kernels) = groups of G consecutive blocks per XCD, all XCDs
- * moving through the lattice together; xcd_map -1 (kernels 2, 4, 6; the
- * default) = each XCD owns one of eight bands of rows j (equal segment
- * counts) and sweeps it plane by plane.  Every variant computes the same
- * values bit for bit. */
-int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int chunk, int nt);
+ * kernel: 4 = AA pattern in one lattice (the default; one position per lane,
+ *             four active 64-position segments per workgroup),
+ *         2 = the same segments with two lattices and pull streaming.
+ * xcd_map -1 (the default) = each XCD owns one of eight bands of rows j
+ * (equal segment counts) and sweeps it plane by plane; 0 = blocks in
+ * lattice order, round robin over the XCDs; 1 = each XCD sweeps one
+ * contiguous part of the lattice; G > 1 = groups of G consecutive blocks per
+ * XCD, all XCDs moving through the lattice together.
+ * nt (AA pattern, float storage): non-temporal population accesses, bits
+ * 0/1 = loads/stores of the even step, bits 2/3 = loads/stores of the odd
+ * step.  Every choice computes the same values bit for bit. */
+int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
 /* Row layout of simulations created afterwards (process-wide; no reference
  * counterpart): enable = 1 (the default) starts each lattice row at the
  * offset that makes the fewest 128-byte lines hold fluid, 0 puts cell k = 0
  * right after a one-position halo.  Results are the same bit for bit. */
 int abx_lattice_set_row_shift(int enable);
-/* Plane-pipelined AA pairs (kernel 4; no reference counterpart): planes > 0
- * issues each pair of steps as launches over bands of `planes` lattice
- * planes, the second step trailing the first by a little more than one
- * plane, so that it re-reads the first step's output from the last-level
- * cache.  0 (the default) = one launch per step.  The values are the same
- * bit for bit. */
-int abx_sim_set_pipeline(abx_sim* sim, int planes);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
 int abx_sim_set_graph(abx_sim* sim, int enable);
@@ -320,7 +314,11 @@ int abx_evaluate(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, ab
 /* Replaces streamlines (engine.js:721-758): RK2 traces of the averaged field
  * from up to maxSeeds seeds over the inlet opening, one GPU thread per seed.
  * pts (3 floats per point, model frame), spd (one per point) and off (line
- * starts, nlines+1 entries) as the reference's {pts, spd, off}. */
+ * starts, nlines+1 entries) as the reference's {pts, spd, off}.  The seed
+ * stride gives at most maxSeeds+1 lines of at most 2002 points
+ * (engine.js:738, 745-748).  *npts and *nlines are always set; the call fails
+ * when a non-NULL output is shorter than they need (a NULL output with cap 0
+ * only counts). */
 int abx_streamlines(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, int maxSeeds, float* pts,
                     int64_t pts_cap, float* spd, int64_t spd_cap, uint32_t* off, int64_t off_cap,
                     int64_t* npts, int64_t* nlines);

@@ -3,7 +3,11 @@
 // with the reference's golden outputs: SHA-256 of every typed array, exact
 // float64 scalars, the progress objects.
 //
-// Usage: node run_hip_case.js <case.json>   (case: {stl, prepare, run, precision, selftest})
+// Usage: node run_hip_case.js <case.json>
+//   case: {stl, prepare, run, precision, selftest, nanCell, gateTolerance}
+//   nanCell: run with ns = NaN on that fluid cell (cid order) and report the
+//            error the run rejects with;
+//   gateTolerance: call runHIP with this gate and report its error, if any.
 "use strict";
 const fs = require("fs");
 const crypto = require("crypto");
@@ -37,6 +41,31 @@ const hex64 = (x) => { const b = Buffer.alloc(8); b.writeDoubleLE(x, 0); return 
   out.summary = Object.assign({}, g.summary, { openPts: undefined, filterPts: undefined, outletPts: undefined });
   for (const k of ["volume_l", "inlet_cm2", "filter_cm2", "outlet_cm2", "outlet_cm2_real"]) out.hex[k] = hex64(g.summary[k]);
   if (C.selftest) out.selftest = H.gpuSelfTestHIP(dev, g, C.run);
+  if (C.nanCell !== undefined) {
+    const ns = new Float32Array(g.nf);
+    ns[C.nanCell] = NaN;
+    const prog = [];
+    try {
+      await H.simulateHIP(dev, g, C.run, (info) => prog.push(info.step), ns, { precision: C.precision });
+      out.simError = null;
+    } catch (err) {
+      out.simError = err.message;
+    }
+    out.progressSteps = prog;
+    process.stdout.write(JSON.stringify(out));
+    return;
+  }
+  if (C.gateTolerance !== undefined) {
+    try {
+      await H.runHIP(g, C.run, null, { gateTolerance: C.gateTolerance });
+      out.runError = null;
+    } catch (err) {
+      out.runError = err.message;
+    }
+    out.selftest = H.gpuSelfTestHIP(dev, g, C.run);
+    process.stdout.write(JSON.stringify(out));
+    return;
+  }
   if (C.run) {
     const prog = [];
     const res = await H.simulateHIP(dev, g, C.run, (info) => prog.push(info), null, { precision: C.precision });
