@@ -394,11 +394,66 @@ def warm_start_init(net: U.UNet, norm: dict, dev: abx.Device, box: Box, g_fine: 
             gc.free()
 
 
-WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_unet_96x48x64.npz")
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+WEIGHTS = os.path.join(DATA, "surrogate_unet_96x48x64.abxw")                 # trained at FAMILY_DX
+WEIGHTS_DX2 = os.path.join(DATA, "surrogate_unet_96x48x64_dx2targets_1500.abxw")  # 2 mm solutions, restricted
+
+# Inference weight file (.abxw), read by this module and by the C ABI's
+# abx_surrogate_load (csrc/surrogate.cpp); little-endian:
+#   char[8] "ABXUNET1"
+#   int32   n_params, d, h, w, in_ch, head_ch, widths[4], meta_len, 0
+#   float64 family_dx, u0_scale, rho_scale, mean[4], std[4]
+#   meta_len bytes of JSON (training facts), zero padding to a multiple of 16
+#   float32 w[n_params]   (layer by layer: W[co][taps][ci] then bias[co])
+_ABXW_MAGIC = b"ABXUNET1"
+_ABXW_HEAD = "<8s12i11d"
+
+
+def write_weights(path: str, w: np.ndarray, norm: dict, meta: dict, grid=GRID, family_dx: float = FAMILY_DX) -> None:
+    """The inference weight file of a trained network (FP32 weights and the
+    target standardisation)."""
+    import json
+    import struct
+    w = np.ascontiguousarray(w, np.float32)
+    if w.size != U.n_params():
+        raise ValueError(f"{w.size} weights, the network has {U.n_params()}")
+    mb = json.dumps(meta).encode()
+    head = struct.pack(_ABXW_HEAD, _ABXW_MAGIC, w.size, *grid, U.IN_CH, U.HEAD_CH, *U.WIDTHS, len(mb), 0,
+                       float(family_dx), U0, RHO_SCALE, *[float(v) for v in norm["mean"]],
+                       *[float(v) for v in norm["std"]])
+    body = head + mb
+    body += b"\0" * (-len(body) % 16)
+    with open(path, "wb") as fh:
+        fh.write(body)
+        fh.write(w.tobytes())
+
+
+def read_weights(path: str) -> dict:
+    """The contents of an .abxw file: w, norm {mean, std}, meta, grid,
+    family_dx, u0_scale, rho_scale."""
+    import json
+    import struct
+    with open(path, "rb") as fh:
+        data = fh.read()
+    hs = struct.calcsize(_ABXW_HEAD)
+    f = struct.unpack(_ABXW_HEAD, data[:hs])
+    if f[0] != _ABXW_MAGIC:
+        raise ValueError(f"{path}: not a surrogate weight file")
+    n, d, h, w_, in_ch, head_ch = f[1:7]
+    widths, meta_len = tuple(f[7:11]), f[11]
+    if (in_ch, head_ch, widths) != (U.IN_CH, U.HEAD_CH, U.WIDTHS) or n != U.n_params():
+        raise ValueError(f"{path}: network shape {in_ch}/{head_ch}/{widths}/{n} differs from this build's")
+    dd = f[13:]
+    meta = json.loads(data[hs:hs + meta_len].decode())
+    off = hs + meta_len + (-(hs + meta_len) % 16)
+    w = np.frombuffer(data, np.float32, n, off).copy()
+    return dict(w=w, norm=dict(mean=np.array(dd[3:7], np.float32), std=np.array(dd[7:11], np.float32)), meta=meta,
+                grid=(d, h, w_), family_dx=dd[0], u0_scale=dd[1], rho_scale=dd[2])
 
 
 def save(path: str, net: U.UNet, norm: dict, meta: dict) -> None:
-    """Weights (FP32), Adam state and target standardisation as one .npz."""
+    """Training checkpoint: weights (FP32), Adam state and target
+    standardisation as one .npz (write_weights makes the inference file)."""
     import json
     st = net.state()
     np.savez_compressed(path, w=st["w"], m=st["m"], v=st["v"], step=st["step"], norm_mean=norm["mean"],
@@ -406,12 +461,17 @@ def save(path: str, net: U.UNet, norm: dict, meta: dict) -> None:
 
 
 def load(path: str, net: U.UNet) -> tuple:
-    """Loads weights into net; returns (norm, meta)."""
+    """Loads weights into net from an .abxw inference file or an .npz
+    checkpoint; returns (norm, meta)."""
     import json
+    if path.endswith(".abxw"):
+        r = read_weights(path)
+        net.load_state({"w": r["w"]})
+        return r["norm"], r["meta"]
     z = np.load(path)      # plain arrays only (allow_pickle stays off)
     if "m" in z:
         net.load_state({"w": z["w"], "m": z["m"], "v": z["v"], "step": z["step"]})
-    else:                  # weights-only file (inference)
+    else:                  # weights-only file
         net.load_state({"w": z["w"]})
     return dict(mean=z["norm_mean"], std=z["norm_std"]), json.loads(bytes(z["meta"]).decode())
 
@@ -587,10 +647,12 @@ def main(argv=None) -> int:
     tr.add_argument("--init-weights", help="continue training these weights (their standardisation is kept)")
     tr.add_argument("--cosine", action="store_true", help="cosine learning-rate decay to lr / 10")
     tr.add_argument("--keep-best", action="store_true", help="keep the weights of the best held-out epoch")
-    tr.add_argument("--out", required=True, help="weights .npz")
+    tr.add_argument("--out", required=True, help="training checkpoint .npz; the inference file is written "
+                                                 "beside it with the suffix .abxw")
     tr.add_argument("--report", help="JSON report")
     st = sub.add_parser("study", help="cold vs warm time to steady on held-out boxes")
-    st.add_argument("--weights", default=WEIGHTS, help="weights .npz (default: the trained 96x48x64 surrogate)")
+    st.add_argument("--weights", default=WEIGHTS, help="weights .abxw or .npz (default: the trained 96x48x64 "
+                                                       "surrogate)")
     st.add_argument("--boxes", type=int, default=128)
     st.add_argument("--first-seed", type=int, default=1_000_000)
     st.add_argument("--dx", type=float, default=2.0)
@@ -618,6 +680,7 @@ def main(argv=None) -> int:
         if init_meta is not None:
             meta["continued_from"] = init_meta
         save(a.out, net, data["norm"], meta)
+        write_weights(os.path.splitext(a.out)[0] + ".abxw", net.state()["w"], data["norm"], meta)
         rep = dict(meta=meta, dataset_s=t_data, dataset_stats=data["stats"], norm={k: v.tolist() for k, v in
                                                                                    data["norm"].items()},
                    train=res, params=U.n_params(), device=dev.name, torch=torch.__version__)

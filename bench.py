@@ -146,8 +146,7 @@ def bench_warmstart(a, dev):
     from airboxcalc_amd import surrogate as S, unet as U
     dx = a.dx if a.dx != 1.0 else 2.0
     n = a.variants if a.variants != 64 else 16
-    weights = S.WEIGHTS if abs(dx - S.FAMILY_DX) < 1e-9 else os.path.join(
-        ROOT, "airboxcalc_amd", "data", "surrogate_unet_96x48x64_dx2targets_1500.npz")
+    weights = S.WEIGHTS if abs(dx - S.FAMILY_DX) < 1e-9 else S.WEIGHTS_DX2
     net = U.UNet(seed=0)
     norm, meta = S.load(weights, net)
     boxes = S.family(range(1_000_000, 1_000_000 + n))

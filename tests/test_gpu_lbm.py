@@ -135,7 +135,10 @@ def test_simulate_metrics_equal_engine_js(name):
     want = np.array([[GU.x64(e) for e in row] for row in sim["hist"]])
     got = np.array(res.hist)
     assert np.array_equal(got[:, 0], want[:, 0])
-    np.testing.assert_allclose(got[:, 1:], want[:, 1:], rtol=1e-12, atol=1e-15)
+    # the probe's sums are tree reductions on the GPU: rounding at 1e-16,
+    # amplified where the mean filter velocity is still near 0 (seed5_dx3's
+    # first probes, gamma ~ 19)
+    np.testing.assert_allclose(got[:, 1:], want[:, 1:], rtol=1e-11, atol=1e-15)
 
 
 def test_init_eq_cold_start_and_moments():

@@ -107,18 +107,33 @@ def test_resample_same_lattice_is_identity_and_linear_fields_survive():
     np.testing.assert_allclose(ux[inside], S.U0 * 0.01 * fpos[0][inside], rtol=1e-4, atol=1e-6)
 
 
-def test_trained_weights_file_matches_the_network():
+@pytest.mark.parametrize("path", [S.WEIGHTS, S.WEIGHTS_DX2])
+def test_trained_weights_file_matches_the_network(path):
     from airboxcalc_amd import unet as U
-    z = np.load(S.WEIGHTS)
+    z = S.read_weights(path)
     assert z["w"].shape == (U.n_params(),) and np.isfinite(z["w"]).all()
-    assert z["norm_mean"].shape == (4,) and (z["norm_std"] > 0).all()
-    import json
-    meta = json.loads(bytes(z["meta"]).decode())
-    assert meta["grid"] == list(S.GRID) and meta["dx"] == S.FAMILY_DX
+    assert z["norm"]["mean"].shape == (4,) and (z["norm"]["std"] > 0).all()
+    assert z["meta"]["grid"] == list(S.GRID) and z["meta"]["dx"] == S.FAMILY_DX
+    assert z["grid"] == S.GRID and z["family_dx"] == S.FAMILY_DX
+    assert (z["u0_scale"], z["rho_scale"]) == (S.U0, S.RHO_SCALE)
     # the unused head channels stay zero
     head = U.layers()[-1]
     W = z["w"][head.w_off:head.w_off + head.nw].reshape(head.co, head.taps, head.ci)
     assert not W[U.OUT_CH:].any()
+
+
+def test_weight_file_round_trip(tmp_path):
+    from airboxcalc_amd import unet as U
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(U.n_params()).astype(np.float32)
+    norm = dict(mean=rng.standard_normal(4).astype(np.float32), std=rng.uniform(0.1, 1, 4).astype(np.float32))
+    p = str(tmp_path / "w.abxw")
+    S.write_weights(p, w, norm, {"boxes": 3, "note": "x" * 17})
+    r = S.read_weights(p)
+    assert np.array_equal(r["w"], w) and r["meta"] == {"boxes": 3, "note": "x" * 17}
+    assert np.array_equal(r["norm"]["mean"], norm["mean"]) and np.array_equal(r["norm"]["std"], norm["std"])
+    with pytest.raises(ValueError):
+        S.write_weights(p, w[:-1], norm, {})
 
 
 def test_restricted_targets_of_a_same_lattice_solve_are_the_targets():
