@@ -168,6 +168,8 @@ def load() -> C.CDLL:
         "abx_batch_destroy": (None, [vp]),
         "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
         "abx_evaluate": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp, vp, vp, vp, vp, vp, vp]),
+        "abx_evaluate_metrics_noalloc_bytes": (i64, [i32, i32, i32]),
+        "abx_evaluate_metrics_noalloc": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp]),
         "abx_streamlines": (i32, [vp, C.POINTER(EvalGeom), i32, d, i32, vp, i64, vp, i64, vp, i64,
                                   C.POINTER(i64), C.POINTER(i64)]),
         # flow surrogate kernels (stream first; tensors as device addresses)

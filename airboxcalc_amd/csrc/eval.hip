@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "abx_common.hpp"
@@ -413,6 +414,87 @@ int check_geom(abx_sim* s, const abx_eval_geom* g, int b) {
 
 using namespace abx;
 
+namespace {
+
+// Section indices of evaluate and field3d (engine.js:657, 662-665, 704).
+struct Sections {
+  int jb, ic, jc, jlow, kB;
+};
+
+int sections(const abx_eval_geom* g, Sections* sc) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const double h = g->h;
+  sc->jb = first_ge_f(g->dc, ny, g->pleat + 2 * h, true);
+  double si = 0, sj = 0, so = 0;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      if (g->opening[(int64_t)i * ny + j]) {
+        si += i;
+        sj += j;
+        so++;
+      }
+  auto jsround = [](double x) {
+    const double r = floor(x);
+    return (x - r >= 0.5) ? r + 1 : r;
+  };
+  if (!(so > 0)) {
+    set_error("evaluate: no inlet opening");
+    return -1;
+  }
+  sc->ic = (int)jsround(si / so);
+  sc->jc = (int)jsround(sj / so);
+  sc->jlow = first_ge_f(g->dc, ny, g->pleat + 12, true);
+  const int kw = first_ge_f(g->zc, nz, g->mn_z, false);
+  sc->kB = std::max(2, kw - 2);
+  return 0;
+}
+
+// The filter map, core mask and scalar sums of evaluate on the handle's
+// stream, into device buffers fmap (nx*nz floats), core (nx*nz bytes) and
+// out; opening (nx*ny bytes) is copied to dopen first.
+int launch_metrics(abx_sim* s, const abx_eval_geom* g, const EvalDev& e, const Sections& sc, float* fmap,
+                   uint8_t* core, uint8_t* dopen, EvalOut* out) {
+  const SimView v = sim_view(s);
+  hipStream_t st = v.stream;
+  const int64_t L = (int64_t)g->nx * g->nz;
+  ABX_HIP(hipMemcpyAsync(dopen, g->opening, (size_t)g->nx * g->ny, hipMemcpyHostToDevice, st));
+  if (v.ssz == 8) {
+    hipLaunchKernelGGL(k_layer<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, g->jm, fmap, core);
+    hipLaunchKernelGGL(k_eval_scalars<double>, dim3(1), dim3(1024), 0, st, e, sc.jb, sc.kB, dopen, fmap, core, out);
+  } else {
+    hipLaunchKernelGGL(k_layer<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, g->jm, fmap, core);
+    hipLaunchKernelGGL(k_eval_scalars<float>, dim3(1), dim3(1024), 0, st, e, sc.jb, sc.kB, dopen, fmap, core, out);
+  }
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
+// evaluate's metrics from the scalar sums (engine.js:639-660, 682-685).
+void fill_metrics(abx_sim* s, const abx_eval_geom* g, const Sections& sc, const EvalOut& o, abx_metrics* m) {
+  const SimView v = sim_view(s);
+  const double h = g->h, u0 = v.u0;
+  m->gamma = 1 - o.dev / (2 * o.n * o.wm);
+  m->gammaCore = 1 - o.cdev / (2 * o.cn * o.cm);
+  m->lowCore = o.low / o.cn;
+  m->highCore = o.high / o.cn;
+  m->reverse = o.rev / o.n;
+  m->maxToMean = o.mxr;
+  m->zeta = ((o.pin / o.pinN - o.pb / o.pbN) / 3) / (0.5 * u0 * u0);
+  m->inlet_cm2 = v.Ain * h * h / 100;
+  m->filter_cm2 = v.Af * h * h / 100;
+  m->ic = sc.ic;
+  m->jc = sc.jc;
+  m->jlow = sc.jlow;
+  m->jb = sc.jb;
+  m->kB = sc.kB;
+  m->rhoRef = o.pb / o.pbN;
+  m->pin = o.pinf / std::max(o.winf, 1e-12);
+  m->n = (int)o.n;
+  m->cn = (int)o.cn;
+}
+
+}  // namespace
+
 extern "C" int abx_evaluate(abx_sim* s, const abx_eval_geom* g, int b, double divisor, abx_metrics* m,
                             float* fmap, float* planInlet, float* planLow, float* lon, float* spd, float* pst,
                             float* loss) {
@@ -427,55 +509,30 @@ extern "C" int abx_evaluate(abx_sim* s, const abx_eval_geom* g, int b, double di
   EvalDev e = eval_dev(s, b, divisor);
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const int64_t L = (int64_t)nx * nz, N = (int64_t)nx * ny * nz;
-  const double h = g->h, u0 = v.u0;
-  // section positions (engine.js:657, 662-665, 704)
-  const int jb = first_ge_f(g->dc, ny, g->pleat + 2 * h, true);
-  double si = 0, sj = 0, so = 0;
-  for (int i = 0; i < nx; ++i)
-    for (int j = 0; j < ny; ++j)
-      if (g->opening[(int64_t)i * ny + j]) {
-        si += i;
-        sj += j;
-        so++;
-      }
-  auto jsround = [](double x) {
-    const double r = floor(x);
-    return (x - r >= 0.5) ? r + 1 : r;
-  };
-  const int ic = (int)jsround(si / so), jc = (int)jsround(sj / so);
-  const int jlow = first_ge_f(g->dc, ny, g->pleat + 12, true);
-  const int kw = first_ge_f(g->zc, nz, g->mn_z, false);
-  const int kB = std::max(2, kw - 2);
-  if (ic < 0 || ic >= nx || jc < 0 || jc >= ny) {
-    set_error("evaluate: no inlet opening");
-    return -1;
-  }
+  const double u0 = v.u0;
+  Sections sc;
+  if (sections(g, &sc)) return -1;
   Dev dfmap, dcore, dout, dopen, dplanes;
   if (dfmap.alloc(4 * (size_t)L) || dcore.alloc((size_t)L) || dout.alloc(sizeof(EvalOut)) ||
       dopen.alloc((size_t)nx * ny) || dplanes.alloc(4 * (size_t)(6 * L + 3 * (int64_t)nz * ny)))
     return -1;
-  ABX_HIP(hipMemcpyAsync(dopen.p, g->opening, (size_t)nx * ny, hipMemcpyHostToDevice, st));
+  if (launch_metrics(s, g, e, sc, dfmap.as<float>(), dcore.as<uint8_t>(), dopen.as<uint8_t>(), dout.as<EvalOut>()))
+    return -1;
   float* pl = dplanes.as<float>();
   const bool f64 = v.ssz == 8;
   if (f64) {
-    hipLaunchKernelGGL(k_layer<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, g->jm, dfmap.as<float>(),
-                       dcore.as<uint8_t>());
-    hipLaunchKernelGGL(k_eval_scalars<double>, dim3(1), dim3(1024), 0, st, e, jb, kB, dopen.as<uint8_t>(),
-                       dfmap.as<float>(), dcore.as<uint8_t>(), dout.as<EvalOut>());
-    hipLaunchKernelGGL(k_plane<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, jc, u0, pl, pl + L, pl + 2 * L);
-    hipLaunchKernelGGL(k_plane<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, jlow, u0, pl + 3 * L,
+    hipLaunchKernelGGL(k_plane<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, sc.jc, u0, pl, pl + L,
+                       pl + 2 * L);
+    hipLaunchKernelGGL(k_plane<double>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, sc.jlow, u0, pl + 3 * L,
                        pl + 4 * L, pl + 5 * L);
-    hipLaunchKernelGGL(k_plane<double>, dim3(nbk((int64_t)nz * ny, 256)), dim3(256), 0, st, e, 1, ic, u0,
+    hipLaunchKernelGGL(k_plane<double>, dim3(nbk((int64_t)nz * ny, 256)), dim3(256), 0, st, e, 1, sc.ic, u0,
                        pl + 6 * L, pl + 6 * L + (int64_t)nz * ny, pl + 6 * L + 2 * (int64_t)nz * ny);
   } else {
-    hipLaunchKernelGGL(k_layer<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, g->jm, dfmap.as<float>(),
-                       dcore.as<uint8_t>());
-    hipLaunchKernelGGL(k_eval_scalars<float>, dim3(1), dim3(1024), 0, st, e, jb, kB, dopen.as<uint8_t>(),
-                       dfmap.as<float>(), dcore.as<uint8_t>(), dout.as<EvalOut>());
-    hipLaunchKernelGGL(k_plane<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, jc, u0, pl, pl + L, pl + 2 * L);
-    hipLaunchKernelGGL(k_plane<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, jlow, u0, pl + 3 * L, pl + 4 * L,
-                       pl + 5 * L);
-    hipLaunchKernelGGL(k_plane<float>, dim3(nbk((int64_t)nz * ny, 256)), dim3(256), 0, st, e, 1, ic, u0,
+    hipLaunchKernelGGL(k_plane<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, sc.jc, u0, pl, pl + L,
+                       pl + 2 * L);
+    hipLaunchKernelGGL(k_plane<float>, dim3(nbk(L, 256)), dim3(256), 0, st, e, 0, sc.jlow, u0, pl + 3 * L,
+                       pl + 4 * L, pl + 5 * L);
+    hipLaunchKernelGGL(k_plane<float>, dim3(nbk((int64_t)nz * ny, 256)), dim3(256), 0, st, e, 1, sc.ic, u0,
                        pl + 6 * L, pl + 6 * L + (int64_t)nz * ny, pl + 6 * L + 2 * (int64_t)nz * ny);
   }
   ABX_HIP(hipGetLastError());
@@ -486,25 +543,8 @@ extern "C" int abx_evaluate(abx_sim* s, const abx_eval_geom* g, int b, double di
   if (planLow) ABX_HIP(hipMemcpyAsync(planLow, pl + 3 * L, 12 * (size_t)L, hipMemcpyDeviceToHost, st));
   if (lon) ABX_HIP(hipMemcpyAsync(lon, pl + 6 * L, 12 * (size_t)nz * ny, hipMemcpyDeviceToHost, st));
   ABX_HIP(hipStreamSynchronize(st));
+  fill_metrics(s, g, sc, o, m);
   const double q = 0.5 * u0 * u0;
-  m->gamma = 1 - o.dev / (2 * o.n * o.wm);
-  m->gammaCore = 1 - o.cdev / (2 * o.cn * o.cm);
-  m->lowCore = o.low / o.cn;
-  m->highCore = o.high / o.cn;
-  m->reverse = o.rev / o.n;
-  m->maxToMean = o.mxr;
-  m->zeta = ((o.pin / o.pinN - o.pb / o.pbN) / 3) / (0.5 * u0 * u0);
-  m->inlet_cm2 = v.Ain * h * h / 100;
-  m->filter_cm2 = v.Af * h * h / 100;
-  m->ic = ic;
-  m->jc = jc;
-  m->jlow = jlow;
-  m->jb = jb;
-  m->kB = kB;
-  m->rhoRef = o.pb / o.pbN;
-  m->pin = o.pinf / std::max(o.winf, 1e-12);
-  m->n = (int)o.n;
-  m->cn = (int)o.cn;
   if (spd || pst || loss) {
     if (!spd || !pst || !loss) {
       set_error("abx_evaluate: spd, pst and loss go together");
@@ -514,10 +554,10 @@ extern "C" int abx_evaluate(abx_sim* s, const abx_eval_geom* g, int b, double di
     if (d3.alloc(12 * (size_t)N)) return -1;
     float* f3 = d3.as<float>();
     if (f64)
-      hipLaunchKernelGGL(k_field3d<double>, dim3(nbk(N, 256)), dim3(256), 0, st, e, kB, u0, q, m->rhoRef, m->pin,
+      hipLaunchKernelGGL(k_field3d<double>, dim3(nbk(N, 256)), dim3(256), 0, st, e, sc.kB, u0, q, m->rhoRef, m->pin,
                          f3, f3 + N, f3 + 2 * N);
     else
-      hipLaunchKernelGGL(k_field3d<float>, dim3(nbk(N, 256)), dim3(256), 0, st, e, kB, u0, q, m->rhoRef, m->pin,
+      hipLaunchKernelGGL(k_field3d<float>, dim3(nbk(N, 256)), dim3(256), 0, st, e, sc.kB, u0, q, m->rhoRef, m->pin,
                          f3, f3 + N, f3 + 2 * N);
     ABX_HIP(hipGetLastError());
     ABX_HIP(hipMemcpyAsync(spd, f3, 4 * (size_t)N, hipMemcpyDeviceToHost, st));
@@ -612,5 +652,86 @@ extern "C" int abx_streamlines(abx_sim* s, const abx_eval_geom* g, int b, double
                (long long)nlines);
     return -1;
   }
+  return 0;
+}
+
+// ---- evaluate's metrics without device allocations ------------------------
+// The buffers abx_evaluate allocates per call for its metrics (the filter map,
+// the core mask, the inlet opening and the scalar sums) live here in one
+// fixed device array per GPU.  The host never passes `scratch` itself to a
+// kernel or a copy: it takes the array's device address once per device with
+// hipGetSymbolAddress and uses that.
+constexpr size_t kScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Device address of this GPU's scratch array (the device is current).
+int scratch_address(int ordinal, unsigned char** out) {
+  if (ordinal < 0 || ordinal >= kMaxDevices) {
+    set_errorf("abx_evaluate_metrics_noalloc: device %d out of range", ordinal);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[ordinal]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_errorf("abx_evaluate_metrics_noalloc: scratch address: %s", hipGetErrorString(e));
+      return -1;
+    }
+    g_scratch_ptr[ordinal] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[ordinal];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int64_t abx_evaluate_metrics_noalloc_bytes(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const size_t L = (size_t)nx * nz;
+  return (int64_t)(align256(sizeof(EvalOut)) + align256(4 * L) + align256(L) + align256((size_t)nx * ny));
+}
+
+extern "C" int abx_evaluate_metrics_noalloc(abx_sim* s, const abx_eval_geom* g, int b, double divisor,
+                                            abx_metrics* m, float* fmap) {
+  if (!s || !m) {
+    set_error("abx_evaluate_metrics_noalloc: null argument");
+    return -1;
+  }
+  if (check_geom(s, g, b)) return -1;
+  const int64_t need = abx_evaluate_metrics_noalloc_bytes(g->nx, g->ny, g->nz);
+  if (need < 0 || (size_t)need > kScratchBytes) {
+    set_errorf("abx_evaluate_metrics_noalloc: a %d x %d x %d grid needs %lld bytes of scratch, more than the "
+               "%zu bytes available",
+               g->nx, g->ny, g->nz, (long long)need, kScratchBytes);
+    return -1;
+  }
+  const SimView v = sim_view(s);
+  ABX_HIP(hipSetDevice(v.ordinal));
+  unsigned char* base = nullptr;
+  if (scratch_address(v.ordinal, &base)) return -1;
+  Sections sc;
+  if (sections(g, &sc)) return -1;
+  const size_t L = (size_t)g->nx * g->nz;
+  // every byte used below is written before it is read in this call
+  EvalOut* dout = (EvalOut*)base;
+  float* dfmap = (float*)(base + align256(sizeof(EvalOut)));
+  uint8_t* dcore = (uint8_t*)dfmap + align256(4 * L);
+  uint8_t* dopen = dcore + align256(L);
+  const EvalDev e = eval_dev(s, b, divisor);
+  if (launch_metrics(s, g, e, sc, dfmap, dcore, dopen, dout)) return -1;
+  EvalOut o;
+  ABX_HIP(hipMemcpyAsync(&o, dout, sizeof o, hipMemcpyDeviceToHost, v.stream));
+  if (fmap) ABX_HIP(hipMemcpyAsync(fmap, dfmap, 4 * L, hipMemcpyDeviceToHost, v.stream));
+  ABX_HIP(hipStreamSynchronize(v.stream));
+  fill_metrics(s, g, sc, o, m);
   return 0;
 }

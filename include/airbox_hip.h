@@ -311,6 +311,20 @@ typedef struct {
 int abx_evaluate(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, abx_metrics* m, float* fmap,
                  float* planInlet, float* planLow, float* lon, float* spd, float* pst, float* loss);
 
+/* No reference counterpart: evaluate's metrics (and, if fmap is not NULL,
+ * the normalised filter map of nx*nz floats) exactly as abx_evaluate computes
+ * them, without allocating device memory, for callers that cannot allocate
+ * (code under HIP graph capture, serving loops that poll a running solve).
+ * The working buffers live in one fixed device array of 8 MiB per GPU, held
+ * until the process ends; a grid needs abx_evaluate_metrics_noalloc_bytes
+ * (nx, ny, nz) bytes of it (about 5*nx*nz + nx*ny) and larger grids fail with
+ * an error.  Every call on a GPU uses that GPU's one array: calls that
+ * overlap on one GPU (different threads, or streams of different handles)
+ * race and must be serialised by the caller. */
+int64_t abx_evaluate_metrics_noalloc_bytes(int nx, int ny, int nz);
+int abx_evaluate_metrics_noalloc(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, abx_metrics* m,
+                                 float* fmap);
+
 /* Replaces streamlines (engine.js:721-758): RK2 traces of the averaged field
  * from up to maxSeeds seeds over the inlet opening, one GPU thread per seed.
  * pts (3 floats per point, model frame), spd (one per point) and off (line
