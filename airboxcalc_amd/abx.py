@@ -96,6 +96,26 @@ class Metrics(C.Structure):
 METRIC_KEYS = ("gamma", "gammaCore", "lowCore", "highCore", "reverse", "maxToMean", "zeta", "inlet_cm2",
                "filter_cm2")
 
+
+class SteadyParams(C.Structure):
+    _fields_ = [("window", C.c_int), ("max_steps", C.c_int), ("step0", C.c_int), ("tol_gamma", C.c_double),
+                ("tol_zeta", C.c_double)]
+
+
+class SteadyWindow(C.Structure):
+    _fields_ = [("step", C.c_int), ("gamma", C.c_double), ("massRatio", C.c_double), ("metrics", Metrics)]
+
+
+class SteadyResult(C.Structure):
+    _fields_ = [("converged", C.c_int), ("steps", C.c_int), ("windows", C.c_int), ("metrics", Metrics)]
+
+
+STEADY_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(SteadyWindow))
+
+
+def metrics_dict(M: Metrics) -> dict:
+    return {k: getattr(M, k) for k in METRIC_KEYS}
+
 G_SOLID, G_FLUID, G_OPENING, G_OUTLET, G_XC, G_DC, G_ZC, G_OPENPTS, G_FILTERPTS, G_OUTLETPTS, G_FINE = range(11)
 
 _lib = None
@@ -172,6 +192,12 @@ def load() -> C.CDLL:
         "abx_evaluate_metrics_noalloc": (i32, [vp, C.POINTER(EvalGeom), i32, d, C.POINTER(Metrics), vp]),
         "abx_streamlines": (i32, [vp, C.POINTER(EvalGeom), i32, d, i32, vp, i64, vp, i64, vp, i64,
                                   C.POINTER(i64), C.POINTER(i64)]),
+        "abx_sim_solve_to_steady": (i32, [vp, C.POINTER(EvalGeom), C.POINTER(SteadyParams), STEADY_CB, vp,
+                                          C.POINTER(SteadyResult)]),
+        "abx_surrogate_load": (i32, [vp, C.c_char_p, C.POINTER(vp)]),
+        "abx_surrogate_info": (i32, [vp, C.POINTER(i32), C.POINTER(d)]),
+        "abx_surrogate_free": (None, [vp]),
+        "abx_surrogate_warm_start": (i32, [vp, vp, vp, vp, i64, C.POINTER(PrepParams), d, d]),
         # flow surrogate kernels (stream first; tensors as device addresses)
         "abx_conv3d_pack": (i32, [vp, vp, i32, i32, i32, i32, vp]),
         "abx_conv3d_pack_all": (i32, [vp, i32, vp]),
@@ -317,6 +343,7 @@ class Geometry:
         P = prep_params(p, max_cells, cy, sy)
         self.ptr = C.c_void_p()
         _check(L.abx_prepare(dev.h, _p(tris), tris.size, C.byref(P), C.byref(self.ptr)))
+        self.tris, self.prep = tris, P   # what a warm start needs to prepare the box at another cell size
         I = GeomInfo()
         _check(L.abx_geom_info_get(self.ptr, C.byref(I)))
         self.info = I
@@ -492,6 +519,36 @@ class Sim:
             out["vol"] = dict(spd=vol[0], pst=vol[1], loss=vol[2], kB=M.kB)
         return out
 
+    def solve_to_steady(self, eg: "EvalGeom", window: int = 1000, tol_gamma: float = 0.005, tol_zeta: float = 0.02,
+                        max_steps: int = 60000, step0: int = 0, on_window=None) -> dict:
+        """abx_sim_solve_to_steady: windows of `window` averaged steps until
+        gamma-core and zeta agree between two consecutive windows.  Returns
+        {converged, steps, metrics (mean of the two agreeing windows),
+        windows: [(step, gamma, massRatio, gammaCore, zeta)]}; on_window(dict)
+        is called after every window (returning True stops the solve)."""
+        wins, raised = [], []
+
+        def cb(_user, wp):
+            w = wp.contents
+            m = metrics_dict(w.metrics)
+            wins.append((w.step, w.gamma, w.massRatio, m["gammaCore"], m["zeta"]))
+            if on_window is not None:
+                try:
+                    return 1 if on_window(dict(step=w.step, gamma=w.gamma, massRatio=w.massRatio, metrics=m)) else 0
+                except BaseException as e:  # stop the solve, re-raise below
+                    raised.append(e)
+                    return 1
+            return 0
+
+        P = SteadyParams(int(window), int(max_steps), int(step0), float(tol_gamma), float(tol_zeta))
+        R = SteadyResult()
+        fn = STEADY_CB(cb)
+        _check(load().abx_sim_solve_to_steady(self.h, C.byref(eg), C.byref(P), fn, None, C.byref(R)))
+        if raised:
+            raise raised[0]
+        return dict(converged=bool(R.converged), steps=R.steps, metrics=metrics_dict(R.metrics), windows=wins,
+                    ic=R.metrics.ic, jc=R.metrics.jc, jlow=R.metrics.jlow, jb=R.metrics.jb, kB=R.metrics.kB)
+
     def streamlines(self, eg: "EvalGeom", divisor: float = 1.0, b: int = 0, max_seeds: int = 220) -> dict:
         """streamlines (engine.js:721-758) on the GPU."""
         cap = 2002 * max_seeds + 16
@@ -511,13 +568,13 @@ class Sim:
         nsa = None if ns is None else np.ascontiguousarray(ns, np.float32)
         _check(load().abx_sim_set_ns(self.h, _p(nsa)))
 
-    def set_launch(self, kernel: int = 4, xcd_map: int = -1, nt: int = 0) -> None:
+    def set_launch(self, kernel: int = 4, xcd_map: int = -1, nt: int = 19) -> None:
         """kernel 4 = AA pattern in one lattice (default), 2 = two lattices
         with pull streaming.  xcd_map: block order across XCDs (0 round
         robin, 1 contiguous, G > 1 groups of G blocks, -1 one band of rows j
-        per XCD: the library's default).  nt: non-temporal population
-        accesses of the AA steps (bits 0/1 even-step loads/stores, bits 2/3
-        odd-step loads/stores)."""
+        per XCD: the library's default).  nt: non-temporal access flags of
+        the AA steps, even step in bits 0-7, odd step in bits 8-15
+        (abx_sim_set_launch; default 19)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, nt))
 
     def set_graph(self, enable: bool = True) -> None:
@@ -585,6 +642,37 @@ class Batch:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """The trained flow surrogate on the device (abx_surrogate_load): the
+    warm start of the C ABI, without torch."""
+
+    def __init__(self, dev: Device, path: str):
+        self.h = C.c_void_p()
+        self.dev = dev
+        _check(load().abx_surrogate_load(dev.h, path.encode(), C.byref(self.h)))
+        dims = (C.c_int * 3)()
+        fdx = C.c_double()
+        _check(load().abx_surrogate_info(self.h, dims, C.byref(fdx)))
+        self.dims, self.family_dx = tuple(dims), fdx.value
+
+    def warm_start(self, sim: "Sim", g: "Geometry", K: float, aniso: float) -> None:
+        """Seed sim (on g's lattice) at the equilibrium of the prediction
+        (abx_surrogate_warm_start)."""
+        _check(load().abx_surrogate_warm_start(self.h, sim.h, g.ptr, _p(g.tris), g.tris.size, C.byref(g.prep),
+                                               float(K), float(aniso)))
+
+    def free(self):
+        if self.h:
+            load().abx_surrogate_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

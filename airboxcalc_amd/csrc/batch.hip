@@ -37,7 +37,9 @@ __global__ __launch_bounds__(64 * SPB) void k_batch_step(const StepArgs* __restr
     }
     __syncthreads();
     const int seg = work[blk * SPB + wv].y;
-    if (seg >= 0) seg_cell<S, M, ACC, false, MODE>(a, win[wv], seg, l, 0);
+    // non-temporal accesses as a single handle's default: the even AA step's
+    // populations and averaged fields (float storage)
+    if (seg >= 0) seg_cell<S, M, ACC, false, MODE, (sizeof(S) == 4 && MODE == 1) ? 19 : 0>(a, win[wv], seg, l, 0);
     if (!PERSIST) break;
     __syncthreads();
   }

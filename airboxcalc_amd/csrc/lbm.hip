@@ -293,9 +293,9 @@ struct abx_sim {
   int32_t* chunks_slab = nullptr;
   int nslab = 0;
   // kernel 4: AA pattern in one lattice (the default); 2: two lattices, pull.
-  // nt: non-temporal population accesses of the AA steps, bits 0/1 = loads /
-  // stores of the even step, bits 2/3 = loads / stores of the odd step.
-  int kernel = 4, nt = 0;
+  // nt: non-temporal accesses of the AA steps, seg_cell's NT flags of the even
+  // step in bits 0-7 and of the odd step in bits 8-15.
+  int kernel = 4, nt = 19;
   // stored form of the populations: 0 plain (f[cur][q](x) = f_q), 1 AA
   // swapped (F[opp q](x) = f_q), 2 AA streamed (after an odd AA step)
   int form = 1;
@@ -308,8 +308,8 @@ struct abx_sim {
   double u0 = 0, tau0 = 0, sig = 0, aniso = 0, cs2s = 0;
   int64_t device_bytes = 0;
   int use_graph = 0;
-  // captured chunks of steps: (n, doacc, form, cur, kernel) -> executable graph
-  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
+  // captured chunks of steps: (n, doacc, form, cur, kernel, nt, xcd_map) -> executable graph
+  std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -421,20 +421,51 @@ void launch_seg(abx_sim* s, StepArgs a, int doacc) {
   }
 }
 
-// AA step of mode MODE with the non-temporal flags nt (float storage only;
-// double storage always uses cached accesses).
+// Non-temporal flag sets of the AA steps (seg_cell's NT) that are compiled:
+// the even step (mode 1) takes the first list, the odd step (mode 2) the
+// second.  Double storage always uses cached accesses.
+constexpr int kNtEven[] = {0, 3, 19};
+constexpr int kNtOdd[] = {0, 1, 2, 3, 8, 11, 16, 19};
+
+bool nt_supported(int nt) {
+  const int ev = nt & 0xff, od = nt >> 8;
+  bool e = false, o = false;
+  for (int v : kNtEven) e |= v == ev;
+  for (int v : kNtOdd) o |= v == od;
+  return e && o && nt >= 0 && nt < (1 << 16);
+}
+
 template <typename S, typename M, int MODE>
 void launch_aa(abx_sim* s, const StepArgs& a, int doacc, int nt) {
   if constexpr (sizeof(S) == 4) {
-    switch (nt & 3) {
-      case 1:
-        return launch_seg<S, M, MODE, 1>(s, a, doacc);
-      case 2:
-        return launch_seg<S, M, MODE, 2>(s, a, doacc);
-      case 3:
-        return launch_seg<S, M, MODE, 3>(s, a, doacc);
-      default:
-        break;
+    if constexpr (MODE == 1) {
+      switch (nt) {
+        case 3:
+          return launch_seg<S, M, MODE, 3>(s, a, doacc);
+        case 19:
+          return launch_seg<S, M, MODE, 19>(s, a, doacc);
+        default:
+          break;
+      }
+    } else {
+      switch (nt) {
+        case 1:
+          return launch_seg<S, M, MODE, 1>(s, a, doacc);
+        case 2:
+          return launch_seg<S, M, MODE, 2>(s, a, doacc);
+        case 3:
+          return launch_seg<S, M, MODE, 3>(s, a, doacc);
+        case 8:
+          return launch_seg<S, M, MODE, 8>(s, a, doacc);
+        case 11:
+          return launch_seg<S, M, MODE, 11>(s, a, doacc);
+        case 16:
+          return launch_seg<S, M, MODE, 16>(s, a, doacc);
+        case 19:
+          return launch_seg<S, M, MODE, 19>(s, a, doacc);
+        default:
+          break;
+      }
     }
   }
   launch_seg<S, M, MODE, 0>(s, a, doacc);
@@ -445,9 +476,9 @@ void launch_step_t(abx_sim* s, const StepArgs& a, int doacc) {
   if (s->kernel == 4) {
     // AA pattern: an odd step follows the swapped form, an even one the streamed form
     if (s->form == 2)
-      launch_aa<S, M, 1>(s, a, doacc, s->nt);
+      launch_aa<S, M, 1>(s, a, doacc, s->nt & 0xff);
     else
-      launch_aa<S, M, 2>(s, a, doacc, s->nt >> 2);
+      launch_aa<S, M, 2>(s, a, doacc, s->nt >> 8);
   } else {
     launch_seg<S, M, 0, 0>(s, a, doacc);
   }
@@ -788,7 +819,7 @@ extern "C" int abx_sim_steps(abx_sim* s, int n, int doacc) {
   // lattice, kernel): capture it once, replay it afterwards
   if (s->kernel != 4 && ensure_second_lattice(s)) return -1;
   const int form0 = s->form, cur0 = s->cur;
-  const auto key = std::make_tuple(n, doacc, form0, cur0, s->kernel + 8 * s->nt + 256 * (s->xcd_map + 1));
+  const auto key = std::make_tuple(n, doacc, form0, cur0, s->kernel, s->nt, s->xcd_map);
   auto it = s->graphs.find(key);
   if (it == s->graphs.end()) {
     hipGraph_t g;
@@ -1105,8 +1136,9 @@ extern "C" int abx_sim_sync(abx_sim* s) {
 }
 
 extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
-  if (!s || (kernel != 2 && kernel != 4) || nt < 0 || nt > 15) {
-    set_error("abx_sim_set_launch: bad argument (kernel 2 or 4, nt 0..15)");
+  if (!s || (kernel != 2 && kernel != 4) || !nt_supported(nt)) {
+    set_error("abx_sim_set_launch: bad argument (kernel 2 or 4; nt: even-step flags 0, 3 or 19 plus 256 x odd-step "
+              "flags 0, 1, 2, 3, 8, 11, 16 or 19)");
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));

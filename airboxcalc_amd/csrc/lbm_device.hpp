@@ -5,6 +5,9 @@
 // bit-identical to the same solve run alone.
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "abx_common.hpp"
 
 namespace abx {
@@ -31,13 +34,18 @@ __device__ __forceinline__ M min09(M x) {
   return x > (M)0.9 ? (M)0.9 : x;
 }
 
-template <typename T>
-__device__ __forceinline__ T ld(const T* p, bool nt) {
-  return nt ? __builtin_nontemporal_load(p) : *p;
+// Load / store, non-temporal (streamed through the caches without keeping
+// the line) when NT.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T* p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
 }
-template <typename T>
-__device__ __forceinline__ void st(T* p, T v, bool nt) {
-  if (nt)
+template <bool NT, typename T>
+__device__ __forceinline__ void st(T* p, T v) {
+  if constexpr (NT)
     __builtin_nontemporal_store(v, p);
   else
     *p = v;
@@ -308,21 +316,38 @@ __device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int 
   return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? q * P + dn : OPP[q] * P + d;
 }
 
+// f(integral_constant<int, q>) for q = 0 .. 18 in order, q a constant
+// expression in each call.
+template <typename F, int... Q>
+__device__ __forceinline__ void for_each_q(F&& f, std::integer_sequence<int, Q...>) {
+  (f(std::integral_constant<int, Q>{}), ...);
+}
+template <typename F>
+__device__ __forceinline__ void for_each_q(F&& f) {
+  for_each_q(f, std::make_integer_sequence<int, 19>{});
+}
+
 // One lane's cell of a 64-position segment starting at seg: the fused
 // stream-collide of the scheme MODE (see k_stream_collide_seg).  win holds
 // the segment's nine neighbour-row bit windows (positions seg-32 .. seg+96).
-// NT bit 0 / bit 1: population loads / stores are non-temporal (streamed
-// through the caches without keeping the lines).
+// NT selects non-temporal accesses (streamed through the caches without
+// keeping the lines): bit 0 / bit 1 = population loads / stores at
+// line-aligned offsets (every population of the even AA step, the cz = 0
+// ones of the odd step), bit 2 / bit 3 = the odd step's other loads /
+// stores, bit 4 = the averaged fields.
 template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0>
 __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win)[9][4], int seg, int l, int b) {
   const int p0 = l + 32;
   // AA steps: a non-fluid position (solid or padding) whose 32-position half
-  // of the segment holds fluid moves its own 19 slots through unchanged (read
-  // slot q, write slot opp q, as a fluid cell walled in on all sides).  No
-  // fluid cell ever reads or writes those slots; this only makes every memory
-  // line that holds fluid fully read and fully written (no partially written
-  // lines at the ragged ends of the fluid runs), while lines without fluid
-  // stay untouched.
+  // of the segment holds fluid writes its own 19 slots (slot opp q, as a
+  // fluid cell walled in on all sides): in the even step the values it reads
+  // from its own slots, in the odd step zeros, without reading.  No fluid
+  // cell ever reads or writes those slots and nothing else reads them, so
+  // their values do not matter; this only makes every memory line that holds
+  // fluid fully written (no partially written lines at the ragged ends of the
+  // fluid runs).  Lines without fluid stay untouched.  (Both choices were
+  // timed: the even step's unconditional aligned loads are cheaper than
+  // masked ones, the odd step's masked gathers cheaper than full ones.)
   const uint32_t own = win[4][p0 >> 5];
   const bool fluid = (own >> (p0 & 31)) & 1u;
   if (!fluid && (MODE == 0 || own == 0u)) return;
@@ -342,8 +367,7 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
 #pragma unroll
   for (int wi = 0; wi < 9; ++wi) wb[wi] = (uint64_t)win[wi][w0] | (uint64_t)win[wi][w0 + 1] << 32;
   uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
+  auto load = [&](int q, auto nt) {
     const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
     const bool nbf = fluid & (bool)((wb[wi] >> (sh0 - CZ[q])) & 1u);
     nbm |= (uint32_t)nbf << q;
@@ -355,14 +379,25 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
       src = fin + q * P + d;
     else
       src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
-    fi[q] = (M)ld(src, NT & 1);
+    fi[q] = (M)ld<decltype(nt)::value>(src);
+  };
+  // in the odd step only the populations with cz = 0 sit a whole number of
+  // 128-byte lines away
+  if (MODE != 2 || fluid) {
+    for_each_q([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr bool aligned = MODE != 2 || CZ[q] == 0;
+      load(q, std::integral_constant<bool, aligned ? (NT & 1) != 0 : (NT & 4) != 0>{});
+    });
+  } else {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fi[q] = 0;
   }
   M s = 0;
   if (NS) s = (M)a.ns[(int64_t)b * P + d];
   M ux, uy, uz, rho;
   collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
-#pragma unroll
-  for (int q = 0; q < 19; ++q) {
+  auto store = [&](int q, auto nt) {
     S* dst;
     if (MODE == 0) {
       dst = fout + q * P + d;
@@ -372,16 +407,28 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
       const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
       dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
     }
-    st(dst, fluid ? (S)fo[q] : (S)fi[q], NT & 2);
-  }
+    st<decltype(nt)::value>(dst, fluid ? (S)fo[q] : (S)fi[q]);
+  };
+  for_each_q([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr bool aligned = MODE != 2 || CZ[q] == 0;
+    store(q, std::integral_constant<bool, aligned ? (NT & 2) != 0 : (NT & 8) != 0>{});
+  });
   if (ACC) {
+    constexpr bool NTC = (NT & 16) != 0;
     S* ac = (S*)a.acc + (int64_t)b * 4 * P;
-    // non-fluid positions rewrite their (unused) sums unchanged
-    const S a0 = ac[d], a1 = ac[P + d], a2 = ac[2 * P + d], a3 = ac[3 * P + d];
-    ac[d] = fluid ? (S)((M)a0 + ux) : a0;
-    ac[P + d] = fluid ? (S)((M)a1 + uy) : a1;
-    ac[2 * P + d] = fluid ? (S)((M)a2 + uz) : a2;
-    ac[3 * P + d] = fluid ? (S)((M)a3 + rho) : a3;
+    // non-fluid positions write zero sums (never read) without reading
+    S a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (fluid) {
+      a0 = (S)((M)ld<NTC>(ac + d) + ux);
+      a1 = (S)((M)ld<NTC>(ac + P + d) + uy);
+      a2 = (S)((M)ld<NTC>(ac + 2 * P + d) + uz);
+      a3 = (S)((M)ld<NTC>(ac + 3 * P + d) + rho);
+    }
+    st<NTC>(ac + d, a0);
+    st<NTC>(ac + P + d, a1);
+    st<NTC>(ac + 2 * P + d, a2);
+    st<NTC>(ac + 3 * P + d, a3);
   }
 }
 

@@ -91,6 +91,8 @@ function prepare(p, dev) {
   const I = r.info, h = p.dx;
   const out = p.outlet && p.outlet.enabled ? p.outlet : null;
   return {
+    // the inputs, kept for a surrogate warm start at another cell size
+    source: { tris: p.tris, params: Object.assign({}, p, { tris: undefined, cy: fr.cy, sy: fr.sy }) },
     fr, h, nx: I.nx, ny: I.ny, nz: I.nz, N: I.N, xc: r.xc, dc: r.dc, zc: r.zc,
     solid: r.solid, fluid: r.fluid, opening: r.opening, outlet: r.outlet,
     top: I.top, jm: I.jm, nf: I.nf, mn: I.mn, mx: I.mx, pleat: p.pleat,
@@ -152,16 +154,48 @@ const plane = (buf, L, n1, n2, key, value) => {
   return o;
 };
 
+/* The trained surrogate networks shipped with this backend (no reference
+   counterpart): "family" predicts at its own cell size of 4.8 mm, "dx2" was
+   trained on 2 mm solutions (use it for solves at 2 mm). */
+const SURROGATE_WEIGHTS = {
+  family: path.join(__dirname, "..", "data", "surrogate_unet_96x48x64.abxw"),
+  dx2: path.join(__dirname, "..", "data", "surrogate_unet_96x48x64_dx2targets_1500.abxw"),
+};
+
+/* loadSurrogate(dev, file) -> {net, dims, familyDx}: the network on the GPU
+   (abx_surrogate_load), for simulateHIP's warmStart option. */
+function loadSurrogate(dev, file) {
+  const d = deviceOrDefault(dev);
+  return A.surrogateLoad(d.device, file || SURROGATE_WEIGHTS.family);
+}
+
 /* simulateHIP(dev, g, p, onProgress, ns, opts) -> result: the reference's
    simulateGPU (engine.js:613-630) on the gfx950 kernel.  p = {K, aniso, u,
-   tau, steps, avg, every}; opts.precision "f32" (default), "f32_f64" (the
-   CPU backend's arithmetic) or "f64". */
+   tau, steps, avg, every}; opts (all optional):
+     precision  "f32" (default), "f32_f64" (the CPU backend's arithmetic) or
+                "f64";
+     warmStart  a loadSurrogate() network: the lattice starts at the
+                equilibrium of its prediction instead of at rest
+                (engine.js:260-261);
+     criterion  {window, tolGamma, tolZeta, maxSteps}: instead of p.steps /
+                p.avg, run windows of `window` averaged steps until gamma-core
+                moves by less than tolGamma and zeta by at most tolZeta
+                (relative) between two windows (abx_sim_solve_to_steady).  The
+                result's metrics are then the mean of those two windows, its
+                fields the last window's average, hist one row per window, and
+                result.steady = {converged, steps, windows}; a progress object
+                per window adds gammaCore and zeta. */
 async function simulateHIP(dev, g, p, onProgress, ns, opts) {
   const d = deviceOrDefault(dev);
   const prec = PRECISION[(opts && opts.precision) || "f32"];
   if (prec === undefined) throw new Error("unknown precision " + opts.precision);
   const c = A.simCreate(d.device, g.native, p, prec, 1, ns || null), sim = c.sim, NF = c.info.NF;
   try {
+    if (opts && opts.warmStart) {
+      const src = g.source || {};
+      A.surrogateWarmStart(opts.warmStart.net, sim, g.native, src.tris || null, src.params || null, p.K, p.aniso);
+    }
+    if (opts && opts.criterion) return await steadyRun(d, g, p, onProgress, sim, NF, opts.criterion);
     const t0 = Date.now(), hist = [];
     let step = 0, nacc = 0;
     const startAcc = p.steps - p.avg;
@@ -181,23 +215,55 @@ async function simulateHIP(dev, g, p, onProgress, ns, opts) {
         if (onProgress) onProgress(info);
       }
     }
-    const div = Math.max(nacc, 1);
-    const E = A.evaluate(sim, g.native, 0, div, true), L = A.streamlines(sim, g.native, 0, div, 220);
-    const { nx, ny, nz, xc, dc, zc, fr } = g, Lxz = nx * nz;
-    return {
-      metrics: E.metrics, hist, fmap: E.fmap, nx, nz, ny, xc: Array.from(xc), zc: Array.from(zc), dc: Array.from(dc),
-      pleat: g.pleat,
-      planInlet: plane(E.planInlet, Lxz, nx, nz, "depth", dc[E.jc]),
-      planLow: plane(E.planLow, Lxz, nx, nz, "depth", dc[E.jlow]),
-      lon: plane(E.lon, nz * ny, nz, ny, "x", xc[E.ic]),
-      frame: { dAx: fr.dAx, iAx: fr.iAx, oAx: fr.oAx, sd: fr.sd, si: fr.si, seal: fr.seal, yaw: fr.yaw },
-      vol: { nx, ny, nz, h: g.h, x0: xc[0], d0: dc[0], z0: zc[0], spd: E.spd, pst: E.pst, loss: E.loss, jm: g.jm,
-        top: g.top, pleat: g.pleat, kB: E.kB, zIn: g.mn[2] },
-      lines: L, device: "GPU · " + d.name,
-    };
+    return resultObject(d, g, sim, Math.max(nacc, 1), hist);
   } finally {
     A.destroy(sim);
   }
+}
+
+/* The result object of evaluate (engine.js:681-690) from the handle's
+   averaged fields divided by div. */
+function resultObject(d, g, sim, div, hist) {
+  const E = A.evaluate(sim, g.native, 0, div, true), L = A.streamlines(sim, g.native, 0, div, 220);
+  const { nx, ny, nz, xc, dc, zc, fr } = g, Lxz = nx * nz;
+  return {
+    metrics: E.metrics, hist, fmap: E.fmap, nx, nz, ny, xc: Array.from(xc), zc: Array.from(zc), dc: Array.from(dc),
+    pleat: g.pleat,
+    planInlet: plane(E.planInlet, Lxz, nx, nz, "depth", dc[E.jc]),
+    planLow: plane(E.planLow, Lxz, nx, nz, "depth", dc[E.jlow]),
+    lon: plane(E.lon, nz * ny, nz, ny, "x", xc[E.ic]),
+    frame: { dAx: fr.dAx, iAx: fr.iAx, oAx: fr.oAx, sd: fr.sd, si: fr.si, seal: fr.seal, yaw: fr.yaw },
+    vol: { nx, ny, nz, h: g.h, x0: xc[0], d0: dc[0], z0: zc[0], spd: E.spd, pst: E.pst, loss: E.loss, jm: g.jm,
+      top: g.top, pleat: g.pleat, kB: E.kB, zIn: g.mn[2] },
+    lines: L, device: "GPU · " + d.name,
+  };
+}
+
+/* simulateHIP's criterion run: the library's steady solve on a pool thread,
+   a progress object per window on this thread. */
+async function steadyRun(d, g, p, onProgress, sim, NF, crit) {
+  const window = crit.window || 1000, maxSteps = crit.maxSteps || 60000;
+  const t0 = Date.now(), hist = [], windows = [];
+  let allSeen = null;
+  const onWindow = (w) => {
+    const el = (Date.now() - t0) / 1000;
+    hist.push([w.step, w.gamma, w.massRatio]);
+    windows.push([w.step, w.gamma, w.massRatio, w.metrics.gammaCore, w.metrics.zeta]);
+    if (onProgress) {
+      onProgress({ step: w.step, steps: maxSteps, gamma: w.gamma, massRatio: w.massRatio, elapsed: el,
+        eta: el / w.step * (maxSteps - w.step), mlups: NF * w.step / el / 1e6, device: "GPU",
+        gammaCore: w.metrics.gammaCore, zeta: w.metrics.zeta });
+    }
+    if (allSeen && windows.length >= allSeen.n) allSeen.resolve();
+  };
+  const r = await A.solveToSteady(sim, g.native, { window, maxSteps, tolGamma: crit.tolGamma === undefined ? 0.005 :
+    crit.tolGamma, tolZeta: crit.tolZeta === undefined ? 0.02 : crit.tolZeta }, onWindow);
+  // the per-window calls may trail the solve's completion
+  if (windows.length < r.windows) await new Promise((resolve) => { allSeen = { n: r.windows, resolve }; });
+  const res = resultObject(d, g, sim, window, hist);
+  res.metrics = r.metrics;
+  res.steady = { converged: r.converged, steps: r.steps, windows };
+  return res;
 }
 
 /* runHIP(g, p, onProgress, opts) -> result: the HIP path of the reference's
@@ -214,4 +280,4 @@ async function runHIP(g, p, onProgress, opts) {
 }
 
 module.exports = { hipDevice, parseSTL, makeFrame, prepare, simulateHIP, gpuSelfTestHIP, runHIP, cellFlags,
-  native: A };
+  loadSurrogate, SURROGATE_WEIGHTS, native: A };

@@ -38,7 +38,7 @@ napi_value throw_msg(napi_env env, const std::string& m) {
 }
 
 // ---- handles -----------------------------------------------------------------
-enum Kind { K_DEV = 1, K_GEOM = 2, K_SIM = 3 };
+enum Kind { K_DEV = 1, K_GEOM = 2, K_SIM = 3, K_NET = 4 };
 // busy: a simChunk of this simulation handle is running on a pool thread.
 // While it runs, synchronous calls on the handle throw and destroy() is
 // deferred to the chunk's completion (the handle's stream and state belong to
@@ -55,6 +55,7 @@ void release(Handle* h) {
     if (h->kind == K_SIM) abx_sim_destroy((abx_sim*)h->p);
     if (h->kind == K_GEOM) abx_geom_free((abx_geom*)h->p);
     if (h->kind == K_DEV) abx_device_close((abx_dev*)h->p);
+    if (h->kind == K_NET) abx_surrogate_free((abx_surrogate*)h->p);
     h->p = nullptr;
   }
 }
@@ -341,14 +342,8 @@ napi_value geom_object(napi_env env, abx_geom* g) {
   return o;
 }
 
-napi_value Prepare(napi_env env, napi_callback_info info) {
-  std::vector<napi_value> a;
-  if (!get_args(env, info, 3, a)) return nullptr;
-  Handle* d = unwrap(env, a[0], K_DEV);
-  if (!d) return nullptr;
-  View T = view(env, a[1]);
-  if (!want(env, T, napi_float32_array, (size_t)-1, "prepare tris")) return nullptr;
-  napi_value p = a[2];
+// prepare's parameter object (ui.html:477-479) plus cy/sy and maxCells.
+abx_prep_params prep_params(napi_env env, napi_value p) {
   abx_prep_params P;
   memset(&P, 0, sizeof P);
   P.dx = num(env, prop(env, p, "dx"));
@@ -388,6 +383,17 @@ napi_value Prepare(napi_env env, napi_callback_info info) {
       P.out_height = num(env, prop(env, out, "height"));
     }
   }
+  return P;
+}
+
+napi_value Prepare(napi_env env, napi_callback_info info) {
+  std::vector<napi_value> a;
+  if (!get_args(env, info, 3, a)) return nullptr;
+  Handle* d = unwrap(env, a[0], K_DEV);
+  if (!d) return nullptr;
+  View T = view(env, a[1]);
+  if (!want(env, T, napi_float32_array, (size_t)-1, "prepare tris")) return nullptr;
+  const abx_prep_params P = prep_params(env, a[2]);
   abx_geom* g = nullptr;
   if (abx_prepare((abx_dev*)d->p, (const float*)T.data, (int64_t)T.length, &P, &g)) return throw_lib(env);
   return geom_object(env, g);
@@ -474,10 +480,10 @@ napi_value SimSetLaunch(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 2, a)) return nullptr;
   abx_sim* s = sim_arg(env, a[0]);
   if (!s) return nullptr;
-  int kernel, xm = -1, nt = 0;
+  int kernel, xm = -1, nt = 19;
   if (!int_arg(env, a[1], 0, 16, "simSetLaunch kernel", &kernel)) return nullptr;
   if (a.size() > 2 && !int_arg(env, a[2], -1, 4096, "simSetLaunch xcdMap", &xm)) return nullptr;
-  if (a.size() > 3 && !int_arg(env, a[3], 0, 15, "simSetLaunch nt", &nt)) return nullptr;
+  if (a.size() > 3 && !int_arg(env, a[3], 0, 65535, "simSetLaunch nt", &nt)) return nullptr;
   if (abx_sim_set_launch(s, kernel, xm, nt)) return throw_lib(env);
   return nullptr;
 }
@@ -665,6 +671,20 @@ napi_value SimInfo(napi_env env, napi_callback_info info) {
   return sim_info(env, s);
 }
 
+napi_value metrics_object(napi_env env, const abx_metrics& m) {
+  napi_value met = obj(env);
+  setd(env, met, "gamma", m.gamma);
+  setd(env, met, "gammaCore", m.gammaCore);
+  setd(env, met, "lowCore", m.lowCore);
+  setd(env, met, "highCore", m.highCore);
+  setd(env, met, "reverse", m.reverse);
+  setd(env, met, "maxToMean", m.maxToMean);
+  setd(env, met, "zeta", m.zeta);
+  setd(env, met, "inlet_cm2", m.inlet_cm2);
+  setd(env, met, "filter_cm2", m.filter_cm2);
+  return met;
+}
+
 napi_value Evaluate(napi_env env, napi_callback_info info) {
   std::vector<napi_value> a;
   if (!get_args(env, info, 4, a)) return nullptr;
@@ -692,17 +712,7 @@ napi_value Evaluate(napi_env env, napi_callback_info info) {
   }
   abx_metrics m;
   if (abx_evaluate(s, &eg, b, div, &m, fmap, pI, pL, lon, spd, pst, loss)) return throw_lib(env);
-  napi_value met = obj(env);
-  setd(env, met, "gamma", m.gamma);
-  setd(env, met, "gammaCore", m.gammaCore);
-  setd(env, met, "lowCore", m.lowCore);
-  setd(env, met, "highCore", m.highCore);
-  setd(env, met, "reverse", m.reverse);
-  setd(env, met, "maxToMean", m.maxToMean);
-  setd(env, met, "zeta", m.zeta);
-  setd(env, met, "inlet_cm2", m.inlet_cm2);
-  setd(env, met, "filter_cm2", m.filter_cm2);
-  set(env, o, "metrics", met);
+  set(env, o, "metrics", metrics_object(env, m));
   seti(env, o, "ic", m.ic);
   seti(env, o, "jc", m.jc);
   seti(env, o, "jlow", m.jlow);
@@ -742,6 +752,174 @@ napi_value Streamlines(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// surrogateLoad(dev, path) -> {net, dims, familyDx}: abx_surrogate_load.
+napi_value SurrogateLoad(napi_env env, napi_callback_info info) {
+  std::vector<napi_value> a;
+  if (!get_args(env, info, 2, a)) return nullptr;
+  Handle* d = unwrap(env, a[0], K_DEV);
+  if (!d) return nullptr;
+  size_t len = 0;
+  if (napi_get_value_string_utf8(env, a[1], nullptr, 0, &len) != napi_ok) return throw_msg(env, "surrogateLoad: path");
+  std::string path(len, '\0');
+  napi_get_value_string_utf8(env, a[1], &path[0], len + 1, &len);
+  abx_surrogate* n = nullptr;
+  if (abx_surrogate_load((abx_dev*)d->p, path.c_str(), &n)) return throw_lib(env);
+  int dims[3];
+  double fdx = 0;
+  abx_surrogate_info(n, dims, &fdx);
+  napi_value o = obj(env), da;
+  set(env, o, "net", wrap(env, K_NET, n));
+  napi_create_array_with_length(env, 3, &da);
+  for (uint32_t k = 0; k < 3; ++k) {
+    napi_value e;
+    napi_create_int32(env, dims[k], &e);
+    napi_set_element(env, da, k, e);
+  }
+  set(env, o, "dims", da);
+  setd(env, o, "familyDx", fdx);
+  return o;
+}
+
+// surrogateWarmStart(net, sim, geom, tris, prepParams, K, aniso):
+// abx_surrogate_warm_start (synchronous; a few ms).
+napi_value SurrogateWarmStart(napi_env env, napi_callback_info info) {
+  std::vector<napi_value> a;
+  if (!get_args(env, info, 7, a)) return nullptr;
+  Handle* n = unwrap(env, a[0], K_NET);
+  if (!n) return nullptr;
+  abx_sim* s = sim_arg(env, a[1]);
+  if (!s) return nullptr;
+  Handle* g = unwrap(env, a[2], K_GEOM);
+  if (!g) return nullptr;
+  const float* tris = nullptr;
+  int64_t nf = 0;
+  abx_prep_params P;
+  const abx_prep_params* pp = nullptr;
+  if (!is_nullish(env, a[3])) {
+    View T = view(env, a[3]);
+    if (!want(env, T, napi_float32_array, (size_t)-1, "surrogateWarmStart tris")) return nullptr;
+    tris = (const float*)T.data;
+    nf = (int64_t)T.length;
+    P = prep_params(env, a[4]);
+    pp = &P;
+  }
+  if (abx_surrogate_warm_start((abx_surrogate*)n->p, s, (abx_geom*)g->p, tris, nf, pp, num(env, a[5]),
+                               num(env, a[6])))
+    return throw_lib(env);
+  return nullptr;
+}
+
+// solveToSteady(sim, geom, {window, maxSteps, step0, tolGamma, tolZeta},
+// onWindow) -> Promise<{converged, steps, windows, metrics}>: the steady
+// criterion (abx_sim_solve_to_steady) on a pool thread; onWindow gets
+// {step, gamma, massRatio, metrics} of every window on the JS thread.
+struct SteadyWork {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref simref, geomref;
+  Handle *hs, *hg;
+  abx_sim* sim;
+  abx_eval_geom eg;
+  abx_steady_params p;
+  napi_threadsafe_function tsfn = nullptr;
+  int rc = 0;
+  std::string err;
+  abx_steady_result res;
+};
+
+int steady_window_cb(void* user, const abx_steady_window* w) {
+  SteadyWork* sw = (SteadyWork*)user;
+  if (sw->tsfn) napi_call_threadsafe_function(sw->tsfn, new abx_steady_window(*w), napi_tsfn_blocking);
+  return 0;
+}
+
+void steady_call_js(napi_env env, napi_value cb, void*, void* data) {
+  abx_steady_window* w = (abx_steady_window*)data;
+  if (env && cb) {
+    napi_value o = obj(env), u;
+    seti(env, o, "step", w->step);
+    setd(env, o, "gamma", w->gamma);
+    setd(env, o, "massRatio", w->massRatio);
+    set(env, o, "metrics", metrics_object(env, w->metrics));
+    napi_get_undefined(env, &u);
+    napi_call_function(env, u, cb, 1, &o, nullptr);
+  }
+  delete w;
+}
+
+void steady_exec(napi_env, void* data) {
+  SteadyWork* w = (SteadyWork*)data;
+  w->rc = abx_sim_solve_to_steady(w->sim, &w->eg, &w->p, steady_window_cb, w, &w->res);
+  if (w->rc) w->err = abx_last_error();
+}
+
+void steady_done(napi_env env, napi_status, void* data) {
+  SteadyWork* w = (SteadyWork*)data;
+  if (w->tsfn) napi_release_threadsafe_function(w->tsfn, napi_tsfn_release);
+  if (w->rc) {
+    napi_value msg, err;
+    napi_create_string_utf8(env, w->err.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, w->deferred, err);
+  } else {
+    napi_value o = obj(env), b;
+    napi_get_boolean(env, w->res.converged != 0, &b);
+    set(env, o, "converged", b);
+    seti(env, o, "steps", w->res.steps);
+    seti(env, o, "windows", w->res.windows);
+    set(env, o, "metrics", metrics_object(env, w->res.metrics));
+    napi_resolve_deferred(env, w->deferred, o);
+  }
+  if (--w->hs->busy == 0 && w->hs->destroy_pending) release(w->hs);
+  if (--w->hg->busy == 0 && w->hg->destroy_pending) release(w->hg);
+  napi_delete_reference(env, w->simref);
+  napi_delete_reference(env, w->geomref);
+  napi_delete_async_work(env, w->work);
+  delete w;
+}
+
+napi_value SolveToSteady(napi_env env, napi_callback_info info) {
+  std::vector<napi_value> a;
+  if (!get_args(env, info, 3, a)) return nullptr;
+  abx_sim* s = sim_arg(env, a[0]);
+  if (!s) return nullptr;
+  Handle* hg = unwrap(env, a[1], K_GEOM);
+  if (!hg) return nullptr;
+  abx_steady_params P;
+  if (!int_arg(env, prop(env, a[2], "window"), 1, 1 << 30, "solveToSteady window", &P.window) ||
+      !int_arg(env, prop(env, a[2], "maxSteps"), 1, INT32_MAX, "solveToSteady maxSteps", &P.max_steps))
+    return nullptr;
+  P.step0 = 0;
+  napi_value s0 = prop(env, a[2], "step0");
+  if (s0 && !int_arg(env, s0, 0, INT32_MAX, "solveToSteady step0", &P.step0)) return nullptr;
+  P.tol_gamma = num(env, prop(env, a[2], "tolGamma"), 0.005);
+  P.tol_zeta = num(env, prop(env, a[2], "tolZeta"), 0.02);
+  SteadyWork* w = new SteadyWork();
+  w->hs = unwrap(env, a[0], K_SIM);
+  w->hg = hg;
+  w->sim = s;
+  w->p = P;
+  abx_geom_eval_view((abx_geom*)hg->p, &w->eg);
+  napi_value name, promise;
+  napi_create_string_utf8(env, "abx_solve_to_steady", NAPI_AUTO_LENGTH, &name);
+  napi_valuetype t = napi_undefined;
+  if (a.size() > 3) napi_typeof(env, a[3], &t);
+  if (t == napi_function &&
+      napi_create_threadsafe_function(env, a[3], nullptr, name, 0, 1, nullptr, nullptr, nullptr, steady_call_js,
+                                      &w->tsfn) != napi_ok) {
+    delete w;
+    return throw_msg(env, "solveToSteady: progress callback");
+  }
+  w->hs->busy++;
+  w->hg->busy++;
+  napi_create_promise(env, &w->deferred, &promise);
+  napi_create_reference(env, a[0], 1, &w->simref);  // keep the handles alive
+  napi_create_reference(env, a[1], 1, &w->geomref);
+  napi_create_async_work(env, nullptr, name, steady_exec, steady_done, w, &w->work);
+  napi_queue_async_work(env, w->work);
+  return promise;
+}
+
 napi_value Destroy(napi_env env, napi_callback_info info) {
   std::vector<napi_value> a;
   if (!get_args(env, info, 1, a)) return nullptr;
@@ -765,7 +943,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"simProbe", SimProbe},     {"simReadAcc", SimReadAcc}, {"simReadF", SimReadF},
              {"simWriteF", SimWriteF},   {"simInitEq", SimInitEq},   {"simSetNS", SimSetNS},
              {"simClearAcc", SimClearAcc}, {"evaluate", Evaluate},   {"streamlines", Streamlines},
-             {"simSetLaunch", SimSetLaunch}, {"destroy", Destroy}};
+             {"simSetLaunch", SimSetLaunch}, {"surrogateLoad", SurrogateLoad},
+             {"surrogateWarmStart", SurrogateWarmStart}, {"solveToSteady", SolveToSteady}, {"destroy", Destroy}};
   for (auto& f : fns) {
     napi_value fn;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);

@@ -49,7 +49,8 @@ def solve_to_steady(dev: abx.Device, g: abx.Geometry, p: dict, window: int = 500
                     tol_zeta: float = 0.02, max_steps: int = 60000, precision: str = "f32",
                     init_rho_u: Optional[np.ndarray] = None, keep_acc: bool = False,
                     init_fn: Optional[Callable[[abx.Sim], None]] = None) -> SteadyResult:
-    """init_rho_u: equilibrium start from (rho, u) in cid order; init_fn: any
+    """The criterion above, run by the library (abx_sim_solve_to_steady).
+    init_rho_u: equilibrium start from (rho, u) in cid order; init_fn: any
     start set on the fresh handle (its time is not part of wall_s)."""
     sim = abx.Sim(dev, g.grid(), p["K"], p["aniso"], p["u"], p["tau"], precision=precision)
     try:
@@ -57,30 +58,15 @@ def solve_to_steady(dev: abx.Device, g: abx.Geometry, p: dict, window: int = 500
             sim.init_eq(init_rho_u)
         if init_fn is not None:
             init_fn(sim)
-        eg = g.eval_view()
-        wins = []
-        prev = None
-        steps = 0
-        converged = False
         sim.sync()
         t0 = time.perf_counter()
-        while steps < max_steps:
-            sim.clear_acc()
-            sim.steps(window, True)
-            steps += window
-            sim.probe(steps)  # raises the reference's instability error on non-finite flow
-            m = sim.evaluate(eg, divisor=window, field3d=False)["metrics"]
-            wins.append((steps, m["gammaCore"], m["zeta"]))
-            if prev is not None and abs(m["gammaCore"] - prev["gammaCore"]) < tol_gamma and \
-                    abs(m["zeta"] - prev["zeta"]) <= tol_zeta * abs(prev["zeta"]):
-                converged = True
-                prev = {k: 0.5 * (v + prev[k]) if isinstance(v, float) and k in prev else v for k, v in m.items()}
-                break
-            prev = m
+        r = sim.solve_to_steady(g.eval_view(), window=window, tol_gamma=tol_gamma, tol_zeta=tol_zeta,
+                                max_steps=max_steps)
         sim.sync()
         wall = time.perf_counter() - t0
         acc = sim.read_acc(window) if keep_acc else None
-        return SteadyResult(converged, steps, wall, prev, wins, acc)
+        wins = [(w[0], w[3], w[4]) for w in r["windows"]]
+        return SteadyResult(r["converged"], r["steps"], wall, r["metrics"], wins, acc)
     finally:
         sim.destroy()
 

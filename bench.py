@@ -409,7 +409,7 @@ def main():
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
     if a.kernel is not None or a.xcd_map is not None or a.nt is not None:
         sim.set_launch(kernel=4 if a.kernel is None else a.kernel, xcd_map=-1 if a.xcd_map is None else a.xcd_map,
-                       nt=0 if a.nt is None else a.nt)
+                       nt=19 if a.nt is None else a.nt)
     NF = sim.NF
     I = sim.info
     log(f"sim NF={NF} device bytes {I.device_bytes / 2**30:.2f} GiB, lattice plane {I.plane} positions "

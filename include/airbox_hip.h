@@ -37,6 +37,7 @@ typedef struct abx_dev abx_dev;
 typedef struct abx_geom abx_geom;
 typedef struct abx_sim abx_sim;
 typedef struct abx_batch abx_batch;
+typedef struct abx_surrogate abx_surrogate;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* abx_last_error(void);
@@ -236,9 +237,15 @@ int abx_sim_sync(abx_sim* sim);
  * lattice order, round robin over the XCDs; 1 = each XCD sweeps one
  * contiguous part of the lattice; G > 1 = groups of G consecutive blocks per
  * XCD, all XCDs moving through the lattice together.
- * nt (AA pattern, float storage): non-temporal population accesses, bits
- * 0/1 = loads/stores of the even step, bits 2/3 = loads/stores of the odd
- * step.  Every choice computes the same values bit for bit. */
+ * nt (AA pattern, float storage): non-temporal accesses (streamed through
+ * the caches without keeping the lines), flags of the even step in bits 0-7
+ * and of the odd step in bits 8-15: 1 / 2 = population loads / stores at
+ * line-aligned offsets (all of the even step's, the odd step's with cz = 0),
+ * 4 / 8 = the odd step's other loads / stores, 16 = the averaged fields.
+ * Compiled sets: even 0, 3, 19; odd 0, 1, 2, 3, 8, 11, 16, 19.  Default 19
+ * (the even step's populations and averaged fields non-temporal, the odd
+ * step's accesses cached: its gathers reuse neighbouring rows' lines).  Every
+ * choice computes the same values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
 /* Row layout of simulations created afterwards (process-wide; no reference
  * counterpart): enable = 1 (the default) starts each lattice row at the
@@ -336,6 +343,37 @@ int abx_evaluate_metrics_noalloc(abx_sim* sim, const abx_eval_geom* g, int b, do
 int abx_streamlines(abx_sim* sim, const abx_eval_geom* g, int b, double divisor, int maxSeeds, float* pts,
                     int64_t pts_cap, float* spd, int64_t spd_cap, uint32_t* off, int64_t off_cap,
                     int64_t* npts, int64_t* nlines);
+
+/* ---- time to steady ----------------------------------------------------------
+ * No reference counterpart: the reference always runs a fixed number of steps
+ * averaging the last `avg` (engine.js:341-358, 613-630).  Runs variant 0 of
+ * sim in windows of `window` steps, each averaged on the GPU; after each
+ * window the probe runs (failing with the reference's instability message,
+ * engine.js:335) and evaluate's metrics of the window's average are taken
+ * (engine.js:632-691).  Steady: gamma-core moved by less than tol_gamma and
+ * zeta by at most tol_zeta * |zeta| between two consecutive windows; the
+ * result is then the mean of those two windows' metrics (section indices and
+ * counts of the last).  Stops unsteady after max_steps (windows that fit).
+ * step0 is added to the step numbers reported (steps already run).  cb, if
+ * not NULL, gets every window on the calling thread and stops the solve by
+ * returning nonzero.  On return the handle holds the last window's average in
+ * acc (divisor window) and its populations after out->steps more steps. */
+typedef struct {
+  int window, max_steps, step0;
+  double tol_gamma, tol_zeta;
+} abx_steady_params;
+typedef struct {
+  int step;                 /* step0 + steps run at the end of the window */
+  double gamma, massRatio;  /* the probe (progressInfo, engine.js:330-339) */
+  abx_metrics metrics;      /* evaluate of the window's average */
+} abx_steady_window;
+typedef struct {
+  int converged, steps, windows;
+  abx_metrics metrics;
+} abx_steady_result;
+typedef int (*abx_steady_cb)(void* user, const abx_steady_window* w);
+int abx_sim_solve_to_steady(abx_sim* sim, const abx_eval_geom* g, const abx_steady_params* p, abx_steady_cb cb,
+                            void* user, abx_steady_result* out);
 
 /* ---- Flow surrogate (3-D U-Net) kernels ---------------------------------
  * The reference has no surrogate: every solve starts from rest, f = w_q
@@ -453,6 +491,26 @@ int64_t abx_surrogate_features_workspace(int d, int h, int w);
 int abx_surrogate_features(void* stream, const uint8_t* fluid, const uint8_t* opening, const uint8_t* outlet,
                            const float* dc, int nx, int ny, int nz, int top, double pleat, double K, double aniso,
                            int d, int h, int w, int32_t* workspace, int64_t ws_bytes, uint16_t* x, int xs, int c);
+
+/* The trained network as a handle (no reference counterpart): reads an
+ * inference weight file (.abxw: "ABXUNET1", shape, cell size of the
+ * network's volume, target standardisation, FP32 weights; written by
+ * airboxcalc_amd/surrogate.py write_weights), packs the weights to bf16 and
+ * allocates one pass's activations on dev.  abx_surrogate_info gives the
+ * volume d x h x w and the cell size (mm) the network reads boxes at. */
+int abx_surrogate_load(abx_dev* dev, const char* path, abx_surrogate** out);
+int abx_surrogate_info(const abx_surrogate* net, int dims[3], double* family_dx);
+void abx_surrogate_free(abx_surrogate* net);
+/* Warm start (the reference starts from rest, engine.js:260-261): prepares
+ * the box at the network's cell size (prepare as abx_prepare from tris and p
+ * with dx replaced, the cap lifted; skipped when g_fine already has that cell
+ * size, and then tris and p may be NULL), builds the input volume
+ * (abx_surrogate_features, with K and aniso of the run), runs the network on
+ * the library's kernels and seeds every variant of sim, whose lattice is
+ * g_fine's, at the equilibrium of the prediction (abx_sim_init_eq_resample).
+ * Returns when sim is set.  Calls on one network handle must not overlap. */
+int abx_surrogate_warm_start(abx_surrogate* net, abx_sim* sim, const abx_geom* g_fine, const float* tris,
+                             int64_t nfloat, const abx_prep_params* p, double K, double aniso);
 
 /* One Adam step (bias-corrected, step >= 1) on n FP32 weights. */
 int abx_adam(void* stream, float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,

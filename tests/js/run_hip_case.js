@@ -7,7 +7,10 @@
 //   case: {stl, prepare, run, precision, selftest, nanCell, gateTolerance}
 //   nanCell: run with ns = NaN on that fluid cell (cid order) and report the
 //            error the run rejects with;
-//   gateTolerance: call runHIP with this gate and report its error, if any.
+//   gateTolerance: call runHIP with this gate and report its error, if any;
+//   warmStart: {weights, criterion}: seed a handle from the surrogate and
+//            report its populations' SHA-256, then run simulateHIP with
+//            {warmStart, criterion} and report its steady result.
 "use strict";
 const fs = require("fs");
 const crypto = require("crypto");
@@ -52,6 +55,27 @@ const hex64 = (x) => { const b = Buffer.alloc(8); b.writeDoubleLE(x, 0); return 
       out.simError = err.message;
     }
     out.progressSteps = prog;
+    process.stdout.write(JSON.stringify(out));
+    return;
+  }
+  if (C.warmStart) {
+    const net = H.loadSurrogate(dev, C.warmStart.weights);
+    out.net = { dims: net.dims, familyDx: net.familyDx };
+    const s = H.native.simCreate(dev.device, g.native, C.run, 0, 1, null);
+    H.native.surrogateWarmStart(net.net, s.sim, g.native, g.source.tris, g.source.params, C.run.K, C.run.aniso);
+    out.sha.warm_f = sha(H.native.simReadF(s.sim, 0));
+    H.native.destroy(s.sim);
+    const prog = [];
+    const res = await H.simulateHIP(dev, g, C.run, (info) => prog.push(info), null,
+      { warmStart: net, criterion: C.warmStart.criterion });
+    out.steady = res.steady;
+    out.metrics = res.metrics;
+    out.hex.metrics = {};
+    for (const k of Object.keys(res.metrics)) out.hex.metrics[k] = hex64(res.metrics[k]);
+    out.hist = res.hist;
+    out.progress = prog;
+    out.sha.vol_spd = sha(res.vol.spd);
+    H.native.destroy(net.net);
     process.stdout.write(JSON.stringify(out));
     return;
   }

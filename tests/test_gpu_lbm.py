@@ -182,8 +182,9 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     assert np.array_equal(a.read_f(), f)
     a.steps(10, doacc=True)
     want_f, want_acc = a.read_f(), a.read_acc()
-    for kern, xm, nt in ((4, 0, 0), (4, 1, 0), (4, 3, 0), (4, -1, 0), (4, -1, 3), (4, -1, 12), (4, 8, 5),
-                         (4, -1, 15), (2, 0, 0), (2, 5, 0), (2, -1, 0)):
+    for kern, xm, nt in ((4, 0, 3), (4, 1, 0), (4, 3, 3), (4, -1, 0), (4, -1, 3), (4, -1, 19 + 256 * 19),
+                         (4, 8, 3 + 256 * 11), (4, -1, 256 * 8), (4, -1, 3 + 256 * 1), (4, -1, 3 + 256 * 2),
+                         (4, -1, 3 + 256 * 16), (4, -1, 256 * 3), (2, 0, 0), (2, 5, 0), (2, -1, 0)):
         _, b = _sim(fx, precision)
         b.set_launch(kern, xm, nt)
         b.write_f(f)
@@ -197,7 +198,7 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     c.steps(3, doacc=True)
     c.set_launch(kernel=2)
     c.steps(4, doacc=True)
-    c.set_launch(kernel=4, nt=15)
+    c.set_launch(kernel=4, nt=19 + 256 * 11)
     c.steps(1, doacc=True)
     c.set_launch(kernel=2)
     c.steps(1, doacc=True)
