@@ -74,8 +74,7 @@ class SimInfo(C.Structure):
     _fields_ = [("NF", C.c_int64), ("Ain", C.c_int), ("Af", C.c_int), ("Lp", C.c_int), ("nM", C.c_int),
                 ("nT", C.c_int), ("u0", C.c_double), ("tau0", C.c_double), ("sig", C.c_double),
                 ("aniso", C.c_double), ("cs2s", C.c_double), ("precision", C.c_int), ("B", C.c_int),
-                ("device_bytes", C.c_int64), ("plane", C.c_int64), ("nyp", C.c_int), ("nzp", C.c_int),
-                ("kofs", C.c_int), ("nseg", C.c_int)]
+                ("device_bytes", C.c_int64), ("plane", C.c_int64), ("nrows", C.c_int), ("nseg", C.c_int)]
 
 
 class EvalGeom(C.Structure):
@@ -173,7 +172,6 @@ def load() -> C.CDLL:
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32]),
-        "abx_lattice_set_row_shift": (i32, [i32]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
         "abx_sim_reset": (i32, [vp]),

@@ -26,22 +26,14 @@ namespace abx {
 template <typename S, typename M, bool ACC, int MODE, int SPB, bool PERSIST>
 __global__ __launch_bounds__(64 * SPB) void k_batch_step(const StepArgs* __restrict__ args,
                                                          const int2* __restrict__ work, int nblocks) {
-  __shared__ uint32_t win[SPB][9][4];
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   for (int blk = blockIdx.x; blk < nblocks; blk += PERSIST ? gridDim.x : nblocks) {
-    const StepArgs a = args[work[blk * SPB].x];  // one member per workgroup
-    if (t < SPB * 36) {
-      const int ss = t / 36;
-      const int seg = work[blk * SPB + ss].y;
-      (&win[ss][0][0])[t - 36 * ss] = seg >= 0 ? seg_window_word(a, seg, t - 36 * ss) : 0u;
-    }
-    __syncthreads();
-    const int seg = work[blk * SPB + wv].y;
+    const int2 w = work[blk * SPB + wv];  // (member, segment); segment -1 = padding
+    const int m = __builtin_amdgcn_readfirstlane(w.x), seg = __builtin_amdgcn_readfirstlane(w.y);
     // non-temporal accesses as a single handle's default: the even AA step's
     // populations and averaged fields (float storage)
-    if (seg >= 0) seg_cell<S, M, ACC, false, MODE, (sizeof(S) == 4 && MODE == 1) ? 19 : 0>(a, win[wv], seg, l, 0);
+    if (seg >= 0) seg_cell<S, M, ACC, false, MODE, (sizeof(S) == 4 && MODE == 1) ? 19 : 0>(args[m], seg, l, 0);
     if (!PERSIST) break;
-    __syncthreads();
   }
 }
 
@@ -174,36 +166,25 @@ extern "C" int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_b
     bt->members.push_back(s);
     bt->cells += I.NF;
   }
-  // work lists.  Workgroup b runs on XCD b % 8.  Each member's segments are
-  // cut into eight bands of lattice rows j with equal segment counts; XCD x
-  // takes band x of every member, in member order, each band padded to whole
+  // work lists.  Workgroup b runs on XCD b % 8.  Each member's segments lie
+  // in eight bands of lattice rows j with equal cell counts; XCD x takes band
+  // x of every member, in member order, each band padded to whole
   // workgroups.  All XCDs move through the members together, and the odd
   // step's gathers from neighbouring rows stay in the XCD's own L2.
   const int spbs[3] = {2, 4, 8};
   std::vector<std::vector<int32_t>> segs((size_t)n);
-  std::vector<int> nyp((size_t)n), nzp((size_t)n);
+  std::vector<std::vector<int8_t>> bands((size_t)n);
   for (int m = 0; m < n; ++m) {
     segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
-    const SimView v = sim_view(bt->members[(size_t)m]);
-    nyp[(size_t)m] = v.nyp;
-    nzp[(size_t)m] = v.nzp;
+    bands[(size_t)m] = sim_segment_bands(bt->members[(size_t)m]);
   }
   for (int k = 0; k < 3; ++k) {
     std::vector<std::vector<int2>> q(8);
     for (int m = 0; m < n; ++m) {
       const std::vector<int32_t>& sg = segs[(size_t)m];
-      std::vector<int64_t> per_j((size_t)nyp[(size_t)m], 0);
-      for (int32_t d : sg) per_j[(size_t)((d / nzp[(size_t)m]) % nyp[(size_t)m])]++;
-      std::vector<int> band((size_t)nyp[(size_t)m], 0);
-      int64_t cum = 0;
-      const int64_t tot = std::max<int64_t>((int64_t)sg.size(), 1);
-      for (int j = 0; j < nyp[(size_t)m]; ++j) {
-        band[(size_t)j] = (int)std::min<int64_t>(7, cum * 8 / tot);
-        cum += per_j[(size_t)j];
-      }
       const size_t start[8] = {q[0].size(), q[1].size(), q[2].size(), q[3].size(),
                                q[4].size(), q[5].size(), q[6].size(), q[7].size()};
-      for (int32_t d : sg) q[(size_t)band[(size_t)((d / nzp[(size_t)m]) % nyp[(size_t)m])]].push_back(make_int2(m, d));
+      for (size_t i = 0; i < sg.size(); ++i) q[(size_t)bands[(size_t)m][i]].push_back(make_int2(m, sg[i]));
       for (int x = 0; x < 8; ++x)
         if (q[(size_t)x].size() > start[x])
           while (q[(size_t)x].size() % spbs[k]) q[(size_t)x].push_back(make_int2(m, -1));

@@ -25,17 +25,24 @@ struct EvalDev {
   const void* acc;  // [4][plane] of the variant
   const float* ns;  // [plane] of the variant or null
   const uint32_t* fluid_bits;
+  const int32_t* rowid;   // nx*ny: row of (i, j), -1 without fluid
+  const int4* rowinfo;    // per row {rowoff, kmin, kmax, i*ny + j}
   int64_t plane;
-  int nx, ny, nz, nyp, nzp, kofs;
+  int nx, ny, nz;
   double div;
 };
 
+// Lattice position of cell (i, j, k) of the grid, -1 where no position is
+// stored (a row without fluid, or k outside its row's span).
 __device__ __forceinline__ int64_t dpos(const EvalDev& e, int64_t i, int64_t j, int64_t k) {
-  return ((i + 1) * e.nyp + (j + 1)) * e.nzp + (k + e.kofs);
+  const int r = e.rowid[i * e.ny + j];
+  if (r < 0) return -1;
+  const int4 ri = e.rowinfo[r];
+  return k < ri.y || k > ri.z ? -1 : (int64_t)ri.x + k;
 }
 // fluid of evaluate: g.fluid minus cells with ns > 0.5 (engine.js:634)
 __device__ __forceinline__ bool efluid(const EvalDev& e, int64_t d) {
-  if (!((e.fluid_bits[d >> 5] >> (d & 31)) & 1u)) return false;
+  if (d < 0 || !((e.fluid_bits[d >> 5] >> (d & 31)) & 1u)) return false;
   return !(e.ns && e.ns[d] > 0.5f);
 }
 // averaged component value as the reference's acc holds it (acc / nacc)
@@ -390,9 +397,8 @@ EvalDev eval_dev(abx_sim* s, int b, double div) {
   e.nx = v.nx;
   e.ny = v.ny;
   e.nz = v.nz;
-  e.nyp = v.nyp;
-  e.nzp = v.nzp;
-  e.kofs = v.kofs;
+  e.rowid = v.rowid;
+  e.rowinfo = v.rowinfo;
   e.div = div;
   return e;
 }

@@ -3,23 +3,24 @@
 //
 // Lattice layout (HBM).  The reference stores the compacted fluid list with a
 // 19-entry neighbour table (engine.js:241-251), 76 B of indices per cell that
-// every step re-reads.  Here the lattice is dense: the nx*ny*nz grid with a
-// one-cell halo on every side and the k extent padded to a multiple of 32,
-//     D(i,j,k) = ((i+1)*nyp + (j+1))*nzp + (k+kofs),  nyp = ny+2,
-//     nzp = 32*ceil((nz+kofs+1)/32), kofs in [1, 32] (best_row_offset),
-// one plane of `plane` elements per population q (structure of arrays,
-// f[q*plane + D]).  Streaming pulls f_q(x - c_q) at the constant offset
-// off_q = (cx*nyp + cy)*nzp + cz; a neighbour that is not fluid (a wall, the
-// halo, or outside the box) turns into halfway bounce-back f_opp(q)(x) of the
-// previous step, which is exactly the reference's "nb = -1" case: prepare
-// keeps a solid shell between the fluid and the grid faces (engine.js:177-178,
-// 190-195), so its flat-index wrap-around never reaches a fluid cell.
+// every step re-reads.  Here every grid row (i, j) that holds fluid stores
+// the span of k from its first to its last fluid cell, and the rows follow
+// one another without gaps (in eight bands of rows j, then i, then j), so
+// nearly every stored position is a fluid cell and every 128-byte line of a
+// population plane is almost entirely fluid: cell (i, j, k) sits at position
+// rowoff(i, j) + k of each plane (structure of arrays, f[q*plane + d]).  The
+// neighbour x - c_q lies in row (i - cx, j - cy) at position
+// d + delta(row, -cx, -cy) - cz, with the nine deltas of each row in a table;
+// one meta word per position holds the neighbours' fluid bits and the row
+// (relative to its 64-position segment).  A neighbour that is not fluid (a
+// wall, or outside the grid) turns into halfway bounce-back f_opp(q)(x) of
+// the previous step, which is exactly the reference's "nb = -1" case.
 //
-// Per cell, the only geometry data are bits: a fluid bit (bit-packed, 32
-// cells per word, staged per workgroup in LDS for the 18 neighbour tests) and
-// three type bits (inlet / outlet / porous, engine.js:252-254).  A wavefront
-// owns 64 consecutive lattice positions (a segment); only segments that hold
-// fluid are launched (the segment list).  Algorithmic traffic per fluid cell
+// Per cell the even AA step reads only bits: a fluid bit (32 cells per word)
+// and three type bits (inlet / outlet / porous, engine.js:252-254); the odd
+// step also reads the cell's meta word and its row's deltas.  A wavefront
+// owns 64 consecutive positions (a segment); only segments that hold fluid
+// are launched (the segment list).  Algorithmic traffic per fluid cell
 // update: 19 loads + 19 stores of the storage type (152 B in float).
 #include <math.h>
 
@@ -54,9 +55,7 @@ namespace abx {
 // loads (bit 0) / stores (bit 1).
 template <typename S, typename M, bool ACC, bool NS, int MODE, int NT>
 __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
-  __shared__ uint32_t win[4][9][4];
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int b = blockIdx.y;
   const int nblk = (a.nchunks + 3) >> 2;
   // Block order across the 8 XCDs (workgroup x runs on XCD x % 8 as the
   // x-th of that XCD's workgroups, s = x / 8):
@@ -71,26 +70,20 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
     const int sq = (int)(blockIdx.x >> 3), G = a.xcd_map;
     blk = (sq / G) * 8 * G + (int)(blockIdx.x & 7) * G + sq % G;
   }
-  if (blk >= nblk) return;  // whole workgroup leaves together
-  if (t < 4 * 36) {
-    const int ss = t / 36, sg = 4 * blk + ss;
-    (&win[ss][0][0])[t - 36 * ss] = sg < a.nchunks ? seg_window_word(a, a.chunks[sg], t - 36 * ss) : 0u;
-  }
-  __syncthreads();
+  if (blk >= nblk) return;
   const int sg = 4 * blk + wv;
-  if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE, NT>(a, win[wv], a.chunks[sg], l, b);
+  if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE, NT>(a, a.chunks[sg], l, blockIdx.y);
 }
 
 template <typename S, typename M>
-__global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy, const uint32_t* fb, int form,
-                        int nyp, int nzp) {
+__global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy, LatView L, int form) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= np) return;
   const int d = plist[n];
   M r = 0, my = 0;
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
-    const M v = (M)post_f(f, P, fb, d, q, form, nyp, nzp);
+    const M v = (M)post_f(f, P, L, d, q, form);
     r += v;
     my += (M)CY[q] * v;
   }
@@ -147,13 +140,12 @@ __global__ __launch_bounds__(1024) void k_gamma(const double* vy, int nM, int nT
 // out[q*NF + a] = post-collision f_q of fluid cell a (the reference's gather
 // kernel, engine.js:468-476, over whole components), one thread per cell.
 template <typename S>
-__global__ void k_gather_f(const S* F, int64_t P, const int32_t* cells, int64_t NF, S* out, const uint32_t* fb,
-                           int form, int nyp, int nzp) {
+__global__ void k_gather_f(const S* F, int64_t P, const int32_t* cells, int64_t NF, S* out, LatView L, int form) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= NF) return;
   const int d = cells[a];
 #pragma unroll
-  for (int q = 0; q < 19; ++q) out[q * NF + a] = post_f(F, P, fb, d, q, form, nyp, nzp);
+  for (int q = 0; q < 19; ++q) out[q * NF + a] = post_f(F, P, L, d, q, form);
 }
 
 // acc / divisor in double, stored in T (engine.js:628: Float32Array /= nacc).
@@ -167,19 +159,17 @@ __global__ void k_gather_div(const S* src, int64_t P, const int32_t* cells, int6
 
 // Store post-collision populations given in cid order in the stored form.
 template <typename S>
-__global__ void k_scatter_f(const S* in, int64_t P, const int32_t* cells, int64_t NF, S* F, const uint32_t* fb,
-                            int form, int nyp, int nzp) {
+__global__ void k_scatter_f(const S* in, int64_t P, const int32_t* cells, int64_t NF, S* F, LatView L, int form) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= NF) return;
   const int d = cells[a];
 #pragma unroll
-  for (int q = 0; q < 19; ++q) F[post_slot(P, fb, d, q, form, nyp, nzp)] = in[q * NF + a];
+  for (int q = 0; q < 19; ++q) F[post_slot(P, L, d, q, form)] = in[q * NF + a];
 }
 
 // f_q = feq_q(rho, u) at the fluid cells (feq as engine.js:296).
 template <typename S, typename M>
-__global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, int64_t P, S* f, const uint32_t* fb,
-                          int form, int nyp, int nzp) {
+__global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, int64_t P, S* f, LatView L, int form) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= NF) return;
   const M rho = (M)rho_u[a], ux = (M)rho_u[NF + a], uy = (M)rho_u[2 * NF + a], uz = (M)rho_u[3 * NF + a];
@@ -188,7 +178,7 @@ __global__ void k_init_eq(const float* rho_u, const int32_t* cells, int64_t NF, 
 #pragma unroll
   for (int q = 0; q < 19; ++q) {
     const M cu = (M)3 * ((M)CX[q] * ux + (M)CY[q] * uy + (M)CZ[q] * uz);
-    f[post_slot(P, fb, d, q, form, nyp, nzp)] = (S)((M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq));
+    f[post_slot(P, L, d, q, form)] = (S)((M)wt(q) * rho * ((M)1 + cu + (M)0.5 * cu * cu - usq));
   }
 }
 
@@ -206,12 +196,13 @@ struct ResampleArgs {
   float a[4], b[4];
 };
 
-__global__ void k_resample_rho_u(ResampleArgs r, const int32_t* cells, int64_t NF, int nyp, int nzp, int kofs,
-                                 float* rho_u) {
+__global__ void k_resample_rho_u(ResampleArgs r, const int32_t* cells, int64_t NF, LatView L, const int4* rowinfo,
+                                 int ny, float* rho_u) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= NF) return;
   const int d = cells[e];
-  const int k = d % nzp - kofs, t = d / nzp, j = t % nyp - 1, i = t / nyp - 1;
+  const int4 ri = rowinfo[lat_row(L, d, L.meta[d])];  // {rowoff, kmin, kmax, i ny + j}
+  const int k = d - ri.x, i = ri.w / ny, j = ri.w - i * ny;
   const int n3[3] = {r.nxc, r.nyc, r.nzc};
   const int idx[3] = {i, j, k};
   int i0[3];
@@ -272,24 +263,32 @@ struct abx_sim {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int prec = ABX_F32, B = 1, xcd_map = -1;  // default launch: one band of rows j per XCD
-  int nx = 0, ny = 0, nz = 0, nxp = 0, nyp = 0, nzp = 0;
-  // position of cell k within its row: k + kofs (kofs - 1 halo positions in front)
-  int kofs = 1;
-  int64_t ntot = 0, plane = 0, NF = 0;
+  int nx = 0, ny = 0, nz = 0;
+  int nrows = 0;
+  int64_t plane = 0, NF = 0;
   size_t ssz = 4;  // bytes per stored value
   void* f[2] = {nullptr, nullptr};
   int cur = 0;
   void* acc = nullptr;
   float* ns = nullptr;
-  uint32_t* fluid_alloc = nullptr;
-  const uint32_t* fluid_bits = nullptr;
+  uint32_t* fluid_bits = nullptr;
   uint32_t* type_bits = nullptr;
+  // the row-span layout (LatView): per-position meta words, the first row of
+  // every segment, per-row neighbour deltas; per-row {rowoff, kmin, kmax,
+  // i ny + j} and the row of every (i, j) (-1 without fluid)
+  uint32_t* meta = nullptr;
+  int32_t* seg_row0 = nullptr;
+  int32_t* rdelta = nullptr;
+  int4* rowinfo = nullptr;
+  int32_t* rowid = nullptr;
   int32_t* chunks64 = nullptr;    // active 64-position segments
   int nchunks64 = 0;
   std::vector<int32_t> seg_host;  // host copy of chunks64 (ascending positions)
+  std::vector<int8_t> seg_band;   // band of rows j of each active segment
+  int32_t seg_idle = 0;           // first position of a segment without fluid
   // xcd_map -1: the segments reordered so that XCD x (workgroups x, x + 8, ...)
-  // owns the x-th of eight bands of lattice rows j (equal segment counts) and
-  // sweeps it plane by plane; padded with position 0 (never fluid)
+  // owns the x-th of eight bands of lattice rows j (equal cell counts) and
+  // sweeps it plane by plane; padded with position 0 of a segment without fluid
   int32_t* chunks_slab = nullptr;
   int nslab = 0;
   // kernel 4: AA pattern in one lattice (the default); 2: two lattices, pull.
@@ -314,45 +313,6 @@ struct abx_sim {
 
 namespace {
 
-// Row layout of new simulations (abx_lattice_set_row_shift).
-int g_row_shift = 1;
-
-// Offset kofs in [1, 32] of cell k = 0 within its lattice row that makes the
-// fewest 128-byte lines (32 positions) hold fluid: the stream-collide reads and
-// writes every such line whole, so at the ragged ends of the fluid runs a
-// line-aligned run start saves a partly used line.  Ties go to the shorter row.
-int best_row_offset(const uint8_t* fluid, int nx, int ny, int nz) {
-  int64_t cnt[32] = {0};
-  for (int64_t r = 0; r < (int64_t)nx * ny; ++r) {
-    const uint8_t* f = fluid + r * nz;
-    int last[32];
-    for (int t = 0; t < 32; ++t) last[t] = -1;
-    int k = 0;
-    while (k < nz) {  // fluid runs [k0, k1] in increasing k
-      if (!f[k]) {
-        ++k;
-        continue;
-      }
-      const int k0 = k;
-      while (k < nz && f[k]) ++k;
-      const int k1 = k - 1;
-      for (int t = 0; t < 32; ++t) {
-        int lo = (k0 + 1 + t) >> 5;
-        const int hi = (k1 + 1 + t) >> 5;
-        if (lo == last[t]) ++lo;
-        if (hi >= lo) cnt[t] += hi - lo + 1;
-        last[t] = hi;
-      }
-    }
-  }
-  int best = 0;
-  for (int t = 1; t < 32; ++t) {
-    const int64_t rt = (nz + t + 2 + 31) / 32, rb = (nz + best + 2 + 31) / 32;
-    if (cnt[t] < cnt[best] || (cnt[t] == cnt[best] && rt < rb)) best = t;
-  }
-  return 1 + best;
-}
-
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T>
@@ -369,6 +329,14 @@ int dalloc(abx_sim* s, T** p, size_t count) {
 }
 
 int ensure_second_lattice(abx_sim* s);
+
+LatView lat_view(const abx_sim* s) {
+  LatView L;
+  L.meta = s->meta;
+  L.seg_row0 = s->seg_row0;
+  L.rdelta = s->rdelta;
+  return L;
+}
 
 StepArgs step_args(abx_sim* s, int doacc) {
   StepArgs a;
@@ -389,8 +357,7 @@ StepArgs step_args(abx_sim* s, int doacc) {
     a.xcd_map = 0;
   }
   a.per_xcd = (a.nchunks + 7) / 8;
-  a.nyp = s->nyp;
-  a.nzp = s->nzp;
+  a.lat = lat_view(s);
   a.plane = s->plane;
   a.u0 = s->u0;
   a.tau0 = s->tau0;
@@ -534,22 +501,14 @@ int fill_cold(abx_sim* s) {
 int ensure_slab_order(abx_sim* s) {
   if (s->chunks_slab || s->seg_host.empty()) return 0;
   const std::vector<int32_t>& sg = s->seg_host;
-  std::vector<int64_t> per_j((size_t)s->nyp + 1, 0);
-  for (int32_t d : sg) per_j[(size_t)((d / s->nzp) % s->nyp)]++;
-  // band of row j: cut the cumulative segment count into eight equal parts
-  std::vector<int> band((size_t)s->nyp, 0);
-  int64_t cum = 0;
-  const int64_t tot = (int64_t)sg.size();
-  for (int j = 0; j < s->nyp; ++j) {
-    band[(size_t)j] = (int)std::min<int64_t>(7, (cum * 8) / std::max<int64_t>(tot, 1));
-    cum += per_j[(size_t)j];
-  }
   std::vector<std::vector<int32_t>> lists(8);
-  for (int32_t d : sg) lists[(size_t)band[(size_t)((d / s->nzp) % s->nyp)]].push_back(d);
+  for (size_t n = 0; n < sg.size(); ++n) lists[(size_t)s->seg_band[n]].push_back(sg[n]);
   size_t maxb = 0;
   for (const auto& l : lists) maxb = std::max(maxb, (l.size() + 3) / 4);
-  // workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th workgroup
-  std::vector<int32_t> perm(maxb * 8 * 4, 0);
+  // workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th workgroup; empty
+  // places take a segment without fluid (its lanes leave at once)
+  const int32_t idle = s->seg_idle;
+  std::vector<int32_t> perm(maxb * 8 * 4, idle);
   for (size_t x = 0; x < 8; ++x)
     for (size_t q = 0; q < lists[x].size(); ++q) perm[((q / 4) * 8 + x) * 4 + q % 4] = lists[x][q];
   if (dalloc(s, &s->chunks_slab, perm.size())) return -1;
@@ -604,8 +563,9 @@ void sim_free(abx_sim* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   sim_drop_graphs(s);
-  void* ptrs[] = {s->f[0],   s->f[1], s->acc,   s->ns, s->fluid_alloc, s->type_bits,
-                  s->chunks64, s->cells, s->plist, s->vy, s->red,         s->chunks_slab};
+  void* ptrs[] = {s->f[0],  s->f[1],     s->acc,      s->ns,    s->fluid_bits, s->type_bits, s->meta,
+                  s->seg_row0, s->rdelta, s->rowinfo, s->rowid, s->chunks64,  s->cells,     s->plist,
+                  s->vy,     s->red,      s->chunks_slab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->red_host) (void)hipHostFree(s->red_host);
@@ -650,19 +610,92 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->nx = nx;
   s->ny = ny;
   s->nz = nz;
-  s->nxp = nx + 2;
-  s->nyp = ny + 2;
-  s->kofs = g_row_shift ? best_row_offset(g->fluid, nx, ny, nz) : 1;
-  s->nzp = ((nz + s->kofs + 1 + 31) / 32) * 32;
-  s->ntot = (int64_t)s->nxp * s->nyp * s->nzp;
-  s->plane = ((s->ntot + CHUNK - 1) / CHUNK) * CHUNK;
-  if ((int64_t)s->plane * 19 * B > ((int64_t)1 << 40) || s->plane + (int64_t)(s->nyp + 2) * s->nzp > INT32_MAX) {
+  // rows (i, j) that hold fluid, and the span [kmin, kmax] of each
+  const int64_t N = (int64_t)nx * ny * nz, NR = (int64_t)nx * ny;
+  std::vector<int> kmin((size_t)NR, -1), kmax((size_t)NR, -1);
+  std::vector<int64_t> per_j((size_t)ny, 0);
+  for (int64_t ij = 0; ij < NR; ++ij) {
+    const uint8_t* fr = g->fluid + ij * nz;
+    for (int k = 0; k < nz; ++k)
+      if (fr[k]) {
+        if (kmin[(size_t)ij] < 0) kmin[(size_t)ij] = k;
+        kmax[(size_t)ij] = k;
+        per_j[(size_t)(ij % ny)]++;
+      }
+  }
+  // eight bands of rows j with equal cell counts, one per XCD
+  std::vector<int8_t> band((size_t)ny, 0);
+  {
+    int64_t tot = 0, cum = 0;
+    for (int64_t v : per_j) tot += v;
+    tot = std::max<int64_t>(tot, 1);
+    for (int j = 0; j < ny; ++j) {
+      band[(size_t)j] = (int8_t)std::min<int64_t>(7, cum * 8 / tot);
+      cum += per_j[(size_t)j];
+    }
+  }
+  // rows stored in the order band, i, j without gaps; each band starts on a
+  // 64-position segment
+  std::vector<int32_t> rowid((size_t)NR, -1);
+  std::vector<int4> rows;  // {rowoff, kmin, kmax, i ny + j}: cell k of the row at position rowoff + k
+  std::vector<int8_t> row_band;
+  int64_t pos = 0;
+  for (int bnd = 0; bnd < 8; ++bnd) {
+    pos = (pos + 63) / 64 * 64;
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        const int64_t ij = (int64_t)i * ny + j;
+        if (band[(size_t)j] != bnd || kmin[(size_t)ij] < 0) continue;
+        rowid[(size_t)ij] = (int32_t)rows.size();
+        rows.push_back(make_int4((int)(pos - kmin[(size_t)ij]), kmin[(size_t)ij], kmax[(size_t)ij], (int)ij));
+        row_band.push_back((int8_t)bnd);
+        pos += kmax[(size_t)ij] - kmin[(size_t)ij] + 1;
+      }
+  }
+  // after the last band one segment without fluid (the band order's filler)
+  pos = (pos + 63) / 64 * 64;
+  s->seg_idle = (int32_t)std::min<int64_t>(pos, INT32_MAX);
+  s->plane = ((pos + 64 + CHUNK - 1) / CHUNK) * CHUNK;
+  if ((int64_t)s->plane * 19 * B > ((int64_t)1 << 40) || s->plane > INT32_MAX / 2 || NR > INT32_MAX) {
     set_error("abx_sim_create: grid too large for 32-bit lattice positions");
     delete s;
     return -1;
   }
-  const int64_t N = (int64_t)nx * ny * nz;
-  auto D = [&](int64_t i, int64_t j, int64_t k) { return ((i + 1) * s->nyp + (j + 1)) * s->nzp + (k + s->kofs); };
+  s->nrows = (int)rows.size();
+  const int64_t nseg = s->plane / 64;
+  std::vector<int32_t> seg_row0((size_t)nseg, 0);
+  {
+    std::vector<uint8_t> set((size_t)nseg, 0);
+    for (size_t r = 0; r < rows.size(); ++r)
+      for (int64_t sg = ((int64_t)rows[r].x + rows[r].y) >> 6; sg <= ((int64_t)rows[r].x + rows[r].z) >> 6; ++sg)
+        if (!set[(size_t)sg]) {
+          set[(size_t)sg] = 1;
+          seg_row0[(size_t)sg] = (int32_t)r;
+        }
+  }
+  // meta words: the row of every stored position relative to its segment's
+  // first row (at most 63: a segment holds at most 64 rows)
+  std::vector<uint32_t> meta((size_t)s->plane, 0u);
+  for (size_t r = 0; r < rows.size(); ++r)
+    for (int k = rows[r].y; k <= rows[r].z; ++k) {
+      const int64_t p = (int64_t)rows[r].x + k;
+      meta[(size_t)p] = (uint32_t)((int64_t)r - seg_row0[(size_t)(p >> 6)]) << 19;
+    }
+  // position deltas to the nine neighbour rows
+  std::vector<int32_t> rdelta(12 * rows.size(), 0);
+  for (size_t r = 0; r < rows.size(); ++r) {
+    const int i = rows[r].w / ny, j = rows[r].w % ny;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int ni = i + di, nj = j + dj;
+        if (ni < 0 || ni >= nx || nj < 0 || nj >= ny) continue;
+        const int32_t r2 = rowid[(size_t)((int64_t)ni * ny + nj)];
+        if (r2 >= 0) rdelta[12 * r + (size_t)((di + 1) * 3 + (dj + 1))] = rows[(size_t)r2].x - rows[r].x;
+      }
+  }
+  auto fluid_at = [&](int64_t i, int64_t j, int64_t k) {
+    return i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz && g->fluid[(i * ny + j) * nz + k];
+  };
 
   // setupSim (engine.js:239-264): compact order, flags, Lp, Ain, layers, sigma
   const float* dc = g->dc;
@@ -674,14 +707,12 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   std::vector<int32_t> lM, lT;
   const int64_t nwords = s->plane / 32;
   std::vector<uint32_t> typ((size_t)(3 * nwords), 0u);
-  const int64_t margin = ((int64_t)(s->nyp + 1) * s->nzp) / 32 + 16;
-  std::vector<uint32_t> fb((size_t)(nwords + 2 * margin), 0u);
-  uint32_t* fbits = fb.data() + margin;
+  std::vector<uint32_t> fbits((size_t)nwords, 0u);
   int Ain = 0;
   for (int64_t c = 0; c < N; ++c) {
     if (!g->fluid[c]) continue;
     const int64_t k = c % nz, ij = c / nz, j = ij % ny, i = ij / ny;
-    const int64_t d = D(i, j, k);
+    const int64_t d = (int64_t)rows[(size_t)rowid[(size_t)ij]].x + k;
     if ((int64_t)cells.size() >= INT32_MAX) {
       set_error("abx_sim_create: too many fluid cells");
       delete s;
@@ -689,7 +720,9 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     }
     const int32_t a = (int32_t)cells.size();
     cells.push_back((int32_t)d);
-    fbits[d >> 5] |= 1u << (d & 31);
+    fbits[(size_t)(d >> 5)] |= 1u << (d & 31);
+    for (int q = 0; q < 19; ++q)
+      if (fluid_at(i - CX[q], j - CY[q], k - CZ[q])) meta[(size_t)d] |= 1u << q;
     int t = -1;
     if (k == 0 && g->opening[ij]) {
       t = 0;
@@ -716,9 +749,13 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->sig = rp->K * 0.5 * s->u0 * s->u0 / (uface * std::max(Lp, 1));
   s->cs2s = 18 * 1.4142135623730951 * 0.17 * 0.17;
 
+  // the segments that hold fluid, and the band of each
   std::vector<int32_t> chunks64;
-  for (int64_t c0 = 0; c0 < s->plane; c0 += 64)
-    if (fbits[c0 >> 5] | fbits[(c0 >> 5) + 1]) chunks64.push_back((int32_t)c0);
+  for (int64_t sg = 0; sg < nseg; ++sg)
+    if (fbits[(size_t)(2 * sg)] | fbits[(size_t)(2 * sg + 1)]) {
+      chunks64.push_back((int32_t)(sg * 64));
+      s->seg_band.push_back(row_band[(size_t)seg_row0[(size_t)sg]]);
+    }
   s->nchunks64 = (int)chunks64.size();
   s->seg_host = chunks64;
   std::vector<int32_t> plist((size_t)(s->nM + s->nT));
@@ -740,8 +777,13 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     rc |= dalloc(s, (float**)&s->f[0], fcount);
     rc |= rc ? 0 : dalloc(s, (float**)&s->acc, (size_t)(s->plane * 4 * B));
   }
-  rc |= rc ? 0 : dalloc(s, &s->fluid_alloc, fb.size());
+  rc |= rc ? 0 : dalloc(s, &s->fluid_bits, fbits.size());
   rc |= rc ? 0 : dalloc(s, &s->type_bits, typ.size());
+  rc |= rc ? 0 : dalloc(s, &s->meta, meta.size());
+  rc |= rc ? 0 : dalloc(s, &s->seg_row0, seg_row0.size());
+  rc |= rc ? 0 : dalloc(s, &s->rdelta, std::max<size_t>(rdelta.size(), 12));
+  rc |= rc ? 0 : dalloc(s, &s->rowinfo, std::max<size_t>(rows.size(), 1));
+  rc |= rc ? 0 : dalloc(s, &s->rowid, rowid.size());
   rc |= rc ? 0 : dalloc(s, &s->chunks64, std::max<size_t>(chunks64.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->cells, std::max<size_t>(cells.size(), 1));
   rc |= rc ? 0 : dalloc(s, &s->plist, std::max<size_t>(plist.size(), 1));
@@ -756,11 +798,13 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     sim_free(s);
     return -1;
   }
-  s->fluid_bits = s->fluid_alloc + margin;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream) : hipSuccess;
   };
-  if (up(s->fluid_alloc, fb.data(), fb.size() * 4) != hipSuccess || up(s->type_bits, typ.data(), typ.size() * 4) ||
+  if (up(s->fluid_bits, fbits.data(), fbits.size() * 4) != hipSuccess ||
+      up(s->type_bits, typ.data(), typ.size() * 4) != hipSuccess || up(s->meta, meta.data(), meta.size() * 4) ||
+      up(s->seg_row0, seg_row0.data(), seg_row0.size() * 4) || up(s->rdelta, rdelta.data(), rdelta.size() * 4) ||
+      up(s->rowinfo, rows.data(), rows.size() * sizeof(int4)) || up(s->rowid, rowid.data(), rowid.size() * 4) ||
       up(s->chunks64, chunks64.data(), chunks64.size() * 4) || up(s->cells, cells.data(), cells.size() * 4) ||
       up(s->plist, plist.data(), plist.size() * 4)) {
     set_error("abx_sim_create: upload failed");
@@ -800,9 +844,7 @@ extern "C" int abx_sim_info_get(const abx_sim* s, abx_sim_info* o) {
   o->B = s->B;
   o->device_bytes = s->device_bytes;
   o->plane = s->plane;
-  o->nyp = s->nyp;
-  o->nzp = s->nzp;
-  o->kofs = s->kofs;
+  o->nrows = s->nrows;
   o->nseg = s->nchunks64;
   return 0;
 }
@@ -886,16 +928,13 @@ extern "C" int abx_sim_probe(abx_sim* s, int step, double out[3]) {
   if (np > 0) {
     if (s->prec == ABX_F32)
       hipLaunchKernelGGL((k_probe<float, float>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
-                         s->nzp);
+                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, lat_view(s), s->form);
     else if (s->prec == ABX_F32_F64)
       hipLaunchKernelGGL((k_probe<float, double>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
-                         s->nzp);
+                         (const float*)s->f[s->cur], s->plane, s->plist, np, s->vy, lat_view(s), s->form);
     else
       hipLaunchKernelGGL((k_probe<double, double>), dim3(nblk(np, 256)), dim3(256), 0, s->stream,
-                         (const double*)s->f[s->cur], s->plane, s->plist, np, s->vy, s->fluid_bits, s->form, s->nyp,
-                         s->nzp);
+                         (const double*)s->f[s->cur], s->plane, s->plist, np, s->vy, lat_view(s), s->form);
     hipLaunchKernelGGL(k_gamma, dim3(1), dim3(1024), 0, s->stream, s->vy, s->nM, s->nT, (double)s->Ain * s->u0,
                        s->red);
     ABX_HIP(hipGetLastError());
@@ -961,11 +1000,11 @@ extern "C" int abx_sim_read_f(abx_sim* s, int b, void* out) {
   if (s->prec == ABX_F64)
     hipLaunchKernelGGL(k_gather_f<double>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream,
                        (const double*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, s->cells, s->NF,
-                       (double*)dout, s->fluid_bits, s->form, s->nyp, s->nzp);
+                       (double*)dout, lat_view(s), s->form);
   else
     hipLaunchKernelGGL(k_gather_f<float>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream,
                        (const float*)s->f[s->cur] + (int64_t)b * 19 * s->plane, s->plane, s->cells, s->NF,
-                       (float*)dout, s->fluid_bits, s->form, s->nyp, s->nzp);
+                       (float*)dout, lat_view(s), s->form);
   ABX_HIP(hipGetLastError());
   ABX_HIP(hipMemcpyAsync(out, dout, (size_t)n * s->ssz, hipMemcpyDeviceToHost, s->stream));
   ABX_HIP(hipStreamSynchronize(s->stream));
@@ -988,11 +1027,11 @@ extern "C" int abx_sim_write_f(abx_sim* s, int b, const void* in) {
     if (s->prec == ABX_F64)
       hipLaunchKernelGGL(k_scatter_f<double>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream, (const double*)din,
                          s->plane, s->cells, s->NF, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane,
-                         s->fluid_bits, s->form, s->nyp, s->nzp);
+                         lat_view(s), s->form);
     else
       hipLaunchKernelGGL(k_scatter_f<float>, dim3(nblk(s->NF, 256)), dim3(256), 0, s->stream, (const float*)din,
                          s->plane, s->cells, s->NF, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane,
-                         s->fluid_bits, s->form, s->nyp, s->nzp);
+                         lat_view(s), s->form);
   }
   ABX_HIP(hipGetLastError());
   ABX_HIP(hipStreamSynchronize(s->stream));
@@ -1007,16 +1046,13 @@ int sim_init_eq_dev(abx_sim* s, const float* d) {
   for (int v = 0; v < s->B; ++v) {
     if (s->prec == ABX_F32)
       hipLaunchKernelGGL((k_init_eq<float, float>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells, NF,
-                         s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
-                         s->nyp, s->nzp);
+                         s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, lat_view(s), s->form);
     else if (s->prec == ABX_F32_F64)
       hipLaunchKernelGGL((k_init_eq<float, double>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells,
-                         NF, s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
-                         s->nyp, s->nzp);
+                         NF, s->plane, (float*)s->f[s->cur] + (int64_t)v * 19 * s->plane, lat_view(s), s->form);
     else
       hipLaunchKernelGGL((k_init_eq<double, double>), dim3(nblk(NF, 256)), dim3(256), 0, s->stream, d, s->cells,
-                         NF, s->plane, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane, s->fluid_bits, s->form,
-                         s->nyp, s->nzp);
+                         NF, s->plane, (double*)s->f[s->cur] + (int64_t)v * 19 * s->plane, lat_view(s), s->form);
   }
   ABX_HIP(hipGetLastError());
   return 0;
@@ -1075,8 +1111,8 @@ extern "C" int abx_sim_init_eq_resample(abx_sim* s, const uint16_t* vol, int cst
   }
   float* d = nullptr;
   ABX_HIP(hipMalloc((void**)&d, (size_t)(4 * NF) * sizeof(float)));
-  hipLaunchKernelGGL(k_resample_rho_u, dim3(nblk(NF, 256)), dim3(256), 0, s->stream, r, s->cells, NF, s->nyp, s->nzp,
-                     s->kofs, d);
+  hipLaunchKernelGGL(k_resample_rho_u, dim3(nblk(NF, 256)), dim3(256), 0, s->stream, r, s->cells, NF, lat_view(s),
+                     s->rowinfo, s->ny, d);
   const hipError_t le = hipGetLastError();
   int rc = 0;
   if (le != hipSuccess) {
@@ -1162,10 +1198,6 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
   return 0;
 }
 
-extern "C" int abx_lattice_set_row_shift(int enable) {
-  g_row_shift = enable ? 1 : 0;
-  return 0;
-}
 
 extern "C" void abx_sim_destroy(abx_sim* s) { sim_free(s); }
 
@@ -1184,6 +1216,7 @@ int sim_normalize(abx_sim* s) {
     if (abx_sim_write_f(s, b, host.data() + (size_t)b * n)) return -1;
   return 0;
 }
+std::vector<int8_t> sim_segment_bands(abx_sim* s) { return s->seg_band; }
 std::vector<int32_t> sim_segments(abx_sim* s) {
   std::vector<int32_t> v((size_t)s->nchunks64);
   if (s->nchunks64) (void)hipMemcpy(v.data(), s->chunks64, v.size() * 4, hipMemcpyDeviceToHost);
@@ -1197,9 +1230,8 @@ SimView sim_view(abx_sim* s) {
   v.nx = s->nx;
   v.ny = s->ny;
   v.nz = s->nz;
-  v.nyp = s->nyp;
-  v.nzp = s->nzp;
-  v.kofs = s->kofs;
+  v.rowid = s->rowid;
+  v.rowinfo = s->rowinfo;
   v.plane = s->plane;
   v.ssz = s->ssz;
   v.acc = s->acc;

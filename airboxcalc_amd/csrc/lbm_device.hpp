@@ -14,16 +14,39 @@ namespace abx {
 
 constexpr int CHUNK = 256;
 
+// The lattice of a simulation (row-span layout).  Every row (i, j) of the
+// grid that holds fluid stores the span of k from its first to its last fluid
+// cell, and the rows follow one another without gaps, so cell (i, j, k) sits
+// at position rowoff(i, j) + k of every population plane and nearly every
+// stored position is a fluid cell.  The neighbour x + c of a cell lies in row
+// (i + cx, j + cy) at position d + delta(row, cx, cy) + cz.
+//   meta[d]      bit q (q < 19): the cell x - c_q is fluid (0 at non-fluid
+//                positions); bits 19-25: the position's row minus the first
+//                row of its 64-position segment
+//   seg_row0[s]  first row overlapping positions 64 s .. 64 s + 63
+//   rdelta[12 r + (di + 1) * 3 + (dj + 1)]  position delta from row r to the
+//                row (i + di, j + dj) (0 where that row holds no fluid)
+struct LatView {
+  const uint32_t* meta;
+  const int32_t* seg_row0;
+  const int32_t* rdelta;
+};
+
+// Row of position d.
+__device__ __forceinline__ int lat_row(const LatView& L, int d, uint32_t meta_d) {
+  return L.seg_row0[d >> 6] + (int)((meta_d >> 19) & 127u);
+}
+
 struct StepArgs {
   const void* fin;
   void* fout;
   void* acc;
   const float* ns;
-  const uint32_t* fluid_bits;  // word w holds positions 32w..32w+31; readable from -margin
+  const uint32_t* fluid_bits;  // word w holds positions 32w..32w+31
   const uint32_t* type_bits;   // per 32 positions: inlet, outlet, porous words
-  const int32_t* chunks;       // first position of each active chunk
+  const int32_t* chunks;       // first position of each active 64-position segment
   int nchunks, per_xcd, xcd_map;
-  int nyp, nzp;
+  LatView lat;
   int64_t plane;
   double u0, tau0, sig, aniso, cs2s;
 };
@@ -294,26 +317,29 @@ __device__ __forceinline__ void collide_any(const M (&fi)[19], bool inlet, bool 
     collide<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
 }
 
-// Post-collision population q of the cell at d, whatever the stored form:
-// two lattices / AA after an even step ("swapped": F[opp q](x)) or AA after
-// an odd step ("streamed": F[q](x + c_q), or F[opp q](x) towards a wall).
+// Post-collision population q of the fluid cell at d, whatever the stored
+// form: two lattices / AA after an even step ("swapped": F[opp q](x)) or AA
+// after an odd step ("streamed": F[q](x + c_q), or F[opp q](x) towards a
+// wall; x + c_q = x - c_opp(q) is fluid iff meta bit opp(q) of x is set).
 template <typename S>
-__device__ __forceinline__ S post_f(const S* F, int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
-                                    int nzp) {
+__device__ __forceinline__ S post_f(const S* F, int64_t P, const LatView& L, int d, int q, int form) {
   if (form == 0) return F[q * P + d];
   if (form == 1) return F[OPP[q] * P + d];
-  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
-  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? F[q * P + dn] : F[OPP[q] * P + d];
+  const uint32_t m = L.meta[d];
+  if (!((m >> OPP[q]) & 1u)) return F[OPP[q] * P + d];
+  const int r = lat_row(L, d, m);
+  return F[q * P + d + L.rdelta[12 * r + (1 + CX[q]) * 3 + (1 + CY[q])] + CZ[q]];
 }
 
-// Slot that holds post-collision population q of the cell at d in a given
-// form (the inverse of post_f).
-__device__ __forceinline__ int64_t post_slot(int64_t P, const uint32_t* fb, int d, int q, int form, int nyp,
-                                             int nzp) {
+// Slot that holds post-collision population q of the fluid cell at d in a
+// given form (the inverse of post_f).
+__device__ __forceinline__ int64_t post_slot(int64_t P, const LatView& L, int d, int q, int form) {
   if (form == 0) return q * P + d;
   if (form == 1) return OPP[q] * P + d;
-  const int dn = d + (CX[q] * nyp + CY[q]) * nzp + CZ[q];
-  return ((fb[dn >> 5] >> (dn & 31)) & 1u) ? q * P + dn : OPP[q] * P + d;
+  const uint32_t m = L.meta[d];
+  if (!((m >> OPP[q]) & 1u)) return OPP[q] * P + d;
+  const int r = lat_row(L, d, m);
+  return q * P + d + L.rdelta[12 * r + (1 + CX[q]) * 3 + (1 + CY[q])] + CZ[q];
 }
 
 // f(integral_constant<int, q>) for q = 0 .. 18 in order, q a constant
@@ -328,66 +354,71 @@ __device__ __forceinline__ void for_each_q(F&& f) {
 }
 
 // One lane's cell of a 64-position segment starting at seg: the fused
-// stream-collide of the scheme MODE (see k_stream_collide_seg).  win holds
-// the segment's nine neighbour-row bit windows (positions seg-32 .. seg+96).
+// stream-collide of the scheme MODE (see k_stream_collide_seg).
 // NT selects non-temporal accesses (streamed through the caches without
-// keeping the lines): bit 0 / bit 1 = population loads / stores at
-// line-aligned offsets (every population of the even AA step, the cz = 0
-// ones of the odd step), bit 2 / bit 3 = the odd step's other loads /
-// stores, bit 4 = the averaged fields.
+// keeping the lines): bit 0 / bit 1 = population loads / stores at the
+// cell's own position (every population of the even AA step), bit 2 / bit 3
+// = the odd step's loads / stores at neighbour positions, bit 4 = the
+// averaged fields.
 template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0>
-__device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win)[9][4], int seg, int l, int b) {
-  const int p0 = l + 32;
-  // AA steps: a non-fluid position (solid or padding) whose 32-position half
-  // of the segment holds fluid writes its own 19 slots (slot opp q, as a
-  // fluid cell walled in on all sides): in the even step the values it reads
-  // from its own slots, in the odd step zeros, without reading.  No fluid
-  // cell ever reads or writes those slots and nothing else reads them, so
-  // their values do not matter; this only makes every memory line that holds
-  // fluid fully written (no partially written lines at the ragged ends of the
-  // fluid runs).  Lines without fluid stay untouched.  (Both choices were
+__device__ __forceinline__ void seg_cell(const StepArgs& a, int seg, int l, int b) {
+  const int d = seg + l;
+  // AA steps: a non-fluid position (a solid cell inside a row's span, or
+  // padding) whose 32-position half of the segment holds fluid writes its own
+  // 19 slots (slot opp q, as a fluid cell walled in on all sides): in the even
+  // step the values it reads from its own slots, in the odd step zeros,
+  // without reading.  No fluid cell ever reads or writes those slots and
+  // nothing else reads them, so their values do not matter; this only makes
+  // every memory line that holds fluid fully written.  (Both choices were
   // timed: the even step's unconditional aligned loads are cheaper than
   // masked ones, the odd step's masked gathers cheaper than full ones.)
-  const uint32_t own = win[4][p0 >> 5];
-  const bool fluid = (own >> (p0 & 31)) & 1u;
+  const uint32_t own = a.fluid_bits[d >> 5];
+  const bool fluid = (own >> (d & 31)) & 1u;
   if (!fluid && (MODE == 0 || own == 0u)) return;
   const int64_t P = a.plane;
   const S* fin = (const S*)a.fin + (int64_t)b * 19 * P;
   S* fout = (S*)a.fout + (int64_t)b * 19 * P;
-  const int d = seg + l;
   const uint32_t tb = (uint32_t)d & 31u;
   const uint32_t* tw = a.type_bits + 3 * (d >> 5);
   const bool inlet = (tw[0] >> tb) & 1u, outlet = (tw[1] >> tb) & 1u, porous = (tw[2] >> tb) & 1u;
   M fi[19], fo[19];
-  // the fluid bits at positions p0 - 1 .. p0 + 1 of the nine neighbour rows,
-  // read from LDS before any population load (two words per row), so no wait
-  // for an LDS read sits between the 19 loads
-  uint64_t wb[9];
-  const int w0 = (p0 - 1) >> 5, sh0 = p0 - 32 * w0;
-#pragma unroll
-  for (int wi = 0; wi < 9; ++wi) wb[wi] = (uint64_t)win[wi][w0] | (uint64_t)win[wi][w0 + 1] << 32;
+  // neighbour bits and the position deltas to the nine neighbour rows (the
+  // even AA step touches only the cell's own position)
   uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
+  int dl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if constexpr (MODE != 1) {
+    const uint32_t m = a.lat.meta[d];
+    nbm = m & 0x7ffffu;
+    const int4* rd = (const int4*)(a.lat.rdelta + 12 * lat_row(a.lat, d, m));
+    const int4 v0 = rd[0], v1 = rd[1];
+    dl[0] = v0.x;
+    dl[1] = v0.y;
+    dl[2] = v0.z;
+    dl[3] = v0.w;
+    dl[4] = v1.x;
+    dl[5] = v1.y;
+    dl[6] = v1.z;
+    dl[7] = v1.w;
+    dl[8] = rd[2].x;
+  }
   auto load = [&](int q, auto nt) {
-    const int wi = (CX[q] + 1) * 3 + (CY[q] + 1);
-    const bool nbf = fluid & (bool)((wb[wi] >> (sh0 - CZ[q])) & 1u);
-    nbm |= (uint32_t)nbf << q;
-    const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
+    const bool nbf = (nbm >> q) & 1u;
+    // x - c_q: row (i - cx, j - cy), k - cz
+    const int src_nb = d + dl[(1 - CX[q]) * 3 + (1 - CY[q])] - CZ[q];
     const S* src;
     if (MODE == 0)
-      src = nbf ? fin + q * P + (d - off) : fin + OPP[q] * P + d;
+      src = nbf ? fin + q * P + src_nb : fin + OPP[q] * P + d;
     else if (MODE == 1)
       src = fin + q * P + d;
     else
-      src = nbf ? fin + OPP[q] * P + (d - off) : fin + q * P + d;
+      src = nbf ? fin + OPP[q] * P + src_nb : fin + q * P + d;
     fi[q] = (M)ld<decltype(nt)::value>(src);
   };
-  // in the odd step only the populations with cz = 0 sit a whole number of
-  // 128-byte lines away
   if (MODE != 2 || fluid) {
     for_each_q([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      constexpr bool aligned = MODE != 2 || CZ[q] == 0;
-      load(q, std::integral_constant<bool, aligned ? (NT & 1) != 0 : (NT & 4) != 0>{});
+      constexpr bool here = MODE == 1 || (CX[q] == 0 && CY[q] == 0 && CZ[q] == 0);
+      load(q, std::integral_constant<bool, here ? (NT & 1) != 0 : (NT & 4) != 0>{});
     });
   } else {
 #pragma unroll
@@ -404,15 +435,16 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
     } else if (MODE == 1) {
       dst = fout + OPP[q] * P + d;
     } else {
-      const int off = (CX[q] * a.nyp + CY[q]) * a.nzp + CZ[q];
-      dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + (d + off) : fout + OPP[q] * P + d;
+      // x + c_q: row (i + cx, j + cy), k + cz; fluid iff x - c_opp(q) is
+      const int dst_nb = d + dl[(1 + CX[q]) * 3 + (1 + CY[q])] + CZ[q];
+      dst = ((nbm >> OPP[q]) & 1u) ? fout + q * P + dst_nb : fout + OPP[q] * P + d;
     }
     st<decltype(nt)::value>(dst, fluid ? (S)fo[q] : (S)fi[q]);
   };
   for_each_q([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    constexpr bool aligned = MODE != 2 || CZ[q] == 0;
-    store(q, std::integral_constant<bool, aligned ? (NT & 2) != 0 : (NT & 8) != 0>{});
+    constexpr bool here = MODE != 2 || (CX[q] == 0 && CY[q] == 0 && CZ[q] == 0);
+    store(q, std::integral_constant<bool, here ? (NT & 2) != 0 : (NT & 8) != 0>{});
   });
   if (ACC) {
     constexpr bool NTC = (NT & 16) != 0;
@@ -430,15 +462,6 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, const uint32_t (&win
     st<NTC>(ac + 2 * P + d, a2);
     st<NTC>(ac + 3 * P + d, a3);
   }
-}
-
-// Stage the neighbour-row bit windows of segment seg (64 positions) for
-// window word u (0..35) of the segment.
-__device__ __forceinline__ uint32_t seg_window_word(const StepArgs& a, int seg, int u) {
-  const int wi = u >> 2, wo = u & 3;
-  const int di = wi / 3 - 1, dj = wi - (wi / 3) * 3 - 1;
-  const int64_t base = (int64_t)seg - ((int64_t)di * a.nyp + dj) * a.nzp - 32;
-  return a.fluid_bits[(base >> 5) + wo];
 }
 
 }  // namespace abx

@@ -11,8 +11,9 @@ struct SimView {
   int ordinal;
   hipStream_t stream;
   int B;
-  int nx, ny, nz, nyp, nzp;
-  int kofs;                // position of cell k within its row: k + kofs
+  int nx, ny, nz;
+  const int32_t* rowid;    // nx*ny: row of (i, j), -1 without fluid
+  const int4* rowinfo;     // per row {rowoff, kmin, kmax, i*ny + j}: cell k at rowoff + k
   int64_t plane;
   size_t ssz;              // bytes per stored value (4 or 8)
   const void* acc;         // [B][4][plane]
@@ -32,7 +33,9 @@ int sim_form(abx_sim* s);
 void sim_set_form(abx_sim* s, int form);
 // Bring an AA-pattern handle to the swapped form (host round trip if needed).
 int sim_normalize(abx_sim* s);
-// Host copy of the handle's active 64-position segments.
+// Host copy of the handle's active 64-position segments, and the band of
+// rows j (0..7) of each.
 std::vector<int32_t> sim_segments(abx_sim* s);
+std::vector<int8_t> sim_segment_bands(abx_sim* s);
 
 }  // namespace abx

@@ -413,7 +413,7 @@ def main():
     NF = sim.NF
     I = sim.info
     log(f"sim NF={NF} device bytes {I.device_bytes / 2**30:.2f} GiB, lattice plane {I.plane} positions "
-        f"(rows of {I.nzp}, cell k=0 at {I.kofs}), {I.nseg} active segments")
+        f"({I.nrows} rows), {I.nseg} active segments")
 
     # Spin-up (untimed): the flow develops from rest until it has crossed the
     # filter layer, so that the probes of the timed steps read a real flow.
@@ -493,7 +493,8 @@ def main():
                                f"{steps} steps averaging {avg}, probe every {every}, after {a.spinup} untimed "
                                f"spin-up steps from rest",
                    "grid": [g.nx, g.ny, g.nz], "fluid_cells": NF, "triangles": int(tris.size // 9),
-                   "lattice_positions": int(I.plane), "active_segments": int(I.nseg)},
+                   "lattice_positions": int(I.plane), "lattice_rows": int(I.nrows),
+                   "active_segments": int(I.nseg)},
         "value_definition": "fluid cells x steps / time from the start of the loop to its last probe (engine.js:338)",
         "readback_ms": round(t_read * 1e3, 2),
         "wall_s_total": round(t_solve + t_read, 4),

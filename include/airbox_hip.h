@@ -162,10 +162,11 @@ typedef struct {
   int precision, B;
   int64_t device_bytes;
   /* the device lattice (no reference counterpart): positions per population
-   * plane, padded row and column extents, position of cell k = 0 within its
-   * row, and the number of 64-position segments that hold fluid */
+   * plane (the grid rows that hold fluid, each stored from its first to its
+   * last fluid cell, one after another), the number of such rows, and the
+   * number of 64-position segments that hold fluid */
   int64_t plane;
-  int nyp, nzp, kofs, nseg;
+  int nrows, nseg;
 } abx_sim_info;
 
 /* Replaces setupSim + new GpuSim(dev, S, B, ns) (engine.js:239-264,
@@ -247,11 +248,6 @@ int abx_sim_sync(abx_sim* sim);
  * step's accesses cached: its gathers reuse neighbouring rows' lines).  Every
  * choice computes the same values bit for bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
-/* Row layout of simulations created afterwards (process-wide; no reference
- * counterpart): enable = 1 (the default) starts each lattice row at the
- * offset that makes the fewest 128-byte lines hold fluid, 0 puts cell k = 0
- * right after a one-position halo.  Results are the same bit for bit. */
-int abx_lattice_set_row_shift(int enable);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
 int abx_sim_set_graph(abx_sim* sim, int enable);

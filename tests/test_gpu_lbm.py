@@ -208,40 +208,78 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     assert np.array_equal(c.read_acc(), want_acc)
 
 
-@pytest.mark.parametrize("name", ["small_yaw", "c1_seed3"])
-def test_row_offset_layout_gives_the_same_results(name):
-    """Lattice rows placed at the line-saving offset (the default) and right
-    after a one-position halo: the same populations, averages, probe and
-    evaluate/field3d/streamlines results, bit for bit."""
-    fx = GU.load(name)
-    say("row offset layout", name)
-    g = gpu_geometry(fx)
-    r = fx["desc"]["run"]
-    lib = abx.load()
-    out = []
-    try:
-        for shift in (1, 0):
-            assert lib.abx_lattice_set_row_shift(shift) == 0
-            s = abx.Sim(dev(), g.grid(), r["K"], r["aniso"], r["u"], r["tau"], precision="f32")
-            rng = np.random.default_rng(12)
-            f = (np.repeat([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, s.NF) *
-                 rng.uniform(0.99, 1.01, 19 * s.NF)).astype(np.float32)
-            s.write_f(f)
-            s.steps(5)
-            s.steps(6, doacc=True)
-            ev = s.evaluate(g.eval_view(), divisor=6.0)
-            sl = s.streamlines(g.eval_view(), divisor=6.0)
-            out.append((s.read_f(), s.read_acc(), s.probe(11), ev, sl))
-            s.destroy()
-    finally:
-        lib.abx_lattice_set_row_shift(1)
-    (f1, a1, p1, e1, s1), (f0, a0, p0, e0, s0) = out
-    assert np.array_equal(f1, f0) and np.array_equal(a1, a0) and p1 == p0
-    assert e1["metrics"] == e0["metrics"]
-    for k in e1["vol"]:
-        assert np.array_equal(e1["vol"][k], e0["vol"][k], equal_nan=True), k
-    for k in s1:
-        assert np.array_equal(np.asarray(s1[k]), np.asarray(s0[k])), k
+def _ragged_grid(seed, nx, ny, nz):
+    """A grid of random fluid inside a solid shell: rows with several fluid
+    runs (solid cells inside a row's span), rows of one cell, rows without
+    fluid, many rows per 64-position segment when nz is small; inlet cells at
+    k = 0 under an opening, outlet cells, a porous layer."""
+    rng = np.random.default_rng(seed)
+    fl = np.zeros((nx, ny, nz), np.uint8)
+    fl[1:-1, 1:-1, 1:-1] = rng.random((nx - 2, ny - 2, nz - 2)) < 0.7
+    fl[3, :, :] = 0                                   # a plane without fluid
+    fl[5, 4, :] = 0
+    fl[5, 4, nz // 2] = 1                             # a row of one cell
+    opening = np.zeros((nx, ny), np.uint8)
+    opening[2:nx - 2, 2:ny - 2] = rng.random((nx - 4, ny - 4)) < 0.5
+    fl[:, :, 0] = opening                             # inlet cells
+    outlet = np.zeros_like(fl)
+    outlet[:, ny - 2, 1:-1] = fl[:, ny - 2, 1:-1]
+    dc = np.arange(ny, dtype=np.float32) * 2.0
+    top, jm, pleat = 1, 4, 9.0
+    return fl, opening, outlet, dc, top, jm, pleat
+
+
+def _oracle_sim_from_arrays(fl, opening, outlet, dc, top, jm, pleat, K, aniso, u, tau):
+    """The oracle's setupSim on a grid given as arrays (the arrays stay
+    owned here)."""
+    import ctypes as C
+    nx, ny, nz = fl.shape
+    g = O.GeomS()
+    g.nx, g.ny, g.nz, g.N, g.nf, g.top, g.jm, g.pleat = nx, ny, nz, fl.size, int(fl.sum()), top, jm, pleat
+    g.fluid = fl.ctypes.data_as(C.POINTER(C.c_uint8))
+    g.opening = opening.ctypes.data_as(C.POINTER(C.c_uint8))
+    g.outlet = outlet.ctypes.data_as(C.POINTER(C.c_uint8))
+    g.dc = dc.ctypes.data_as(C.POINTER(C.c_float))
+    S = O.SimS()
+    assert O.lib().orc_setup_sim(C.byref(g), K, aniso, u, tau, C.byref(S)) == 0
+    NF = S.NF
+    return O.Sim(s=S, NF=NF, cid=None, cells=None, nb=None, flag=None, layerM=None, layerT=None, Ain=S.Ain,
+                 Af=S.Af, Lp=S.Lp, sig=S.sig, cs2s=S.cs2s, u0=S.u0)
+
+
+@pytest.mark.parametrize("seed,dims", [(1, (23, 19, 70)), (2, (17, 13, 9)), (3, (12, 40, 130))])
+def test_row_span_layout_on_ragged_fluid(seed, dims):
+    """The row-span lattice (rows stored from their first to their last fluid
+    cell, one after another) on ragged random fluid: the reference's
+    arithmetic equals the oracle's cpuStep bit for bit after even and odd
+    steps with averaging, as do the two-lattice scheme and every XCD order,
+    and the probe equals the oracle's."""
+    fl, opening, outlet, dc, top, jm, pleat = _ragged_grid(seed, *dims)
+    K, aniso, u, tau = 3.0, 10.0, 0.06, 0.51
+    S = _oracle_sim_from_arrays(fl, opening, outlet, dc, top, jm, pleat, K, aniso, u, tau)
+    assert S.Ain > 0 and S.Af > 0
+    grid = abx.GridArrays(*dims, fl.ravel(), opening.ravel(), outlet.ravel(), dc, top, jm, pleat)
+    rng = np.random.default_rng(seed)
+    f = (S.f0().astype(np.float64) * rng.uniform(0.99, 1.01, 19 * S.NF)).astype(np.float32)
+    f_ref = f.copy()
+    f2 = np.empty_like(f_ref)
+    acc_ref = np.zeros(4 * S.NF, np.float32)
+    for st in range(1, 8):
+        O.cpu_step(S, f_ref, f2, acc_ref if st > 3 else None)
+        f_ref, f2 = f2, f_ref
+    say("row-span layout", seed, dims)
+    for kern, xm in ((4, -1), (4, 0), (4, 1), (2, -1), (2, 0)):
+        sim = abx.Sim(dev(), grid, K, aniso, u, tau, precision="f32_f64")
+        assert sim.NF == S.NF and sim.info.plane >= S.NF
+        sim.set_launch(kern, xm)
+        sim.write_f(f)
+        sim.steps(3)
+        sim.steps(4, doacc=True)
+        assert np.array_equal(sim.read_f(), f_ref), (kern, xm)
+        assert np.array_equal(sim.read_acc(1.0), acc_ref), (kern, xm)
+        if S.s.nM and S.s.nT:
+            assert sim.probe(7)[:2] == pytest.approx(O.probe(S, f_ref, 7)[:2], rel=1e-12, nan_ok=True)
+        sim.destroy()
 
 
 @pytest.mark.parametrize("name", ["small", "small_yaw", "c1_seed3"])
