@@ -569,8 +569,8 @@ class Sim:
     def set_launch(self, kernel: int = 4, xcd_map: int = -1, nt: int = 19) -> None:
         """kernel 4 = AA pattern in one lattice (default), 2 = two lattices
         with pull streaming.  xcd_map: block order across XCDs (0 round
-        robin, 1 contiguous, G > 1 groups of G blocks, -1 one band of rows j
-        per XCD: the library's default).  nt: non-temporal access flags of
+        robin, 1 contiguous, G > 1 groups of G blocks per XCD; negative: the
+        library's default, groups of 4).  nt: non-temporal access flags of
         the AA steps, even step in bits 0-7, odd step in bits 8-15
         (abx_sim_set_launch; default 19)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, nt))

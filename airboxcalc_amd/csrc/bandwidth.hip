@@ -62,7 +62,8 @@ namespace {
 
 // The memory pattern of an AA even step with every position active: each
 // lane reads the 19 population planes at its position and writes them back
-// permuted (slot q -> slot opp q), in place.  No collision; 152 B per
+// permuted (slot q -> slot opp q), in place, with non-temporal accesses as
+// the stream-collide's even step makes them.  No collision; 152 B per
 // position.  This is the ceiling of the stream-collide's access pattern, as
 // k_copy4 is the ceiling of a plain copy.
 __global__ __launch_bounds__(256) void k_aa_pattern(float* __restrict__ f, int64_t P) {
@@ -70,9 +71,9 @@ __global__ __launch_bounds__(256) void k_aa_pattern(float* __restrict__ f, int64
   if (i >= P) return;
   float v[19];
 #pragma unroll
-  for (int q = 0; q < 19; ++q) v[q] = f[q * P + i];
+  for (int q = 0; q < 19; ++q) v[q] = __builtin_nontemporal_load(f + q * P + i);
 #pragma unroll
-  for (int q = 0; q < 19; ++q) f[OPP[q] * P + i] = v[q];
+  for (int q = 0; q < 19; ++q) __builtin_nontemporal_store(v[q], f + OPP[q] * P + i);
 }
 
 }  // namespace

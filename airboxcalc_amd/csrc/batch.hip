@@ -166,34 +166,30 @@ extern "C" int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_b
     bt->members.push_back(s);
     bt->cells += I.NF;
   }
-  // work lists.  Workgroup b runs on XCD b % 8.  Each member's segments lie
-  // in eight bands of lattice rows j with equal cell counts; XCD x takes band
-  // x of every member, in member order, each band padded to whole
-  // workgroups.  All XCDs move through the members together, and the odd
-  // step's gathers from neighbouring rows stay in the XCD's own L2.
+  // work lists: every member's active segments in lattice order, in member
+  // order, each member's part padded to whole workgroups.  Workgroup b runs
+  // on XCD b % 8 as that XCD's s-th workgroup (s = b / 8); it takes list
+  // block (s / 4) 32 + (b % 8) 4 + s % 4, so each XCD steps through groups
+  // of four consecutive blocks and all XCDs sweep the lattices together (the
+  // single-handle launch's default order).
   const int spbs[3] = {2, 4, 8};
   std::vector<std::vector<int32_t>> segs((size_t)n);
-  std::vector<std::vector<int8_t>> bands((size_t)n);
-  for (int m = 0; m < n; ++m) {
-    segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
-    bands[(size_t)m] = sim_segment_bands(bt->members[(size_t)m]);
-  }
+  for (int m = 0; m < n; ++m) segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
+  constexpr size_t G = 4;
   for (int k = 0; k < 3; ++k) {
-    std::vector<std::vector<int2>> q(8);
+    const size_t spb = (size_t)spbs[k];
+    std::vector<int2> lin;
     for (int m = 0; m < n; ++m) {
-      const std::vector<int32_t>& sg = segs[(size_t)m];
-      const size_t start[8] = {q[0].size(), q[1].size(), q[2].size(), q[3].size(),
-                               q[4].size(), q[5].size(), q[6].size(), q[7].size()};
-      for (size_t i = 0; i < sg.size(); ++i) q[(size_t)bands[(size_t)m][i]].push_back(make_int2(m, sg[i]));
-      for (int x = 0; x < 8; ++x)
-        if (q[(size_t)x].size() > start[x])
-          while (q[(size_t)x].size() % spbs[k]) q[(size_t)x].push_back(make_int2(m, -1));
+      for (int32_t sg : segs[(size_t)m]) lin.push_back(make_int2(m, sg));
+      while (lin.size() % spb) lin.push_back(make_int2(m, -1));
     }
-    size_t maxb = 0;
-    for (const auto& qx : q) maxb = std::max(maxb, qx.size() / spbs[k]);
-    std::vector<int2> w(maxb * 8 * spbs[k], make_int2(0, -1));
-    for (size_t x = 0; x < 8; ++x)
-      for (size_t i = 0; i < q[x].size(); ++i) w[((i / spbs[k]) * 8 + x) * spbs[k] + i % spbs[k]] = q[x][i];
+    const size_t nlin = lin.size() / spb, nblk = (nlin + 8 * G - 1) / (8 * G) * (8 * G);
+    std::vector<int2> w(nblk * spb, make_int2(0, -1));
+    for (size_t blk = 0; blk < nblk; ++blk) {
+      const size_t sq = blk / 8, src = (sq / G) * 8 * G + (blk % 8) * G + sq % G;
+      if (src < nlin)
+        for (size_t i = 0; i < spb; ++i) w[blk * spb + i] = lin[src * spb + i];
+    }
     bt->nblocks[k] = (int)(w.size() / spbs[k]);
     if (hipMalloc((void**)&bt->d_work[k], std::max<size_t>(w.size(), 1) * sizeof(int2)) != hipSuccess ||
         hipMemcpy(bt->d_work[k], w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {

@@ -5,7 +5,7 @@
 // 19-entry neighbour table (engine.js:241-251), 76 B of indices per cell that
 // every step re-reads.  Here every grid row (i, j) that holds fluid stores
 // the span of k from its first to its last fluid cell, and the rows follow
-// one another without gaps (in eight bands of rows j, then i, then j), so
+// one another without gaps (in the order i, j), so
 // nearly every stored position is a fluid cell and every 128-byte line of a
 // population plane is almost entirely fluid: cell (i, j, k) sits at position
 // rowoff(i, j) + k of each plane (structure of arrays, f[q*plane + d]).  The
@@ -58,9 +58,8 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   const int nblk = (a.nchunks + 3) >> 2;
   // Block order across the 8 XCDs (workgroup x runs on XCD x % 8 as the
-  // x-th of that XCD's workgroups, s = x / 8):
-  //   xcd_map 0: block x (round robin; the band order of the segment list
-  //   then gives each XCD one band of rows j);
+  // s-th of that XCD's workgroups, s = x / 8):
+  //   xcd_map 0: block x (round robin);
   //   xcd_map 1: XCD x % 8 owns the contiguous range [(x % 8) per_xcd, ...);
   //   xcd_map G > 1: groups of G consecutive blocks per XCD, all XCDs moving
   //   through the lattice together (block (s / G) 8G + (x % 8) G + s % G).
@@ -262,7 +261,9 @@ struct abx_sim {
   abx_dev* dev = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int prec = ABX_F32, B = 1, xcd_map = -1;  // default launch: one band of rows j per XCD
+  // default launch: groups of 4 consecutive blocks per XCD, all XCDs sweeping
+  // the lattice together
+  int prec = ABX_F32, B = 1, xcd_map = 4;
   int nx = 0, ny = 0, nz = 0;
   int nrows = 0;
   int64_t plane = 0, NF = 0;
@@ -284,13 +285,6 @@ struct abx_sim {
   int32_t* chunks64 = nullptr;    // active 64-position segments
   int nchunks64 = 0;
   std::vector<int32_t> seg_host;  // host copy of chunks64 (ascending positions)
-  std::vector<int8_t> seg_band;   // band of rows j of each active segment
-  int32_t seg_idle = 0;           // first position of a segment without fluid
-  // xcd_map -1: the segments reordered so that XCD x (workgroups x, x + 8, ...)
-  // owns the x-th of eight bands of lattice rows j (equal cell counts) and
-  // sweeps it plane by plane; padded with position 0 of a segment without fluid
-  int32_t* chunks_slab = nullptr;
-  int nslab = 0;
   // kernel 4: AA pattern in one lattice (the default); 2: two lattices, pull.
   // nt: non-temporal accesses of the AA steps, seg_cell's NT flags of the even
   // step in bits 0-7 and of the odd step in bits 8-15.
@@ -349,13 +343,6 @@ StepArgs step_args(abx_sim* s, int doacc) {
   a.chunks = s->chunks64;
   a.nchunks = s->nchunks64;
   a.xcd_map = s->xcd_map;
-  if (s->xcd_map < 0) {
-    if (s->chunks_slab) {
-      a.chunks = s->chunks_slab;
-      a.nchunks = s->nslab;
-    }
-    a.xcd_map = 0;
-  }
   a.per_xcd = (a.nchunks + 7) / 8;
   a.lat = lat_view(s);
   a.plane = s->plane;
@@ -392,7 +379,7 @@ void launch_seg(abx_sim* s, StepArgs a, int doacc) {
 // the even step (mode 1) takes the first list, the odd step (mode 2) the
 // second.  Double storage always uses cached accesses.
 constexpr int kNtEven[] = {0, 3, 19};
-constexpr int kNtOdd[] = {0, 1, 2, 3, 8, 11, 16, 19};
+constexpr int kNtOdd[] = {0, 1, 2, 3, 8, 11, 16, 19, 32, 48};
 
 bool nt_supported(int nt) {
   const int ev = nt & 0xff, od = nt >> 8;
@@ -430,6 +417,10 @@ void launch_aa(abx_sim* s, const StepArgs& a, int doacc, int nt) {
           return launch_seg<S, M, MODE, 16>(s, a, doacc);
         case 19:
           return launch_seg<S, M, MODE, 19>(s, a, doacc);
+        case 32:
+          return launch_seg<S, M, MODE, 32>(s, a, doacc);
+        case 48:
+          return launch_seg<S, M, MODE, 48>(s, a, doacc);
         default:
           break;
       }
@@ -497,26 +488,6 @@ int fill_cold(abx_sim* s) {
   return 0;
 }
 
-// Segment order of xcd_map -1 (see abx_sim::chunks_slab).
-int ensure_slab_order(abx_sim* s) {
-  if (s->chunks_slab || s->seg_host.empty()) return 0;
-  const std::vector<int32_t>& sg = s->seg_host;
-  std::vector<std::vector<int32_t>> lists(8);
-  for (size_t n = 0; n < sg.size(); ++n) lists[(size_t)s->seg_band[n]].push_back(sg[n]);
-  size_t maxb = 0;
-  for (const auto& l : lists) maxb = std::max(maxb, (l.size() + 3) / 4);
-  // workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th workgroup; empty
-  // places take a segment without fluid (its lanes leave at once)
-  const int32_t idle = s->seg_idle;
-  std::vector<int32_t> perm(maxb * 8 * 4, idle);
-  for (size_t x = 0; x < 8; ++x)
-    for (size_t q = 0; q < lists[x].size(); ++q) perm[((q / 4) * 8 + x) * 4 + q % 4] = lists[x][q];
-  if (dalloc(s, &s->chunks_slab, perm.size())) return -1;
-  s->nslab = (int)perm.size();
-  ABX_HIP(hipMemcpyAsync(s->chunks_slab, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, s->stream));
-  return 0;
-}
-
 // The second lattice, needed only by the two-lattice kernels.
 int ensure_second_lattice(abx_sim* s) {
   if (s->f[1]) return 0;
@@ -565,7 +536,7 @@ void sim_free(abx_sim* s) {
   sim_drop_graphs(s);
   void* ptrs[] = {s->f[0],  s->f[1],     s->acc,      s->ns,    s->fluid_bits, s->type_bits, s->meta,
                   s->seg_row0, s->rdelta, s->rowinfo, s->rowid, s->chunks64,  s->cells,     s->plist,
-                  s->vy,     s->red,      s->chunks_slab};
+                  s->vy,     s->red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->red_host) (void)hipHostFree(s->red_host);
@@ -613,48 +584,26 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   // rows (i, j) that hold fluid, and the span [kmin, kmax] of each
   const int64_t N = (int64_t)nx * ny * nz, NR = (int64_t)nx * ny;
   std::vector<int> kmin((size_t)NR, -1), kmax((size_t)NR, -1);
-  std::vector<int64_t> per_j((size_t)ny, 0);
   for (int64_t ij = 0; ij < NR; ++ij) {
     const uint8_t* fr = g->fluid + ij * nz;
     for (int k = 0; k < nz; ++k)
       if (fr[k]) {
         if (kmin[(size_t)ij] < 0) kmin[(size_t)ij] = k;
         kmax[(size_t)ij] = k;
-        per_j[(size_t)(ij % ny)]++;
       }
   }
-  // eight bands of rows j with equal cell counts, one per XCD
-  std::vector<int8_t> band((size_t)ny, 0);
-  {
-    int64_t tot = 0, cum = 0;
-    for (int64_t v : per_j) tot += v;
-    tot = std::max<int64_t>(tot, 1);
-    for (int j = 0; j < ny; ++j) {
-      band[(size_t)j] = (int8_t)std::min<int64_t>(7, cum * 8 / tot);
-      cum += per_j[(size_t)j];
-    }
-  }
-  // rows stored in the order band, i, j without gaps; each band starts on a
-  // 64-position segment
+  // rows stored in the order i, j without gaps
   std::vector<int32_t> rowid((size_t)NR, -1);
   std::vector<int4> rows;  // {rowoff, kmin, kmax, i ny + j}: cell k of the row at position rowoff + k
-  std::vector<int8_t> row_band;
   int64_t pos = 0;
-  for (int bnd = 0; bnd < 8; ++bnd) {
-    pos = (pos + 63) / 64 * 64;
-    for (int i = 0; i < nx; ++i)
-      for (int j = 0; j < ny; ++j) {
-        const int64_t ij = (int64_t)i * ny + j;
-        if (band[(size_t)j] != bnd || kmin[(size_t)ij] < 0) continue;
-        rowid[(size_t)ij] = (int32_t)rows.size();
-        rows.push_back(make_int4((int)(pos - kmin[(size_t)ij]), kmin[(size_t)ij], kmax[(size_t)ij], (int)ij));
-        row_band.push_back((int8_t)bnd);
-        pos += kmax[(size_t)ij] - kmin[(size_t)ij] + 1;
-      }
+  for (int64_t ij = 0; ij < NR; ++ij) {
+    if (kmin[(size_t)ij] < 0) continue;
+    rowid[(size_t)ij] = (int32_t)rows.size();
+    rows.push_back(make_int4((int)(pos - kmin[(size_t)ij]), kmin[(size_t)ij], kmax[(size_t)ij], (int)ij));
+    pos += kmax[(size_t)ij] - kmin[(size_t)ij] + 1;
   }
-  // after the last band one segment without fluid (the band order's filler)
+  // at least one segment without fluid after the last row
   pos = (pos + 63) / 64 * 64;
-  s->seg_idle = (int32_t)std::min<int64_t>(pos, INT32_MAX);
   s->plane = ((pos + 64 + CHUNK - 1) / CHUNK) * CHUNK;
   if ((int64_t)s->plane * 19 * B > ((int64_t)1 << 40) || s->plane > INT32_MAX / 2 || NR > INT32_MAX) {
     set_error("abx_sim_create: grid too large for 32-bit lattice positions");
@@ -749,13 +698,10 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
   s->sig = rp->K * 0.5 * s->u0 * s->u0 / (uface * std::max(Lp, 1));
   s->cs2s = 18 * 1.4142135623730951 * 0.17 * 0.17;
 
-  // the segments that hold fluid, and the band of each
+  // the segments that hold fluid
   std::vector<int32_t> chunks64;
   for (int64_t sg = 0; sg < nseg; ++sg)
-    if (fbits[(size_t)(2 * sg)] | fbits[(size_t)(2 * sg + 1)]) {
-      chunks64.push_back((int32_t)(sg * 64));
-      s->seg_band.push_back(row_band[(size_t)seg_row0[(size_t)sg]]);
-    }
+    if (fbits[(size_t)(2 * sg)] | fbits[(size_t)(2 * sg + 1)]) chunks64.push_back((int32_t)(sg * 64));
   s->nchunks64 = (int)chunks64.size();
   s->seg_host = chunks64;
   std::vector<int32_t> plist((size_t)(s->nM + s->nT));
@@ -811,7 +757,7 @@ extern "C" int abx_sim_create(abx_dev* dev, const abx_grid* g, const abx_run_par
     sim_free(s);
     return -1;
   }
-  if (ensure_slab_order(s) || fill_cold(s) || zero_acc(s) || upload_ns(s, ns, cells)) {
+  if (fill_cold(s) || zero_acc(s) || upload_ns(s, ns, cells)) {
     sim_free(s);
     return -1;
   }
@@ -1174,13 +1120,12 @@ extern "C" int abx_sim_sync(abx_sim* s) {
 extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
   if (!s || (kernel != 2 && kernel != 4) || !nt_supported(nt)) {
     set_error("abx_sim_set_launch: bad argument (kernel 2 or 4; nt: even-step flags 0, 3 or 19 plus 256 x odd-step "
-              "flags 0, 1, 2, 3, 8, 11, 16 or 19)");
+              "flags 0, 1, 2, 3, 8, 11, 16, 19, 32 or 48)");
     return -1;
   }
   ABX_HIP(hipSetDevice(s->dev->ordinal));
   sim_drop_graphs(s);
   if (kernel == 2 && ensure_second_lattice(s)) return -1;
-  if (xcd_map < 0 && ensure_slab_order(s)) return -1;
   if (kernel != s->kernel) {
     // carry the state over: read it in the old form, store it in the new one
     const size_t n = (size_t)(19 * s->NF);
@@ -1193,7 +1138,7 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
     for (int b = 0; b < s->B; ++b)
       if (abx_sim_write_f(s, b, host.data() + (size_t)b * n * s->ssz)) return -1;
   }
-  s->xcd_map = xcd_map < 0 ? -1 : std::min(xcd_map, 4096);
+  s->xcd_map = xcd_map < 0 ? 4 : std::min(xcd_map, 4096);
   s->nt = nt;
   return 0;
 }
@@ -1216,7 +1161,6 @@ int sim_normalize(abx_sim* s) {
     if (abx_sim_write_f(s, b, host.data() + (size_t)b * n)) return -1;
   return 0;
 }
-std::vector<int8_t> sim_segment_bands(abx_sim* s) { return s->seg_band; }
 std::vector<int32_t> sim_segments(abx_sim* s) {
   std::vector<int32_t> v((size_t)s->nchunks64);
   if (s->nchunks64) (void)hipMemcpy(v.data(), s->chunks64, v.size() * 4, hipMemcpyDeviceToHost);

@@ -359,7 +359,7 @@ __device__ __forceinline__ void for_each_q(F&& f) {
 // keeping the lines): bit 0 / bit 1 = population loads / stores at the
 // cell's own position (every population of the even AA step), bit 2 / bit 3
 // = the odd step's loads / stores at neighbour positions, bit 4 = the
-// averaged fields.
+// averaged fields, bit 5 = the meta words.
 template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0>
 __device__ __forceinline__ void seg_cell(const StepArgs& a, int seg, int l, int b) {
   const int d = seg + l;
@@ -387,7 +387,7 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, int seg, int l, int 
   uint32_t nbm = 0;  // bit q: the cell x - c_q is fluid
   int dl[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if constexpr (MODE != 1) {
-    const uint32_t m = a.lat.meta[d];
+    const uint32_t m = ld<(NT & 32) != 0>(a.lat.meta + d);
     nbm = m & 0x7ffffu;
     const int4* rd = (const int4*)(a.lat.rdelta + 12 * lat_row(a.lat, d, m));
     const int4 v0 = rd[0], v1 = rd[1];

@@ -33,9 +33,7 @@ int sim_form(abx_sim* s);
 void sim_set_form(abx_sim* s, int form);
 // Bring an AA-pattern handle to the swapped form (host round trip if needed).
 int sim_normalize(abx_sim* s);
-// Host copy of the handle's active 64-position segments, and the band of
-// rows j (0..7) of each.
+// Host copy of the handle's active 64-position segments (ascending).
 std::vector<int32_t> sim_segments(abx_sim* s);
-std::vector<int8_t> sim_segment_bands(abx_sim* s);
 
 }  // namespace abx

@@ -58,8 +58,9 @@ int abx_copy_bandwidth(abx_dev* dev, int64_t bytes, int iters, double* gbps, dou
 
 /* Ceiling of the stream-collide's access pattern (no reference counterpart):
  * GB/s (152 B per position) of an in-place pass over 19 planes of `positions`
- * floats that reads slot q and writes slot opp q at every position, the AA
- * even step without its collision, averaged over iters launches. */
+ * floats that reads slot q and writes slot opp q at every position with
+ * non-temporal accesses, the AA even step without its collision, averaged
+ * over iters launches. */
 int abx_aa_pattern_bandwidth(abx_dev* dev, int64_t positions, int iters, double* gbps, double* ms_per_pass);
 
 /* ---- STL ------------------------------------------------------------------
@@ -233,20 +234,21 @@ int abx_sim_sync(abx_sim* sim);
  * kernel: 4 = AA pattern in one lattice (the default; one position per lane,
  *             four active 64-position segments per workgroup),
  *         2 = the same segments with two lattices and pull streaming.
- * xcd_map -1 (the default) = each XCD owns one of eight bands of rows j
- * (equal segment counts) and sweeps it plane by plane; 0 = blocks in
- * lattice order, round robin over the XCDs; 1 = each XCD sweeps one
- * contiguous part of the lattice; G > 1 = groups of G consecutive blocks per
- * XCD, all XCDs moving through the lattice together.
+ * xcd_map: order of the workgroups over the 8 XCDs.  G > 1 = groups of G
+ * consecutive blocks per XCD, all XCDs moving through the lattice together
+ * (4, the default, also taken for any negative value); 0 = blocks in lattice
+ * order, round robin over the XCDs; 1 = each XCD sweeps one contiguous part
+ * of the lattice.
  * nt (AA pattern, float storage): non-temporal accesses (streamed through
  * the caches without keeping the lines), flags of the even step in bits 0-7
- * and of the odd step in bits 8-15: 1 / 2 = population loads / stores at
- * line-aligned offsets (all of the even step's, the odd step's with cz = 0),
- * 4 / 8 = the odd step's other loads / stores, 16 = the averaged fields.
- * Compiled sets: even 0, 3, 19; odd 0, 1, 2, 3, 8, 11, 16, 19.  Default 19
- * (the even step's populations and averaged fields non-temporal, the odd
- * step's accesses cached: its gathers reuse neighbouring rows' lines).  Every
- * choice computes the same values bit for bit. */
+ * and of the odd step in bits 8-15: 1 / 2 = a cell's own population loads /
+ * stores (all of the even step's, the odd step's rest population), 4 / 8 =
+ * the odd step's neighbour loads / stores, 16 = the averaged fields, 32 = the
+ * odd step's meta words.  Compiled sets: even 0, 3, 19; odd 0, 1, 2, 3, 8,
+ * 11, 16, 19, 32, 48.  Default 19 (the even step's populations and averaged
+ * fields non-temporal, the odd step's accesses cached: its gathers reuse
+ * neighbouring rows' lines).  Every choice computes the same values bit for
+ * bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */

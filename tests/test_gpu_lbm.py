@@ -184,7 +184,8 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
     want_f, want_acc = a.read_f(), a.read_acc()
     for kern, xm, nt in ((4, 0, 3), (4, 1, 0), (4, 3, 3), (4, -1, 0), (4, -1, 3), (4, -1, 19 + 256 * 19),
                          (4, 8, 3 + 256 * 11), (4, -1, 256 * 8), (4, -1, 3 + 256 * 1), (4, -1, 3 + 256 * 2),
-                         (4, -1, 3 + 256 * 16), (4, -1, 256 * 3), (2, 0, 0), (2, 5, 0), (2, -1, 0)):
+                         (4, -1, 3 + 256 * 16), (4, -1, 256 * 3), (4, 4, 19 + 256 * 32), (4, 16, 3 + 256 * 48),
+                         (2, 0, 0), (2, 5, 0), (2, -1, 0)):
         _, b = _sim(fx, precision)
         b.set_launch(kern, xm, nt)
         b.write_f(f)
@@ -268,7 +269,7 @@ def test_row_span_layout_on_ragged_fluid(seed, dims):
         O.cpu_step(S, f_ref, f2, acc_ref if st > 3 else None)
         f_ref, f2 = f2, f_ref
     say("row-span layout", seed, dims)
-    for kern, xm in ((4, -1), (4, 0), (4, 1), (2, -1), (2, 0)):
+    for kern, xm in ((4, -1), (4, 0), (4, 1), (4, 2), (2, -1), (2, 0)):
         sim = abx.Sim(dev(), grid, K, aniso, u, tau, precision="f32_f64")
         assert sim.NF == S.NF and sim.info.plane >= S.NF
         sim.set_launch(kern, xm)
