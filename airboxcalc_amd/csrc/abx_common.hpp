@@ -9,13 +9,9 @@
 #include <string>
 
 #include "../../include/airbox_hip.h"
+#include "abx_error.hpp"
 
 namespace abx {
-
-// Thread-local message of the last failure (abx_last_error).
-void set_error(const std::string& msg);
-void set_errorf(const char* fmt, ...);
-const char* last_error();
 
 #define ABX_HIP(call)                                                                   \
   do {                                                                                  \

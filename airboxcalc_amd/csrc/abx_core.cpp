@@ -109,7 +109,7 @@ double to_number(const char* s, size_t n) {
 }  // namespace
 
 extern "C" int abx_parse_stl(const uint8_t* buf, int64_t len, float* out, int64_t cap, int64_t* nfloat) {
-  if (!buf || !nfloat || len < 0) {
+  if ((!buf && len > 0) || !nfloat || len < 0) {
     set_error("abx_parse_stl: bad argument");
     return -1;
   }

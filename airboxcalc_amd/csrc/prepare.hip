@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "abx_common.hpp"
+#include "prepare_host.hpp"
 
 namespace abx {
 
@@ -374,49 +375,6 @@ int voxelize_device(const float* T_dev, int64_t ntri, const double origin[3], in
   return 0;
 }
 
-// ---- host side of prepare --------------------------------------------------
-struct Frame {
-  int dAx, iAx, oAx, sd, si;
-  double seal, cy, sy, yaw;
-  void to_local(const double q[3], double L[3]) const {
-    const double a = q[oAx], b = -si * q[iAx];
-    L[0] = cy * a + sy * b;
-    L[1] = -sd * (q[dAx] - seal);
-    L[2] = -sy * a + cy * b;
-  }
-  void to_model(double X, double D, double Z, double q[3]) const {
-    q[oAx] = cy * X - sy * Z;
-    q[dAx] = seal - sd * D;
-    q[iAx] = -si * (sy * X + cy * Z);
-  }
-};
-
-// Number#toString for the cell sizes the leak message quotes.
-std::string js_num(double v) {
-  char buf[64];
-  for (int prec = 1; prec <= 17; ++prec) {
-    snprintf(buf, sizeof buf, "%.*g", prec, v);
-    if (strtod(buf, nullptr) == v) break;
-  }
-  return buf;
-}
-
-// Number#toFixed(1): the nearest one-decimal value, ties away from zero for
-// positive x.
-std::string js_fixed1(double x) {
-  const double y = x * 10, fl = floor(y);
-  const double n = (y - fl == 0.5) ? fl + 1 : nearbyint(y);
-  char buf[64];
-  snprintf(buf, sizeof buf, "%.1f", n / 10);
-  return buf;
-}
-
-int first_ge(const std::vector<float>& a, double v, bool or_last) {
-  for (size_t i = 0; i < a.size(); ++i)
-    if ((double)a[i] >= v) return (int)i;
-  return or_last ? (int)a.size() - 1 : -1;
-}
-
 // Inlet duct, floor, flood fill and leak check on the GPU (engine.js:173-195).
 // solid (N bytes) is updated in place with the duct and floor; fluid receives
 // the flooded cells, *nfluid their number and *leaked whether air reaches an
@@ -499,11 +457,7 @@ int flood_device(std::vector<uint8_t>& solid, const std::vector<uint8_t>& openin
 
 using namespace abx;
 
-struct abx_geom {
-  abx_geom_info info;
-  std::vector<uint8_t> solid, fluid, opening, outlet;
-  std::vector<float> xc, dc, zc;
-  std::vector<double> openPts, filterPts, outletPts;
+struct abx_geom : GeomHost {
   uint8_t* fine_dev = nullptr;
   int64_t nfine = 0;
   int ordinal = 0;
@@ -548,93 +502,30 @@ extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const 
     return -1;
   }
   *out = nullptr;
-  Frame fr;
-  fr.dAx = p->sealAxis;
-  fr.iAx = p->inletAxis;
-  if (fr.dAx < 0 || fr.dAx > 2 || fr.iAx < 0 || fr.iAx > 2) {
-    set_error("abx_prepare: axis out of range");
-    return -1;
-  }
-  if (fr.dAx == fr.iAx) {
-    set_error("Einlassseite und Filterebene liegen auf derselben Achse.");
-    return -1;
-  }
-  fr.oAx = 3 - fr.dAx - fr.iAx;
-  fr.sd = p->sealSign;
-  fr.si = p->inletSign;
-  fr.seal = p->seal;
-  fr.yaw = p->inletYaw;
-  const double yr = (std::isnan(p->inletYaw) ? 0.0 : p->inletYaw) * 3.141592653589793 / 180;
-  fr.cy = std::isnan(p->cy) ? cos(yr) : p->cy;
-  fr.sy = std::isnan(p->sy) ? sin(yr) : p->sy;
-  ABX_HIP(hipSetDevice(dev->ordinal));
-
-  // local frame and bounding box (engine.js:112-118)
-  const int64_t ntri = nfloat / 9;
-  std::vector<float> T((size_t)std::max<int64_t>(nfloat, 1));
-  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t v = 0; v < ntri * 3; ++v) {
-    const double q[3] = {T0[3 * v], T0[3 * v + 1], T0[3 * v + 2]};
-    double L[3];
-    fr.to_local(q, L);
-    for (int a = 0; a < 3; ++a) {
-      T[(size_t)(3 * v + a)] = (float)L[a];
-      if (L[a] < mn[a]) mn[a] = L[a];
-      if (L[a] > mx[a]) mx[a] = L[a];
-    }
-  }
-  // grid (engine.js:119-125)
-  const double h = p->dx, hf = h / 3;
-  const bool has_out = p->has_outlet != 0;
-  double origin[3] = {mn[0] - h, (has_out ? std::min(mn[1], -p->out_height) : mn[1]) - h, mn[2] - 10 * h};
-  const double ndx = ceil((mx[0] - origin[0] + h) / h), ndy = ceil((mx[1] - origin[1] + h) / h),
-               ndz = ceil((mx[2] - origin[2] + h) / h);
-  if (!(ndx >= 1 && ndy >= 1 && ndz >= 1) || ndx * ndy * ndz > 4.0e9) {
-    set_error("abx_prepare: degenerate or unbounded grid");
-    return -1;
-  }
-  const int nx = (int)ndx, ny = (int)ndy, nz = (int)ndz;
-  const int64_t N = (int64_t)nx * ny * nz;
-  if (p->max_cells > 0 && (double)N > p->max_cells) {
-    set_errorf("Gitter zu groß (%s Mio. Zellen) – Zellgröße erhöhen.", js_fixed1((double)N / 1e6).c_str());
-    return -1;
-  }
-  const int nxf = 3 * nx, nyf = 3 * ny, nzf = 3 * nz;
   abx_geom* g = new abx_geom();
   g->ordinal = dev->ordinal;
-  abx_geom_info& I = g->info;
-  memset(&I, 0, sizeof I);
-  I.nx = nx;
-  I.ny = ny;
-  I.nz = nz;
-  I.N = N;
-  I.h = h;
-  for (int a = 0; a < 3; ++a) {
-    I.origin[a] = origin[a];
-    I.mn[a] = mn[a];
-    I.mx[a] = mx[a];
-  }
-  I.dAx = fr.dAx;
-  I.iAx = fr.iAx;
-  I.oAx = fr.oAx;
-  I.sd = fr.sd;
-  I.si = fr.si;
-  I.seal = fr.seal;
-  I.yaw = std::isnan(p->inletYaw) ? 0.0 : p->inletYaw;
-  I.cy = fr.cy;
-  I.sy = fr.sy;
-  I.pleat = p->pleat;
-  I.has_outlet = has_out ? 1 : 0;
-  auto fail = [&](const std::string& m) {
-    set_error(m);
+  auto fail = [&]() {
     delete g;
     return -1;
   };
+  // frame, bounding box and grid (engine.js:111-125)
+  std::vector<float> T;
+  if (prep_grid(T0, nfloat, p, *g, T)) return fail();
+  const abx_geom_info& I = g->info;
+  const int nx = I.nx, ny = I.ny, nz = I.nz, nxf = 3 * nx, nyf = 3 * ny, nzf = 3 * nz;
+  const int64_t N = I.N, ntri = nfloat / 9;
+  if (hipSetDevice(dev->ordinal) != hipSuccess) {
+    set_error("abx_prepare: hipSetDevice failed");
+    return fail();
+  }
 
   // voxelize and vote on the GPU (engine.js:126-135), slice kf for the openings
   hipStream_t st;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail("abx_prepare: stream");
-  const int kf = (int)floor((mn[2] + 1.5 - origin[2]) / hf);
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("abx_prepare: stream");
+    return fail();
+  }
+  const int kf = prep_slice_k(*g);
   std::vector<uint8_t> empty((size_t)nxf * nyf);
   g->solid.assign((size_t)N, 0);
   {
@@ -651,7 +542,7 @@ extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const 
       set_error("abx_prepare: triangle upload failed");
       rc = -1;
     }
-    if (!rc) rc = voxelize_device(dT.as<float>(), ntri, origin, nxf, nyf, nzf, hf, g->fine_dev, st);
+    if (!rc) rc = voxelize_device(dT.as<float>(), ntri, I.origin, nxf, nyf, nzf, I.h / 3, g->fine_dev, st);
     if (!rc) {
       hipLaunchKernelGGL(k_vote, dim3(nb(N, 256)), dim3(256), 0, st, g->fine_dev, nx, ny, nz, dsolid.as<uint8_t>());
       hipLaunchKernelGGL(k_slice, dim3(nb((int64_t)empty.size(), 256)), dim3(256), 0, st, g->fine_dev, nxf, nyf,
@@ -665,201 +556,20 @@ extern "C" int abx_prepare(abx_dev* dev, const float* T0, int64_t nfloat, const 
       }
     }
     (void)hipStreamDestroy(st);
-    if (rc) {
-      std::string m = last_error();
-      return fail(m);
-    }
+    if (rc) return fail();
   }
-  std::vector<uint8_t>& solid = g->solid;
-  g->xc.resize((size_t)nx);
-  g->dc.resize((size_t)ny);
-  g->zc.resize((size_t)nz);
-  for (int i = 0; i < nx; ++i) g->xc[(size_t)i] = (float)(origin[0] + (i + 0.5) * h);
-  for (int j = 0; j < ny; ++j) g->dc[(size_t)j] = (float)(origin[1] + (j + 0.5) * h);
-  for (int k = 0; k < nz; ++k) g->zc[(size_t)k] = (float)(origin[2] + (k + 0.5) * h);
-  const std::vector<float>&xc = g->xc, &dc = g->dc, &zc = g->zc;
-
-  // openings: holes of the first material layer not connected to the border
-  // (engine.js:141-154)
-  const int64_t n2 = (int64_t)nxf * nyf;
-  std::vector<uint8_t> outside((size_t)n2, 0);
-  std::vector<int32_t> q((size_t)n2);
-  int64_t qh = 0, qt = 0;
-  auto push = [&](int64_t c) {
-    if (empty[(size_t)c] && !outside[(size_t)c]) {
-      outside[(size_t)c] = 1;
-      q[(size_t)qt++] = (int32_t)c;
-    }
-  };
-  for (int i = 0; i < nxf; ++i) {
-    push((int64_t)i * nyf);
-    push((int64_t)i * nyf + nyf - 1);
-  }
-  for (int j = 0; j < nyf; ++j) {
-    push(j);
-    push((int64_t)(nxf - 1) * nyf + j);
-  }
-  while (qh < qt) {
-    const int64_t c = q[(size_t)qh++];
-    const int i = (int)(c / nyf), j = (int)(c % nyf);
-    if (i > 0) push(c - nyf);
-    if (i < nxf - 1) push(c + nyf);
-    if (j > 0) push(c - 1);
-    if (j < nyf - 1) push(c + 1);
-  }
-  std::vector<uint8_t> holes((size_t)n2);
-  for (int64_t c = 0; c < n2; ++c)
-    holes[(size_t)c] = (empty[(size_t)c] && !outside[(size_t)c] && (origin[1] + ((double)(c % nyf) + 0.5) * hf > 0));
-  // 4-connected components in scan order, cells in BFS order (engine.js:95-108)
-  std::vector<int32_t> lab((size_t)n2, -1);
-  std::vector<std::vector<int32_t>> comps;
-  for (int64_t s0 = 0; s0 < n2; ++s0) {
-    if (!holes[(size_t)s0] || lab[(size_t)s0] >= 0) continue;
-    const int32_t id = (int32_t)comps.size();
-    std::vector<int32_t> cells;
-    cells.push_back((int32_t)s0);
-    lab[(size_t)s0] = id;
-    for (size_t h0 = 0; h0 < cells.size(); ++h0) {
-      const int32_t c = cells[h0];
-      const int i = c / nyf, j = c % nyf;
-      const int32_t nbs[4] = {i > 0 ? c - nyf : -1, i < nxf - 1 ? c + nyf : -1, j > 0 ? c - 1 : -1,
-                              j < nyf - 1 ? c + 1 : -1};
-      for (int32_t d : nbs)
-        if (d >= 0 && holes[(size_t)d] && lab[(size_t)d] < 0) {
-          lab[(size_t)d] = id;
-          cells.push_back(d);
-        }
-    }
-    if ((double)cells.size() * hf * hf > 25) comps.push_back(std::move(cells));
-  }
-  I.nHoles = (int)comps.size();
-  if (comps.empty()) return fail("Auf der gewählten Einlassseite wurde keine Öffnung gefunden.");
-  size_t best = 0;
-  if (p->has_click) {
-    double L[3];
-    fr.to_local(p->click, L);
-    double bd = INFINITY;
-    for (size_t ci = 0; ci < comps.size(); ++ci)
-      for (size_t s = 0; s < comps[ci].size(); s += 3) {
-        const int32_t c = comps[ci][s];
-        const double x = origin[0] + ((double)(c / nyf) + 0.5) * hf, y = origin[1] + ((double)(c % nyf) + 0.5) * hf;
-        const double ex = x - L[0], ey = y - L[1];
-        const double d = ex * ex + ey * ey;
-        if (d < bd) {
-          bd = d;
-          best = ci;
-        }
-      }
-  } else {
-    for (size_t ci = 1; ci < comps.size(); ++ci)
-      if (comps[ci].size() > comps[best].size()) best = ci;
-  }
-  std::vector<uint8_t> sel((size_t)nx * ny, 0);
-  for (int32_t c : comps[best]) sel[(size_t)((c / nyf) / 3) * ny + (size_t)((c % nyf) / 3)]++;
-  g->opening.assign((size_t)nx * ny, 0);
-  int nOpen = 0;
-  for (size_t c = 0; c < sel.size(); ++c)
-    if (sel[c] >= 5) {
-      g->opening[c] = 1;
-      nOpen++;
-    }
-  I.nOpen = nOpen;
-  if (nOpen < 8) return fail("Einlassöffnung ist zu klein für die gewählte Zellgröße.");
-  const std::vector<uint8_t>& opening = g->opening;
-
-  // inlet duct, floor, flood fill of the air and leak check, on the GPU
+  // openings (engine.js:141-171), duct depth, seal row and floor rows
+  std::vector<uint8_t> floor_row;
+  if (prep_openings(p, empty.data(), *g) || prep_flood_inputs(*g, floor_row)) return fail();
+  // inlet duct, floor, flood fill of the air and leak check on the GPU
   // (engine.js:173-195).  The reference applies duct and floor before it
   // checks the seal row; neither changes whether that check fails.
-  auto IDX = [&](int64_t i, int64_t j, int64_t k) { return (i * ny + j) * nz + k; };
-  const int kw0 = first_ge(zc, mn[2], false);
-  I.kw0 = kw0;
-  const int top = first_ge(dc, 0.0, false);
-  if (top < 0) return fail("Dichtebene liegt außerhalb des Modells.");
-  I.top = top;
-  std::vector<uint8_t> floor_row((size_t)ny);
-  for (int j = 0; j < ny; ++j) floor_row[(size_t)j] = (double)dc[(size_t)j] > mx[1] - 0.5 * h ? 1 : 0;
-  std::vector<uint8_t>& fluid = g->fluid;
+  int64_t nfluid = 0;
   bool leaked = false;
-  if (flood_device(solid, opening, floor_row, nx, ny, nz, kw0, top, fluid, &qt, &leaked)) {
-    std::string m = last_error();
-    return fail(m);
-  }
-  if (leaked)
-    return fail("Bei " + js_num(h) +
-                " mm Zellgröße läuft die Luft durch eine zu dünne Wand nach außen (Wände sollten mindestens so "
-                "dick wie eine Zelle sein). Bitte eine kleinere Zellgröße wählen.");
-  // clean-air side: lid plenum and outlet disc, or the whole top row (engine.js:197-213)
-  g->outlet.assign((size_t)N, 0);
-  int nOut = 0;
-  if (has_out) {
-    const int jpt = first_ge(dc, -p->out_height, true);
-    if (jpt >= top) return fail("Deckelhöhe ist kleiner als eine Zelle.");
-    double L[3];
-    fr.to_local(p->out_center, L);
-    const double rr = p->out_diameter / 2, r2 = rr * rr;
-    for (int i = 0; i < nx; ++i)
-      for (int k = 0; k < nz; ++k) {
-        if (!fluid[(size_t)IDX(i, top, k)]) continue;
-        for (int j = jpt; j < top; ++j) {
-          const int64_t c = IDX(i, j, k);
-          fluid[(size_t)c] = 1;
-          solid[(size_t)c] = 0;
-          qt++;
-        }
-        const double ex = (double)xc[(size_t)i] - L[0], ez = (double)zc[(size_t)k] - L[2];
-        if (ex * ex + ez * ez <= r2) {
-          g->outlet[(size_t)IDX(i, jpt, k)] = 1;
-          nOut++;
-          double m[3];
-          fr.to_model(xc[(size_t)i], dc[(size_t)jpt], zc[(size_t)k], m);
-          g->outletPts.insert(g->outletPts.end(), m, m + 3);
-        }
-      }
-    if (nOut < 4) return fail("Der Auslass liegt nicht über der Filterfläche oder ist zu klein.");
-  } else {
-    for (int i = 0; i < nx; ++i)
-      for (int k = 0; k < nz; ++k) {
-        const int64_t c = IDX(i, top, k);
-        if (fluid[(size_t)c]) {
-          g->outlet[(size_t)c] = 1;
-          nOut++;
-        }
-      }
-  }
-  I.nOut = nOut;
-  I.nf = qt;
-  const int jm = first_ge(dc, p->pleat / 2, true);
-  I.jm = jm;
-  int Af = 0;
-  for (int i = 0; i < nx; ++i)
-    for (int k = 0; k < nz; ++k) Af += fluid[(size_t)IDX(i, jm, k)];
-  I.Af = Af;
-  for (int i = 0; i < nx; ++i)
-    for (int k = 0; k < nz; ++k)
-      if (fluid[(size_t)IDX(i, jm, k)]) {
-        double m[3];
-        fr.to_model(xc[(size_t)i], dc[(size_t)jm], zc[(size_t)k], m);
-        g->filterPts.insert(g->filterPts.end(), m, m + 3);
-      }
-  for (int64_t c = 0; c < (int64_t)nx * ny; ++c)
-    if (opening[(size_t)c]) {
-      double m[3];
-      fr.to_model(xc[(size_t)(c / ny)], dc[(size_t)(c % ny)], mn[2], m);
-      g->openPts.insert(g->openPts.end(), m, m + 3);
-    }
-  int leak = 0;
-  for (int i = 0; i < nx; ++i)
-    for (int k = 0; k < nz; ++k) leak += fluid[(size_t)IDX(i, top, k)];
-  I.leak = leak;
-  I.nOpenPts = (int64_t)g->openPts.size() / 3;
-  I.nFilterPts = (int64_t)g->filterPts.size() / 3;
-  I.nOutletPts = (int64_t)g->outletPts.size() / 3;
-  I.volume_l = (double)I.nf * pow(h, 3) / 1e6;
-  I.inlet_cm2 = nOpen * h * h / 100;
-  I.filter_cm2 = Af * h * h / 100;
-  I.outlet_cm2 = leak * h * h / 100;
-  I.outlet_cm2_real = has_out ? nOut * h * h / 100 : 0;
-  I.pleatOk = jm > top;
+  if (flood_device(g->solid, g->opening, floor_row, nx, ny, nz, I.kw0, I.top, g->fluid, &nfluid, &leaked))
+    return fail();
+  // lid plenum and outlet, filter plane, points and summary (engine.js:196-234)
+  if (prep_finish(p, nfluid, leaked, *g)) return fail();
   *out = g;
   return 0;
 }
