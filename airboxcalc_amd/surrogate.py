@@ -32,7 +32,7 @@ import copy
 import os
 import time
 from dataclasses import dataclass
-from typing import Callable, List, Optional
+from typing import Callable, List, Optional, Tuple
 
 import numpy as np
 
@@ -483,6 +483,7 @@ STUDY_SUBDIV = 57        # quads split 57 x 57: about 200 k triangles per box
 def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box], dx: float = 2.0,
                      window: int = 1000, max_steps: int = 60000, fixed_steps: int = 6000, fixed_avg: int = 2000,
                      fp64_every: int = 4, subdiv: int = STUDY_SUBDIV, warm_steps: int = 2000, warm_avg: int = 2000,
+                     ideal: Optional[Tuple[float, float]] = None, ideal_alt: bool = False,
                      log: Optional[Callable[[str], None]] = None) -> dict:
     """Cold against surrogate-warm time to steady for held-out boxes.
 
@@ -498,6 +499,15 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                   start needs no transient to discard);
       cold_short  the same shortened protocol from rest (what the warm start
                   is worth at that length);
+      ideal       (ideal = (a_rho, a_u)) to the criterion from the
+                  equilibrium of fixed + a (prediction - fixed), a_rho for rho
+                  and a_u for u: a = 0 takes the fixed run's own average, a
+                  perfect prediction of the mean flow, 0 < a < 1 a prediction
+                  that far off, a = 1 the network's; with ideal_alt the
+                  "fixed" average is replaced by an independent one of the
+                  same flow (steps fixed_steps .. fixed_steps + fixed_avg of a
+                  longer run from rest), the best a predictor of the mean
+                  flow could seed;
       fp64        (every fp64_every-th box) the fixed protocol in FP64.
     Velocity-field errors are relative L2 norms over the fluid cells."""
     from . import steady
@@ -530,6 +540,15 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                 warm_fixed = steady.fixed_run(dev, g, p, warm_steps, warm_avg,
                                               init_fn=lambda sim: warm_start_init(net, norm, dev, box, g, sim, tris))
                 cold_short = steady.fixed_run(dev, g, p, warm_steps, warm_avg)
+                best = None
+                if ideal is not None:
+                    nf = fixed.acc.size // 4
+                    src = steady.fixed_run(dev, g, p, fixed_steps + fixed_avg, fixed_avg) if ideal_alt else fixed
+                    fru = np.concatenate([src.acc[3 * nf:], src.acc[:3 * nf]]).astype(np.float64)
+                    a = np.repeat(np.array(ideal, np.float64), [nf, 3 * nf])
+                    ru = fru + a * (np.asarray(timing["rho_u"], np.float64) - fru)
+                    best = steady.solve_to_steady(dev, g, p, window=window, max_steps=max_steps, keep_acc=True,
+                                                  init_rho_u=ru.astype(np.float32))
                 f64 = steady.fixed_run(dev, g, p, fixed_steps, fixed_avg, precision="f64") \
                     if fp64_every and bi % fp64_every == 0 else None
             except abx.AbxError as e:       # e.g. the reference's instability error
@@ -553,7 +572,7 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
 
         def run(r):
             return dict(converged=r.converged, steps=r.steps, wall_s=r.wall_s,
-                        gammaCore=r.metrics["gammaCore"], zeta=r.metrics["zeta"])
+                        gammaCore=r.metrics["gammaCore"], zeta=r.metrics["zeta"], windows=r.windows)
 
         NF = fixed.acc.size // 4
         init = np.concatenate([rho_u[NF:], rho_u[:NF]])     # (ux, uy, uz, rho), the order of acc
@@ -564,6 +583,9 @@ def warm_start_study(dev: abx.Device, net: U.UNet, norm: dict, boxes: List[Box],
                    warm_fixed=run(warm_fixed), warm_fixed_vs_fixed=compare(warm_fixed, fixed),
                    cold_short_vs_fixed=compare(cold_short, fixed),
                    prediction_u_err_vs_fixed=uerr(init, fixed.acc))
+        if best is not None:
+            row["ideal"] = run(best)
+            row["ideal_vs_fixed"] = compare(best, fixed)
         if f64 is not None:
             row["fp64"] = run(f64)
             row["fixed_vs_fp64"] = compare(fixed, f64)
@@ -589,6 +611,10 @@ def summarize(rows: List[dict]) -> dict:
     cw = np.array([r["cold"]["wall_s"] for r in ok])
     ww = np.array([r["warm"]["wall_s"] + r["predict_s"] for r in ok])
     col = lambda key, f: np.array([r[key][f] for r in ok], float)
+    # the speedup over the boxes where both answers agree with the fixed result
+    both = np.array([r["warm_vs_fixed"]["within"] and r["cold_vs_fixed"]["within"] for r in ok])
+    out.update(n_both_within=int(both.sum()),
+               wall_speedup_both_within=float(cw[both].sum() / ww[both].sum()) if both.any() else None)
     out.update(steps_cold_median=float(np.median(cs)), steps_warm_median=float(np.median(ws)),
                step_speedup_median=float(np.median(cs / ws)), step_speedup_total=float(cs.sum() / ws.sum()),
                wall_speedup_total_incl_predict=float(cw.sum() / ww.sum()),
@@ -614,6 +640,17 @@ def summarize(rows: List[dict]) -> dict:
                    warm_fixed_dzeta_vs_fixed_median=float(np.median([r["warm_fixed_vs_fixed"]["zeta_rel"] for r in wf])),
                    cold_short_dzeta_vs_fixed_median=float(np.median([r["cold_short_vs_fixed"]["zeta_rel"] for r in wf])),
                    warm_fixed_u_err_vs_fixed_median=float(np.median([r["warm_fixed_vs_fixed"]["u_err"] for r in wf])))
+    iw = [r for r in ok if "ideal" in r]
+    if iw:
+        ic = np.array([r["cold"]["wall_s"] for r in iw])
+        ii = np.array([r["ideal"]["wall_s"] for r in iw])
+        out.update(ideal_steps_median=float(np.median([r["ideal"]["steps"] for r in iw])),
+                   ideal_converged=float(np.mean([r["ideal"]["converged"] for r in iw])),
+                   ideal_step_speedup_total=float(sum(r["cold"]["steps"] for r in iw) /
+                                                  sum(r["ideal"]["steps"] for r in iw)),
+                   ideal_wall_speedup_total=float(ic.sum() / ii.sum()),
+                   ideal_within_tol_of_fixed=float(np.mean([r["ideal_vs_fixed"]["within"] for r in iw])),
+                   ideal_dzeta_vs_fixed_median=float(np.median([r["ideal_vs_fixed"]["zeta_rel"] for r in iw])))
     f = [r for r in ok if "fp64" in r]
     if f:
         out.update(fp64_boxes=len(f),
@@ -661,6 +698,12 @@ def main(argv=None) -> int:
     st.add_argument("--subdiv", type=int, default=STUDY_SUBDIV)
     st.add_argument("--warm-steps", type=int, default=2000, help="steps of the shortened fixed protocol")
     st.add_argument("--warm-avg", type=int, default=2000, help="steps it averages")
+    st.add_argument("--ideal", type=float, nargs=2, default=None, metavar=("A_RHO", "A_U"),
+                    help="also start from fixed + A (prediction - fixed), A_RHO for rho and A_U for u; A = 0: the "
+                         "fixed run's own average, a perfect mean-flow prediction")
+    st.add_argument("--ideal-alt", action="store_true",
+                    help="--ideal blends toward an independent average of the same flow (the fixed protocol's "
+                         "window moved past its end) instead of the fixed result itself")
     st.add_argument("--report", help="JSON report")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
@@ -690,7 +733,9 @@ def main(argv=None) -> int:
         norm, meta = load(a.weights, net)
         boxes = family(range(a.first_seed, a.first_seed + a.boxes))
         rep = warm_start_study(dev, net, norm, boxes, dx=a.dx, window=a.window, fp64_every=a.fp64_every,
-                               subdiv=a.subdiv, warm_steps=a.warm_steps, warm_avg=a.warm_avg, log=log)
+                               subdiv=a.subdiv, warm_steps=a.warm_steps, warm_avg=a.warm_avg,
+                               ideal=tuple(a.ideal) if a.ideal else None, ideal_alt=a.ideal_alt,
+                               log=log)
         rep["weights_meta"] = meta
         rep["device"] = dev.name
         log(json.dumps(rep["summary"]))

@@ -6,10 +6,9 @@
   float-storage/double-arithmetic kernel equals engine.js exactly there
   (test_gpu_lbm.test_simulate_metrics_equal_engine_js).
 - The 1 mm airbox of the throughput benchmark (seed 1, 10.5 M fluid cells in
-  the row-span lattice with the band-per-XCD block order): 12 steps against
-  the oracle's cpuStep,
-  bit for bit in the reference's arithmetic, within the reference's gate in
-  FP32.
+  the row-span lattice with the default block order): 12 steps against the
+  oracle's cpuStep, bit for bit in the reference's arithmetic, within the
+  reference's gate in FP32.
 - The 64 seeded batch variants (200-360 mm, dx 2-4 mm): batched members
   equal solo runs bit for bit, and members in the reference's arithmetic
   equal the oracle after 12 steps.
@@ -85,7 +84,7 @@ def test_dx1_benchmark_lattice_matches_oracle():
         I = sim.info
         assert sim.NF == S.NF == 10493429
         # the row-span layout: a stored position per fluid cell plus the solid
-        # cells inside the rows' spans and the band padding
+        # cells inside the rows' spans and the final segment's padding
         assert S.NF <= I.plane < 1.02 * S.NF
         sim.steps(6)
         sim.steps(6, doacc=True)
