@@ -21,6 +21,8 @@
 // all 27 taps.  Weight gradients are summed per workgroup in registers over a
 // strided range of tiles and written as partial sums that one more kernel
 // reduces, so the result does not depend on scheduling.
+#include <utility>
+
 #include "abx_common.hpp"
 
 namespace abx {
@@ -200,14 +202,25 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // weight load, and on the small deep levels its wide channel tiles leave too
 // few workgroups).  A tile is 16 x 4 x 4 (x, y, z) output voxels; wave w computes
 // z-slice w, four rows of 16 voxels.  Input channels are taken 16 at a time:
-// the k-steps of 32 then cover two taps each (k = 32 s + 8 lg: tap
-// 2 s + lg / 2, channels 8 (lg % 2) .. + 8), 14 steps for 27 taps (the last
-// half empty).  Every k-step reads one input fragment per row from the LDS
-// brick, at an offset fixed at compile time up to the lane's tap parity, and
-// issues four MFMAs against a weight fragment held in registers.
+// the k-steps of 32 then cover a pair of taps each (k = 32 s + 8 lg: tap A of
+// pair s for lg < 2, tap B for lg >= 2, channels 8 (lg % 2) .. + 8), 14
+// pairs for 27 taps (the last half empty), with one weight fragment per pair.
+// The 14 weight fragments (14 KB) sit in LDS beside the brick, not in
+// registers: 117 registers leave 4 waves per SIMD (4 workgroups per CU),
+// where holding them in registers left 2, and the brick loads of one
+// workgroup overlap the MFMAs of the others.
+//
+// The pairs are chosen so that one input fragment read from the LDS brick
+// serves several MFMAs: pairs 0-8 are the x shifts 0 and 1 of tap row
+// (dz, dy) = (s / 3, s % 3), pairs 9-11 the x shift 2 of dz = 0 and 1 at
+// dy = s - 9, pair 12 the x shift 2 of dz = 2 at dy = 0 and 1, pair 13 the
+// last tap.  A fragment of brick row rho (pairs 0-11) is then the same for
+// every output row r = rho - dy and serves up to three MFMAs: 32 fragment
+// reads per wave and tile instead of one per MFMA (56).
 //
 // Workgroups are persistent over tiles.  With 16 input channels the weights
-// are loaded once per workgroup; each thread owns fixed slots of the brick
+// are loaded once per workgroup (else once per 16-channel chunk, between the
+// brick's two barriers); each thread owns fixed slots of the brick
 // (decoded once).  The bias is read once per workgroup, not per tile, so the
 // epilogue issues no load that it has to wait for (unless it accumulates or
 // is gated).
@@ -218,6 +231,57 @@ constexpr int F16SLOTS = (2 * F16BV + 255) / 256;  // 16-byte brick pieces per t
 
 __host__ __device__ constexpr int f16_tapoff(int tap) {
   return tap < 27 ? ((tap / 9) * F16BY + (tap / 3) % 3) * F16BX + tap % 3 : 0;
+}
+
+// Taps of pair s (27: none); tap = (dz * 3 + dy) * 3 + dx.
+__host__ __device__ constexpr int f16_tapA(int s) {
+  return s < 9 ? s * 3 : (s < 12 ? (s - 9) * 3 + 2 : (s == 12 ? 20 : 26));
+}
+__host__ __device__ constexpr int f16_tapB(int s) {
+  return s < 9 ? s * 3 + 1 : (s < 12 ? 9 + (s - 9) * 3 + 2 : (s == 12 ? 23 : 27));
+}
+
+// Fragment read i (0-31) of a wave's tile: brick voxel offsets of its lanes
+// with lg < 2 (lo) and lg >= 2 (hi, -1: zero), relative to the wave's own
+// output voxel, and its MFMAs (pair w, output row r; w = -1: none).
+struct F16Read {
+  int lo, hi, w[3], r[3];
+};
+__host__ __device__ constexpr F16Read f16_read(int i) {
+  constexpr int Z = F16BY * F16BX;
+  F16Read f{0, -1, {-1, -1, -1}, {0, 0, 0}};
+  if (i < 24) {
+    // pairs 0-8 (dz = i / 6, x shifts 0 / 1) or 9-11 (x shift 2, dz 0 / 1):
+    // brick row rho, serving output rows rho - dy
+    const int rho = i < 18 ? i % 6 : i - 18;
+    f.lo = i < 18 ? (i / 6) * Z + rho * F16BX : rho * F16BX + 2;
+    f.hi = i < 18 ? f.lo + 1 : Z + rho * F16BX + 2;
+    for (int dy = 0; dy < 3; ++dy)
+      if (rho - dy >= 0 && rho - dy < F16Y) {
+        f.w[dy] = i < 18 ? (i / 6) * 3 + dy : 9 + dy;
+        f.r[dy] = rho - dy;
+      }
+  } else if (i < 28) {
+    // pair 12: dz = 2, x shift 2, rows r (dy = 0) and r + 1 (dy = 1)
+    const int r = i - 24;
+    f.lo = 2 * Z + r * F16BX + 2;
+    f.hi = f.lo + F16BX;
+    f.w[0] = 12;
+    f.r[0] = r;
+  } else {
+    // pair 13: tap 26 (dz = dy = 2, x shift 2) alone
+    const int r = i - 28;
+    f.lo = 2 * Z + (r + 2) * F16BX + 2;
+    f.w[0] = 13;
+    f.r[0] = r;
+  }
+  return f;
+}
+constexpr int F16READS = 32;
+
+template <int... I, typename Fn>
+__device__ __forceinline__ void f16_unroll(std::integer_sequence<int, I...>, Fn&& fn) {
+  (fn(std::integral_constant<int, I>{}), ...);
 }
 
 struct F16Tile {
@@ -250,9 +314,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
                                                     const u16* __restrict__ gate, int gs, int D, int H, int W, int Ci,
                                                     int ntiles, int relu, int accum) {
   __shared__ __attribute__((aligned(16))) u16 sx[F16BV * F16SP];
+  __shared__ u32x4 sw[14 * 64];
   const int co0 = blockIdx.y * 16, nch = Ci / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4, hi = lg >> 1;
   const int lbase = (wave * F16BY * F16BX + lr) * F16SP + 8 * (lg & 1);
+  const int bx1 = lbase + (hi ? F16SP : 0), bxy = lbase + (hi ? F16BX * F16SP : 0),
+            bxz = lbase + (hi ? F16BY * F16BX * F16SP : 0);
 
   // brick slots: piece e = thread + 256 m is voxel e / 2 (x fastest), channels 8 (e % 2) .. + 8
   int sb[F16SLOTS];  // bx | by << 5 | bz << 8 | g << 11, or -1
@@ -276,17 +343,23 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
                                 8 * (sb[m] >> 11));
     }
   };
-  u32x4 wf[14];
   auto load_weights = [&](int c) {
 #pragma unroll
-    for (int s = 0; s < 14; ++s) {
-      const int tap = 2 * s + hi;
-      wf[s] = tap < 27 ? *(const u32x4*)(wt + ((int64_t)(co0 + lr) * 27 + tap) * Ci + 16 * c + 8 * (lg & 1))
-                       : (u32x4){0u, 0u, 0u, 0u};
+    for (int m = 0; m < 4; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      if (e < 14 * 64) {
+        const int s = e >> 6, ln = e & 63, l_r = ln & 15, l_g = ln >> 4;
+        const int tap = (l_g >> 1) ? f16_tapB(s) : f16_tapA(s);
+        sw[e] = tap < 27 ? *(const u32x4*)(wt + ((int64_t)(co0 + l_r) * 27 + tap) * Ci + 16 * c + 8 * (l_g & 1))
+                         : (u32x4){0u, 0u, 0u, 0u};
+      }
     }
   };
 
-  if (nch == 1) load_weights(0);
+  if (nch == 1) {
+    load_weights(0);
+    __syncthreads();
+  }
   // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
   const int co = co0 + 4 * lg;
   float bv[4];
@@ -305,32 +378,42 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
           const int vb = ((sb[m] >> 8) & 7) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
           *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 11)) = pf[m];
         }
-      __syncthreads();
       if (nch > 1) load_weights(c);
-      // the fragments of tap pair s + 1 are read while the MFMAs of pair s
-      // run (registers are free up to the 2-wave limit), so LDS latency is
-      // not waited out once per pair of MFMAs
-      u32x4 bf[2][F16Y];
-#pragma unroll
-      for (int r = 0; r < F16Y; ++r)
-        bf[0][r] = *(const u32x4*)(sx + lbase + (hi ? f16_tapoff(1) : f16_tapoff(0)) * F16SP + r * F16BX * F16SP);
-#pragma unroll
-      for (int s = 0; s < 14; ++s) {
-        if (s + 1 < 14) {
-          const int offn = (hi ? f16_tapoff(2 * s + 3) : f16_tapoff(2 * s + 2)) * F16SP;
-#pragma unroll
-          for (int r = 0; r < F16Y; ++r) bf[(s + 1) & 1][r] = *(const u32x4*)(sx + lbase + offn + r * F16BX * F16SP);
+      __syncthreads();
+      // fragment read i + 2 is issued before the MFMAs of read i, so LDS
+      // latency is not waited out at every read; scheduling barriers keep the
+      // compiler from sinking the reads below those MFMAs
+      // the hi lanes' voxel is the lo lanes' plus 1, a plane or a row: one
+      // lane base per case, and the read's own offset is a constant
+      u32x4 fr[3];
+      auto read = [&](auto ic) {
+        constexpr F16Read f = f16_read(decltype(ic)::value);
+        constexpr int d = f.hi - f.lo;
+        static_assert(f.hi < 0 || d == 1 || d == F16BY * F16BX || d == F16BX, "hi lanes: x, z or y neighbour");
+        if constexpr (f.hi < 0) {
+          const u32x4 v = *(const u32x4*)(sx + lbase + f.lo * F16SP);
+          return hi ? (u32x4){0u, 0u, 0u, 0u} : v;
+        } else {
+          const int b = d == 1 ? bx1 : (d == F16BX ? bxy : bxz);
+          return *(const u32x4*)(sx + b + f.lo * F16SP);
         }
-        // keep the scheduler from sinking those reads below this pair's MFMAs
+      };
+      fr[0] = read(std::integral_constant<int, 0>{});
+      fr[1] = read(std::integral_constant<int, 1>{});
+      f16_unroll(std::make_integer_sequence<int, F16READS>{}, [&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if constexpr (i + 2 < F16READS) fr[(i + 2) % 3] = read(std::integral_constant<int, i + 2>{});
         __builtin_amdgcn_sched_barrier(0);
-        const bool none = s == 13 && hi;  // tap 27 does not exist
+        constexpr F16Read f = f16_read(i);
+        u32x4 wa[3];
 #pragma unroll
-        for (int r = 0; r < F16Y; ++r) {
-          const u32x4 b = none ? (u32x4){0u, 0u, 0u, 0u} : bf[s & 1][r];
-          acc[r] = mfma16(wf[s], b, acc[r]);
-        }
+        for (int k = 0; k < 3; ++k)
+          if (f.w[k] >= 0) wa[k] = sw[f.w[k] * 64 + lane];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (f.w[k] >= 0) acc[f.r[k]] = mfma16(wa[k], fr[i % 3], acc[f.r[k]]);
         __builtin_amdgcn_sched_barrier(0);
-      }
+      });
     }
 
     const F16Tile tl = f16_tile(t, D, H, W);
