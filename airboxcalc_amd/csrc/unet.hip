@@ -200,15 +200,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // Forward convolution, 3x3x3 taps, one 16-channel output tile per workgroup
 // (every 3x3x3 layer: on the 16-channel layers k_conv_fwd has one MFMA per
 // weight load, and on the small deep levels its wide channel tiles leave too
-// few workgroups).  A tile is 16 x 4 x 4 (x, y, z) output voxels; wave w computes
-// z-slice w, four rows of 16 voxels.  Input channels are taken 16 at a time:
+// few workgroups).  A tile is 16 x 4 x 8 (x, y, z) output voxels, eight
+// waves; wave w computes z-slice w, four rows of 16 voxels.  Input channels are taken 16 at a time:
 // the k-steps of 32 then cover a pair of taps each (k = 32 s + 8 lg: tap A of
 // pair s for lg < 2, tap B for lg >= 2, channels 8 (lg % 2) .. + 8), 14
 // pairs for 27 taps (the last half empty), with one weight fragment per pair.
 // The 14 weight fragments (14 KB) sit in LDS beside the brick, not in
-// registers: 117 registers leave 4 waves per SIMD (4 workgroups per CU),
-// where holding them in registers left 2, and the brick loads of one
-// workgroup overlap the MFMAs of the others.
+// registers: 98 registers leave 4 waves per SIMD (two 8-wave workgroups per
+// CU, 48 KB of LDS each), where holding them in registers left 2, and the
+// brick loads of one workgroup overlap the MFMAs of the other.  Eight
+// z-slices per workgroup share one brick (halo 2.1 voxels loaded per output
+// voxel, 2.5 with four) and one copy of the weights.
 //
 // The pairs are chosen so that one input fragment read from the LDS brick
 // serves several MFMAs: pairs 0-8 are the x shifts 0 and 1 of tap row
@@ -224,10 +226,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // (decoded once).  The bias is read once per workgroup, not per tile, so the
 // epilogue issues no load that it has to wait for (unless it accumulates or
 // is gated).
-constexpr int F16X = 16, F16Y = 4, F16Z = 4;
+constexpr int F16X = 16, F16Y = 4, F16Z = 8;
 constexpr int F16BX = F16X + 2, F16BY = F16Y + 2, F16BZ = F16Z + 2, F16BV = F16BX * F16BY * F16BZ;
 constexpr int F16SP = 16;  // u16 per brick voxel: 16 channels, unpadded (measured faster than 24, 32, 40)
-constexpr int F16SLOTS = (2 * F16BV + 255) / 256;  // 16-byte brick pieces per thread
+constexpr int F16SLOTS = (2 * F16BV + 511) / 512;  // 16-byte brick pieces per thread
 
 __host__ __device__ constexpr int f16_tapoff(int tap) {
   return tap < 27 ? ((tap / 9) * F16BY + (tap / 3) % 3) * F16BX + tap % 3 : 0;
@@ -309,7 +311,7 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-__global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
+__global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
                                                     const float* __restrict__ bias, u16* __restrict__ y, int ys,
                                                     const u16* __restrict__ gate, int gs, int D, int H, int W, int Ci,
                                                     int ntiles, int relu, int accum) {
@@ -321,13 +323,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
   const int bx1 = lbase + (hi ? F16SP : 0), bxy = lbase + (hi ? F16BX * F16SP : 0),
             bxz = lbase + (hi ? F16BY * F16BX * F16SP : 0);
 
-  // brick slots: piece e = thread + 256 m is voxel e / 2 (x fastest), channels 8 (e % 2) .. + 8
-  int sb[F16SLOTS];  // bx | by << 5 | bz << 8 | g << 11, or -1
+  // brick slots: piece e = thread + 512 m is voxel e / 2 (x fastest), channels 8 (e % 2) .. + 8
+  int sb[F16SLOTS];  // bx | by << 5 | bz << 8 | g << 12, or -1
 #pragma unroll
   for (int m = 0; m < F16SLOTS; ++m) {
-    const int e = threadIdx.x + 256 * m, vb = e >> 1;
+    const int e = threadIdx.x + 512 * m, vb = e >> 1;
     const int bx = vb % F16BX, r = vb / F16BX, by = r % F16BY, bz = r / F16BY;
-    sb[m] = e < 2 * F16BV ? bx | by << 5 | bz << 8 | (e & 1) << 11 : -1;
+    sb[m] = e < 2 * F16BV ? bx | by << 5 | bz << 8 | (e & 1) << 12 : -1;
   }
   u32x4 pf[F16SLOTS];
   auto fetch = [&](int t, int c) {
@@ -337,16 +339,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
       pf[m] = (u32x4){0u, 0u, 0u, 0u};
       if (sb[m] < 0) continue;
       const int gx = tl.x0 + (sb[m] & 31) - 1, gy = tl.y0 + ((sb[m] >> 5) & 7) - 1,
-                gz = tl.z0 + ((sb[m] >> 8) & 7) - 1;
+                gz = tl.z0 + ((sb[m] >> 8) & 15) - 1;
       if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
         pf[m] = *(const u32x4*)(x + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xs + 16 * c +
-                                8 * (sb[m] >> 11));
+                                8 * (sb[m] >> 12));
     }
   };
   auto load_weights = [&](int c) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const int e = threadIdx.x + 256 * m;
+      const int e = threadIdx.x + 512 * m;
       if (e < 14 * 64) {
         const int s = e >> 6, ln = e & 63, l_r = ln & 15, l_g = ln >> 4;
         const int tap = (l_g >> 1) ? f16_tapB(s) : f16_tapA(s);
@@ -375,8 +377,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const u16* __restrict__ x, i
 #pragma unroll
       for (int m = 0; m < F16SLOTS; ++m)
         if (sb[m] >= 0) {
-          const int vb = ((sb[m] >> 8) & 7) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
-          *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 11)) = pf[m];
+          const int vb = ((sb[m] >> 8) & 15) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
+          *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 12)) = pf[m];
         }
       if (nch > 1) load_weights(c);
       __syncthreads();
@@ -1129,10 +1131,10 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int n,
   hipStream_t s = (hipStream_t)stream;
   if (ksize == 3 && g_fwd16) {
     const int64_t tiles = (int64_t)n * ((d + F16Z - 1) / F16Z) * ((h + F16Y - 1) / F16Y) * ((w + F16X - 1) / F16X);
-    // persistent workgroups: about four per CU across the co tiles
-    const int64_t per_co = std::max<int64_t>(1, (int64_t)4 * device_cus() / (co / 16));
+    // persistent workgroups: about two (of eight waves) per CU across the co tiles
+    const int64_t per_co = std::max<int64_t>(1, (int64_t)2 * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(256), 0, s, x, xs, wpack, bias, y, ys, gate,
+    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, wpack, bias, y, ys, gate,
                        gs, d, h, w, ci, (int)tiles, relu, accumulate);
     ABX_HIP(hipGetLastError());
     return 0;
