@@ -740,6 +740,30 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
   if (bias_wave && lr == 0) bias_store(pp + nw + co0 + bwt * 16, accb, lg);
 }
 
+// First pass of the partial-sum reduction, in place: row c * chunk of part
+// becomes the sum of rows c * chunk .. c * chunk + chunk - 1 (the last chunk
+// may be shorter), element by element in row order.  One thread per
+// (element, chunk), so every CU takes part; k_reduce_parts then sums the
+// S / chunk leading rows.
+__global__ __launch_bounds__(256) void k_sum_chunks(float* __restrict__ part, int S, int64_t n, int64_t stride,
+                                                    int chunk) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int s0 = (int)blockIdx.y * chunk, s1 = min(S, s0 + chunk);
+  float* p = part + (int64_t)s0 * stride + i;
+  float acc = 0.f;
+  int s = s0;
+  for (; s + 8 <= s1; s += 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(s - s0 + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; s < s1; ++s) acc += p[(int64_t)(s - s0) * stride];
+  *p = acc;
+}
+
 // r[i] = (accum ? r[i] : 0) + sum_s part[s*stride + i] for i < n, where r is
 // out for i < n1 and out2 (from index n1 on) otherwise.  A workgroup owns 64
 // consecutive i; its 4 waves sum the partials s = g, g+4, g+8, ... (8
@@ -1029,9 +1053,11 @@ WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
   const int ngr = (taps + s.tg - 1) / s.tg;
   s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci)) * ngr;
   const int64_t nt = tiles_of(n, d, h, w);
-  // enough workgroups to fill the GPU, but at most about 4 M partial sums
-  const int64_t by_size = std::max<int64_t>(1, ((int64_t)4 << 20) / ((int64_t)co * taps * ci + co));
-  s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, (int64_t)std::max(1, 2048 / s.blocks_y), by_size}));
+  // enough workgroups to hide each tile's load latency, but at most about 8 M
+  // partial sums (4 M and 16 M measured slower in a training step: fewer
+  // workgroups wait on their loads, more partials cost their reduction)
+  const int64_t by_size = std::max<int64_t>(1, ((int64_t)8 << 20) / ((int64_t)co * taps * ci + co));
+  s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, (int64_t)std::max(1, 8192 / s.blocks_y), by_size}));
   return s;
 }
 
@@ -1240,9 +1266,19 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
 #undef ABX_WG
 #undef ABX_LAUNCH
   ABX_HIP(hipGetLastError());
-  // weights and bias in one reduction (rows of nw + co partials)
+  // weights and bias in one reduction (rows of nw + co partials): chunks of
+  // kChunk rows summed in place, then the chunk sums
   const int64_t nred = nw + (db ? co : 0);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nred, 64)), dim3(256), 0, s, part, sh.S, nred, nw + co, nw, dw,
+  constexpr int kChunk = 32;
+  int rows = sh.S;
+  int64_t stride = nw + co;
+  if (sh.S > kChunk) {
+    rows = (sh.S + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nred), (unsigned)rows), dim3(256), 0, s, part, sh.S, nred, stride,
+                       kChunk);
+    stride *= kChunk;
+  }
+  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nred, 64)), dim3(256), 0, s, part, rows, nred, stride, nw, dw,
                      db, accumulate);
   ABX_HIP(hipGetLastError());
   return 0;
