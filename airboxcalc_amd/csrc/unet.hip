@@ -877,6 +877,160 @@ __global__ void k_maxpool2_bwd(const u16* __restrict__ x, int xs, const u16* __r
   }
 }
 
+// The pooling-window kernels below with x-pair lanes: thread (coarse voxel,
+// half c, 8-channel group g), g fastest, then c, then the coarse voxel, so
+// consecutive lanes read consecutive 16-byte pieces of a fine row (fine x =
+// 2 cx + c).  The two halves of a coarse voxel lie G lanes apart in one wave
+// (2G divides 64) and are combined by a lane shuffle.  Indices are 32-bit:
+// the host uses these kernels when the thread count fits.
+struct PoolIx {
+  int n, cz, cy, cx, c, g;
+};
+
+__device__ __forceinline__ PoolIx pool_ix(uint32_t i, int G, int d, int h, int w) {
+  PoolIx r;
+  r.g = (int)(i % (uint32_t)G);
+  i /= (uint32_t)G;
+  r.c = (int)(i & 1u);
+  i >>= 1;
+  r.cx = (int)(i % (uint32_t)w);
+  i /= (uint32_t)w;
+  r.cy = (int)(i % (uint32_t)h);
+  i /= (uint32_t)h;
+  r.cz = (int)(i % (uint32_t)d);
+  r.n = (int)(i / (uint32_t)d);
+  return r;
+}
+
+// k_maxpool2 with x-pair lanes (same result: the maximum is taken in any order).
+__global__ __launch_bounds__(256) void k_maxpool2_lx(const u16* __restrict__ x, int xs, int N, int D, int H, int W,
+                                                     int C, u16* __restrict__ y, int ys, uint32_t total) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  const int G = C >> 3, d = D >> 1, h = H >> 1, w = W >> 1;
+  const PoolIx p = pool_ix(i, G, d, h, w);
+  const int partner = (int)(threadIdx.x & 63) + (p.c ? -G : G);
+  float m[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) m[k] = -__builtin_inff();
+  u16x8 a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t fv = (((int64_t)p.n * D + 2 * p.cz + (j >> 1)) * H + 2 * p.cy + (j & 1)) * W + 2 * p.cx + p.c;
+    a[j] = *(const u16x8*)(x + fv * xs + 8 * p.g);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m[k] = fmaxf(m[k], bf2f(a[j][k]));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) m[k] = fmaxf(m[k], __shfl(m[k], partner, 64));
+  if (p.c) return;
+  u16x8 o8;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o8[k] = f2bf(m[k]);
+  const int64_t v = (((int64_t)p.n * d + p.cz) * h + p.cy) * w + p.cx;
+  *(u16x8*)(y + v * ys + 8 * p.g) = o8;
+}
+
+// k_maxpool2_bwd with x-pair lanes: each half finds its first maximum in the
+// fine order (z, y, x), the pair keeps the larger, the earlier on a tie, and
+// each lane writes its own four fine voxels: the same values as k_maxpool2_bwd.
+__global__ __launch_bounds__(256) void k_maxpool2_bwd_lx(const u16* __restrict__ x, int xs, const u16* __restrict__ dy,
+                                                         int dys, int N, int D, int H, int W, int C,
+                                                         u16* __restrict__ dx, int dxs, int accum, int gate,
+                                                         uint32_t total) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  const int G = C >> 3, d = D >> 1, h = H >> 1, w = W >> 1;
+  const PoolIx p = pool_ix(i, G, d, h, w);
+  const int partner = (int)(threadIdx.x & 63) + (p.c ? -G : G);
+  float m[8];
+  int am[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    m[k] = -__builtin_inff();
+    am[k] = 0;
+  }
+  int64_t fv[4];
+  u16x8 xa[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    fv[j] = (((int64_t)p.n * D + 2 * p.cz + (j >> 1)) * H + 2 * p.cy + (j & 1)) * W + 2 * p.cx + p.c;
+    xa[j] = *(const u16x8*)(x + fv[j] * xs + 8 * p.g);
+  }
+  const int64_t v = (((int64_t)p.n * d + p.cz) * h + p.cy) * w + p.cx;
+  const u16x8 gy = *(const u16x8*)(dy + v * dys + 8 * p.g);
+  // the values accumulated onto, loaded with the rest: a load right before
+  // its store would wait out a round trip per fine voxel
+  u16x8 cur[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    cur[j] = accum ? *(const u16x8*)(dx + fv[j] * dxs + 8 * p.g) : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (bf2f(xa[j][k]) > m[k]) {
+        m[k] = bf2f(xa[j][k]);
+        am[k] = 2 * j + p.c;  // window order o = (dz << 2) | (dy << 1) | dx
+      }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float mo = __shfl(m[k], partner, 64);
+    const int ao = __shfl(am[k], partner, 64);
+    if (mo > m[k] || (mo == m[k] && ao < am[k])) am[k] = ao;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    u16* q = dx + fv[j] * dxs + 8 * p.g;
+    u16x8 out;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      float val = accum ? bf2f(cur[j][k]) : 0.f;
+      if (am[k] == 2 * j + p.c) val += bf2f(gy[k]);
+      out[k] = gate && !(bf2f(xa[j][k]) > 0.f) ? (u16)0 : f2bf(val);
+    }
+    *(u16x8*)q = out;
+  }
+}
+
+// k_upsample2_bwd with x-pair lanes (D, H, W coarse): each half sums its four
+// fine voxels in window order, the pair adds the two sums (FP32).
+__global__ __launch_bounds__(256) void k_upsample2_bwd_lx(const u16* __restrict__ dy, int dys, int N, int D, int H,
+                                                          int W, int C, u16* __restrict__ dx, int dxs,
+                                                          const u16* __restrict__ gate, int gs, uint32_t total) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  const int G = C >> 3;
+  const PoolIx p = pool_ix(i, G, D, H, W);
+  const int partner = (int)(threadIdx.x & 63) + (p.c ? -G : G);
+  u16x8 a[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t fv =
+        (((int64_t)p.n * 2 * D + 2 * p.cz + (j >> 1)) * (2 * H) + 2 * p.cy + (j & 1)) * (2 * W) + 2 * p.cx + p.c;
+    a[j] = *(const u16x8*)(dy + fv * dys + 8 * p.g);
+  }
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += bf2f(a[j][k]);
+  float t[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) t[k] = __shfl(s[k], partner, 64);
+  if (p.c) return;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] += t[k];
+  const int64_t v = (((int64_t)p.n * D + p.cz) * H + p.cy) * W + p.cx;
+  const u16x8 gv = gate ? *(const u16x8*)(gate + v * gs + 8 * p.g) : (u16x8){0, 0, 0, 0, 0, 0, 0, 0};
+  u16x8 o8;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o8[k] = gate && !(bf2f(gv[k]) > 0.f) ? (u16)0 : f2bf(s[k]);
+  *(u16x8*)(dx + v * dxs + 8 * p.g) = o8;
+}
+
 // Nearest 2x upsampling; one thread per (fine voxel, 8-channel group).
 // D, H, W: coarse grid.
 __global__ void k_upsample2(const u16* __restrict__ x, int xs, int N, int D, int H, int W, int C, u16* __restrict__ y,
@@ -890,6 +1044,19 @@ __global__ void k_upsample2(const u16* __restrict__ x, int xs, int N, int D, int
   const int n = (int)(v / ((int64_t)Wf * Hf * Df));
   const int64_t cv = (((int64_t)n * D + (fz >> 1)) * H + (fy >> 1)) * W + (fx >> 1);
   *(u16x8*)(y + v * ys + 8 * g) = *(const u16x8*)(x + cv * xs + 8 * g);
+}
+
+// k_upsample2 with 32-bit index arithmetic (the host uses it when the thread
+// count fits): the same copy, without 64-bit divisions per thread.
+__global__ __launch_bounds__(256) void k_upsample2_32(const u16* __restrict__ x, int xs, int N, int D, int H, int W,
+                                                      int C, u16* __restrict__ y, int ys, uint32_t total) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= total) return;
+  const uint32_t G = (uint32_t)C >> 3, Wf = 2u * W, Hf = 2u * H, Df = 2u * D;
+  const uint32_t g = i % G, v = i / G;
+  const uint32_t fx = v % Wf, r = v / Wf, fy = r % Hf, r2 = r / Hf, fz = r2 % Df, n = r2 / Df;
+  const int64_t cv = (((int64_t)n * D + (fz >> 1)) * H + (fy >> 1)) * W + (fx >> 1);
+  *(u16x8*)(y + (int64_t)v * ys + 8 * g) = *(const u16x8*)(x + cv * xs + 8 * g);
 }
 
 // Gradient of k_upsample2: dx[coarse] = sum of the 8 fine dy (FP32 sum).
@@ -1285,6 +1452,13 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
 }
 
 namespace {
+// The x-pair-lane kernels need a coarse voxel's 2G lanes inside one wave
+// (2G divides 64) and a 32-bit thread index.
+bool pair_lanes_ok(int c, int64_t tot) {
+  const int G = c / 8;
+  return 64 % (2 * G) == 0 && 2 * tot < ((int64_t)1 << 31);
+}
+
 bool ew_ok(const char* fn, std::initializer_list<const void*> ptrs, std::initializer_list<int> strides, int c) {
   for (const void* p : ptrs)
     if (!p || !aligned16(p)) {
@@ -1312,7 +1486,12 @@ extern "C" int abx_maxpool2(void* stream, const uint16_t* x, int xs, int n, int 
     return -1;
   }
   const int64_t tot = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * (c / 8);
-  hipLaunchKernelGGL(k_maxpool2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y, ys);
+  if (pair_lanes_ok(c, tot))
+    hipLaunchKernelGGL(k_maxpool2_lx, dim3(blocks_for(2 * tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w,
+                       c, y, ys, (uint32_t)(2 * tot));
+  else
+    hipLaunchKernelGGL(k_maxpool2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y,
+                       ys);
   ABX_HIP(hipGetLastError());
   return 0;
 }
@@ -1326,8 +1505,12 @@ int maxpool2_backward(const char* fn, void* stream, const uint16_t* x, int xs, c
     return -1;
   }
   const int64_t tot = (int64_t)n * (d / 2) * (h / 2) * (w / 2) * (c / 8);
-  hipLaunchKernelGGL(k_maxpool2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, dy, dys, n, d,
-                     h, w, c, dx, dxs, accumulate, gate);
+  if (pair_lanes_ok(c, tot))
+    hipLaunchKernelGGL(k_maxpool2_bwd_lx, dim3(blocks_for(2 * tot)), dim3(256), 0, (hipStream_t)stream, x, xs, dy, dys,
+                       n, d, h, w, c, dx, dxs, accumulate, gate, (uint32_t)(2 * tot));
+  else
+    hipLaunchKernelGGL(k_maxpool2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, dy, dys, n, d,
+                       h, w, c, dx, dxs, accumulate, gate);
   ABX_HIP(hipGetLastError());
   return 0;
 }
@@ -1340,8 +1523,12 @@ int upsample2_backward(const char* fn, void* stream, const uint16_t* dy, int dys
     return -1;
   }
   const int64_t tot = (int64_t)n * d * h * w * (c / 8);
-  hipLaunchKernelGGL(k_upsample2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d, h, w,
-                     c, dx, dxs, gate, gs);
+  if (pair_lanes_ok(c, tot))
+    hipLaunchKernelGGL(k_upsample2_bwd_lx, dim3(blocks_for(2 * tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d,
+                       h, w, c, dx, dxs, gate, gs, (uint32_t)(2 * tot));
+  else
+    hipLaunchKernelGGL(k_upsample2_bwd, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, dy, dys, n, d, h, w,
+                       c, dx, dxs, gate, gs);
   ABX_HIP(hipGetLastError());
   return 0;
 }
@@ -1366,8 +1553,12 @@ extern "C" int abx_upsample2(void* stream, const uint16_t* x, int xs, int n, int
     return -1;
   }
   const int64_t tot = (int64_t)n * 8 * d * h * w * (c / 8);
-  hipLaunchKernelGGL(k_upsample2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y,
-                     ys);
+  if (tot < ((int64_t)1 << 31))
+    hipLaunchKernelGGL(k_upsample2_32, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c,
+                       y, ys, (uint32_t)tot);
+  else
+    hipLaunchKernelGGL(k_upsample2, dim3(blocks_for(tot)), dim3(256), 0, (hipStream_t)stream, x, xs, n, d, h, w, c, y,
+                       ys);
   ABX_HIP(hipGetLastError());
   return 0;
 }

@@ -394,10 +394,13 @@ def _distinct_windows(n, d, h, w, c, rng):
     return (sign * (base * 8 + pos)).astype(np.float32)
 
 
-def test_pool_and_upsample_match_torch():
+@pytest.mark.parametrize("c", [24, 16, 32, 64])
+def test_pool_and_upsample_match_torch(c):
+    """Pooling and upsampling against torch; 16, 32 and 64 channels take the
+    x-pair-lane kernels, 24 the per-voxel ones."""
     import torch.nn.functional as F
-    n, d, h, w, c = 2, 4, 6, 10, 24
-    say("pool / upsample")
+    n, d, h, w = 2, 4, 6, 10
+    say("pool / upsample", c)
     rng = np.random.default_rng(5)
     lib = abx.load()
     X = bf(_distinct_windows(n, d, h, w, c, rng))
@@ -423,6 +426,32 @@ def test_pool_and_upsample_match_torch():
     assert np.array_equal(f32(U), Ur.detach().permute(0, 2, 3, 4, 1).cpu().numpy())
     Ur.backward(dU.float().permute(0, 4, 1, 2, 3).contiguous())
     close_bf16(f32(dXu), Xr2.grad.permute(0, 2, 3, 4, 1).cpu().numpy(), "upsample backward")
+
+
+@pytest.mark.parametrize("c", [16, 24, 32])
+def test_maxpool_backward_ties_go_to_the_first_maximum(c):
+    """With ties in every window (values 0, 1, 2), the gradient goes to the
+    first maximum in the fine order (z, y, x), whichever kernel runs; the
+    rest of the window gets zero."""
+    n, d, h, w = 2, 4, 6, 8
+    say("maxpool backward ties", c)
+    rng = np.random.default_rng(17)
+    xv = rng.integers(0, 3, (n, d, h, w, c)).astype(np.float32)
+    gv = rng.standard_normal((n, d // 2, h // 2, w // 2, c)).astype(np.float32)
+    X, dY = bf(xv), bf(gv)
+    gv = f32(dY)
+    dX = torch.empty_like(X)
+    ck(abx.load().abx_maxpool2_backward(stream(), X.data_ptr(), c, dY.data_ptr(), c, n, d, h, w, c, dX.data_ptr(), c,
+                                        0))
+    torch.cuda.synchronize()
+    ref = np.zeros((n, d, h, w, c), np.float32)
+    win = xv.reshape(n, d // 2, 2, h // 2, 2, w // 2, 2, c).transpose(0, 1, 3, 5, 2, 4, 6, 7)
+    win = win.reshape(n, d // 2, h // 2, w // 2, 8, c)
+    first = win.argmax(axis=4)          # numpy returns the first maximum
+    for o in range(8):
+        a, b, e = o >> 2, (o >> 1) & 1, o & 1
+        ref[:, a::2, b::2, e::2, :] = np.where(first == o, gv, 0.0)
+    assert np.array_equal(f32(dX), ref)
 
 
 def _bfr(a):
