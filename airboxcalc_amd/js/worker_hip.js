@@ -29,8 +29,19 @@ const getDev = () => {
 };
 const post = (m, tr) => parentPort.postMessage(m, tr || []);
 
+/* The trained networks loaded so far, by weight file. */
+const nets = new Map();
+const surrogate = (d, which) => {
+  const file = H.SURROGATE_WEIGHTS[which] || which;
+  if (!nets.has(file)) nets.set(file, H.loadSurrogate(d, file));
+  return nets.get(file);
+};
+
 /* The run parameters are the reference's (engine.js:765-779); a caller may
-   add gateTolerance to tighten the self-test gate (default 1e-3). */
+   add gateTolerance to tighten the self-test gate (default 1e-3), warmStart
+   ("family", "dx2" or a weight file: start from the surrogate's prediction)
+   and criterion ({window, tolGamma, tolZeta, maxSteps}: run to the steady
+   criterion instead of steps / avg; the result then carries .steady). */
 async function runSim(p) {
   const prog = (info) => post({ type: "progress", info });
   const d = getDev();
@@ -40,7 +51,13 @@ async function runSim(p) {
   const md = H.gpuSelfTestHIP(d, geo, p);
   if (!(md < tol)) throw new Error("Abweichung GPU/CPU " + md.toExponential(1));
   post({ type: "status", text: "Rechnet auf der Grafikkarte (" + d.name + ")" });
-  const res = await H.simulateHIP(d, geo, p, prog, null);
+  let opts = null;
+  if (p.warmStart || p.criterion) {
+    opts = {};
+    if (p.warmStart) opts.warmStart = surrogate(d, p.warmStart);
+    if (p.criterion) opts.criterion = p.criterion;
+  }
+  const res = await H.simulateHIP(d, geo, p, prog, null, opts);
   res.device = "GPU · " + d.name;
   return res;
 }

@@ -38,14 +38,18 @@ const stlBuffer = () => { const b = fs.readFileSync(C.stl); return b.buffer.slic
     const rs = await ask({ type: "run", params: C.run });
     if (rs.type === "result") {
       const r = rs.res;
-      out.result = { metrics: r.metrics, hist: r.hist, device: r.device, sha: {} };
+      out.result = { metrics: r.metrics, hist: r.hist, device: r.device, steady: r.steady || null, sha: {} };
       for (const k of ["spd", "pst", "loss"]) out.result.sha["vol_" + k] = sha(r.vol[k]);
       for (const k of ["pts", "spd", "off"]) out.result.sha["lines_" + k] = sha(r.lines[k]);
       // the direct call in this thread, for comparison
       const tris = H.parseSTL(stlBuffer());
       const g = H.prepare(Object.assign({ tris }, C.prepare), H.hipDevice(0));
-      const d = await H.runHIP(g, C.run, null);
-      out.direct = { metrics: d.metrics, hist: d.hist, sha: {} };
+      const opts = {};
+      if (C.run.warmStart)
+        opts.warmStart = H.loadSurrogate(H.hipDevice(0), H.SURROGATE_WEIGHTS[C.run.warmStart] || C.run.warmStart);
+      if (C.run.criterion) opts.criterion = C.run.criterion;
+      const d = await H.runHIP(g, C.run, null, opts);
+      out.direct = { metrics: d.metrics, hist: d.hist, steady: d.steady || null, sha: {} };
       for (const k of ["spd", "pst", "loss"]) out.direct.sha["vol_" + k] = sha(d.vol[k]);
       for (const k of ["pts", "spd", "off"]) out.direct.sha["lines_" + k] = sha(d.lines[k]);
     }

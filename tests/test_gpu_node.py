@@ -98,3 +98,30 @@ def test_worker_protocol_run_matches_direct_call(tmp_path):
     assert out["progress"] == len(r["hist"])
     assert r["metrics"] == dr["metrics"] and r["hist"] == dr["hist"]
     assert r["sha"] == dr["sha"]
+
+
+def test_worker_warm_start_and_criterion_match_direct_call(tmp_path):
+    """The worker's run message may ask for the surrogate warm start and the
+    steady criterion (warmStart, criterion in the run parameters): its result
+    equals runHIP called directly with the same options, bit for bit,
+    including the criterion's windows."""
+    fx = GU.load("small_yaw")
+    d = fx["desc"]
+    stl = str(tmp_path / "box.stl")
+    G.write_stl_binary(stl, G.airbox_triangles(G.spec_from_dict(d["spec"])))
+    run = dict(d["run"], warmStart="family", criterion={"window": 200, "maxSteps": 2000})
+    cj = str(tmp_path / "case.json")
+    with open(cj, "w") as fh:
+        json.dump({"stl": stl, "prepare": d["prepare"], "run": run}, fh)
+    print("[gpu] node worker warm start + criterion", flush=True)
+    p = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "run_worker_case.js"), cj], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = json.loads(p.stdout)
+    assert not out["errors"], out["errors"]
+    r, dr = out["result"], out["direct"]
+    assert r["steady"] is not None and r["steady"] == dr["steady"]
+    assert r["steady"]["steps"] <= 2000 and len(r["steady"]["windows"]) == len(r["hist"])
+    assert r["metrics"] == dr["metrics"] and r["hist"] == dr["hist"]
+    assert r["sha"] == dr["sha"]
+

@@ -3,9 +3,9 @@
 tests/sanitize/Makefile builds the library's host sources (abx_core.cpp: error
 state, device query, STL reader; prepare_host.cpp: the host stages of
 abx_prepare) and the oracle's C restatement with -fsanitize=address,undefined
-into one driver, tests/sanitize/host_checks.cpp.  Any sanitizer report aborts
-the driver (-fno-sanitize-recover=all), so every case here fails on a memory
-error, a leak or undefined behaviour as well as on a mismatch.
+into one checking program, tests/sanitize/host_checks.cpp.  Any sanitizer
+report aborts it (-fno-sanitize-recover=all), so every case here fails on a
+memory error, a leak or undefined behaviour as well as on a mismatch.
 
 - prepare: the product's host stages, with CPU stand-ins for the GPU stages
   (vote, slice, duct, floor, flood fill), equal the oracle's prepare array for
