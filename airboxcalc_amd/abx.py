@@ -172,6 +172,7 @@ def load() -> C.CDLL:
         "abx_sim_set_ns": (i32, [vp, vp]),
         "abx_sim_sync": (i32, [vp]),
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32]),
+        "abx_sim_pattern_time": (i32, [vp, i32, C.POINTER(d), C.POINTER(d)]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_sim_destroy": (None, [vp]),
         "abx_sim_reset": (i32, [vp]),
@@ -574,6 +575,13 @@ class Sim:
         the AA steps, even step in bits 0-7, odd step in bits 8-15
         (abx_sim_set_launch; default 19)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, nt))
+
+    def pattern_time(self, iters: int = 20) -> tuple:
+        """(ms even, ms odd): the AA steps' memory pattern alone on this
+        handle's lattice (abx_sim_pattern_time).  Moves the populations."""
+        me, mo = C.c_double(), C.c_double()
+        _check(load().abx_sim_pattern_time(self.h, iters, C.byref(me), C.byref(mo)))
+        return me.value, mo.value
 
     def set_graph(self, enable: bool = True) -> None:
         _check(load().abx_sim_set_graph(self.h, 1 if enable else 0))

@@ -74,6 +74,23 @@ __global__ __launch_bounds__(256) void k_stream_collide_seg(StepArgs a) {
   if (sg < a.nchunks) seg_cell<S, M, ACC, NS, MODE, NT>(a, a.chunks[sg], l, blockIdx.y);
 }
 
+// The AA steps' memory pattern alone on a handle's lattice (abx_sim_pattern_time):
+// k_stream_collide_seg's loads and stores with the collision left out.
+template <int MODE, int NT>
+__global__ __launch_bounds__(256) void k_pattern_seg(StepArgs a) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int nblk = (a.nchunks + 3) >> 2;
+  int blk = (int)blockIdx.x;
+  if (a.xcd_map == 1) blk = (int)(blockIdx.x & 7) * a.per_xcd + (int)(blockIdx.x >> 3);
+  if (a.xcd_map > 1) {
+    const int sq = (int)(blockIdx.x >> 3), G = a.xcd_map;
+    blk = (sq / G) * 8 * G + (int)(blockIdx.x & 7) * G + sq % G;
+  }
+  if (blk >= nblk) return;
+  const int sg = 4 * blk + wv;
+  if (sg < a.nchunks) seg_cell<float, float, false, false, MODE, NT, true>(a, a.chunks[sg], l, 0);
+}
+
 template <typename S, typename M>
 __global__ void k_probe(const S* f, int64_t P, const int32_t* plist, int np, double* vy, LatView L, int form) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,13 +371,18 @@ StepArgs step_args(abx_sim* s, int doacc) {
   return a;
 }
 
-template <typename S, typename M, int MODE, int NT>
-void launch_seg(abx_sim* s, StepArgs a, int doacc) {
+// Grid of the segment kernels for a's block order (sets a.per_xcd).
+unsigned seg_grid_x(StepArgs& a) {
   const int nblk = (a.nchunks + 3) / 4;
   a.per_xcd = (nblk + 7) / 8;
   unsigned gx = a.xcd_map == 1 ? (unsigned)(8 * a.per_xcd) : (unsigned)nblk;
   if (a.xcd_map > 1) gx = (unsigned)((nblk + 8 * a.xcd_map - 1) / (8 * a.xcd_map) * (8 * a.xcd_map));
-  dim3 grid(gx, (unsigned)s->B), block(256);
+  return gx;
+}
+
+template <typename S, typename M, int MODE, int NT>
+void launch_seg(abx_sim* s, StepArgs a, int doacc) {
+  dim3 grid(seg_grid_x(a), (unsigned)s->B), block(256);
   const bool ns = a.ns != nullptr;
   if (doacc) {
     if (ns)
@@ -1143,6 +1165,56 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
   return 0;
 }
 
+
+extern "C" int abx_sim_pattern_time(abx_sim* s, int iters, double* ms_even, double* ms_odd) {
+  if (!s || iters < 1 || !ms_even || !ms_odd) {
+    set_error("abx_sim_pattern_time: bad argument");
+    return -1;
+  }
+  if (s->kernel != 4 || s->prec == ABX_F64 || s->B != 1) {
+    set_error("abx_sim_pattern_time: needs a single float-storage handle on the AA pattern");
+    return -1;
+  }
+  ABX_HIP(hipSetDevice(s->dev->ordinal));
+  sim_drop_graphs(s);
+  StepArgs a = step_args(s, 0);
+  const dim3 grid(seg_grid_x(a)), block(256);
+  hipEvent_t ev[3];
+  for (auto& e : ev) ABX_HIP(hipEventCreate(&e));
+  // pairs of steps in the order the stored form asks for, so the form after
+  // every pair is the one before (the populations themselves are moved)
+  const bool even_first = s->form == 2;
+  float te = 0, to = 0;
+  int rc = 0;
+  for (int i = 0; i <= iters && !rc; ++i) {  // pair 0 warms up, untimed
+    for (int k = 0; k < 2; ++k) {
+      const bool even = (k == 0) == even_first;
+      if (hipEventRecord(ev[k], s->stream) != hipSuccess) rc = -1;
+      if (even)
+        hipLaunchKernelGGL((k_pattern_seg<1, 19>), grid, block, 0, s->stream, a);
+      else
+        hipLaunchKernelGGL((k_pattern_seg<2, 0>), grid, block, 0, s->stream, a);
+    }
+    if (hipEventRecord(ev[2], s->stream) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
+        hipGetLastError() != hipSuccess)
+      rc = -1;
+    float m0 = 0, m1 = 0;
+    (void)hipEventElapsedTime(&m0, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&m1, ev[1], ev[2]);
+    if (i > 0) {
+      te += even_first ? m0 : m1;
+      to += even_first ? m1 : m0;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) {
+    set_error("abx_sim_pattern_time: launch failed");
+    return -1;
+  }
+  *ms_even = te / iters;
+  *ms_odd = to / iters;
+  return 0;
+}
 
 extern "C" void abx_sim_destroy(abx_sim* s) { sim_free(s); }
 

@@ -359,8 +359,9 @@ __device__ __forceinline__ void for_each_q(F&& f) {
 // keeping the lines): bit 0 / bit 1 = population loads / stores at the
 // cell's own position (every population of the even AA step), bit 2 / bit 3
 // = the odd step's loads / stores at neighbour positions, bit 4 = the
-// averaged fields, bit 5 = the meta words.
-template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0>
+// averaged fields, bit 5 = the meta words.  MOVE: no collision, every
+// population leaves as it came (the step's memory pattern alone, for timing).
+template <typename S, typename M, bool ACC, bool NS, int MODE, int NT = 0, bool MOVE = false>
 __device__ __forceinline__ void seg_cell(const StepArgs& a, int seg, int l, int b) {
   const int d = seg + l;
   // AA steps: a non-fluid position (a solid cell inside a row's span, or
@@ -426,8 +427,13 @@ __device__ __forceinline__ void seg_cell(const StepArgs& a, int seg, int l, int 
   }
   M s = 0;
   if (NS) s = (M)a.ns[(int64_t)b * P + d];
-  M ux, uy, uz, rho;
-  collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+  M ux = 0, uy = 0, uz = 0, rho = 0;
+  if constexpr (MOVE) {
+#pragma unroll
+    for (int q = 0; q < 19; ++q) fo[q] = fi[q];
+  } else {
+    collide_any<M, NS>(fi, inlet, outlet, porous, s, a, fo, ux, uy, uz, rho);
+  }
   auto store = [&](int q, auto nt) {
     S* dst;
     if (MODE == 0) {

@@ -250,6 +250,14 @@ int abx_sim_sync(abx_sim* sim);
  * neighbouring rows' lines).  Every choice computes the same values bit for
  * bit. */
 int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
+/* Ceiling of the stream-collide's own memory pattern (no reference
+ * counterpart): average ms of the even and of the odd AA step's loads and
+ * stores on this handle's lattice, in its launch order, with the collision
+ * left out (every population moves as it came), over iters pairs of steps.
+ * Default non-temporal flags.  Needs a single float-storage handle on the AA
+ * pattern; it moves the populations (a streaming without collision), so the
+ * state afterwards is not a solve's: use a handle made for it, or reset. */
+int abx_sim_pattern_time(abx_sim* sim, int iters, double* ms_even, double* ms_odd);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
 int abx_sim_set_graph(abx_sim* sim, int enable);

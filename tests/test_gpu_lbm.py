@@ -313,3 +313,22 @@ def test_gpu_evaluate_field3d_streamlines(name):
     for k in ("pts", "spd", "off"):
         assert GU.sha(L[k]) == fx["sha"][f"sim_lines_{k}"], k
     sim.destroy()
+
+
+def test_pattern_time_moves_populations_without_collision():
+    """abx_sim_pattern_time runs the AA steps' loads and stores with the
+    collision left out: it returns positive per-step times, and the fluid
+    populations afterwards are the same values, moved (a streaming with
+    bounce-back only permutes them)."""
+    fx = GU.load("small_yaw")
+    say("pattern time")
+    _, sim = _sim(fx, "f32")
+    rng = np.random.default_rng(21)
+    f = rng.uniform(0.0, 1.0, 19 * sim.NF).astype(np.float32)
+    sim.write_f(f)
+    me, mo = sim.pattern_time(3)
+    assert me > 0 and mo > 0
+    g = sim.read_f()
+    assert not np.array_equal(g, f)
+    assert np.array_equal(np.sort(g), np.sort(f))
+    sim.destroy()
