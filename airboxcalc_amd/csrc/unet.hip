@@ -740,6 +740,171 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
   if (bias_wave && lr == 0) bias_store(pp + nw + co0 + bwt * 16, accb, lg);
 }
 
+// Weight gradient of a 3x3x3 convolution with every tap in one workgroup.
+// The output tile is 16 x 4 x 4 voxels (eight k steps of 32), its input
+// brick 18 x 6 x 6.  Workgroup (blockIdx.x = partial row, blockIdx.y =
+// block) owns output channels [co0, co0 + 16 NCT) and input channels
+// [ci0, ci0 + 16 NCI) for all 27 taps, so each tile is staged once per
+// block and not once per tap group.  Its 27 NCI (tap, input tile) fragments
+// are dealt to the 8 waves (item q = wave + 8 u); each meets the NCT dZ
+// fragments of the k step, so one B fragment read feeds NCT MFMAs.  The next
+// tile's global loads are issued before this tile's MFMAs and land in
+// registers, which the next iteration stores to LDS.  Fragments are built
+// with transposing LDS reads as in k_conv_wgrad_tr; partial rows
+// [S][Co 27 Ci + Co] as there.  Bias partials come from the ci block 0
+// workgroups: their wave 7 multiplies every dZ fragment by a fragment of
+// ones (27 NCI is never a multiple of 8, so wave 7 has one item less than
+// wave 0, and waves skip the MFMAs of an item they do not own).
+constexpr int W27X = 16, W27Y = 4, W27Z = 4, W27V = W27X * W27Y * W27Z;
+constexpr int W27BX = W27X + 2, W27BY = W27Y + 2, W27BZ = W27Z + 2, W27BV = W27BX * W27BY * W27BZ;
+
+__device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W) {
+  const int ntx = (W + W27X - 1) / W27X, nty = (H + W27Y - 1) / W27Y, ntz = (D + W27Z - 1) / W27Z;
+  Tile r;
+  r.x0 = (t % ntx) * W27X;
+  t /= ntx;
+  r.y0 = (t % nty) * W27Y;
+  t /= nty;
+  r.z0 = (t % ntz) * W27Z;
+  r.n = t / ntz;
+  return r;
+}
+
+template <int NCT, int NCI>
+__global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* __restrict__ dz, int dzs,
+                                                 float* __restrict__ part, float* __restrict__ partb, int D, int H,
+                                                 int W, int Ci, int Co, int ntiles) {
+  constexpr int CO = 16 * NCT, CI = 16 * NCI, PZ = CO, PX = CI;
+  constexpr int NI = 27 * NCI, MAXI = (NI + 7) / 8;
+  constexpr int ZP = W27V * NCT * 2, XP = W27BV * NCI * 2;  // 16-byte pieces of the dZ tile and the brick
+  constexpr int ZSL = (ZP + 511) / 512, XSL = (XP + 511) / 512;
+  static_assert(ZP % 512 == 0, "dZ pieces fill whole slots");
+  __shared__ __attribute__((aligned(16))) u16 sz[W27V * PZ];
+  __shared__ __attribute__((aligned(16))) u16 sx[W27BV * PX];
+  const int ncb = Co / CO;
+  const int cb = (int)blockIdx.y % ncb, cc = (int)blockIdx.y / ncb;
+  const int co0 = cb * CO, ci0 = cc * CI;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  static_assert(NI % 8 != 0, "wave 7 has a spare item slot for the bias");
+  const bool bias_wave = partb && cc == 0 && wave == 7;
+  const bool last_item = wave + 8 * (MAXI - 1) < NI;
+  const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
+
+  // this wave's items: brick offset of the tap plus the input tile's column
+  int boff[MAXI];
+#pragma unroll
+  for (int u = 0; u < MAXI; ++u) {
+    const int q = wave + 8 * u, qq = q < NI ? q : 0;
+    const int tap = qq / NCI, it = qq % NCI;
+    boff[u] = (((tap / 9) * W27BY + (tap / 3) % 3) * W27BX + tap % 3) * PX + it * 16;
+  }
+  // staging slots of this thread: dZ piece threadIdx + 512 m is voxel
+  // e / (2 NCT), channels 8 (e % (2 NCT)) .. + 8; brick piece likewise with
+  // the brick voxel packed as bx | by << 5 | bz << 8 | piece << 11
+  int xsl[XSL];
+#pragma unroll
+  for (int m = 0; m < XSL; ++m) {
+    const int e = min((int)threadIdx.x + 512 * m, XP - 1);
+    const int vb = e / (NCI * 2), q = e - vb * (NCI * 2);
+    const int bx = vb % W27BX, row = vb / W27BX, by = row % W27BY, bz = row / W27BY;
+    xsl[m] = bx | by << 5 | bz << 8 | q << 11;
+  }
+  const bool xlast = (int)threadIdx.x + 512 * (XSL - 1) < XP;
+  u32x4 zv[ZSL], xv[XSL];
+  uint32_t zok = 0, xok = 0;
+  // loads from clamped in-grid addresses; out-of-grid values are zeroed when stored
+  auto fetch = [&](int t) {
+    const Tile tl = w27_tile(t, D, H, W);
+    zok = 0;
+    xok = 0;
+#pragma unroll
+    for (int m = 0; m < ZSL; ++m) {
+      const int e = (int)threadIdx.x + 512 * m;
+      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
+      zok |= (uint32_t)(gx < W && gy < H && gz < D) << m;
+      zv[m] = *(const u32x4*)(dz + vox(tl, D, H, W, min(gz, D - 1), min(gy, H - 1), min(gx, W - 1)) * dzs + co0 +
+                              8 * q);
+    }
+#pragma unroll
+    for (int m = 0; m < XSL; ++m) {
+      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
+      const int gx = tl.x0 + bx - 1, gy = tl.y0 + by - 1, gz = tl.z0 + bz - 1;
+      xok |= (uint32_t)((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) << m;
+      xv[m] = *(const u32x4*)(x + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
+                                      min(max(gx, 0), W - 1)) * xs + ci0 + 8 * q);
+    }
+  };
+  const u32x4 zero4 = {0u, 0u, 0u, 0u};
+  const int lrow = lr >> 2, lcol = 4 * (lr & 3);
+  f32x4 acc[MAXI][NCT];
+#pragma unroll
+  for (int u = 0; u < MAXI; ++u)
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) acc[u][c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  f32x4 accb[NCT];
+#pragma unroll
+  for (int c = 0; c < NCT; ++c) accb[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  int t = blockIdx.x;
+  if (t < ntiles) fetch(t);
+  for (; t < ntiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's fragment reads are done
+#pragma unroll
+    for (int m = 0; m < ZSL; ++m) {
+      const int e = (int)threadIdx.x + 512 * m;
+      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+      *(u32x4*)(sz + v * PZ + 8 * q) = ((zok >> m) & 1u) ? zv[m] : zero4;
+    }
+#pragma unroll
+    for (int m = 0; m < XSL; ++m) {
+      if (m == XSL - 1 && !xlast) break;
+      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
+      *(u32x4*)(sx + ((bz * W27BY + by) * W27BX + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
+    }
+    __syncthreads();
+    if (t + (int)gridDim.x < ntiles) fetch(t + (int)gridDim.x);
+#pragma unroll 1
+    for (int s = 0; s < W27V / 32; ++s) {
+      // k = voxels 32 s + 8 lg + e (e = 0..7), consecutive in x
+      const int vb0 = 32 * s + 8 * lg;
+      const int tx0 = vb0 & 15, ty = (vb0 >> 4) & 3, tz = vb0 >> 6;
+      u32x4 av[NCT];
+#pragma unroll
+      for (int c = 0; c < NCT; ++c) av[c] = frag_tr(sz + (vb0 + lrow) * PZ + c * 16 + lcol, PZ);
+      const u16* lbase = sx + ((tz * W27BY + ty) * W27BX + tx0 + lrow) * PX + lcol;
+#pragma unroll
+      for (int u = 0; u < MAXI; ++u) {
+        const u32x4 b = frag_tr(lbase + boff[u], PX);
+        if (u + 1 < MAXI || last_item)
+#pragma unroll
+          for (int c = 0; c < NCT; ++c) acc[u][c] = mfma16(av[c], b, acc[u][c]);
+      }
+      if (bias_wave)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) accb[c] = mfma16(av[c], ones, accb[c]);
+    }
+  }
+
+  // C element i of lane l: output channel 4 lg + i, input channel lr of the tiles
+  const int64_t nw = (int64_t)Co * 27 * Ci;
+  float* pp = part + (int64_t)blockIdx.x * (nw + Co);
+#pragma unroll
+  for (int u = 0; u < MAXI; ++u) {
+    const int q = wave + 8 * u;
+    if (q >= NI) break;
+    const int tap = q / NCI, ci = ci0 + (q % NCI) * 16 + lr;
+#pragma unroll
+    for (int c = 0; c < NCT; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pp[((int64_t)(co0 + c * 16 + 4 * lg + i) * 27 + tap) * Ci + ci] = acc[u][c][i];
+  }
+  if (bias_wave && lr == 0)
+#pragma unroll
+    for (int c = 0; c < NCT; ++c) bias_store(pp + nw + co0 + c * 16, accb[c], lg);
+}
+
 // First pass of the partial-sum reduction, in place: row c * chunk of part
 // becomes the sum of rows c * chunk .. c * chunk + chunk - 1 (the last chunk
 // may be shorter), element by element in row order.  One thread per
@@ -1184,6 +1349,9 @@ int g_fwd16 = 1;
 // Weight gradients use k_conv_wgrad_tr (transposing LDS reads); 0 = the
 // gathering kernel k_conv_wgrad, for comparison.
 int g_wgrad_tr = 1;
+// 3x3x3 weight gradients use k_wgrad27 (every tap in one workgroup); 0 = the
+// tap-group kernels above, for comparison.
+int g_wgrad27 = 1;
 
 // Compute units of the current device (cached per device ordinal).
 int device_cus() {
@@ -1225,6 +1393,24 @@ WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
   // workgroups wait on their loads, more partials cost their reduction)
   const int64_t by_size = std::max<int64_t>(1, ((int64_t)8 << 20) / ((int64_t)co * taps * ci + co));
   s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, (int64_t)std::max(1, 8192 / s.blocks_y), by_size}));
+  return s;
+}
+
+// k_wgrad27's launch shape: channel tiles per workgroup and partial rows.
+struct W27Shape {
+  int nct, nci, S, blocks_y;
+};
+
+W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
+  W27Shape s;
+  s.nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
+  s.nci = (ci % 48 == 0 && s.nct <= 2) ? 3 : (ci % 32 == 0 ? 2 : 1);
+  s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci));
+  const int64_t nt = (int64_t)n * ((d + W27Z - 1) / W27Z) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X);
+  // about two workgroups per CU, at most about 16 M partial sums
+  const int64_t want = (2 * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
+  const int64_t by_size = std::max<int64_t>(1, ((int64_t)16 << 20) / ((int64_t)co * 27 * ci + co));
+  s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, want, by_size}));
   return s;
 }
 
@@ -1377,6 +1563,7 @@ extern "C" int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs,
 extern "C" int abx_conv3d_set_kernel(int flags) {
   g_fwd16 = (flags & 1) ? 1 : 0;
   g_wgrad_tr = (flags & 2) ? 1 : 0;
+  g_wgrad27 = (flags & 4) ? 0 : 1;
   return 0;
 }
 
@@ -1385,7 +1572,10 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
     return -1;
   const int T = ksize == 3 ? 27 : 1;
   const WgShape s = wgrad_shape(n, d, h, w, ci, co, T);
-  return (int64_t)s.S * ((int64_t)co * T * ci + co) * 4;
+  int64_t rows = s.S;
+  // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects
+  if (T == 27) rows = std::max<int64_t>(rows, wgrad27_shape(n, d, h, w, ci, co).S);
+  return rows * ((int64_t)co * T * ci + co) * 4;
 }
 
 extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d,
@@ -1398,16 +1588,34 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   }
   const int T = ksize == 3 ? 27 : 1;
   const WgShape sh = wgrad_shape(n, d, h, w, ci, co, T);
+  const bool use27 = T == 27 && g_wgrad27;
+  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 1, 1};
+  const int nrows = use27 ? s27.S : sh.S;
   const int64_t nw = (int64_t)co * T * ci;
-  if (ws_bytes < (int64_t)sh.S * (nw + co) * 4) {
+  if (ws_bytes < (int64_t)nrows * (nw + co) * 4) {
     set_errorf("abx_conv3d_wgrad: workspace of %lld bytes is smaller than the %lld needed", (long long)ws_bytes,
-               (long long)((int64_t)sh.S * (nw + co) * 4));
+               (long long)((int64_t)nrows * (nw + co) * 4));
     return -1;
   }
   float* part = workspace;              // rows of nw weight partials, then co bias partials
   float* partb = db ? workspace : nullptr;  // a flag: the bias partials are written
-  const int nt = (int)tiles_of(n, d, h, w);
   hipStream_t s = (hipStream_t)stream;
+  if (use27) {
+    const int nt27 = (int)((int64_t)n * ((d + W27Z - 1) / W27Z) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
+    const dim3 g27((unsigned)s27.S, (unsigned)s27.blocks_y);
+#define ABX_W27(A, B)                                                                                            \
+  hipLaunchKernelGGL((k_wgrad27<A, B>), g27, dim3(512), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt27)
+    if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2);
+    else if (s27.nct == 4) ABX_W27(4, 1);
+    else if (s27.nct == 2 && s27.nci == 3) ABX_W27(2, 3);
+    else if (s27.nct == 2 && s27.nci == 2) ABX_W27(2, 2);
+    else if (s27.nct == 2) ABX_W27(2, 1);
+    else if (s27.nci == 3) ABX_W27(1, 3);
+    else if (s27.nci == 2) ABX_W27(1, 2);
+    else ABX_W27(1, 1);
+#undef ABX_W27
+  }
+  const int nt = (int)tiles_of(n, d, h, w);
   const dim3 grid((unsigned)sh.S, (unsigned)sh.blocks_y);
 #define ABX_LAUNCH(KERNEL)                                                                                     \
   hipLaunchKernelGGL(KERNEL, grid, dim3(256), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt, \
@@ -1425,7 +1633,9 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   else if (sh.nci == 2) ABX_WG(T_, 1, 2);                 \
   else ABX_WG(T_, 1, 1);
   if (T == 27) {
-    ABX_WG_T(27)
+    if (!use27) {
+      ABX_WG_T(27)
+    }
   } else {
     ABX_WG_T(1)
   }
@@ -1437,11 +1647,11 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   // kChunk rows summed in place, then the chunk sums
   const int64_t nred = nw + (db ? co : 0);
   constexpr int kChunk = 32;
-  int rows = sh.S;
+  int rows = nrows;
   int64_t stride = nw + co;
-  if (sh.S > kChunk) {
-    rows = (sh.S + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nred), (unsigned)rows), dim3(256), 0, s, part, sh.S, nred, stride,
+  if (nrows > kChunk) {
+    rows = (nrows + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nred), (unsigned)rows), dim3(256), 0, s, part, nrows, nred, stride,
                        kChunk);
     stride *= kChunk;
   }

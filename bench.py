@@ -281,8 +281,9 @@ def bench_unet(a, dev):
 def bench_conv(a, dev):
     """The surrogate's 16-channel 3x3x3 convolutions (batch 8, 96 x 48 x 64):
     forward, the 16-channel kernel against the general one, and the weight
-    gradient, transposing LDS reads against gathering; each timed with HIP
-    events over --steps calls."""
+    gradient: every tap per workgroup (k_wgrad27, the default) against the
+    tap-group kernels with transposing LDS reads and with gathering; each
+    timed with HIP events over --steps calls."""
     import torch
     from airboxcalc_amd import abx
     L = abx.load()
@@ -314,7 +315,8 @@ def bench_conv(a, dev):
                 us[k] = e0.elapsed_time(e1) / calls * 1e3
         finally:
             L.abx_conv3d_set_kernel(3)
-        # weight gradient of the same shape: transposing LDS reads (bit 1) against gathering
+        # weight gradient of the same shape: k_wgrad27 (flags 3), the tap-group kernel on transposing LDS
+        # reads (7) and gathering (5)
         dz = torch.randn((n, d, h, w, co), generator=g).to(torch.bfloat16).cuda()
         wsb = L.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
         ws = torch.empty(wsb // 4, dtype=torch.float32, device="cuda")
@@ -322,7 +324,7 @@ def bench_conv(a, dev):
         db = torch.empty(co, dtype=torch.float32, device="cuda")
         wg = {}
         try:
-            for k in (1, 3):
+            for k in (5, 7, 3):
                 L.abx_conv3d_set_kernel(k)
                 for _ in range(3):
                     assert L.abx_conv3d_wgrad(s, x.data_ptr(), ci, dz.data_ptr(), co, n, d, h, w, ci, co, 3,
@@ -340,8 +342,8 @@ def bench_conv(a, dev):
         fl = 2 * n * d * h * w * ci * co * 27
         rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[1], 1), general_us=round(us[0], 1),
                          fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1),
-                         wgrad_tr_us=round(wg[3], 1), wgrad_gather_us=round(wg[1], 1),
-                         wgrad_tr_tflops=round(fl / wg[3] / 1e6, 1)))
+                         wgrad27_us=round(wg[3], 1), wgrad_tr_us=round(wg[7], 1), wgrad_gather_us=round(wg[5], 1),
+                         wgrad27_tflops=round(fl / wg[3] / 1e6, 1), wgrad_tr_tflops=round(fl / wg[7] / 1e6, 1)))
     r0 = rows[0]
     out = {
         "metric": "3x3x3 bf16 convolution TFLOP/s (16 -> 16 channels, 96x48x64, batch 8)",
@@ -349,7 +351,7 @@ def bench_conv(a, dev):
         "ms_per_step": round(r0["fwd16_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random volumes)",
         "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16 against k_conv_fwd) "
-                               "and weight gradient (k_conv_wgrad_tr against k_conv_wgrad)",
+                               "and weight gradient (k_wgrad27 against k_conv_wgrad_tr and k_conv_wgrad)",
                    "shapes": rows},
         "roofline": {"bound": "mfma", "achieved": r0["fwd16_tflops"], "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(r0["fwd16_tflops"] / 2500.0, 4), "traffic": None, "kernel": "k_conv_fwd16"},

@@ -152,8 +152,9 @@ def test_conv_forward16_matches_general_kernel(co, ci):
                                   (1, 3, 5, 16, 16, 64, 3), (1, 2, 3, 16, 32, 64, 1)],
                          ids=lambda c: "x".join(map(str, c)))
 def test_wgrad_transposed_reads_equal_gathered(case):
-    """The weight gradient on transposing LDS reads (k_conv_wgrad_tr, the
-    default) builds the same fragments as the 16-bit gathering kernel and
+    """The tap-group weight gradient on transposing LDS reads
+    (k_conv_wgrad_tr; the default for 1x1x1, selected for 3x3x3 with flag
+    bit 2) builds the same fragments as the 16-bit gathering kernel and
     issues the same MFMAs in the same order: equal bit for bit, weights and
     bias, on grids that are not multiples of the tile, for 16, 32 and 64
     output channels per workgroup."""
@@ -167,7 +168,7 @@ def test_wgrad_transposed_reads_equal_gathered(case):
     ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=_dev())
     outs = []
     try:
-        for flags in (3, 1):
+        for flags in (7, 5):
             ck(lib.abx_conv3d_set_kernel(flags))
             dw = torch.empty(co * k ** 3 * ci, dtype=torch.float32, device=_dev())
             db = torch.empty(co, dtype=torch.float32, device=_dev())
@@ -370,6 +371,55 @@ def test_conv_backward_matches_autograd(case):
                             ws_bytes, dw.data_ptr(), None, 0))
     torch.cuda.synchronize()
     assert np.array_equal(dw.cpu().numpy().reshape(co, k ** 3, ci), got_dw)
+
+
+@pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16), (1, 6, 5, 20, 32, 16), (1, 5, 7, 18, 48, 16),
+                                  (2, 4, 8, 16, 16, 32), (1, 3, 7, 33, 32, 32), (1, 3, 7, 33, 96, 32),
+                                  (1, 3, 6, 18, 16, 64), (1, 3, 5, 16, 32, 64), (1, 3, 3, 12, 192, 64),
+                                  (1, 2, 2, 8, 64, 128), (8, 12, 6, 8, 128, 128)],
+                         ids=lambda c: "x".join(map(str, c)))
+def test_wgrad27_every_channel_blocking(case):
+    """k_wgrad27 (every tap in one workgroup, the 3x3x3 default) for every
+    (output, input) channel blocking it is compiled for, on grids that are
+    not multiples of its 16x4x4 tile: weights and bias within 1e-3 of the
+    range of torch's FP32 autograd on the same bf16 operands, within the
+    same bar of the tap-group kernel, and the same bits on a second call."""
+    n, d, h, w, ci, co = case
+    say("wgrad27", case)
+    rng = np.random.default_rng(8)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    dzt = bf(rng.standard_normal((n, d, h, w, co)))
+    lib = abx.load()
+    ws_bytes = lib.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=_dev())
+
+    def run(flags):
+        ck(lib.abx_conv3d_set_kernel(flags))
+        dw = torch.empty(co * 27 * ci, dtype=torch.float32, device=_dev())
+        db = torch.empty(co, dtype=torch.float32, device=_dev())
+        ck(lib.abx_conv3d_wgrad(stream(), xt.data_ptr(), ci, dzt.data_ptr(), co, n, d, h, w, ci, co, 3,
+                                ws.data_ptr(), ws_bytes, dw.data_ptr(), db.data_ptr(), 0))
+        torch.cuda.synchronize()
+        return dw.cpu().numpy().reshape(co, 27, ci), db.cpu().numpy()
+
+    try:
+        dw, db = run(3)
+        dw2, db2 = run(3)
+        dw_old, db_old = run(7)
+    finally:
+        ck(lib.abx_conv3d_set_kernel(3))
+    assert np.array_equal(dw, dw2) and np.array_equal(db, db2)
+    import torch.nn.functional as F
+    Xr = xt.float().permute(0, 4, 1, 2, 3).contiguous()
+    Wr = torch.zeros((co, ci, 3, 3, 3), device=_dev(), requires_grad=True)
+    Br = torch.zeros(co, device=_dev(), requires_grad=True)
+    F.conv3d(Xr, Wr, Br, padding=1).backward(dzt.float().permute(0, 4, 1, 2, 3).contiguous())
+    ref_dw = Wr.grad.permute(0, 2, 3, 4, 1).reshape(co, 27, ci).cpu().numpy()
+    ref_db = Br.grad.cpu().numpy()
+    rng_w, rng_b = np.abs(ref_dw).max(), max(np.abs(ref_db).max(), 1e-30)
+    assert np.abs(dw - ref_dw).max() <= 1e-3 * rng_w, f"weights {case}: {np.abs(dw - ref_dw).max():.3e} of {rng_w:.3e}"
+    assert np.abs(db - ref_db).max() <= 1e-3 * rng_b, f"bias {case}"
+    assert np.abs(dw - dw_old).max() <= 1e-3 * rng_w and np.abs(db - db_old).max() <= 1e-3 * rng_b
 
 
 def test_wgrad_workspace_too_small_is_an_error():
