@@ -741,8 +741,9 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
 }
 
 // Weight gradient of a 3x3x3 convolution with every tap in one workgroup.
-// The output tile is 16 x 4 x 4 voxels (eight k steps of 32), its input
-// brick 18 x 6 x 6.  Workgroup (blockIdx.x = partial row, blockIdx.y =
+// The output tile is 16 x 4 x TZW voxels (2 TZW k steps of 32), its input
+// brick 18 x 6 x (TZW + 2): TZW = 4, or 8 on large grids, where the deeper
+// brick re-reads fewer halo planes.  Workgroup (blockIdx.x = partial row, blockIdx.y =
 // block) owns output channels [co0, co0 + 16 NCT) and input channels
 // [ci0, ci0 + 16 NCI) for all 27 taps, so each tile is staged once per
 // block and not once per tap group.  Its 27 NCI (tap, input tile) fragments
@@ -755,25 +756,25 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
 // workgroups: their wave 7 multiplies every dZ fragment by a fragment of
 // ones (27 NCI is never a multiple of 8, so wave 7 has one item less than
 // wave 0, and waves skip the MFMAs of an item they do not own).
-constexpr int W27X = 16, W27Y = 4, W27Z = 4, W27V = W27X * W27Y * W27Z;
-constexpr int W27BX = W27X + 2, W27BY = W27Y + 2, W27BZ = W27Z + 2, W27BV = W27BX * W27BY * W27BZ;
+constexpr int W27X = 16, W27Y = 4, W27BX = W27X + 2, W27BY = W27Y + 2;
 
-__device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W) {
-  const int ntx = (W + W27X - 1) / W27X, nty = (H + W27Y - 1) / W27Y, ntz = (D + W27Z - 1) / W27Z;
+__device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W, int TZW) {
+  const int ntx = (W + W27X - 1) / W27X, nty = (H + W27Y - 1) / W27Y, ntz = (D + TZW - 1) / TZW;
   Tile r;
   r.x0 = (t % ntx) * W27X;
   t /= ntx;
   r.y0 = (t % nty) * W27Y;
   t /= nty;
-  r.z0 = (t % ntz) * W27Z;
+  r.z0 = (t % ntz) * TZW;
   r.n = t / ntz;
   return r;
 }
 
-template <int NCT, int NCI>
+template <int NCT, int NCI, int TZW>
 __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* __restrict__ dz, int dzs,
                                                  float* __restrict__ part, float* __restrict__ partb, int D, int H,
                                                  int W, int Ci, int Co, int ntiles) {
+  constexpr int W27V = W27X * W27Y * TZW, W27BZ = TZW + 2, W27BV = W27BX * W27BY * W27BZ;
   constexpr int CO = 16 * NCT, CI = 16 * NCI, PZ = CO, PX = CI;
   constexpr int NI = 27 * NCI, MAXI = (NI + 7) / 8;
   constexpr int ZP = W27V * NCT * 2, XP = W27BV * NCI * 2;  // 16-byte pieces of the dZ tile and the brick
@@ -801,21 +802,21 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
   }
   // staging slots of this thread: dZ piece threadIdx + 512 m is voxel
   // e / (2 NCT), channels 8 (e % (2 NCT)) .. + 8; brick piece likewise with
-  // the brick voxel packed as bx | by << 5 | bz << 8 | piece << 11
+  // the brick voxel packed as bx | by << 5 | bz << 8 | piece << 12
   int xsl[XSL];
 #pragma unroll
   for (int m = 0; m < XSL; ++m) {
     const int e = min((int)threadIdx.x + 512 * m, XP - 1);
     const int vb = e / (NCI * 2), q = e - vb * (NCI * 2);
     const int bx = vb % W27BX, row = vb / W27BX, by = row % W27BY, bz = row / W27BY;
-    xsl[m] = bx | by << 5 | bz << 8 | q << 11;
+    xsl[m] = bx | by << 5 | bz << 8 | q << 12;
   }
   const bool xlast = (int)threadIdx.x + 512 * (XSL - 1) < XP;
   u32x4 zv[ZSL], xv[XSL];
   uint32_t zok = 0, xok = 0;
   // loads from clamped in-grid addresses; out-of-grid values are zeroed when stored
   auto fetch = [&](int t) {
-    const Tile tl = w27_tile(t, D, H, W);
+    const Tile tl = w27_tile(t, D, H, W, TZW);
     zok = 0;
     xok = 0;
 #pragma unroll
@@ -829,7 +830,7 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
     }
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
-      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
+      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
       const int gx = tl.x0 + bx - 1, gy = tl.y0 + by - 1, gz = tl.z0 + bz - 1;
       xok |= (uint32_t)((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) << m;
       xv[m] = *(const u32x4*)(x + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
@@ -860,7 +861,7 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
       if (m == XSL - 1 && !xlast) break;
-      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 7, q = xsl[m] >> 11;
+      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
       *(u32x4*)(sx + ((bz * W27BY + by) * W27BX + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
     }
     __syncthreads();
@@ -1398,7 +1399,7 @@ WgShape wgrad_shape(int n, int d, int h, int w, int ci, int co, int taps) {
 
 // k_wgrad27's launch shape: channel tiles per workgroup and partial rows.
 struct W27Shape {
-  int nct, nci, S, blocks_y;
+  int nct, nci, tz, S, blocks_y;
 };
 
 W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
@@ -1406,7 +1407,13 @@ W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
   s.nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   s.nci = (ci % 48 == 0 && s.nct <= 2) ? 3 : (ci % 32 == 0 ? 2 : 1);
   s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci));
-  const int64_t nt = (int64_t)n * ((d + W27Z - 1) / W27Z) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X);
+  // 8-deep tiles where there are at least 8 per CU, for the one blocking compiled with them (its
+  // staging registers fit; the wider input blocks would spill)
+  const auto tiles = [&](int tz) {
+    return (int64_t)n * ((d + tz - 1) / tz) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X);
+  };
+  s.tz = (s.nct == 1 && s.nci == 1 && tiles(8) >= 8 * (int64_t)device_cus()) ? 8 : 4;
+  const int64_t nt = tiles(s.tz);
   // about two workgroups per CU, at most about 16 M partial sums
   const int64_t want = (2 * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
   const int64_t by_size = std::max<int64_t>(1, ((int64_t)16 << 20) / ((int64_t)co * 27 * ci + co));
@@ -1589,7 +1596,7 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   const int T = ksize == 3 ? 27 : 1;
   const WgShape sh = wgrad_shape(n, d, h, w, ci, co, T);
   const bool use27 = T == 27 && g_wgrad27;
-  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 1, 1};
+  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 4, 1, 1};
   const int nrows = use27 ? s27.S : sh.S;
   const int64_t nw = (int64_t)co * T * ci;
   if (ws_bytes < (int64_t)nrows * (nw + co) * 4) {
@@ -1601,18 +1608,19 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
   float* partb = db ? workspace : nullptr;  // a flag: the bias partials are written
   hipStream_t s = (hipStream_t)stream;
   if (use27) {
-    const int nt27 = (int)((int64_t)n * ((d + W27Z - 1) / W27Z) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
+    const int nt27 = (int)((int64_t)n * ((d + s27.tz - 1) / s27.tz) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
     const dim3 g27((unsigned)s27.S, (unsigned)s27.blocks_y);
-#define ABX_W27(A, B)                                                                                            \
-  hipLaunchKernelGGL((k_wgrad27<A, B>), g27, dim3(512), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt27)
-    if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2);
-    else if (s27.nct == 4) ABX_W27(4, 1);
-    else if (s27.nct == 2 && s27.nci == 3) ABX_W27(2, 3);
-    else if (s27.nct == 2 && s27.nci == 2) ABX_W27(2, 2);
-    else if (s27.nct == 2) ABX_W27(2, 1);
-    else if (s27.nci == 3) ABX_W27(1, 3);
-    else if (s27.nci == 2) ABX_W27(1, 2);
-    else ABX_W27(1, 1);
+#define ABX_W27(A, B, Z)                                                                                         \
+  hipLaunchKernelGGL((k_wgrad27<A, B, Z>), g27, dim3(512), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt27)
+    if (s27.tz == 8) ABX_W27(1, 1, 8);
+    else if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2, 4);
+    else if (s27.nct == 4) ABX_W27(4, 1, 4);
+    else if (s27.nct == 2 && s27.nci == 3) ABX_W27(2, 3, 4);
+    else if (s27.nct == 2 && s27.nci == 2) ABX_W27(2, 2, 4);
+    else if (s27.nct == 2) ABX_W27(2, 1, 4);
+    else if (s27.nci == 3) ABX_W27(1, 3, 4);
+    else if (s27.nci == 2) ABX_W27(1, 2, 4);
+    else ABX_W27(1, 1, 4);
 #undef ABX_W27
   }
   const int nt = (int)tiles_of(n, d, h, w);

@@ -303,6 +303,8 @@ def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3
     ntr = n - nval
     rng = np.random.default_rng(seed)
     cw = torch.ones(U.OUT_CH, dtype=torch.float32, device=X.device)
+    # fluid voxels per sample, so a batch's mask count needs no device read-back
+    mcount = M.reshape(n, -1).sum(dim=1, dtype=torch.int64).cpu().numpy()
     hist = []
     t_start = time.perf_counter()
     step_times = []
@@ -318,7 +320,7 @@ def train(net: U.UNet, data: dict, epochs: int, batch: int = 8, lr: float = 1e-3
             xb, yb, mb = X[idx].contiguous(), Y[idx].contiguous(), M[idx].contiguous()
             torch.cuda.synchronize()
             ts = time.perf_counter()
-            loss = net.train_step(xb, yb, mb, cw, lr=lr_ep)
+            loss = net.train_step(xb, yb, mb, cw, lr=lr_ep, count=int(mcount[order[b0:b0 + batch]].sum()))
             losses.append(loss)
             torch.cuda.synchronize()
             step_times.append(time.perf_counter() - ts)

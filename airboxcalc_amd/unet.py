@@ -371,10 +371,12 @@ class UNet:
                                 self.v.data_ptr(), self.w.numel(), lr, b1, b2, eps, self.step_no))
         self.repack()
 
-    def loss_and_grad(self, target, mask, chan_w) -> Tuple["object", "object"]:
+    def loss_and_grad(self, target, mask, chan_w, count: Optional[int] = None) -> Tuple["object", "object"]:
         """Masked weighted MSE of the head output against target (n, d, h, w,
         OUT_CH) bf16 over voxels where mask (n, d, h, w) uint8 is set.  Returns
-        (loss as a device scalar, gradient of the head output)."""
+        (loss as a device scalar, gradient of the head output).  count: the
+        number of set mask voxels when the caller knows it; otherwise it is
+        counted on the device and read back, which waits for the GPU."""
         torch = self.torch
         out = self.A["out"]
         nvox = out.numel() // HEAD_CH
@@ -383,16 +385,16 @@ class UNet:
         if not hasattr(self, "_lp") or self._lp.numel() != nparts:
             self._lp = torch.empty(nparts, dtype=torch.float32, device=self.dev)
             self._dout = torch.empty_like(out)
-        cnt = float(mask.sum().item())
+        cnt = float(mask.sum().item()) if count is None else float(count)
         scale = 1.0 / max(cnt * OUT_CH, 1.0)
         _ck(lib.abx_mse_loss(self.stream, out.data_ptr(), HEAD_CH, target.data_ptr(), target.shape[-1],
                              mask.data_ptr(), nvox, OUT_CH, chan_w.data_ptr(), scale, self._lp.data_ptr(),
                              self._dout.data_ptr(), HEAD_CH))
         return self._lp.sum(), self._dout
 
-    def train_step(self, x, target, mask, chan_w, lr: float = 1e-3) -> "object":
+    def train_step(self, x, target, mask, chan_w, lr: float = 1e-3, count: Optional[int] = None) -> "object":
         self.forward(x)
-        loss, dout = self.loss_and_grad(target, mask, chan_w)
+        loss, dout = self.loss_and_grad(target, mask, chan_w, count)
         self.backward(dout)
         self.adam(lr)
         return loss
