@@ -1414,8 +1414,11 @@ W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
   };
   s.tz = (s.nct == 1 && s.nci == 1 && tiles(8) >= 8 * (int64_t)device_cus()) ? 8 : 4;
   const int64_t nt = tiles(s.tz);
-  // about two workgroups per CU, at most about 16 M partial sums
-  const int64_t want = (2 * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
+  // one round of resident workgroups (two per CU for the blockings within
+  // 128 registers, one for the wider ones; more rounds only add partial
+  // sums to reduce), at most about 16 M partial sums
+  const int res = s.nct * s.nci <= 2 ? 2 : 1;
+  const int64_t want = (res * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
   const int64_t by_size = std::max<int64_t>(1, ((int64_t)16 << 20) / ((int64_t)co * 27 * ci + co));
   s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, want, by_size}));
   return s;
