@@ -209,6 +209,10 @@ def load() -> C.CDLL:
                                            vp, i32]),
         "abx_conv3d_wgrad_workspace": (i64, [i32, i32, i32, i32, i32, i32, i32]),
         "abx_conv3d_wgrad": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp, i32]),
+        "abx_conv3d_forward_parts": (i32, [vp, vp, i32, i32, vp, i32, i32, i32, i32, i32, i32, vp, vp, i32, i32, i32,
+                                           vp, i32, i32, vp, i32]),
+        "abx_conv3d_wgrad_parts": (i32, [vp, vp, i32, i32, vp, i32, vp, i32, i32, i32, i32, i32, i32, i32, vp, i64,
+                                         vp, vp, i32]),
         "abx_maxpool2": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32]),
         "abx_maxpool2_backward": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),
         "abx_maxpool2_backward_gated": (i32, [vp, vp, i32, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),

@@ -311,10 +311,14 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-__global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* __restrict__ wt,
-                                                    const float* __restrict__ bias, u16* __restrict__ y, int ys,
-                                                    const u16* __restrict__ gate, int gs, int D, int H, int W, int Ci,
-                                                    int ntiles, int relu, int accum) {
+// Channel parts: input chunks c < nch1 come from x, the others from x2
+// (chunk c - nch1 there); output tiles blockIdx.y < ncot1 go to y, the others
+// to y2 (tile blockIdx.y - ncot1), so a concatenation can be two tensors.
+__global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
+                                                    int nch1, const u16* __restrict__ wt,
+                                                    const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2,
+                                                    int ncot1, const u16* __restrict__ gate, int gs, int D, int H,
+                                                    int W, int Ci, int ntiles, int relu, int accum) {
   __shared__ __attribute__((aligned(16))) u16 sx[F16BV * F16SP];
   __shared__ u32x4 sw[14 * 64];
   const int co0 = blockIdx.y * 16, nch = Ci / 16;
@@ -334,6 +338,9 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
   u32x4 pf[F16SLOTS];
   auto fetch = [&](int t, int c) {
     const F16Tile tl = f16_tile(t, D, H, W);
+    const bool p2 = c >= nch1;
+    const u16* xb = p2 ? x2 : x;
+    const int xsb = p2 ? xs2 : xs, cb = p2 ? c - nch1 : c;
 #pragma unroll
     for (int m = 0; m < F16SLOTS; ++m) {
       pf[m] = (u32x4){0u, 0u, 0u, 0u};
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
       const int gx = tl.x0 + (sb[m] & 31) - 1, gy = tl.y0 + ((sb[m] >> 5) & 7) - 1,
                 gz = tl.z0 + ((sb[m] >> 8) & 15) - 1;
       if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
-        pf[m] = *(const u32x4*)(x + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xs + 16 * c +
+        pf[m] = *(const u32x4*)(xb + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xsb + 16 * cb +
                                 8 * (sb[m] >> 12));
     }
   };
@@ -420,11 +427,14 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
 
     const F16Tile tl = f16_tile(t, D, H, W);
     const int gx = tl.x0 + lr, gz = tl.z0 + wave;
+    const bool o2 = (int)blockIdx.y >= ncot1;
+    u16* yb = o2 ? y2 : y;
+    const int ysb = o2 ? ys2 : ys, cob = (o2 ? ((int)blockIdx.y - ncot1) * 16 : co0) + 4 * lg;
 #pragma unroll
     for (int r = 0; r < F16Y; ++r) {
       const int gy = tl.y0 + r;
       if (gx >= W || gy >= H || gz >= D) continue;
-      u16* yp = y + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * ys + co;
+      u16* yp = yb + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * ysb + cob;
       float v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) v[i] = acc[r][i] + bv[i];
@@ -770,10 +780,13 @@ __device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W, int TZW) {
   return r;
 }
 
+// Channel parts as in k_conv_fwd16: input channels c < ci1 come from x, the
+// others from x2 (channel c - ci1 there); no input block straddles ci1.
 template <int NCT, int NCI, int TZW>
-__global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* __restrict__ dz, int dzs,
-                                                 float* __restrict__ part, float* __restrict__ partb, int D, int H,
-                                                 int W, int Ci, int Co, int ntiles) {
+__global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* x2, int xs2, int ci1,
+                                                 const u16* __restrict__ dz, int dzs, float* __restrict__ part,
+                                                 float* __restrict__ partb, int D, int H, int W, int Ci, int Co,
+                                                 int ntiles) {
   constexpr int W27V = W27X * W27Y * TZW, W27BZ = TZW + 2, W27BV = W27BX * W27BY * W27BZ;
   constexpr int CO = 16 * NCT, CI = 16 * NCI, PZ = CO, PX = CI;
   constexpr int NI = 27 * NCI, MAXI = (NI + 7) / 8;
@@ -785,6 +798,9 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
   const int ncb = Co / CO;
   const int cb = (int)blockIdx.y % ncb, cc = (int)blockIdx.y / ncb;
   const int co0 = cb * CO, ci0 = cc * CI;
+  const bool p2 = ci0 >= ci1;
+  const u16* xb = p2 ? x2 : x;
+  const int xsb = p2 ? xs2 : xs, cx0 = p2 ? ci0 - ci1 : ci0;
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   static_assert(NI % 8 != 0, "wave 7 has a spare item slot for the bias");
@@ -833,8 +849,8 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
       const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
       const int gx = tl.x0 + bx - 1, gy = tl.y0 + by - 1, gz = tl.z0 + bz - 1;
       xok |= (uint32_t)((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) << m;
-      xv[m] = *(const u32x4*)(x + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
-                                      min(max(gx, 0), W - 1)) * xs + ci0 + 8 * q);
+      xv[m] = *(const u32x4*)(xb + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
+                                       min(max(gx, 0), W - 1)) * xsb + cx0 + 8 * q);
     }
   };
   const u32x4 zero4 = {0u, 0u, 0u, 0u};
@@ -1402,10 +1418,12 @@ struct W27Shape {
   int nct, nci, tz, S, blocks_y;
 };
 
-W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
+// Channel blocking nci (1-3 tiles of 16 input channels) with the given
+// output blocking nct.
+W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, int nci) {
   W27Shape s;
-  s.nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
-  s.nci = (ci % 48 == 0 && s.nct <= 2) ? 3 : (ci % 32 == 0 ? 2 : 1);
+  s.nct = nct;
+  s.nci = nci;
   s.blocks_y = (co / (16 * s.nct)) * (ci / (16 * s.nci));
   // 8-deep tiles where there are at least 8 per CU, for the one blocking compiled with them (its
   // staging registers fit; the wider input blocks would spill)
@@ -1424,9 +1442,21 @@ W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
   return s;
 }
 
-bool conv_args_ok(const char* fn, const void* x, int xs, int n, int d, int h, int w, int ci, int co, int ksize) {
+int w27_nct(int co) { return co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1); }
+
+// Input channels [0, ci1) and [ci1, ci) in two tensors (ci1 = ci: one): the
+// widest input block (48, 32 or 16 channels) that divides both parts.
+W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co, int ci1) {
+  const int nct = w27_nct(co), c2 = ci - ci1;
+  const int nci = (ci1 % 48 == 0 && c2 % 48 == 0 && nct <= 2) ? 3 : (ci1 % 32 == 0 && c2 % 32 == 0 ? 2 : 1);
+  return wgrad27_shape_nci(n, d, h, w, ci, co, nct, nci);
+}
+
+// cx: the input channels that x itself holds (ci, or the first part's)
+bool conv_args_ok(const char* fn, const void* x, int xs, int cx, int n, int d, int h, int w, int ci, int co,
+                  int ksize) {
   if (!x || n < 1 || d < 1 || h < 1 || w < 1 || ci < 16 || co < 16 || ci % 16 || co % 16 || (ksize != 1 && ksize != 3) ||
-      xs < ci || xs % 8 || !aligned16(x) || (int64_t)n * d * h * w > ((int64_t)1 << 31) / 2) {
+      xs < cx || xs % 8 || !aligned16(x) || (int64_t)n * d * h * w > ((int64_t)1 << 31) / 2) {
     set_errorf("%s: bad argument (channels must be multiples of 16, strides of 8, pointers 16-byte aligned)", fn);
     return false;
   }
@@ -1502,17 +1532,45 @@ extern "C" int abx_conv3d_pack(void* stream, const float* w, int co, int ksize, 
 }
 
 namespace {
-int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
-                 const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate, uint16_t* y,
-                 int ys, const uint16_t* gate, int gs) {
-  if (!conv_args_ok(fn, x, xs, n, d, h, w, ci, co, ksize)) return -1;
-  if (!wpack || !y || ys < co || ys % 8 || !aligned16(wpack) || !aligned16(y)) {
+// Two-part input / output bounds: channels [0, c1) in the first tensor, [c1, c) in the second.
+bool parts_ok(const char* fn, const void* p2, int s2, int c1, int c) {
+  if (c1 < 16 || c1 > c || c1 % 16) {
+    set_errorf("%s: the first channel part must be a multiple of 16 of at most all channels", fn);
+    return false;
+  }
+  if (c1 < c && (!p2 || !aligned16(p2) || s2 < c - c1 || s2 % 8)) {
+    set_errorf("%s: bad second channel part (NULL, unaligned, or its stride below its channels)", fn);
+    return false;
+  }
+  return true;
+}
+
+int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2, int n,
+                 int d, int h, int w, int ci, const uint16_t* wpack, const float* bias, int co, int ksize, int relu,
+                 int accumulate, uint16_t* y, int ys, int co1, uint16_t* y2, int ys2, const uint16_t* gate, int gs) {
+  if (!conv_args_ok(fn, x, xs, ci1, n, d, h, w, ci, co, ksize) || !parts_ok(fn, x2, xs2, ci1, ci) ||
+      !parts_ok(fn, y2, ys2, co1, co))
+    return -1;
+  if (!wpack || !y || ys < co1 || ys % 8 || !aligned16(wpack) || !aligned16(y)) {
     set_errorf("%s: bad output or weight argument", fn);
     return -1;
   }
   if (gate && (gs < co || gs % 8 || !aligned16(gate))) {
     set_errorf("%s: gate stride must be >= co and a multiple of 8, the pointer 16-byte aligned", fn);
     return -1;
+  }
+  const bool parts = ci1 < ci || co1 < co;
+  if (parts && (ksize != 3 || !g_fwd16 || gate)) {
+    set_errorf("%s: channel parts need the 16-channel 3x3x3 kernel and no gate", fn);
+    return -1;
+  }
+  if (ci1 == ci) {
+    x2 = x;
+    xs2 = xs;
+  }
+  if (co1 == co) {
+    y2 = y;
+    ys2 = ys;
   }
   const int ck = fwd_chunk(ci);
   const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
@@ -1523,8 +1581,8 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int n,
     // persistent workgroups: about two (of eight waves) per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)2 * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, wpack, bias, y, ys, gate,
-                       gs, d, h, w, ci, (int)tiles, relu, accumulate);
+    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2, xs2, ci1 / 16, wpack,
+                       bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu, accumulate);
     ABX_HIP(hipGetLastError());
     return 0;
   }
@@ -1555,8 +1613,16 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int n,
 extern "C" int abx_conv3d_forward(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
                                   const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
                                   uint16_t* y, int ys) {
-  return conv_forward("abx_conv3d_forward", stream, x, xs, n, d, h, w, ci, wpack, bias, co, ksize, relu, accumulate,
-                      y, ys, nullptr, 0);
+  return conv_forward("abx_conv3d_forward", stream, x, xs, ci, nullptr, 0, n, d, h, w, ci, wpack, bias, co, ksize,
+                      relu, accumulate, y, ys, co, nullptr, 0, nullptr, 0);
+}
+
+extern "C" int abx_conv3d_forward_parts(void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2,
+                                        int n, int d, int h, int w, int ci, const uint16_t* wpack, const float* bias,
+                                        int co, int relu, int accumulate, uint16_t* y, int ys, int co1, uint16_t* y2,
+                                        int ys2) {
+  return conv_forward("abx_conv3d_forward_parts", stream, x, xs, ci1, x2, xs2, n, d, h, w, ci, wpack, bias, co, 3,
+                      relu, accumulate, y, ys, co1, y2, ys2, nullptr, 0);
 }
 
 extern "C" int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs, int n, int d, int h, int w, int ci,
@@ -1566,8 +1632,8 @@ extern "C" int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs,
     set_error("abx_conv3d_forward_gated: gate is NULL");
     return -1;
   }
-  return conv_forward("abx_conv3d_forward_gated", stream, x, xs, n, d, h, w, ci, wpack, bias, co, ksize, relu,
-                      accumulate, y, ys, gate, gs);
+  return conv_forward("abx_conv3d_forward_gated", stream, x, xs, ci, nullptr, 0, n, d, h, w, ci, wpack, bias, co,
+                      ksize, relu, accumulate, y, ys, co, nullptr, 0, gate, gs);
 }
 
 extern "C" int abx_conv3d_set_kernel(int flags) {
@@ -1583,27 +1649,41 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
   const int T = ksize == 3 ? 27 : 1;
   const WgShape s = wgrad_shape(n, d, h, w, ci, co, T);
   int64_t rows = s.S;
-  // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects
-  if (T == 27) rows = std::max<int64_t>(rows, wgrad27_shape(n, d, h, w, ci, co).S);
+  // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects,
+  // and for any split of the input channels into two parts
+  if (T == 27)
+    for (int nci = 1; nci <= 3; ++nci)
+      if (ci % (16 * nci) == 0 && (nci < 3 || w27_nct(co) <= 2))
+        rows = std::max<int64_t>(rows, wgrad27_shape_nci(n, d, h, w, ci, co, w27_nct(co), nci).S);
   return rows * ((int64_t)co * T * ci + co) * 4;
 }
 
-extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d,
-                                int h, int w, int ci, int co, int ksize, float* workspace, int64_t ws_bytes, float* dw,
-                                float* db, int accumulate) {
-  if (!conv_args_ok("abx_conv3d_wgrad", x, xs, n, d, h, w, ci, co, ksize)) return -1;
+namespace {
+int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2,
+               const uint16_t* dz, int dzs, int n, int d, int h, int w, int ci, int co, int ksize, float* workspace,
+               int64_t ws_bytes, float* dw, float* db, int accumulate) {
+  if (!conv_args_ok(fn, x, xs, ci1, n, d, h, w, ci, co, ksize) || !parts_ok(fn, x2, xs2, ci1, ci)) return -1;
   if (!dz || !dw || !workspace || dzs < co || dzs % 8 || !aligned16(dz)) {
-    set_error("abx_conv3d_wgrad: bad gradient or output argument");
+    set_errorf("%s: bad gradient or output argument", fn);
     return -1;
+  }
+  if (ci1 < ci && ksize != 3) {
+    set_errorf("%s: channel parts need a 3x3x3 convolution", fn);
+    return -1;
+  }
+  if (ci1 == ci) {
+    x2 = x;
+    xs2 = xs;
   }
   const int T = ksize == 3 ? 27 : 1;
   const WgShape sh = wgrad_shape(n, d, h, w, ci, co, T);
-  const bool use27 = T == 27 && g_wgrad27;
-  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 4, 1, 1};
+  // two-part inputs always take k_wgrad27
+  const bool use27 = T == 27 && (g_wgrad27 || ci1 < ci);
+  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co, ci1) : W27Shape{1, 1, 4, 1, 1};
   const int nrows = use27 ? s27.S : sh.S;
   const int64_t nw = (int64_t)co * T * ci;
   if (ws_bytes < (int64_t)nrows * (nw + co) * 4) {
-    set_errorf("abx_conv3d_wgrad: workspace of %lld bytes is smaller than the %lld needed", (long long)ws_bytes,
+    set_errorf("%s: workspace of %lld bytes is smaller than the %lld needed", fn, (long long)ws_bytes,
                (long long)((int64_t)nrows * (nw + co) * 4));
     return -1;
   }
@@ -1614,7 +1694,8 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
     const int nt27 = (int)((int64_t)n * ((d + s27.tz - 1) / s27.tz) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
     const dim3 g27((unsigned)s27.S, (unsigned)s27.blocks_y);
 #define ABX_W27(A, B, Z)                                                                                         \
-  hipLaunchKernelGGL((k_wgrad27<A, B, Z>), g27, dim3(512), 0, s, x, xs, dz, dzs, part, partb, d, h, w, ci, co, nt27)
+  hipLaunchKernelGGL((k_wgrad27<A, B, Z>), g27, dim3(512), 0, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w, \
+                     ci, co, nt27)
     if (s27.tz == 8) ABX_W27(1, 1, 8);
     else if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2, 4);
     else if (s27.nct == 4) ABX_W27(4, 1, 4);
@@ -1670,6 +1751,21 @@ extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const u
                      db, accumulate);
   ABX_HIP(hipGetLastError());
   return 0;
+}
+}  // namespace
+
+extern "C" int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d,
+                                int h, int w, int ci, int co, int ksize, float* workspace, int64_t ws_bytes, float* dw,
+                                float* db, int accumulate) {
+  return conv_wgrad("abx_conv3d_wgrad", stream, x, xs, ci, nullptr, 0, dz, dzs, n, d, h, w, ci, co, ksize, workspace,
+                    ws_bytes, dw, db, accumulate);
+}
+
+extern "C" int abx_conv3d_wgrad_parts(void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2,
+                                      const uint16_t* dz, int dzs, int n, int d, int h, int w, int ci, int co,
+                                      float* workspace, int64_t ws_bytes, float* dw, float* db, int accumulate) {
+  return conv_wgrad("abx_conv3d_wgrad_parts", stream, x, xs, ci1, x2, xs2, dz, dzs, n, d, h, w, ci, co, 3, workspace,
+                    ws_bytes, dw, db, accumulate);
 }
 
 namespace {

@@ -14,9 +14,11 @@ Every compute step is a kernel of libairbox_hip.so (abx_conv3d_*,
 abx_maxpool2*, abx_upsample2*, abx_mse_loss, abx_adam; each ReLU's backward
 is applied in the store of the gradient kernel before it);
 torch only allocates device memory and provides the stream.  Forward and
-backward are written out by hand; the skip concatenations are channel slices
-of one buffer per level (decoder input = [upsampled | skip]), so no copy is
-made.
+backward are written out by hand.  A skip concatenation (decoder input =
+[upsampled | skip]) is two contiguous tensors per level, "upK" and "eK", that
+the decoder's first convolution, its input gradient and its weight gradient
+read or write as one input (abx_conv3d_*_parts): no copy is made, and the
+pooling and upsampling kernels touch whole rows instead of channel slices.
 """
 from __future__ import annotations
 
@@ -188,9 +190,9 @@ class UNet:
         dims = [(d >> k, h >> k, w >> k) for k in range(4)]
         E = lambda lvl, c: torch.zeros((n,) + dims[lvl] + (c,), dtype=torch.bfloat16, device=self.dev)
         A = {}
-        A["a1"], A["cat1"], A["p1"] = E(0, w1), E(0, w2 + w1), E(1, w1)
-        A["a2"], A["cat2"], A["p2"] = E(1, w2), E(1, w3 + w2), E(2, w2)
-        A["a3"], A["cat3"], A["p3"] = E(2, w3), E(2, w4 + w3), E(3, w3)
+        A["a1"], A["up1"], A["e1"], A["p1"] = E(0, w1), E(0, w2), E(0, w1), E(1, w1)
+        A["a2"], A["up2"], A["e2"], A["p2"] = E(1, w2), E(1, w3), E(1, w2), E(2, w2)
+        A["a3"], A["up3"], A["e3"], A["p3"] = E(2, w3), E(2, w4), E(2, w3), E(3, w3)
         A["a4"], A["b4"] = E(3, w4), E(3, w4)
         A["d3a"], A["d3b"] = E(2, w3), E(2, w3)
         A["d2a"], A["d2b"] = E(1, w2), E(1, w2)
@@ -224,6 +226,34 @@ class UNet:
         assert x.c == l.ci and y.c == l.co, (name, x.c, y.c)
         _ck(abx.load().abx_conv3d_forward(self.stream, x.p, x.s, n, d, h, w, l.ci, self.packs[name][0].data_ptr(),
                                           self.bptr(l), l.co, l.ksize, int(l.relu), int(accumulate), y.p, y.s))
+
+    def _conv_parts(self, name: str, x1: T, x2: T, y: T) -> None:
+        """Layer name over the concatenation [x1 | x2]."""
+        l = self.by[name]
+        n = self.shape[0]
+        d, h, w = self.dims[self._level(l)]
+        assert x1.c + x2.c == l.ci and y.c == l.co, (name, x1.c, x2.c, y.c)
+        _ck(abx.load().abx_conv3d_forward_parts(self.stream, x1.p, x1.s, x1.c, x2.p, x2.s, n, d, h, w, l.ci,
+                                                self.packs[name][0].data_ptr(), self.bptr(l), l.co, int(l.relu), 0,
+                                                y.p, y.s, l.co, None, 0))
+
+    def _dgrad_parts(self, name: str, dz: T, dx1: T, dx2: T) -> None:
+        """Input gradient of layer name into the concatenation [dx1 | dx2]."""
+        l = self.by[name]
+        n = self.shape[0]
+        d, h, w = self.dims[self._level(l)]
+        assert dz.c == l.co and dx1.c + dx2.c == l.ci, (name, dz.c, dx1.c, dx2.c)
+        _ck(abx.load().abx_conv3d_forward_parts(self.stream, dz.p, dz.s, l.co, None, 0, n, d, h, w, l.co,
+                                                self.packs[name][1].data_ptr(), None, l.ci, 0, 0, dx1.p, dx1.s,
+                                                dx1.c, dx2.p, dx2.s))
+
+    def _wgrad_parts(self, name: str, x1: T, x2: T, dz: T) -> None:
+        l = self.by[name]
+        n = self.shape[0]
+        d, h, w = self.dims[self._level(l)]
+        _ck(abx.load().abx_conv3d_wgrad_parts(self.stream, x1.p, x1.s, x1.c, x2.p, x2.s, dz.p, dz.s, n, d, h, w,
+                                              l.ci, l.co, self.ws.data_ptr(), self.ws_bytes, self.wptr(l, self.g),
+                                              self.bptr(l, self.g), 0))
 
     def _dgrad(self, name: str, dz: T, dx: T, gate: Optional[T] = None) -> None:
         """dx = input gradient of layer name; with gate (the ReLU output that
@@ -285,8 +315,7 @@ class UNet:
         A = self.A
         w1, w2, w3, w4 = WIDTHS
         X = T(x)
-        cat1, cat2, cat3 = T(A["cat1"]), T(A["cat2"]), T(A["cat3"])
-        e1, e2, e3 = cat1.sl(w2, w1), cat2.sl(w3, w2), cat3.sl(w4, w3)
+        e1, e2, e3 = T(A["e1"]), T(A["e2"]), T(A["e3"])
         self._conv("enc1a", X, T(A["a1"]))
         self._conv("enc1b", T(A["a1"]), e1)
         self._pool(e1, T(A["p1"]), 0)
@@ -298,14 +327,14 @@ class UNet:
         self._pool(e3, T(A["p3"]), 2)
         self._conv("bota", T(A["p3"]), T(A["a4"]))
         self._conv("botb", T(A["a4"]), T(A["b4"]))
-        self._up(T(A["b4"]), cat3.sl(0, w4), 3)
-        self._conv("dec3a", cat3, T(A["d3a"]))
+        self._up(T(A["b4"]), T(A["up3"]), 3)
+        self._conv_parts("dec3a", T(A["up3"]), e3, T(A["d3a"]))
         self._conv("dec3b", T(A["d3a"]), T(A["d3b"]))
-        self._up(T(A["d3b"]), cat2.sl(0, w3), 2)
-        self._conv("dec2a", cat2, T(A["d2a"]))
+        self._up(T(A["d3b"]), T(A["up2"]), 2)
+        self._conv_parts("dec2a", T(A["up2"]), e2, T(A["d2a"]))
         self._conv("dec2b", T(A["d2a"]), T(A["d2b"]))
-        self._up(T(A["d2b"]), cat1.sl(0, w2), 1)
-        self._conv("dec1a", cat1, T(A["d1a"]))
+        self._up(T(A["d2b"]), T(A["up1"]), 1)
+        self._conv_parts("dec1a", T(A["up1"]), e1, T(A["d1a"]))
         self._conv("dec1b", T(A["d1a"]), T(A["d1b"]))
         self._conv("head", T(A["d1b"]), T(A["out"]))
         self.x_in = x
@@ -324,42 +353,42 @@ class UNet:
         # level 1 decoder
         self._wgrad("dec1b", a("d1a"), g("d1b"))
         self._dgrad("dec1b", g("d1b"), g("d1a"), gate=a("d1a"))
-        self._wgrad("dec1a", a("cat1"), g("d1a"))
-        self._dgrad("dec1a", g("d1a"), g("cat1"))
-        self._up_bwd(g("cat1").sl(0, w2), g("d2b"), 1, gate=a("d2b"))
+        self._wgrad_parts("dec1a", a("up1"), a("e1"), g("d1a"))
+        self._dgrad_parts("dec1a", g("d1a"), g("up1"), g("e1"))
+        self._up_bwd(g("up1"), g("d2b"), 1, gate=a("d2b"))
         # level 2 decoder
         self._wgrad("dec2b", a("d2a"), g("d2b"))
         self._dgrad("dec2b", g("d2b"), g("d2a"), gate=a("d2a"))
-        self._wgrad("dec2a", a("cat2"), g("d2a"))
-        self._dgrad("dec2a", g("d2a"), g("cat2"))
-        self._up_bwd(g("cat2").sl(0, w3), g("d3b"), 2, gate=a("d3b"))
+        self._wgrad_parts("dec2a", a("up2"), a("e2"), g("d2a"))
+        self._dgrad_parts("dec2a", g("d2a"), g("up2"), g("e2"))
+        self._up_bwd(g("up2"), g("d3b"), 2, gate=a("d3b"))
         # level 3 decoder
         self._wgrad("dec3b", a("d3a"), g("d3b"))
         self._dgrad("dec3b", g("d3b"), g("d3a"), gate=a("d3a"))
-        self._wgrad("dec3a", a("cat3"), g("d3a"))
-        self._dgrad("dec3a", g("d3a"), g("cat3"))
-        self._up_bwd(g("cat3").sl(0, w4), g("b4"), 3, gate=a("b4"))
+        self._wgrad_parts("dec3a", a("up3"), a("e3"), g("d3a"))
+        self._dgrad_parts("dec3a", g("d3a"), g("up3"), g("e3"))
+        self._up_bwd(g("up3"), g("b4"), 3, gate=a("b4"))
         # bottleneck
         self._wgrad("botb", a("a4"), g("b4"))
         self._dgrad("botb", g("b4"), g("a4"), gate=a("a4"))
         self._wgrad("bota", a("p3"), g("a4"))
         self._dgrad("bota", g("a4"), g("p3"))
         # level 3 encoder: skip gradient (from dec3a) + pooling gradient
-        e3, de3 = a("cat3").sl(w4, w3), g("cat3").sl(w4, w3)
+        e3, de3 = a("e3"), g("e3")
         self._pool_bwd(e3, g("p3"), de3, 2)
         self._wgrad("enc3b", a("a3"), de3)
         self._dgrad("enc3b", de3, g("a3"), gate=a("a3"))
         self._wgrad("enc3a", a("p2"), g("a3"))
         self._dgrad("enc3a", g("a3"), g("p2"))
         # level 2 encoder
-        e2, de2 = a("cat2").sl(w3, w2), g("cat2").sl(w3, w2)
+        e2, de2 = a("e2"), g("e2")
         self._pool_bwd(e2, g("p2"), de2, 1)
         self._wgrad("enc2b", a("a2"), de2)
         self._dgrad("enc2b", de2, g("a2"), gate=a("a2"))
         self._wgrad("enc2a", a("p1"), g("a2"))
         self._dgrad("enc2a", g("a2"), g("p1"))
         # level 1 encoder
-        e1, de1 = a("cat1").sl(w2, w1), g("cat1").sl(w2, w1)
+        e1, de1 = a("e1"), g("e1")
         self._pool_bwd(e1, g("p1"), de1, 0)
         self._wgrad("enc1b", a("a1"), de1)
         self._dgrad("enc1b", de1, g("a1"), gate=a("a1"))

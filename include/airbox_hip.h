@@ -427,12 +427,27 @@ int abx_conv3d_forward_gated(void* stream, const uint16_t* x, int xs, int n, int
                              const uint16_t* wpack, const float* bias, int co, int ksize, int relu, int accumulate,
                              uint16_t* y, int ys, const uint16_t* gate, int gs);
 
+/* abx_conv3d_forward (3x3x3, no gate) over a concatenation held as two
+ * tensors: input channels [0, ci1) are channels of x (stride xs), [ci1, ci)
+ * channels 0.. of x2 (stride xs2); output channels [0, co1) go to y (stride
+ * ys), [co1, co) to channels 0.. of y2 (stride ys2).  ci1 and co1 are
+ * multiples of 16; with ci1 = ci (co1 = co) x2 (y2) is unused.  Equal bit for
+ * bit to abx_conv3d_forward on the concatenated tensors.  No reference
+ * counterpart (the U-Net's skip connections). */
+int abx_conv3d_forward_parts(void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2, int n,
+                             int d, int h, int w, int ci, const uint16_t* wpack, const float* bias, int co, int relu,
+                             int accumulate, uint16_t* y, int ys, int co1, uint16_t* y2, int ys2);
+
 /* Process-wide choice of convolution kernels, for comparison and timing
  * (no reference counterpart).  flags bit 0: 3x3x3 forward convolutions use the
- * 16-channel-tile kernel (16x4x4 voxel tiles, weights held in registers)
- * instead of the general one; bit 1: weight gradients build their MFMA
+ * 16-channel-tile kernel (16x4x8 voxel tiles, weights in LDS) instead of the
+ * general one; bit 1: the tap-group weight gradients build their MFMA
  * fragments with gfx950's transposing LDS reads (ds_read_b64_tr_b16) instead
- * of gathering 16-bit values.  Default 3.  The choices compute the same sums. */
+ * of gathering 16-bit values; bit 2: 3x3x3 weight gradients use those
+ * tap-group kernels instead of the every-tap kernel (two-part inputs always
+ * take the every-tap kernel).  Default 3.  The 16-channel and general forward
+ * kernels, and the two tap-group weight gradients, compute the same sums; the
+ * every-tap weight gradient sums its partials in another grouping. */
 int abx_conv3d_set_kernel(int flags);
 
 /* Bytes of FP32 workspace abx_conv3d_wgrad needs for this shape (-1 on a bad
@@ -446,6 +461,14 @@ int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci, int co, i
 int abx_conv3d_wgrad(void* stream, const uint16_t* x, int xs, const uint16_t* dz, int dzs, int n, int d, int h,
                      int w, int ci, int co, int ksize, float* workspace, int64_t ws_bytes, float* dw, float* db,
                      int accumulate);
+
+/* abx_conv3d_wgrad (3x3x3) over an input held as two tensors, as in
+ * abx_conv3d_forward_parts (channels [0, ci1) of x, [ci1, ci) of x2); the
+ * workspace of abx_conv3d_wgrad_workspace for the whole ci suffices.  No
+ * reference counterpart. */
+int abx_conv3d_wgrad_parts(void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2,
+                           const uint16_t* dz, int dzs, int n, int d, int h, int w, int ci, int co, float* workspace,
+                           int64_t ws_bytes, float* dw, float* db, int accumulate);
 
 /* 2x2x2 max pooling of an n x d x h x w grid (even sizes), c channels
  * (multiple of 8), and its gradient (to the first maximum in z, y, x order;

@@ -423,6 +423,72 @@ def test_wgrad27_every_channel_blocking(case):
     assert np.abs(dw - dw_old).max() <= 1e-3 * rng_w and np.abs(db - db_old).max() <= 1e-3 * rng_b
 
 
+@pytest.mark.parametrize("case", [(2, 5, 9, 37, 32, 16, 16), (1, 6, 5, 20, 64, 32, 32), (1, 3, 5, 18, 128, 64, 64),
+                                  (2, 4, 6, 16, 16, 32, 16)],
+                         ids=lambda c: "x".join(map(str, c)))
+def test_conv_parts_equal_the_concatenated_tensor(case):
+    """A concatenation held as two tensors (abx_conv3d_forward_parts,
+    abx_conv3d_wgrad_parts) against the same channels in one tensor: the
+    forward and the input gradient written to two parts equal the one-tensor
+    results bit for bit; the weight and bias gradients agree within 1e-3 of
+    the range (their partial sums are grouped by input block, which differs
+    when 48 or 32 channels no longer divide both parts)."""
+    n, d, h, w, c1, c2, co = case
+    ci = c1 + c2
+    say("conv parts", case)
+    rng = np.random.default_rng(9)
+    lib = abx.load()
+    xa = bf(rng.standard_normal((n, d, h, w, c1)))
+    xb = bf(rng.standard_normal((n, d, h, w, c2)))
+    xc = torch.cat([xa, xb], -1).contiguous()
+    W = (rng.standard_normal((co, 27, ci)) / np.sqrt(27 * ci)).astype(np.float32)
+    wp, _ = pack(W, 3)
+    wb, _ = pack(W, 3, flip=True)
+    bias = torch.as_tensor(rng.standard_normal(co).astype(np.float32)).to(_dev())
+    # forward: [xa | xb] -> y
+    y1 = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device=_dev())
+    y2 = torch.empty_like(y1)
+    ck(lib.abx_conv3d_forward(stream(), xc.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bias.data_ptr(), co, 3, 1,
+                              0, y1.data_ptr(), co))
+    ck(lib.abx_conv3d_forward_parts(stream(), xa.data_ptr(), c1, c1, xb.data_ptr(), c2, n, d, h, w, ci, wp.data_ptr(),
+                                    bias.data_ptr(), co, 1, 0, y2.data_ptr(), co, co, None, 0))
+    # input gradient: dz -> [dxa | dxb]
+    dz = bf(rng.standard_normal((n, d, h, w, co)))
+    dxc = torch.empty((n, d, h, w, ci), dtype=torch.bfloat16, device=_dev())
+    dxa = torch.empty((n, d, h, w, c1), dtype=torch.bfloat16, device=_dev())
+    dxb = torch.empty((n, d, h, w, c2), dtype=torch.bfloat16, device=_dev())
+    ck(lib.abx_conv3d_forward(stream(), dz.data_ptr(), co, n, d, h, w, co, wb.data_ptr(), None, ci, 3, 0, 0,
+                              dxc.data_ptr(), ci))
+    ck(lib.abx_conv3d_forward_parts(stream(), dz.data_ptr(), co, co, None, 0, n, d, h, w, co, wb.data_ptr(), None, ci,
+                                    0, 0, dxa.data_ptr(), c1, c1, dxb.data_ptr(), c2))
+    # weight gradient
+    ws_bytes = lib.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
+    ws = torch.empty(ws_bytes // 4, dtype=torch.float32, device=_dev())
+    outs = []
+    for parts in (False, True):
+        dw = torch.empty(co * 27 * ci, dtype=torch.float32, device=_dev())
+        db = torch.empty(co, dtype=torch.float32, device=_dev())
+        if parts:
+            ck(lib.abx_conv3d_wgrad_parts(stream(), xa.data_ptr(), c1, c1, xb.data_ptr(), c2, dz.data_ptr(), co, n, d,
+                                          h, w, ci, co, ws.data_ptr(), ws_bytes, dw.data_ptr(), db.data_ptr(), 0))
+        else:
+            ck(lib.abx_conv3d_wgrad(stream(), xc.data_ptr(), ci, dz.data_ptr(), co, n, d, h, w, ci, co, 3,
+                                    ws.data_ptr(), ws_bytes, dw.data_ptr(), db.data_ptr(), 0))
+        torch.cuda.synchronize()
+        outs.append((dw.cpu().numpy(), db.cpu().numpy()))
+    assert torch.equal(y1.view(torch.int16), y2.view(torch.int16))
+    assert torch.equal(dxc[..., :c1].contiguous().view(torch.int16), dxa.view(torch.int16))
+    assert torch.equal(dxc[..., c1:].contiguous().view(torch.int16), dxb.view(torch.int16))
+    (w1_, b1_), (w2_, b2_) = outs
+    assert np.abs(w2_ - w1_).max() <= 1e-3 * np.abs(w1_).max()
+    assert np.abs(b2_ - b1_).max() <= 1e-3 * np.abs(b1_).max()
+    # parts are refused where they are not supported: a gate, a 1x1x1 convolution, an unaligned split
+    assert lib.abx_conv3d_wgrad_parts(stream(), xa.data_ptr(), c1, 8, xb.data_ptr(), c2, dz.data_ptr(), co, n, d, h,
+                                      w, ci, co, ws.data_ptr(), ws_bytes, dw.data_ptr(), None, 0) != 0
+    assert lib.abx_conv3d_forward_parts(stream(), xa.data_ptr(), c1, c1, None, 0, n, d, h, w, ci, wp.data_ptr(),
+                                        None, co, 0, 0, y2.data_ptr(), co, co, None, 0) != 0
+
+
 def test_wgrad_workspace_too_small_is_an_error():
     say("wgrad workspace check")
     lib = abx.load()
@@ -552,7 +618,8 @@ def test_unet_forward_backward_stepwise():
         bad = np.abs(got - ref) > tol
         assert not bad.any(), f"{tag}: {bad.sum()} of {bad.size} off, max err {np.abs(got - ref).max():.3e}"
 
-    c1, c2, c3 = A["cat1"], A["cat2"], A["cat3"]
+    cat = lambda D, k: np.concatenate([D["up" + k], D["e" + k]], -1)   # [upsampled | skip]
+    c1, c2, c3 = cat(A, "1"), cat(A, "2"), cat(A, "3")
     e1, e2, e3 = c1[..., w2:], c2[..., w3:], c3[..., w4:]
     # forward
     check("a1", A["a1"], conv("enc1a", X))
@@ -578,7 +645,7 @@ def test_unet_forward_backward_stepwise():
     check("out", A["out"], conv("head", A["d1b"]))
     # backward: activation gradients (ReLU-masked where the network masks them)
     m = lambda a: (a > 0).astype(np.float64)
-    gc1, gc2, gc3 = G["cat1"], G["cat2"], G["cat3"]
+    gc1, gc2, gc3 = cat(G, "1"), cat(G, "2"), cat(G, "3")
     check("g d1b", G["d1b"], dgrad("head", DO) * m(A["d1b"]))
     check("g d1a", G["d1a"], dgrad("dec1b", G["d1b"]) * m(A["d1a"]))
     ref_c1 = dgrad("dec1a", G["d1a"])
