@@ -781,7 +781,8 @@ __device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W, int TZW) {
 }
 
 // Channel parts as in k_conv_fwd16: input channels c < ci1 come from x, the
-// others from x2 (channel c - ci1 there); no input block straddles ci1.
+// others from x2 (channel c - ci1 there), chosen per 8-channel staging piece,
+// so an input block may straddle ci1 (a multiple of 16).
 template <int NCT, int NCI, int TZW>
 __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* x2, int xs2, int ci1,
                                                  const u16* __restrict__ dz, int dzs, float* __restrict__ part,
@@ -798,9 +799,6 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
   const int ncb = Co / CO;
   const int cb = (int)blockIdx.y % ncb, cc = (int)blockIdx.y / ncb;
   const int co0 = cb * CO, ci0 = cc * CI;
-  const bool p2 = ci0 >= ci1;
-  const u16* xb = p2 ? x2 : x;
-  const int xsb = p2 ? xs2 : xs, cx0 = p2 ? ci0 - ci1 : ci0;
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   static_assert(NI % 8 != 0, "wave 7 has a spare item slot for the bias");
@@ -849,8 +847,10 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
       const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
       const int gx = tl.x0 + bx - 1, gy = tl.y0 + by - 1, gz = tl.z0 + bz - 1;
       xok |= (uint32_t)((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D) << m;
-      xv[m] = *(const u32x4*)(xb + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
-                                       min(max(gx, 0), W - 1)) * xsb + cx0 + 8 * q);
+      const int gc = ci0 + 8 * q;  // the piece's channel, in the first part or the second
+      const bool p2 = gc >= ci1;
+      xv[m] = *(const u32x4*)((p2 ? x2 : x) + vox(tl, D, H, W, min(max(gz, 0), D - 1), min(max(gy, 0), H - 1),
+                                                  min(max(gx, 0), W - 1)) * (p2 ? xs2 : xs) + (p2 ? gc - ci1 : gc));
     }
   };
   const u32x4 zero4 = {0u, 0u, 0u, 0u};
@@ -1444,11 +1444,11 @@ W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, 
 
 int w27_nct(int co) { return co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1); }
 
-// Input channels [0, ci1) and [ci1, ci) in two tensors (ci1 = ci: one): the
-// widest input block (48, 32 or 16 channels) that divides both parts.
-W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co, int ci1) {
-  const int nct = w27_nct(co), c2 = ci - ci1;
-  const int nci = (ci1 % 48 == 0 && c2 % 48 == 0 && nct <= 2) ? 3 : (ci1 % 32 == 0 && c2 % 32 == 0 ? 2 : 1);
+// The widest input block (48, 32 or 16 channels) that divides ci; blocks may
+// straddle the split of a two-part input.
+W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
+  const int nct = w27_nct(co);
+  const int nci = (ci % 48 == 0 && nct <= 2) ? 3 : (ci % 32 == 0 ? 2 : 1);
   return wgrad27_shape_nci(n, d, h, w, ci, co, nct, nci);
 }
 
@@ -1649,12 +1649,8 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
   const int T = ksize == 3 ? 27 : 1;
   const WgShape s = wgrad_shape(n, d, h, w, ci, co, T);
   int64_t rows = s.S;
-  // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects,
-  // and for any split of the input channels into two parts
-  if (T == 27)
-    for (int nci = 1; nci <= 3; ++nci)
-      if (ci % (16 * nci) == 0 && (nci < 3 || w27_nct(co) <= 2))
-        rows = std::max<int64_t>(rows, wgrad27_shape_nci(n, d, h, w, ci, co, w27_nct(co), nci).S);
+  // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects
+  if (T == 27) rows = std::max<int64_t>(rows, wgrad27_shape(n, d, h, w, ci, co).S);
   return rows * ((int64_t)co * T * ci + co) * 4;
 }
 
@@ -1677,9 +1673,13 @@ int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1,
   }
   const int T = ksize == 3 ? 27 : 1;
   const WgShape sh = wgrad_shape(n, d, h, w, ci, co, T);
-  // two-part inputs always take k_wgrad27
-  const bool use27 = T == 27 && (g_wgrad27 || ci1 < ci);
-  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co, ci1) : W27Shape{1, 1, 4, 1, 1};
+  // two-part inputs always take k_wgrad27; so does every output width but an
+  // odd multiple of 16 above 16, where its one-tile output blocks would stage
+  // the input once per output tile (48 output channels: 272 against 223 us
+  // for the tap-group kernel on the 96x48x64 batch of 8)
+  const bool odd_wide = w27_nct(co) == 1 && co > 16;
+  const bool use27 = T == 27 && ((g_wgrad27 && !odd_wide) || ci1 < ci);
+  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 4, 1, 1};
   const int nrows = use27 ? s27.S : sh.S;
   const int64_t nw = (int64_t)co * T * ci;
   if (ws_bytes < (int64_t)nrows * (nw + co) * 4) {
