@@ -183,6 +183,8 @@ def load() -> C.CDLL:
         "abx_batch_sync": (i32, [vp]),
         "abx_batch_set_launch": (i32, [vp, i32, i32, i32]),
         "abx_batch_tune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(d)]),
+        "abx_batch_set_xcd_group": (i32, [vp, i32]),
+        "abx_batch_get_xcd_group": (i32, [vp]),
         "abx_batch_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_batch_destroy": (None, [vp]),
         "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
@@ -634,10 +636,18 @@ class Batch:
         _check(load().abx_batch_set_launch(self.h, spb, 1 if persistent else 0, 1 if graph else 0))
 
     def tune(self, trial_steps: int = 20):
-        """Autotune the launch shape; members restart from f0 afterwards."""
+        """Autotune the launch shape; members restart from f0 afterwards.
+        The XCD group it picks stays set (set_launch does not change it)."""
         spb, pers, ms = C.c_int(), C.c_int(), C.c_double()
         _check(load().abx_batch_tune(self.h, trial_steps, C.byref(spb), C.byref(pers), C.byref(ms)))
-        return dict(spb=spb.value, persistent=bool(pers.value), ms_per_step=ms.value)
+        return dict(spb=spb.value, persistent=bool(pers.value), xcd_group=self.xcd_group, ms_per_step=ms.value)
+
+    @property
+    def xcd_group(self) -> int:
+        return load().abx_batch_get_xcd_group(self.h)
+
+    def set_xcd_group(self, group: int) -> None:
+        _check(load().abx_batch_set_xcd_group(self.h, group))
 
     def time_steps(self, n: int, doacc: bool = False) -> float:
         ms = C.c_double()

@@ -8,11 +8,13 @@
 // its member's launch arguments from a device table.  Every cell is computed by
 // the same device code as a single-handle launch (seg_cell), so each member's
 // result is bit-identical to solving it alone.  Launch shape (segments per
-// workgroup, persistent grid) is chosen by abx_batch_tune; chunks of steps can
-// be captured once as a HIP graph and replayed.
+// workgroup, persistent grid, consecutive workgroups per XCD) is chosen by
+// abx_batch_tune; chunks of steps can be captured once as a HIP graph and
+// replayed.
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <utility>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -47,17 +49,51 @@ struct abx_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<abx_sim*> members;
   int prec = ABX_F32;
-  int spb = 4, persistent = 0, use_graph = 0;
+  int spb = 4, persistent = 0, use_graph = 0, xcd_group = 4;
   StepArgs* d_args[2] = {nullptr, nullptr};  // member launch arguments without / with averaging
-  int2* d_work[3] = {nullptr, nullptr, nullptr};  // work lists for 2, 4, 8 segments per workgroup
-  int nblocks[3] = {0, 0, 0};
+  std::vector<std::vector<int32_t>> segs;    // every member's active segments (lattice order)
+  // work lists by (segments per workgroup, workgroups per XCD group): the
+  // device list and its number of blocks
+  std::map<std::pair<int, int>, std::pair<int2*, int>> lists;
   int64_t cells = 0;
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (n, doacc, start form)
 };
 
 namespace {
 
-int spb_index(int spb) { return spb == 2 ? 0 : (spb == 4 ? 1 : 2); }
+// Work list for spb segments per workgroup and XCD groups of g workgroups:
+// every member's active segments in lattice order, in member order, each
+// member's part padded to whole workgroups.  Workgroup b runs on XCD b % 8 as
+// that XCD's s-th workgroup (s = b / 8); it takes list block
+// (s / g) 8 g + (b % 8) g + s % g, so each XCD steps through groups of g
+// consecutive blocks and all XCDs sweep the lattices together (the
+// single-handle launch's block order).  Built once per shape, before any
+// launch of it (so a captured graph never uploads).
+int ensure_list(abx_batch* bt, int spb, int g) {
+  const auto key = std::make_pair(spb, g);
+  if (bt->lists.count(key)) return 0;
+  std::vector<int2> lin;
+  for (size_t m = 0; m < bt->segs.size(); ++m) {
+    for (int32_t sg : bt->segs[m]) lin.push_back(make_int2((int)m, sg));
+    while (lin.size() % (size_t)spb) lin.push_back(make_int2((int)m, -1));
+  }
+  const size_t G = (size_t)g, nlin = lin.size() / (size_t)spb, nblk = (nlin + 8 * G - 1) / (8 * G) * (8 * G);
+  std::vector<int2> w(nblk * (size_t)spb, make_int2(0, -1));
+  for (size_t blk = 0; blk < nblk; ++blk) {
+    const size_t sq = blk / 8, src = (sq / G) * 8 * G + (blk % 8) * G + sq % G;
+    if (src < nlin)
+      for (size_t i = 0; i < (size_t)spb; ++i) w[blk * (size_t)spb + i] = lin[src * (size_t)spb + i];
+  }
+  int2* d = nullptr;
+  if (hipMalloc((void**)&d, std::max<size_t>(w.size(), 1) * sizeof(int2)) != hipSuccess ||
+      hipMemcpy(d, w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+    if (d) (void)hipFree(d);
+    set_error("abx_batch: work list upload failed");
+    return -1;
+  }
+  bt->lists[key] = std::make_pair(d, (int)(w.size() / (size_t)spb));
+  return 0;
+}
 
 void drop_graphs(abx_batch* bt) {
   for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -83,8 +119,9 @@ int upload_args(abx_batch* bt) {
 template <typename S, typename M, bool ACC, int MODE>
 void launch_b(abx_batch* bt) {
   const StepArgs* args = bt->d_args[ACC ? 1 : 0];
-  const int k = spb_index(bt->spb), nb = bt->nblocks[k];
-  const int2* w = bt->d_work[k];
+  const auto& L = bt->lists.at(std::make_pair(bt->spb, bt->xcd_group));
+  const int2* w = L.first;
+  const int nb = L.second;
   const unsigned grid = bt->persistent ? (unsigned)std::min(nb, bt->dev->ncu * 8) : (unsigned)nb;
 #define ABX_BL(SPB)                                                                                                \
   do {                                                                                                             \
@@ -117,7 +154,7 @@ void launch_bt(abx_batch* bt, int doacc, int mode) {
 
 // One batched step: AA even when the members hold the streamed form, odd otherwise.
 int batch_step(abx_batch* bt, int doacc) {
-  if (bt->nblocks[spb_index(bt->spb)] == 0) return 0;
+  if (bt->lists.at(std::make_pair(bt->spb, bt->xcd_group)).second == 0) return 0;
   const int form = sim_form(bt->members[0]);
   const int mode = form == 2 ? 1 : 2;
   if (bt->prec == ABX_F32)
@@ -166,39 +203,10 @@ extern "C" int abx_batch_create(abx_dev* dev, abx_sim* const* sims, int n, abx_b
     bt->members.push_back(s);
     bt->cells += I.NF;
   }
-  // work lists: every member's active segments in lattice order, in member
-  // order, each member's part padded to whole workgroups.  Workgroup b runs
-  // on XCD b % 8 as that XCD's s-th workgroup (s = b / 8); it takes list
-  // block (s / 4) 32 + (b % 8) 4 + s % 4, so each XCD steps through groups
-  // of four consecutive blocks and all XCDs sweep the lattices together (the
-  // single-handle launch's default order).
-  const int spbs[3] = {2, 4, 8};
-  std::vector<std::vector<int32_t>> segs((size_t)n);
-  for (int m = 0; m < n; ++m) segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
-  constexpr size_t G = 4;
-  for (int k = 0; k < 3; ++k) {
-    const size_t spb = (size_t)spbs[k];
-    std::vector<int2> lin;
-    for (int m = 0; m < n; ++m) {
-      for (int32_t sg : segs[(size_t)m]) lin.push_back(make_int2(m, sg));
-      while (lin.size() % spb) lin.push_back(make_int2(m, -1));
-    }
-    const size_t nlin = lin.size() / spb, nblk = (nlin + 8 * G - 1) / (8 * G) * (8 * G);
-    std::vector<int2> w(nblk * spb, make_int2(0, -1));
-    for (size_t blk = 0; blk < nblk; ++blk) {
-      const size_t sq = blk / 8, src = (sq / G) * 8 * G + (blk % 8) * G + sq % G;
-      if (src < nlin)
-        for (size_t i = 0; i < spb; ++i) w[blk * spb + i] = lin[src * spb + i];
-    }
-    bt->nblocks[k] = (int)(w.size() / spbs[k]);
-    if (hipMalloc((void**)&bt->d_work[k], std::max<size_t>(w.size(), 1) * sizeof(int2)) != hipSuccess ||
-        hipMemcpy(bt->d_work[k], w.data(), w.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
-      set_error("abx_batch_create: work list upload failed");
-      abx_batch_destroy(bt);
-      return -1;
-    }
-  }
-  if (upload_args(bt)) {
+  // every member's segments, and the default shape's work list
+  bt->segs.resize((size_t)n);
+  for (int m = 0; m < n; ++m) bt->segs[(size_t)m] = sim_segments(bt->members[(size_t)m]);
+  if (ensure_list(bt, bt->spb, bt->xcd_group) || upload_args(bt)) {
     abx_batch_destroy(bt);
     return -1;
   }
@@ -219,12 +227,26 @@ extern "C" int abx_batch_set_launch(abx_batch* bt, int spb, int persistent, int 
     set_error("abx_batch_set_launch: spb must be 2, 4 or 8");
     return -1;
   }
+  if (ensure_list(bt, spb, bt->xcd_group)) return -1;
   bt->spb = spb;
   bt->persistent = persistent ? 1 : 0;
   bt->use_graph = graph ? 1 : 0;
   drop_graphs(bt);
   return 0;
 }
+
+extern "C" int abx_batch_set_xcd_group(abx_batch* bt, int group) {
+  if (!bt || (group != 1 && group != 2 && group != 4 && group != 8 && group != 16)) {
+    set_error("abx_batch_set_xcd_group: group must be 1, 2, 4, 8 or 16");
+    return -1;
+  }
+  if (ensure_list(bt, bt->spb, group)) return -1;
+  bt->xcd_group = group;
+  drop_graphs(bt);
+  return 0;
+}
+
+extern "C" int abx_batch_get_xcd_group(const abx_batch* bt) { return bt ? bt->xcd_group : -1; }
 
 extern "C" int abx_batch_steps(abx_batch* bt, int n, int doacc) {
   if (!bt || n < 0) {
@@ -318,19 +340,22 @@ extern "C" int abx_batch_tune(abx_batch* bt, int trial_steps, int* best_spb, int
   trial_steps += trial_steps % 2;  // whole AA pairs
   const int graph = bt->use_graph;
   double best = 1e300;
-  int bs = 4, bp = 0;
-  for (int spb : {2, 4, 8})
-    for (int pers : {0, 1}) {
-      if (abx_batch_set_launch(bt, spb, pers, 0) || abx_batch_steps(bt, 2, 0)) return -1;
-      double ms = 0;
-      if (abx_batch_time_steps(bt, trial_steps, 0, &ms)) return -1;
-      if (ms < best) {
-        best = ms;
-        bs = spb;
-        bp = pers;
+  int bs = 4, bp = 0, bg = 4;
+  for (int g : {1, 2, 4, 8, 16})
+    for (int spb : {2, 4, 8})
+      for (int pers : {0, 1}) {
+        if (abx_batch_set_xcd_group(bt, g) || abx_batch_set_launch(bt, spb, pers, 0) || abx_batch_steps(bt, 2, 0))
+          return -1;
+        double ms = 0;
+        if (abx_batch_time_steps(bt, trial_steps, 0, &ms)) return -1;
+        if (ms < best) {
+          best = ms;
+          bs = spb;
+          bp = pers;
+          bg = g;
+        }
       }
-    }
-  if (abx_batch_set_launch(bt, bs, bp, graph)) return -1;
+  if (abx_batch_set_xcd_group(bt, bg) || abx_batch_set_launch(bt, bs, bp, graph)) return -1;
   // trials advanced the members: start them again from f0 with empty averages
   for (abx_sim* s : bt->members)
     if (abx_sim_reset(s)) return -1;
@@ -344,8 +369,8 @@ extern "C" void abx_batch_destroy(abx_batch* bt) {
   if (!bt) return;
   if (bt->stream) (void)hipStreamSynchronize(bt->stream);
   drop_graphs(bt);
-  for (auto* p : bt->d_work)
-    if (p) (void)hipFree(p);
+  for (auto& kv : bt->lists)
+    if (kv.second.first) (void)hipFree(kv.second.first);
   for (auto* p : bt->d_args)
     if (p) (void)hipFree(p);
   if (bt->ev0) (void)hipEventDestroy(bt->ev0);

@@ -284,8 +284,16 @@ int abx_batch_sync(abx_batch* batch);
  * persistent grid, graph = capture each chunk of steps once as a HIP graph
  * and replay it. */
 int abx_batch_set_launch(abx_batch* batch, int spb, int persistent, int graph);
-/* Autotuner: times every launch shape over trial_steps steps, keeps the
- * fastest, then resets every member to the cold start (abx_sim_reset). */
+/* Workgroups per XCD group (1, 2, 4, 8 or 16; default 4): consecutive
+ * blocks of the work list that one XCD takes in turn, all XCDs sweeping the
+ * members together (the single-handle launch's block order, abx_sim_set_launch
+ * xcd_map).  abx_batch_set_launch leaves it as it is. */
+int abx_batch_set_xcd_group(abx_batch* batch, int group);
+int abx_batch_get_xcd_group(const abx_batch* batch);
+/* Autotuner: times every launch shape (segments per workgroup x persistent
+ * grid x XCD group, 30 shapes) over trial_steps steps, keeps the fastest (the
+ * group through abx_batch_get_xcd_group), then resets every member to the
+ * cold start (abx_sim_reset). */
 int abx_batch_tune(abx_batch* batch, int trial_steps, int* best_spb, int* best_persistent, double* best_ms);
 /* Time n steps with HIP events on the batch's stream. */
 int abx_batch_time_steps(abx_batch* batch, int n, int doacc, double* ms);

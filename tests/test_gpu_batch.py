@@ -41,17 +41,20 @@ def _advance(stepper, plan):
 PLAN = [(3, False), (4, False), (5, True), (2, True)]
 
 
-@pytest.mark.parametrize("shape", [(2, False, False), (4, False, False), (8, True, False), (4, False, True),
-                                   (8, False, True), (2, True, True)])
+@pytest.mark.parametrize("shape", [(2, False, False, 4), (4, False, False, 4), (8, True, False, 4), (4, False, True, 4),
+                                   (8, False, True, 4), (2, True, True, 4), (4, False, True, 1), (2, False, False, 2),
+                                   (8, True, True, 8), (4, False, False, 16)])
 def test_batch_members_match_solo_runs(shape):
-    spb, pers, graph = shape
+    spb, pers, graph, group = shape
     say("batch shape", shape)
     solo = _sims("f32")
     for s in solo:
         _advance(lambda n, acc: s.steps(n, acc), PLAN)
     members = _sims("f32")
     b = abx.Batch(dev(), members)
+    b.set_xcd_group(group)
     b.set_launch(spb, pers, graph)
+    assert b.xcd_group == group
     _advance(b.steps, PLAN)
     b.sync()
     for i, (s, m) in enumerate(zip(solo, members)):
@@ -111,7 +114,9 @@ def test_batch_tune_picks_a_shape_and_resets():
     b = abx.Batch(dev(), members)
     b.steps(6, True)
     res = b.tune(10)
-    assert res["spb"] in (2, 4, 8) and res["ms_per_step"] > 0
+    assert res["spb"] in (2, 4, 8) and res["ms_per_step"] > 0 and res["xcd_group"] in (1, 2, 4, 8, 16)
+    with pytest.raises(abx.AbxError):
+        b.set_xcd_group(3)
     for m in members:
         f = m.read_f()
         NF = m.NF
