@@ -174,6 +174,7 @@ def load() -> C.CDLL:
         "abx_sim_set_launch": (i32, [vp, i32, i32, i32]),
         "abx_sim_pattern_time": (i32, [vp, i32, C.POINTER(d), C.POINTER(d)]),
         "abx_sim_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
+        "abx_sim_set_occupancy": (i32, [vp, i32, i32]),
         "abx_sim_destroy": (None, [vp]),
         "abx_sim_reset": (i32, [vp]),
         "abx_sim_set_graph": (i32, [vp, i32]),
@@ -185,6 +186,8 @@ def load() -> C.CDLL:
         "abx_batch_tune": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(d)]),
         "abx_batch_set_xcd_group": (i32, [vp, i32]),
         "abx_batch_get_xcd_group": (i32, [vp]),
+        "abx_batch_set_occupancy": (i32, [vp, i32]),
+        "abx_batch_get_occupancy": (i32, [vp]),
         "abx_batch_time_steps": (i32, [vp, i32, i32, C.POINTER(d)]),
         "abx_batch_destroy": (None, [vp]),
         "abx_geom_eval_view": (i32, [vp, C.POINTER(EvalGeom)]),
@@ -582,6 +585,11 @@ class Sim:
         (abx_sim_set_launch; default 19)."""
         _check(load().abx_sim_set_launch(self.h, kernel, xcd_map, nt))
 
+    def set_occupancy(self, even_waves_per_cu: int = 16, odd_waves_per_cu: int = 0) -> None:
+        """Occupancy caps of the AA steps in waves per CU, 0 = none
+        (abx_sim_set_occupancy; the defaults are the library's)."""
+        _check(load().abx_sim_set_occupancy(self.h, even_waves_per_cu, odd_waves_per_cu))
+
     def pattern_time(self, iters: int = 20) -> tuple:
         """(ms even, ms odd): the AA steps' memory pattern alone on this
         handle's lattice (abx_sim_pattern_time).  Moves the populations."""
@@ -640,7 +648,8 @@ class Batch:
         The XCD group it picks stays set (set_launch does not change it)."""
         spb, pers, ms = C.c_int(), C.c_int(), C.c_double()
         _check(load().abx_batch_tune(self.h, trial_steps, C.byref(spb), C.byref(pers), C.byref(ms)))
-        return dict(spb=spb.value, persistent=bool(pers.value), xcd_group=self.xcd_group, ms_per_step=ms.value)
+        return dict(spb=spb.value, persistent=bool(pers.value), xcd_group=self.xcd_group,
+                    even_waves_per_cu=load().abx_batch_get_occupancy(self.h), ms_per_step=ms.value)
 
     @property
     def xcd_group(self) -> int:
@@ -648,6 +657,9 @@ class Batch:
 
     def set_xcd_group(self, group: int) -> None:
         _check(load().abx_batch_set_xcd_group(self.h, group))
+
+    def set_occupancy(self, even_waves_per_cu: int) -> None:
+        _check(load().abx_batch_set_occupancy(self.h, even_waves_per_cu))
 
     def time_steps(self, n: int, doacc: bool = False) -> float:
         ms = C.c_double()

@@ -39,4 +39,5 @@ struct abx_dev {
   char name[256];
   int ncu;
   int xcds;
+  int lds_per_cu;  // bytes of LDS per CU (the workgroups resident on a CU share it)
 };

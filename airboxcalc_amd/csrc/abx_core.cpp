@@ -64,6 +64,7 @@ extern "C" int abx_device_open(int ordinal, abx_dev** dev, char* name, size_t na
            p.multiProcessorCount);
   d->ncu = p.multiProcessorCount;
   d->xcds = 8;
+  d->lds_per_cu = p.maxSharedMemoryPerMultiProcessor > 0 ? (int)p.maxSharedMemoryPerMultiProcessor : 160 * 1024;
   if (name && name_len) snprintf(name, name_len, "%s", d->name);
   *dev = d;
   return 0;

@@ -50,6 +50,7 @@ struct abx_batch {
   std::vector<abx_sim*> members;
   int prec = ABX_F32;
   int spb = 4, persistent = 0, use_graph = 0, xcd_group = 4;
+  int occ_even = 16;  // waves per CU of the even AA step (0: no cap), as a single handle's default
   StepArgs* d_args[2] = {nullptr, nullptr};  // member launch arguments without / with averaging
   std::vector<std::vector<int32_t>> segs;    // every member's active segments (lattice order)
   // work lists by (segments per workgroup, workgroups per XCD group): the
@@ -119,6 +120,10 @@ int upload_args(abx_batch* bt) {
 template <typename S, typename M, bool ACC, int MODE>
 void launch_b(abx_batch* bt) {
   const StepArgs* args = bt->d_args[ACC ? 1 : 0];
+  // the even step's occupancy cap, as reserved LDS (no more workgroups fit on
+  // a CU; the kernel uses no LDS)
+  const int wgs = MODE == 1 && bt->occ_even > 0 ? std::max(1, bt->occ_even / bt->spb) : 0;
+  const size_t lds = wgs ? (size_t)(bt->dev->lds_per_cu / (wgs + 1) + 1024) : 0;
   const auto& L = bt->lists.at(std::make_pair(bt->spb, bt->xcd_group));
   const int2* w = L.first;
   const int nb = L.second;
@@ -126,10 +131,10 @@ void launch_b(abx_batch* bt) {
 #define ABX_BL(SPB)                                                                                                \
   do {                                                                                                             \
     if (bt->persistent)                                                                                            \
-      hipLaunchKernelGGL((k_batch_step<S, M, ACC, MODE, SPB, true>), dim3(grid), dim3(64 * SPB), 0, bt->stream,     \
+      hipLaunchKernelGGL((k_batch_step<S, M, ACC, MODE, SPB, true>), dim3(grid), dim3(64 * SPB), lds, bt->stream,   \
                          args, w, nb);                                                                             \
     else                                                                                                           \
-      hipLaunchKernelGGL((k_batch_step<S, M, ACC, MODE, SPB, false>), dim3(grid), dim3(64 * SPB), 0, bt->stream,    \
+      hipLaunchKernelGGL((k_batch_step<S, M, ACC, MODE, SPB, false>), dim3(grid), dim3(64 * SPB), lds, bt->stream,  \
                          args, w, nb);                                                                             \
   } while (0)
   if (bt->spb == 2)
@@ -248,6 +253,18 @@ extern "C" int abx_batch_set_xcd_group(abx_batch* bt, int group) {
 
 extern "C" int abx_batch_get_xcd_group(const abx_batch* bt) { return bt ? bt->xcd_group : -1; }
 
+extern "C" int abx_batch_set_occupancy(abx_batch* bt, int even_waves_per_cu) {
+  if (!bt || even_waves_per_cu < 0 || even_waves_per_cu > 64) {
+    set_error("abx_batch_set_occupancy: waves per CU must be 0 (no cap) to 64");
+    return -1;
+  }
+  bt->occ_even = even_waves_per_cu;
+  drop_graphs(bt);
+  return 0;
+}
+
+extern "C" int abx_batch_get_occupancy(const abx_batch* bt) { return bt ? bt->occ_even : -1; }
+
 extern "C" int abx_batch_steps(abx_batch* bt, int n, int doacc) {
   if (!bt || n < 0) {
     set_error("abx_batch_steps: bad argument");
@@ -355,7 +372,20 @@ extern "C" int abx_batch_tune(abx_batch* bt, int trial_steps, int* best_spb, int
           bg = g;
         }
       }
-  if (abx_batch_set_xcd_group(bt, bg) || abx_batch_set_launch(bt, bs, bp, graph)) return -1;
+  if (abx_batch_set_xcd_group(bt, bg) || abx_batch_set_launch(bt, bs, bp, 0)) return -1;
+  // the best shape without and with the even step's occupancy cap
+  const int occ0 = bt->occ_even;
+  int bo = occ0;
+  for (int occ : {occ0 == 0 ? 16 : 0}) {
+    if (abx_batch_set_occupancy(bt, occ) || abx_batch_steps(bt, 2, 0)) return -1;
+    double ms = 0;
+    if (abx_batch_time_steps(bt, trial_steps, 0, &ms)) return -1;
+    if (ms < best) {
+      best = ms;
+      bo = occ;
+    }
+  }
+  if (abx_batch_set_occupancy(bt, bo) || abx_batch_set_launch(bt, bs, bp, graph)) return -1;
   // trials advanced the members: start them again from f0 with empty averages
   for (abx_sim* s : bt->members)
     if (abx_sim_reset(s)) return -1;

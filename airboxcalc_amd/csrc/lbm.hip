@@ -306,6 +306,10 @@ struct abx_sim {
   // nt: non-temporal accesses of the AA steps, seg_cell's NT flags of the even
   // step in bits 0-7 and of the odd step in bits 8-15.
   int kernel = 4, nt = 19;
+  // waves per CU of the AA even and odd steps (0: as many as registers
+  // allow).  The even step, a pure stream of 38 accesses per lane, runs
+  // faster with fewer requests in flight: 16 waves (4 workgroups) per CU.
+  int occ_even = 16, occ_odd = 0;
   // stored form of the populations: 0 plain (f[cur][q](x) = f_q), 1 AA
   // swapped (F[opp q](x) = f_q), 2 AA streamed (after an odd AA step)
   int form = 1;
@@ -371,6 +375,15 @@ StepArgs step_args(abx_sim* s, int doacc) {
   return a;
 }
 
+// Dynamic LDS that caps a launch of `waves_per_wg`-wave workgroups at
+// `waves` waves per CU (0: no cap): one more workgroup than allowed can
+// never fit beside the others.  The stream-collide itself uses no LDS.
+size_t occupancy_lds(const abx_dev* d, int waves, int waves_per_wg) {
+  if (waves <= 0) return 0;
+  const int wgs = std::max(1, waves / waves_per_wg);
+  return (size_t)(d->lds_per_cu / (wgs + 1) + 1024);
+}
+
 // Grid of the segment kernels for a's block order (sets a.per_xcd).
 unsigned seg_grid_x(StepArgs& a) {
   const int nblk = (a.nchunks + 3) / 4;
@@ -384,16 +397,17 @@ template <typename S, typename M, int MODE, int NT>
 void launch_seg(abx_sim* s, StepArgs a, int doacc) {
   dim3 grid(seg_grid_x(a), (unsigned)s->B), block(256);
   const bool ns = a.ns != nullptr;
+  const size_t lds = occupancy_lds(s->dev, MODE == 1 ? s->occ_even : (MODE == 2 ? s->occ_odd : 0), 4);
   if (doacc) {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, MODE, NT>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, true, MODE, NT>), grid, block, lds, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, MODE, NT>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, true, false, MODE, NT>), grid, block, lds, s->stream, a);
   } else {
     if (ns)
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, MODE, NT>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, true, MODE, NT>), grid, block, lds, s->stream, a);
     else
-      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, MODE, NT>), grid, block, 0, s->stream, a);
+      hipLaunchKernelGGL((k_stream_collide_seg<S, M, false, false, MODE, NT>), grid, block, lds, s->stream, a);
   }
 }
 
@@ -1166,6 +1180,17 @@ extern "C" int abx_sim_set_launch(abx_sim* s, int kernel, int xcd_map, int nt) {
 }
 
 
+extern "C" int abx_sim_set_occupancy(abx_sim* s, int even_waves_per_cu, int odd_waves_per_cu) {
+  if (!s || even_waves_per_cu < 0 || odd_waves_per_cu < 0 || even_waves_per_cu > 64 || odd_waves_per_cu > 64) {
+    set_error("abx_sim_set_occupancy: waves per CU must be 0 (no cap) to 64");
+    return -1;
+  }
+  sim_drop_graphs(s);
+  s->occ_even = even_waves_per_cu;
+  s->occ_odd = odd_waves_per_cu;
+  return 0;
+}
+
 extern "C" int abx_sim_pattern_time(abx_sim* s, int iters, double* ms_even, double* ms_odd) {
   if (!s || iters < 1 || !ms_even || !ms_odd) {
     set_error("abx_sim_pattern_time: bad argument");
@@ -1184,6 +1209,7 @@ extern "C" int abx_sim_pattern_time(abx_sim* s, int iters, double* ms_even, doub
   // pairs of steps in the order the stored form asks for, so the form after
   // every pair is the one before (the populations themselves are moved)
   const bool even_first = s->form == 2;
+  const size_t lds_even = occupancy_lds(s->dev, s->occ_even, 4), lds_odd = occupancy_lds(s->dev, s->occ_odd, 4);
   float te = 0, to = 0;
   int rc = 0;
   for (int i = 0; i <= iters && !rc; ++i) {  // pair 0 warms up, untimed
@@ -1191,9 +1217,9 @@ extern "C" int abx_sim_pattern_time(abx_sim* s, int iters, double* ms_even, doub
       const bool even = (k == 0) == even_first;
       if (hipEventRecord(ev[k], s->stream) != hipSuccess) rc = -1;
       if (even)
-        hipLaunchKernelGGL((k_pattern_seg<1, 19>), grid, block, 0, s->stream, a);
+        hipLaunchKernelGGL((k_pattern_seg<1, 19>), grid, block, lds_even, s->stream, a);
       else
-        hipLaunchKernelGGL((k_pattern_seg<2, 0>), grid, block, 0, s->stream, a);
+        hipLaunchKernelGGL((k_pattern_seg<2, 0>), grid, block, lds_odd, s->stream, a);
     }
     if (hipEventRecord(ev[2], s->stream) != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
         hipGetLastError() != hipSuccess)

@@ -257,6 +257,14 @@ int abx_sim_set_launch(abx_sim* sim, int kernel, int xcd_map, int nt);
  * Default non-temporal flags.  Needs a single float-storage handle on the AA
  * pattern; it moves the populations (a streaming without collision), so the
  * state afterwards is not a solve's: use a handle made for it, or reset. */
+/* Occupancy caps of the AA steps, in waves per CU (0: no cap; default 16 for
+ * the even step, none for the odd step): a launch reserves LDS it does not
+ * use so that no more workgroups fit on a CU.  The even step streams 38
+ * accesses per lane and measured 2.5 % faster at 16 waves per CU than at the
+ * 32 its registers allow.  Results are the same bit for bit.  No reference
+ * counterpart (a launch knob). */
+int abx_sim_set_occupancy(abx_sim* sim, int even_waves_per_cu, int odd_waves_per_cu);
+
 int abx_sim_pattern_time(abx_sim* sim, int iters, double* ms_even, double* ms_odd);
 /* Capture each chunk of abx_sim_steps once as a HIP graph and replay it
  * (fewer launch overheads on small grids).  Results are unchanged. */
@@ -290,10 +298,15 @@ int abx_batch_set_launch(abx_batch* batch, int spb, int persistent, int graph);
  * xcd_map).  abx_batch_set_launch leaves it as it is. */
 int abx_batch_set_xcd_group(abx_batch* batch, int group);
 int abx_batch_get_xcd_group(const abx_batch* batch);
+/* Occupancy cap of the batch's even AA steps in waves per CU (0: no cap;
+ * default 16, as abx_sim_set_occupancy's). */
+int abx_batch_set_occupancy(abx_batch* batch, int even_waves_per_cu);
+int abx_batch_get_occupancy(const abx_batch* batch);
 /* Autotuner: times every launch shape (segments per workgroup x persistent
- * grid x XCD group, 30 shapes) over trial_steps steps, keeps the fastest (the
- * group through abx_batch_get_xcd_group), then resets every member to the
- * cold start (abx_sim_reset). */
+ * grid x XCD group, 30 shapes) over trial_steps steps, then the fastest with
+ * the even step's occupancy cap toggled, keeps the fastest (the group and the
+ * cap through abx_batch_get_xcd_group / abx_batch_get_occupancy), then resets
+ * every member to the cold start (abx_sim_reset). */
 int abx_batch_tune(abx_batch* batch, int trial_steps, int* best_spb, int* best_persistent, double* best_ms);
 /* Time n steps with HIP events on the batch's stream. */
 int abx_batch_time_steps(abx_batch* batch, int n, int doacc, double* ms);

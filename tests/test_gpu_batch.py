@@ -41,11 +41,12 @@ def _advance(stepper, plan):
 PLAN = [(3, False), (4, False), (5, True), (2, True)]
 
 
-@pytest.mark.parametrize("shape", [(2, False, False, 4), (4, False, False, 4), (8, True, False, 4), (4, False, True, 4),
-                                   (8, False, True, 4), (2, True, True, 4), (4, False, True, 1), (2, False, False, 2),
-                                   (8, True, True, 8), (4, False, False, 16)])
+@pytest.mark.parametrize("shape", [(2, False, False, 4, 16), (4, False, False, 4, 0), (8, True, False, 4, 16),
+                                   (4, False, True, 4, 16), (8, False, True, 4, 0), (2, True, True, 4, 8),
+                                   (4, False, True, 1, 16), (2, False, False, 2, 0), (8, True, True, 8, 32),
+                                   (4, False, False, 16, 16)])
 def test_batch_members_match_solo_runs(shape):
-    spb, pers, graph, group = shape
+    spb, pers, graph, group, occ = shape
     say("batch shape", shape)
     solo = _sims("f32")
     for s in solo:
@@ -53,6 +54,7 @@ def test_batch_members_match_solo_runs(shape):
     members = _sims("f32")
     b = abx.Batch(dev(), members)
     b.set_xcd_group(group)
+    b.set_occupancy(occ)
     b.set_launch(spb, pers, graph)
     assert b.xcd_group == group
     _advance(b.steps, PLAN)
@@ -115,6 +117,7 @@ def test_batch_tune_picks_a_shape_and_resets():
     b.steps(6, True)
     res = b.tune(10)
     assert res["spb"] in (2, 4, 8) and res["ms_per_step"] > 0 and res["xcd_group"] in (1, 2, 4, 8, 16)
+    assert res["even_waves_per_cu"] in (0, 16)
     with pytest.raises(abx.AbxError):
         b.set_xcd_group(3)
     for m in members:

@@ -193,6 +193,15 @@ def test_write_read_round_trip_and_launch_variants_agree(precision):
         assert np.array_equal(b.read_f(), want_f), (kern, xm, nt)
         assert np.array_equal(b.read_acc(), want_acc), (kern, xm, nt)
         b.destroy()
+    # occupancy caps (reserved LDS) change nothing but the launch
+    for occ in ((0, 0), (8, 24), (32, 16), (4, 4)):
+        _, b = _sim(fx, precision)
+        b.set_occupancy(*occ)
+        b.write_f(f)
+        b.steps(10, doacc=True)
+        assert np.array_equal(b.read_f(), want_f), occ
+        assert np.array_equal(b.read_acc(), want_acc), occ
+        b.destroy()
     # switching scheme mid-run (AA pattern <-> two lattices) carries the state over
     _, c = _sim(fx, precision)
     c.write_f(f)
