@@ -383,3 +383,104 @@ def spec_from_dict(d: dict) -> AirboxSpec:
     d = dict(d)
     d["baffles"] = [Baffle(**b) for b in d.get("baffles", [])]
     return AirboxSpec(**d)
+
+
+# -- variant sets on disk: STL files plus one manifest -----------------------
+
+# cell size each family is solved at when its variants carry none
+FAMILY_DX = {"roofline": 1.0, "surrogate": 2.0}
+
+
+def _variant_name(kind: str, seed: int) -> str:
+    return f"{kind}_{int(seed):07d}.stl"
+
+
+def write_variant_set(out_dir: str, kind: str, seeds: Sequence[int], subdiv: int = 1, ascii: bool = False) -> dict:
+    """Write the seeded variants of one family as STL files into out_dir and
+    describe them in out_dir/manifest.json: per variant its seed, file name,
+    triangle count, the SHA-256 of the file, the full spec (enough to
+    regenerate the mesh), the per-variant run parameters (dx, K, aniso), and
+    the parameter objects of the reference's prepare and run for it.  Returns
+    the manifest."""
+    import hashlib
+    import json
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    variants = []
+    for seed in seeds:
+        spec, extra = seeded_variant(int(seed), kind)
+        spec.subdiv = int(subdiv)
+        tris = airbox_triangles(spec)
+        name = _variant_name(kind, seed)
+        path = os.path.join(out_dir, name)
+        (write_stl_ascii if ascii else write_stl_binary)(path, tris)
+        with open(path, "rb") as fh:
+            sha = hashlib.sha256(fh.read()).hexdigest()
+        dx = float(extra.get("dx", FAMILY_DX.get(kind, 2.0)))
+        variants.append({"seed": int(seed), "file": name, "format": "ascii" if ascii else "binary",
+                         "triangles": int(len(tris)), "sha256": sha, "spec": spec_dict(spec),
+                         "run": {"dx": dx, "K": float(extra.get("K", 3.0)), "aniso": float(extra.get("aniso", 10.0))},
+                         "prepare": prepare_params(spec, dx)})
+    manifest = {"generator": "airboxcalc_amd.geometry.seeded_variant", "kind": kind, "subdiv": int(subdiv),
+                "count": len(variants), "variants": variants}
+    with open(os.path.join(out_dir, "manifest.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1)
+    return manifest
+
+
+def verify_variant_set(out_dir: str) -> List[str]:
+    """Check a written set against its manifest: every file's SHA-256, its
+    triangle count, and that regenerating the mesh from the recorded spec
+    gives the file's triangles bit for bit.  Returns the problems found."""
+    import hashlib
+    import json
+    import os
+    with open(os.path.join(out_dir, "manifest.json")) as fh:
+        manifest = json.load(fh)
+    problems = []
+    for v in manifest["variants"]:
+        path = os.path.join(out_dir, v["file"])
+        with open(path, "rb") as fh:
+            data = fh.read()
+        if hashlib.sha256(data).hexdigest() != v["sha256"]:
+            problems.append(f"{v['file']}: SHA-256 differs")
+        tris = airbox_triangles(spec_from_dict(v["spec"]))
+        if len(tris) != v["triangles"]:
+            problems.append(f"{v['file']}: {len(tris)} triangles regenerated, {v['triangles']} recorded")
+        elif v["format"] == "binary" and data != stl_binary_bytes(tris):
+            problems.append(f"{v['file']}: the recorded spec does not regenerate the file")
+    return problems
+
+
+def _parse_seeds(text: str) -> List[int]:
+    """'0-63' or '1,5,9' or a mix ('0-3,10')."""
+    out = []
+    for part in text.split(","):
+        a, _, b = part.partition("-")
+        out.extend(range(int(a), int(b) + 1) if b else [int(a)])
+    return out
+
+
+def main(argv: Optional[Sequence[str]] = None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m airboxcalc_amd.geometry",
+                                 description="write seeded airbox variants as STL with a manifest, or verify a set")
+    ap.add_argument("--kind", default="batch", choices=["roofline", "batch", "surrogate"])
+    ap.add_argument("--seeds", default="0-63", help="'0-63', '1,5,9' or a mix")
+    ap.add_argument("--subdiv", type=int, default=1, help="split every planar quad into subdiv x subdiv cells")
+    ap.add_argument("--ascii", action="store_true")
+    ap.add_argument("--out", required=True, help="output folder (STL files and manifest.json)")
+    ap.add_argument("--verify", action="store_true", help="check an existing folder against its manifest")
+    a = ap.parse_args(argv)
+    if a.verify:
+        problems = verify_variant_set(a.out)
+        for p in problems:
+            print(p)
+        return 1 if problems else 0
+    m = write_variant_set(a.out, a.kind, _parse_seeds(a.seeds), a.subdiv, a.ascii)
+    print(f"{m['count']} {a.kind} variants written to {a.out} ({sum(v['triangles'] for v in m['variants'])} triangles)")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

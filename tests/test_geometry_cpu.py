@@ -109,3 +109,50 @@ def test_plain_box_volume_in_closed_form(dims, h):
     assert int(vox.sum()) == int(per_col.sum())
     # and column by column
     assert np.array_equal(vox.sum(2), per_col)
+
+
+@pytest.mark.parametrize("kind,seeds,subdiv,ascii_", [("batch", [0, 1, 7], 1, False), ("roofline", [2, 5], 2, False),
+                                                      ("surrogate", [1_000_003], 1, True)])
+def test_variant_set_manifest_regenerates_every_file(tmp_path, kind, seeds, subdiv, ascii_):
+    """A written variant set (STL files + manifest.json, the generator's
+    record of seeds and parameters): every file's hash and triangle count
+    match the manifest, the recorded spec regenerates the binary files bit
+    for bit, the recorded run parameters are the seeded draws, a file read
+    back through the oracle's parseSTL keeps every column's hit count even,
+    and a changed byte is reported."""
+    import json
+    out = str(tmp_path / kind)
+    m = G.write_variant_set(out, kind, seeds, subdiv=subdiv, ascii=ascii_)
+    assert m["count"] == len(seeds) and [v["seed"] for v in m["variants"]] == seeds
+    assert G.verify_variant_set(out) == []
+    with open(tmp_path / kind / "manifest.json") as fh:
+        assert json.load(fh) == m
+    for v in m["variants"]:
+        spec, extra = G.seeded_variant(v["seed"], kind)
+        assert v["run"]["K"] == extra.get("K", 3.0) and v["run"]["aniso"] == extra.get("aniso", 10.0)
+        assert v["run"]["dx"] == (extra["dx"] if "dx" in extra else G.FAMILY_DX[kind])
+        assert v["prepare"]["dx"] == v["run"]["dx"]
+    v0 = m["variants"][0]
+    with open(tmp_path / kind / v0["file"], "rb") as fh:
+        data = fh.read()
+    tris = O.parse_stl(data).reshape(-1, 9)
+    assert len(tris) == v0["triangles"]
+    lo, n = _grid(tris, 1.3)
+    cnt = O.voxel_counts(tris, lo, int(n[0]), int(n[1]), 1.3)
+    assert cnt.sum() > 0 and (cnt % 2 == 0).all()
+    # a changed byte in a file no longer matches its recorded hash
+    bad = bytearray(data)
+    bad[-3] ^= 0x01
+    with open(tmp_path / kind / v0["file"], "wb") as fh:
+        fh.write(bytes(bad))
+    problems = G.verify_variant_set(out)
+    assert problems and v0["file"] in problems[0]
+
+
+def test_variant_set_command_line(tmp_path):
+    out = str(tmp_path / "cli")
+    assert G.main(["--kind", "batch", "--seeds", "3-4,9", "--out", out]) == 0
+    assert G.main(["--verify", "--out", out]) == 0
+    import json
+    with open(tmp_path / "cli" / "manifest.json") as fh:
+        assert [v["seed"] for v in json.load(fh)["variants"]] == [3, 4, 9]
