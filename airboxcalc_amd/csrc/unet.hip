@@ -206,9 +206,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // pair s for lg < 2, tap B for lg >= 2, channels 8 (lg % 2) .. + 8), 14
 // pairs for 27 taps (the last half empty), with one weight fragment per pair.
 // The 14 weight fragments (14 KB) sit in LDS beside the brick, not in
-// registers: 98 registers leave 4 waves per SIMD (two 8-wave workgroups per
-// CU, 48 KB of LDS each), where holding them in registers left 2, and the
-// brick loads of one workgroup overlap the MFMAs of the other.  Eight
+// registers (holding them in registers left 2 waves per SIMD).  The brick
+// and the weights are loaded straight into LDS (global_load_lds), so no
+// registers stage them: 73 registers and 48 KB of LDS per workgroup leave
+// three 8-wave workgroups per CU (6 waves per SIMD), whose brick loads overlap
+// each other's MFMAs; staged through registers (98) two fit.  Eight
 // z-slices per workgroup share one brick (halo 2.1 voxels loaded per output
 // voxel, 2.5 with four) and one copy of the weights.
 //
@@ -311,15 +313,33 @@ __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
+// Sixteen zero bytes: the source of the brick pieces outside the grid when the
+// brick is loaded straight into LDS.
+__device__ u32x4 g_zero16[1];
+
+// 16-byte load from global memory straight into LDS: lane l of the wave
+// writes bytes [16 l, 16 l + 16) after lds (wave-uniform).
+__device__ __forceinline__ void load_lds16(const void* src, void* lds) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                   (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+}
+
 // Channel parts: input chunks c < nch1 come from x, the others from x2
 // (chunk c - nch1 there); output tiles blockIdx.y < ncot1 go to y, the others
 // to y2 (tile blockIdx.y - ncot1), so a concatenation can be two tensors.
-__global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
+// DMA: the brick and the weights go from global memory straight into LDS
+// (global_load_lds_dwordx4; brick piece e of the layout below is LDS byte
+// 16 e, so the 64 pieces of a wave's slot are one contiguous kilobyte), with
+// no staging registers; else through registers, the next chunk's loads issued
+// before the barrier that ends this chunk's MFMAs.
+template <bool DMA, int F16DMA_WAVES = 6>
+__global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
                                                     int nch1, const u16* __restrict__ wt,
                                                     const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2,
                                                     int ncot1, const u16* __restrict__ gate, int gs, int D, int H,
                                                     int W, int Ci, int ntiles, int relu, int accum) {
-  __shared__ __attribute__((aligned(16))) u16 sx[F16BV * F16SP];
+  // whole slots of 64 pieces: the last wave of the last slot writes past the brick
+  __shared__ __attribute__((aligned(16))) u16 sx[(2 * F16BV + 63) / 64 * 64 * 8];
   __shared__ u32x4 sw[14 * 64];
   const int co0 = blockIdx.y * 16, nch = Ci / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4, hi = lg >> 1;
@@ -335,7 +355,39 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
     const int bx = vb % F16BX, r = vb / F16BX, by = r % F16BY, bz = r / F16BY;
     sb[m] = e < 2 * F16BV ? bx | by << 5 | bz << 8 | (e & 1) << 12 : -1;
   }
-  u32x4 pf[F16SLOTS];
+  auto fetch_lds = [&](int t, int c) {
+    const F16Tile tl = f16_tile(t, D, H, W);
+    const bool p2 = c >= nch1;
+    const u16* xb = p2 ? x2 : x;
+    const int xsb = p2 ? xs2 : xs, cb = p2 ? c - nch1 : c;
+#pragma unroll
+    for (int m = 0; m < F16SLOTS; ++m) {
+      if (512 * m + 64 * wave >= 2 * F16BV) continue;  // the wave's pieces are all past the brick
+      const void* src = g_zero16;
+      // decode the slot here: hoisted out of the tile loop, its 15 fields
+      // were spilled at six waves per SIMD
+      int sv = sb[m];
+      asm volatile("" : "+v"(sv));
+      if (sv >= 0) {
+        const int gx = tl.x0 + (sv & 31) - 1, gy = tl.y0 + ((sv >> 5) & 7) - 1, gz = tl.z0 + ((sv >> 8) & 15) - 1;
+        if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
+          src = xb + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xsb + 16 * cb + 8 * (sv >> 12);
+      }
+      load_lds16(src, sx + (512 * m + 64 * wave) * 8);
+    }
+  };
+  auto load_weights_lds = [&](int c) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      if (512 * m + 64 * wave >= 14 * 64) continue;
+      const int s = 8 * m + wave, l_r = lane & 15, l_g = lane >> 4;
+      const int tap = (l_g >> 1) ? f16_tapB(s) : f16_tapA(s);
+      const void* src = g_zero16;
+      if (tap < 27) src = wt + ((int64_t)(co0 + l_r) * 27 + tap) * Ci + 16 * c + 8 * (l_g & 1);
+      load_lds16(src, sw + 512 * m + 64 * wave);
+    }
+  };
+  u32x4 pf[F16SLOTS];  // unused with DMA
   auto fetch = [&](int t, int c) {
     const F16Tile tl = f16_tile(t, D, H, W);
     const bool p2 = c >= nch1;
@@ -366,7 +418,12 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
   };
 
   if (nch == 1) {
-    load_weights(0);
+    if constexpr (DMA) {
+      load_weights_lds(0);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
+      load_weights(0);
+    }
     __syncthreads();
   }
   // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
@@ -379,15 +436,22 @@ __global__ __launch_bounds__(512) void k_conv_fwd16(const u16* __restrict__ x, i
 #pragma unroll
     for (int r = 0; r < F16Y; ++r) acc[r] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int c = 0; c < nch; ++c) {
-      fetch(t, c);
-      __syncthreads();  // the previous chunk's MFMAs are done with the brick
+      if constexpr (DMA) {
+        __syncthreads();  // the previous chunk's MFMAs are done with the brick and the weights
+        fetch_lds(t, c);
+        if (nch > 1) load_weights_lds(c);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      } else {
+        fetch(t, c);
+        __syncthreads();  // the previous chunk's MFMAs are done with the brick
 #pragma unroll
-      for (int m = 0; m < F16SLOTS; ++m)
-        if (sb[m] >= 0) {
-          const int vb = ((sb[m] >> 8) & 15) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
-          *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 12)) = pf[m];
-        }
-      if (nch > 1) load_weights(c);
+        for (int m = 0; m < F16SLOTS; ++m)
+          if (sb[m] >= 0) {
+            const int vb = ((sb[m] >> 8) & 15) * F16BY * F16BX + ((sb[m] >> 5) & 7) * F16BX + (sb[m] & 31);
+            *(u32x4*)(sx + vb * F16SP + 8 * (sb[m] >> 12)) = pf[m];
+          }
+        if (nch > 1) load_weights(c);
+      }
       __syncthreads();
       // fragment read i + 2 is issued before the MFMAs of read i, so LDS
       // latency is not waited out at every read; scheduling barriers keep the
@@ -1369,6 +1433,10 @@ int g_wgrad_tr = 1;
 // 3x3x3 weight gradients use k_wgrad27 (every tap in one workgroup); 0 = the
 // tap-group kernels above, for comparison.
 int g_wgrad27 = 1;
+// k_conv_fwd16 loads its brick straight into LDS (0: through registers) and
+// runs g_fwd16_wgs persistent workgroups per CU.
+int g_fwd16_dma = 1;
+int g_fwd16_wgs = 3;
 
 // Compute units of the current device (cached per device ordinal).
 int device_cus() {
@@ -1578,11 +1646,17 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
   hipStream_t s = (hipStream_t)stream;
   if (ksize == 3 && g_fwd16) {
     const int64_t tiles = (int64_t)n * ((d + F16Z - 1) / F16Z) * ((h + F16Y - 1) / F16Y) * ((w + F16X - 1) / F16X);
-    // persistent workgroups: about two (of eight waves) per CU across the co tiles
-    const int64_t per_co = std::max<int64_t>(1, (int64_t)2 * device_cus() / (co / 16));
+    // persistent workgroups: g_fwd16_wgs (of eight waves) per CU across the co tiles
+    const int64_t per_co = std::max<int64_t>(1, (int64_t)g_fwd16_wgs * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    hipLaunchKernelGGL(k_conv_fwd16, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2, xs2, ci1 / 16, wpack,
-                       bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu, accumulate);
+    if (g_fwd16_dma)
+      hipLaunchKernelGGL(k_conv_fwd16<true>, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2, xs2,
+                         ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu,
+                         accumulate);
+    else
+      hipLaunchKernelGGL(k_conv_fwd16<false>, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2, xs2,
+                         ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu,
+                         accumulate);
     ABX_HIP(hipGetLastError());
     return 0;
   }
@@ -1640,6 +1714,9 @@ extern "C" int abx_conv3d_set_kernel(int flags) {
   g_fwd16 = (flags & 1) ? 1 : 0;
   g_wgrad_tr = (flags & 2) ? 1 : 0;
   g_wgrad27 = (flags & 4) ? 0 : 1;
+  g_fwd16_dma = (flags & 8) ? 0 : 1;
+  g_fwd16_wgs = g_fwd16_dma ? 3 : 2;
+  if (flags & 16) g_fwd16_wgs = 2;
   return 0;
 }
 

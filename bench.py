@@ -281,7 +281,8 @@ def bench_unet(a, dev):
 
 def bench_conv(a, dev):
     """The surrogate's 16-channel 3x3x3 convolutions (batch 8, 96 x 48 x 64):
-    forward, the 16-channel kernel against the general one, and the weight
+    forward, the 16-channel kernel (its brick loaded straight into LDS, or
+    through registers) against the general one, and the weight
     gradient: every tap per workgroup (k_wgrad27, the default) against the
     tap-group kernels with transposing LDS reads and with gathering; each
     timed with HIP events over --steps calls."""
@@ -301,8 +302,11 @@ def bench_conv(a, dev):
         y = torch.empty((n, d, h, w, co), dtype=torch.bfloat16, device="cuda")
         us = {}
         try:
-            for k in (0, 1):
-                L.abx_conv3d_set_kernel(k | 2)   # bit 0: k_conv_fwd16; bit 1 (default): transposed weight gradient
+            # flags: bit 0 k_conv_fwd16 (else the general kernel), bit 1 (default) transposed weight gradient,
+            # bit 3 the brick through registers (else straight into LDS, three workgroups per CU), bit 4 two
+            # workgroups per CU
+            for k in (2, 3, 11, 19):
+                L.abx_conv3d_set_kernel(k)
                 for _ in range(3):
                     assert L.abx_conv3d_forward(s, x.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, 3, 1, 0,
                                                 y.data_ptr(), co) == 0
@@ -341,8 +345,9 @@ def bench_conv(a, dev):
         finally:
             L.abx_conv3d_set_kernel(3)
         fl = 2 * n * d * h * w * ci * co * 27
-        rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[1], 1), general_us=round(us[0], 1),
-                         fwd16_tflops=round(fl / us[1] / 1e6, 1), general_tflops=round(fl / us[0] / 1e6, 1),
+        rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[3], 1), general_us=round(us[2], 1),
+                         fwd16_regs_us=round(us[11], 1), fwd16_lds_2wg_us=round(us[19], 1),
+                         fwd16_tflops=round(fl / us[3] / 1e6, 1), general_tflops=round(fl / us[2] / 1e6, 1),
                          wgrad27_us=round(wg[3], 1), wgrad_tr_us=round(wg[7], 1), wgrad_gather_us=round(wg[5], 1),
                          wgrad27_tflops=round(fl / wg[3] / 1e6, 1), wgrad_tr_tflops=round(fl / wg[7] / 1e6, 1)))
     r0 = rows[0]
@@ -351,7 +356,8 @@ def bench_conv(a, dev):
         "value": r0["fwd16_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": calls, "warmup": 3,
         "ms_per_step": round(r0["fwd16_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random volumes)",
-        "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16 against k_conv_fwd) "
+        "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16, its brick straight "
+                               "into LDS or through registers, against k_conv_fwd) "
                                "and weight gradient (k_wgrad27 against k_conv_wgrad_tr and k_conv_wgrad)",
                    "shapes": rows},
         "roofline": {"bound": "mfma", "achieved": r0["fwd16_tflops"], "peak": 2500.0, "unit": "TFLOP/s",

@@ -466,7 +466,11 @@ int abx_conv3d_forward_parts(void* stream, const uint16_t* x, int xs, int ci1, c
  * fragments with gfx950's transposing LDS reads (ds_read_b64_tr_b16) instead
  * of gathering 16-bit values; bit 2: 3x3x3 weight gradients use those
  * tap-group kernels instead of the every-tap kernel (two-part inputs always
- * take the every-tap kernel).  Default 3.  The 16-channel and general forward
+ * take the every-tap kernel); bit 3: the 16-channel forward kernel stages its
+ * input brick and weights through registers (two workgroups per CU) instead
+ * of loading them straight into LDS (three per CU); bit 4: two workgroups per
+ * CU on the direct path too.  Bits 3 and 4 change timing only, not a bit of
+ * the output.  Default 3.  The 16-channel and general forward
  * kernels, and the two tap-group weight gradients, compute the same sums; the
  * every-tap weight gradient sums its partials in another grouping. */
 int abx_conv3d_set_kernel(int flags);

@@ -147,6 +147,48 @@ def test_conv_forward16_matches_general_kernel(co, ci):
     close_bf16(yg, ref, "k_conv_fwd vs float64")
 
 
+@pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 1, 0), (1, 6, 5, 20, 48, 16, 0, 1), (1, 9, 7, 33, 16, 48, 1, 1),
+                                  (1, 3, 6, 18, 32, 32, 0, 0), (1, 17, 13, 40, 16, 16, 0, 1)],
+                         ids=lambda c: "x".join(map(str, c)))
+def test_conv_forward16_brick_paths_equal(case):
+    """k_conv_fwd16 loads its input brick and weights straight into LDS
+    (global_load_lds, the default, three workgroups per CU) or through
+    registers (flag bit 3, two per CU); flag bit 4 keeps the direct loads at
+    two workgroups per CU. The LDS contents and the MFMA order are the same,
+    so the outputs are equal bit for bit: with and without accumulation and
+    ReLU, over several 16-channel input chunks, on grids that are not
+    multiples of the 16 x 4 x 8 tile (the zero halo comes from a zero source
+    on the direct path)."""
+    n, d, h, w, ci, co, relu, acc = case
+    say("conv forward16 brick paths", case)
+    rng = np.random.default_rng(12)
+    xt = bf(rng.standard_normal((n, d, h, w, ci)))
+    W = (rng.standard_normal((co, 27, ci)) / 20).astype(np.float32)
+    wp, _ = pack(W, 3)
+    bt = torch.as_tensor(rng.standard_normal(co).astype(np.float32)).to(_dev())
+    y0 = bf(rng.standard_normal((n, d, h, w, co)))
+    lib = abx.load()
+    outs = []
+    try:
+        for flags in (3, 11, 19):
+            ck(lib.abx_conv3d_set_kernel(flags))
+            y = y0.clone()
+            ck(lib.abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(), co,
+                                      3, relu, acc, y.data_ptr(), co))
+            torch.cuda.synchronize()
+            outs.append(y.view(torch.int16).cpu())
+    finally:
+        ck(lib.abx_conv3d_set_kernel(3))
+    assert torch.equal(outs[0], outs[1])
+    assert torch.equal(outs[0], outs[2])
+    ref = R.direct_conv3d(f32(xt), f32(wp).reshape(co, 27, ci), bt.cpu().numpy(), 3, relu=False)
+    if acc:
+        ref = ref + f32(y0)
+    if relu:
+        ref = np.maximum(ref, 0)
+    close_bf16(f32(outs[0].view(torch.bfloat16)), ref, "k_conv_fwd16 (LDS-direct brick) vs float64")
+
+
 @pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 3), (1, 6, 5, 20, 32, 16, 3), (1, 4, 8, 16, 16, 32, 3),
                                   (1, 3, 7, 33, 32, 32, 3), (2, 5, 4, 17, 16, 32, 1), (1, 3, 6, 18, 32, 64, 3),
                                   (1, 3, 5, 16, 16, 64, 3), (1, 2, 3, 16, 32, 64, 1)],
