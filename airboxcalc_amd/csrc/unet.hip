@@ -824,7 +824,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tr(const u16* __restrict__ x
 // are dealt to the 8 waves (item q = wave + 8 u); each meets the NCT dZ
 // fragments of the k step, so one B fragment read feeds NCT MFMAs.  The next
 // tile's global loads are issued before this tile's MFMAs and land in
-// registers, which the next iteration stores to LDS.  Fragments are built
+// registers, which the next iteration stores to LDS; or, where the channel
+// blocking then fits more workgroups per CU (w27_wgs), the tiles are loaded
+// straight into LDS and the other workgroups' MFMAs cover the loads (16 ->
+// 16 channels at 96x48x64: 61 against 67 us).  Fragments are built
 // with transposing LDS reads as in k_conv_wgrad_tr; partial rows
 // [S][Co 27 Ci + Co] as there.  Bias partials come from the ci block 0
 // workgroups: their wave 7 multiplies every dZ fragment by a fragment of
@@ -847,11 +850,21 @@ __device__ __forceinline__ Tile w27_tile(int t, int D, int H, int W, int TZW) {
 // Channel parts as in k_conv_fwd16: input channels c < ci1 come from x, the
 // others from x2 (channel c - ci1 there), chosen per 8-channel staging piece,
 // so an input block may straddle ci1 (a multiple of 16).
+// WGS > 0: the dZ tile and the brick go from global memory straight into LDS
+// (piece e of either is LDS byte 16 e after its base), with no staging
+// registers and no prefetch of the next tile, and WGS workgroups share a CU
+// so that one's loads overlap the others' MFMAs; WGS = 0: staged through
+// registers, the next tile's loads issued before this tile's MFMAs.
 template <int NCT, int NCI, int TZW>
-__global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int xs, const u16* x2, int xs2, int ci1,
-                                                 const u16* __restrict__ dz, int dzs, float* __restrict__ part,
-                                                 float* __restrict__ partb, int D, int H, int W, int Ci, int Co,
-                                                 int ntiles) {
+constexpr int w27_lds_bytes() {
+  return (W27X * W27Y * TZW * 16 * NCT + ((W27BX * W27BY * (TZW + 2) * NCI * 2 + 63) / 64 * 64) * 8) * 2;
+}
+
+template <int NCT, int NCI, int TZW, int WGS>
+__global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
+    const u16* __restrict__ x, int xs, const u16* x2, int xs2, int ci1, const u16* __restrict__ dz, int dzs,
+    float* __restrict__ part, float* __restrict__ partb, int D, int H, int W, int Ci, int Co, int ntiles) {
+  constexpr bool DMA = WGS > 0;
   constexpr int W27V = W27X * W27Y * TZW, W27BZ = TZW + 2, W27BV = W27BX * W27BY * W27BZ;
   constexpr int CO = 16 * NCT, CI = 16 * NCI, PZ = CO, PX = CI;
   constexpr int NI = 27 * NCI, MAXI = (NI + 7) / 8;
@@ -859,7 +872,9 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
   constexpr int ZSL = (ZP + 511) / 512, XSL = (XP + 511) / 512;
   static_assert(ZP % 512 == 0, "dZ pieces fill whole slots");
   __shared__ __attribute__((aligned(16))) u16 sz[W27V * PZ];
-  __shared__ __attribute__((aligned(16))) u16 sx[W27BV * PX];
+  // whole slots of 64 pieces: with direct loads the last wave of the last slot writes past the brick
+  __shared__ __attribute__((aligned(16))) u16 sx[(XP + 63) / 64 * 64 * 8];
+  static_assert(sizeof(sz) + sizeof(sx) == w27_lds_bytes<NCT, NCI, TZW>(), "LDS per workgroup");
   const int ncb = Co / CO;
   const int cb = (int)blockIdx.y % ncb, cc = (int)blockIdx.y / ncb;
   const int co0 = cb * CO, ci0 = cc * CI;
@@ -917,6 +932,35 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
                                                   min(max(gx, 0), W - 1)) * (p2 ? xs2 : xs) + (p2 ? gc - ci1 : gc));
     }
   };
+  auto fetch_lds = [&](int t) {
+    const Tile tl = w27_tile(t, D, H, W, TZW);
+#pragma unroll
+    for (int m = 0; m < ZSL; ++m) {
+      const int e = (int)threadIdx.x + 512 * m;
+      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
+      const void* src = g_zero16;
+      if (gx < W && gy < H && gz < D) src = dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q;
+      load_lds16(src, sz + (512 * m + 64 * wave) * 8);
+    }
+#pragma unroll
+    for (int m = 0; m < XSL; ++m) {
+      if (512 * m + 64 * wave >= XP) continue;  // the wave's pieces are all past the brick
+      // decoded here, not hoisted out of the tile loop into registers
+      int sv = xsl[m];
+      asm volatile("" : "+v"(sv));
+      const int bx = sv & 31, by = (sv >> 5) & 7, bz = (sv >> 8) & 15, q = sv >> 12;
+      const int gx = tl.x0 + bx - 1, gy = tl.y0 + by - 1, gz = tl.z0 + bz - 1;
+      const void* src = g_zero16;
+      const bool in = (unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D;
+      if (in && (int)threadIdx.x + 512 * m < XP) {
+        const int gc = ci0 + 8 * q;
+        const bool p2 = gc >= ci1;
+        src = (p2 ? x2 : x) + vox(tl, D, H, W, gz, gy, gx) * (p2 ? xs2 : xs) + (p2 ? gc - ci1 : gc);
+      }
+      load_lds16(src, sx + (512 * m + 64 * wave) * 8);
+    }
+  };
   const u32x4 zero4 = {0u, 0u, 0u, 0u};
   const int lrow = lr >> 2, lcol = 4 * (lr & 3);
   f32x4 acc[MAXI][NCT];
@@ -929,23 +973,30 @@ __global__ __launch_bounds__(512) void k_wgrad27(const u16* __restrict__ x, int 
   for (int c = 0; c < NCT; ++c) accb[c] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
   int t = blockIdx.x;
-  if (t < ntiles) fetch(t);
+  if constexpr (!DMA)
+    if (t < ntiles) fetch(t);
   for (; t < ntiles; t += gridDim.x) {
     __syncthreads();  // the previous tile's fragment reads are done
+    if constexpr (DMA) {
+      fetch_lds(t);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    } else {
 #pragma unroll
-    for (int m = 0; m < ZSL; ++m) {
-      const int e = (int)threadIdx.x + 512 * m;
-      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
-      *(u32x4*)(sz + v * PZ + 8 * q) = ((zok >> m) & 1u) ? zv[m] : zero4;
-    }
+      for (int m = 0; m < ZSL; ++m) {
+        const int e = (int)threadIdx.x + 512 * m;
+        const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+        *(u32x4*)(sz + v * PZ + 8 * q) = ((zok >> m) & 1u) ? zv[m] : zero4;
+      }
 #pragma unroll
-    for (int m = 0; m < XSL; ++m) {
-      if (m == XSL - 1 && !xlast) break;
-      const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
-      *(u32x4*)(sx + ((bz * W27BY + by) * W27BX + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
+      for (int m = 0; m < XSL; ++m) {
+        if (m == XSL - 1 && !xlast) break;
+        const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
+        *(u32x4*)(sx + ((bz * W27BY + by) * W27BX + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
+      }
     }
     __syncthreads();
-    if (t + (int)gridDim.x < ntiles) fetch(t + (int)gridDim.x);
+    if constexpr (!DMA)
+      if (t + (int)gridDim.x < ntiles) fetch(t + (int)gridDim.x);
 #pragma unroll 1
     for (int s = 0; s < W27V / 32; ++s) {
       // k = voxels 32 s + 8 lg + e (e = 0..7), consecutive in x
@@ -1437,6 +1488,9 @@ int g_wgrad27 = 1;
 // runs g_fwd16_wgs persistent workgroups per CU.
 int g_fwd16_dma = 1;
 int g_fwd16_wgs = 3;
+// k_wgrad27 loads its tiles straight into LDS where its blocking allows more
+// than one workgroup per CU (0: staged through registers everywhere).
+int g_wgrad27_dma = 1;
 
 // Compute units of the current device (cached per device ordinal).
 int device_cus() {
@@ -1486,6 +1540,16 @@ struct W27Shape {
   int nct, nci, tz, S, blocks_y;
 };
 
+// Workgroups per CU of k_wgrad27's direct-load variant for a blocking: as many
+// as LDS holds, at most three (80 registers), two with four output tiles (64
+// accumulators: 128 registers); 0 where the blocking keeps the staged variant
+// (2 x 3 and 4 x 2 tiles: their accumulators leave one workgroup per CU, which
+// only the prefetch into registers overlaps, or spill at two).
+constexpr int w27_wgs(int nct, int nci, int tz) {
+  const int lds = (16 * 4 * tz * 16 * nct + ((18 * 6 * (tz + 2) * nci * 2 + 63) / 64 * 64) * 8) * 2;
+  return nct * nci >= 6 ? 0 : std::min(nct >= 4 ? 2 : 3, 160 * 1024 / lds);
+}
+
 // Channel blocking nci (1-3 tiles of 16 input channels) with the given
 // output blocking nct.
 W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, int nci) {
@@ -1503,7 +1567,8 @@ W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, 
   // one round of resident workgroups (two per CU for the blockings within
   // 128 registers, one for the wider ones; more rounds only add partial
   // sums to reduce), at most about 16 M partial sums
-  const int res = s.nct * s.nci <= 2 ? 2 : 1;
+  const int wgs = g_wgrad27_dma ? w27_wgs(s.nct, s.nci, s.tz) : 0;
+  const int res = wgs > 0 ? wgs : (s.nct * s.nci <= 2 ? 2 : 1);
   const int64_t want = (res * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
   const int64_t by_size = std::max<int64_t>(1, ((int64_t)16 << 20) / ((int64_t)co * 27 * ci + co));
   s.S = (int)std::max<int64_t>(1, std::min<int64_t>({nt, want, by_size}));
@@ -1717,6 +1782,7 @@ extern "C" int abx_conv3d_set_kernel(int flags) {
   g_fwd16_dma = (flags & 8) ? 0 : 1;
   g_fwd16_wgs = g_fwd16_dma ? 3 : 2;
   if (flags & 16) g_fwd16_wgs = 2;
+  g_wgrad27_dma = (flags & 32) ? 0 : 1;
   return 0;
 }
 
@@ -1732,6 +1798,21 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
 }
 
 namespace {
+template <int A, int B, int Z>
+void launch_w27(dim3 g, hipStream_t s, const u16* x, int xs, const u16* x2, int xs2, int ci1, const u16* dz, int dzs,
+                float* part, float* partb, int d, int h, int w, int ci, int co, int nt) {
+  constexpr int wgs = w27_wgs(A, B, Z);
+  static_assert(w27_lds_bytes<A, B, Z>() * wgs <= 160 * 1024, "LDS of the resident workgroups");
+  if constexpr (wgs > 0)
+    if (g_wgrad27_dma) {
+      hipLaunchKernelGGL((k_wgrad27<A, B, Z, wgs>), g, dim3(512), 0, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d,
+                         h, w, ci, co, nt);
+      return;
+    }
+  hipLaunchKernelGGL((k_wgrad27<A, B, Z, 0>), g, dim3(512), 0, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w,
+                     ci, co, nt);
+}
+
 int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1, const uint16_t* x2, int xs2,
                const uint16_t* dz, int dzs, int n, int d, int h, int w, int ci, int co, int ksize, float* workspace,
                int64_t ws_bytes, float* dw, float* db, int accumulate) {
@@ -1771,8 +1852,7 @@ int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1,
     const int nt27 = (int)((int64_t)n * ((d + s27.tz - 1) / s27.tz) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
     const dim3 g27((unsigned)s27.S, (unsigned)s27.blocks_y);
 #define ABX_W27(A, B, Z)                                                                                         \
-  hipLaunchKernelGGL((k_wgrad27<A, B, Z>), g27, dim3(512), 0, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w, \
-                     ci, co, nt27)
+  launch_w27<A, B, Z>(g27, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w, ci, co, nt27)
     if (s27.tz == 8) ABX_W27(1, 1, 8);
     else if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2, 4);
     else if (s27.nct == 4) ABX_W27(4, 1, 4);

@@ -320,8 +320,8 @@ def bench_conv(a, dev):
                 us[k] = e0.elapsed_time(e1) / calls * 1e3
         finally:
             L.abx_conv3d_set_kernel(3)
-        # weight gradient of the same shape: k_wgrad27 (flags 3), the tap-group kernel on transposing LDS
-        # reads (7) and gathering (5)
+        # weight gradient of the same shape: k_wgrad27 (flags 3; 35: its tiles staged through registers), the
+        # tap-group kernel on transposing LDS reads (7) and gathering (5)
         dz = torch.randn((n, d, h, w, co), generator=g).to(torch.bfloat16).cuda()
         wsb = L.abx_conv3d_wgrad_workspace(n, d, h, w, ci, co, 3)
         ws = torch.empty(wsb // 4, dtype=torch.float32, device="cuda")
@@ -329,7 +329,7 @@ def bench_conv(a, dev):
         db = torch.empty(co, dtype=torch.float32, device="cuda")
         wg = {}
         try:
-            for k in (5, 7, 3):
+            for k in (5, 7, 3, 35):
                 L.abx_conv3d_set_kernel(k)
                 for _ in range(3):
                     assert L.abx_conv3d_wgrad(s, x.data_ptr(), ci, dz.data_ptr(), co, n, d, h, w, ci, co, 3,
@@ -348,7 +348,8 @@ def bench_conv(a, dev):
         rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[3], 1), general_us=round(us[2], 1),
                          fwd16_regs_us=round(us[11], 1), fwd16_lds_2wg_us=round(us[19], 1),
                          fwd16_tflops=round(fl / us[3] / 1e6, 1), general_tflops=round(fl / us[2] / 1e6, 1),
-                         wgrad27_us=round(wg[3], 1), wgrad_tr_us=round(wg[7], 1), wgrad_gather_us=round(wg[5], 1),
+                         wgrad27_us=round(wg[3], 1), wgrad27_regs_us=round(wg[35], 1), wgrad_tr_us=round(wg[7], 1),
+                         wgrad_gather_us=round(wg[5], 1),
                          wgrad27_tflops=round(fl / wg[3] / 1e6, 1), wgrad_tr_tflops=round(fl / wg[7] / 1e6, 1)))
     r0 = rows[0]
     out = {

@@ -426,7 +426,10 @@ def test_wgrad27_every_channel_blocking(case):
     not multiples of its 16x4x4 tile (the last case has enough tiles for the
     16x4x8 tiles of large grids): weights and bias within 1e-3 of the
     range of torch's FP32 autograd on the same bf16 operands, within the
-    same bar of the tap-group kernel, and the same bits on a second call."""
+    same bar of the tap-group kernel and of k_wgrad27 staging its tiles
+    through registers (flag bit 5; the default loads them straight into LDS
+    where the blocking allows more workgroups per CU, so the partial sums
+    may be grouped in more rows), and the same bits on a second call."""
     n, d, h, w, ci, co = case
     say("wgrad27", case)
     rng = np.random.default_rng(8)
@@ -449,6 +452,7 @@ def test_wgrad27_every_channel_blocking(case):
         dw, db = run(3)
         dw2, db2 = run(3)
         dw_old, db_old = run(7)
+        dw_st, db_st = run(35)
     finally:
         ck(lib.abx_conv3d_set_kernel(3))
     assert np.array_equal(dw, dw2) and np.array_equal(db, db2)
@@ -463,6 +467,7 @@ def test_wgrad27_every_channel_blocking(case):
     assert np.abs(dw - ref_dw).max() <= 1e-3 * rng_w, f"weights {case}: {np.abs(dw - ref_dw).max():.3e} of {rng_w:.3e}"
     assert np.abs(db - ref_db).max() <= 1e-3 * rng_b, f"bias {case}"
     assert np.abs(dw - dw_old).max() <= 1e-3 * rng_w and np.abs(db - db_old).max() <= 1e-3 * rng_b
+    assert np.abs(dw - dw_st).max() <= 1e-3 * rng_w and np.abs(db - db_st).max() <= 1e-3 * rng_b
 
 
 @pytest.mark.parametrize("case", [(2, 5, 9, 37, 32, 16, 16), (1, 6, 5, 20, 64, 32, 32), (1, 3, 5, 18, 128, 64, 64),
