@@ -205,12 +205,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // the k-steps of 32 then cover a pair of taps each (k = 32 s + 8 lg: tap A of
 // pair s for lg < 2, tap B for lg >= 2, channels 8 (lg % 2) .. + 8), 14
 // pairs for 27 taps (the last half empty), with one weight fragment per pair.
-// The 14 weight fragments (14 KB) sit in LDS beside the brick, not in
-// registers (holding them in registers left 2 waves per SIMD).  The brick
-// and the weights are loaded straight into LDS (global_load_lds), so no
-// registers stage them: 73 registers and 48 KB of LDS per workgroup leave
-// three 8-wave workgroups per CU (6 waves per SIMD), whose brick loads overlap
-// each other's MFMAs; staged through registers (98) two fit.  Eight
+// The brick and the 14 weight fragments (14 KB) are loaded straight into LDS
+// (global_load_lds), so no registers stage them, and each wave then reads the
+// weight fragments into 56 registers once per chunk: 32 LDS fragment reads
+// per wave and tile instead of 88 (the weights read from LDS for each of the
+// 56 MFMAs).  124 registers leave two 8-wave workgroups per CU (4 waves per
+// SIMD), whose brick loads overlap each other's MFMAs.  Measured against the
+// weights read from LDS (73 registers, three workgroups per CU) and against
+// brick and weights staged through registers (98, two per CU; with the
+// weights in registers as well only one fitted): 57, 58 and 64 us for 16 ->
+// 16 channels at 96x48x64, 153, 175 and 182 us for 48 -> 16.  Eight
 // z-slices per workgroup share one brick (halo 2.1 voxels loaded per output
 // voxel, 2.5 with four) and one copy of the weights.
 //
@@ -331,8 +335,11 @@ __device__ __forceinline__ void load_lds16(const void* src, void* lds) {
 // (global_load_lds_dwordx4; brick piece e of the layout below is LDS byte
 // 16 e, so the 64 pieces of a wave's slot are one contiguous kilobyte), with
 // no staging registers; else through registers, the next chunk's loads issued
-// before the barrier that ends this chunk's MFMAs.
-template <bool DMA, int F16DMA_WAVES = 6>
+// before the barrier that ends this chunk's MFMAs.  F16DMA_WAVES: waves per
+// SIMD the register allocation must allow.
+// WREG: the 14 weight fragments are held in registers (56 of them), not read
+// from LDS for every MFMA.
+template <bool DMA, int F16DMA_WAVES = 6, bool WREG = false>
 __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
                                                     int nch1, const u16* __restrict__ wt,
                                                     const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2,
@@ -417,6 +424,12 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
     }
   };
 
+  // with WREG the weights still arrive in LDS and are read into registers once per chunk
+  u32x4 wr[WREG ? 14 : 1];
+  auto read_weights = [&]() {
+#pragma unroll
+    for (int s = 0; s < (WREG ? 14 : 0); ++s) wr[s] = sw[s * 64 + lane];
+  };
   if (nch == 1) {
     if constexpr (DMA) {
       load_weights_lds(0);
@@ -425,6 +438,7 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
       load_weights(0);
     }
     __syncthreads();
+    read_weights();
   }
   // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
   const int co = co0 + 4 * lg;
@@ -453,6 +467,7 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
         if (nch > 1) load_weights(c);
       }
       __syncthreads();
+      if (nch > 1) read_weights();
       // fragment read i + 2 is issued before the MFMAs of read i, so LDS
       // latency is not waited out at every read; scheduling barriers keep the
       // compiler from sinking the reads below those MFMAs
@@ -481,7 +496,7 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
         u32x4 wa[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-          if (f.w[k] >= 0) wa[k] = sw[f.w[k] * 64 + lane];
+          if (f.w[k] >= 0) wa[k] = WREG ? wr[WREG ? f.w[k] : 0] : sw[f.w[k] * 64 + lane];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
           if (f.w[k] >= 0) acc[f.r[k]] = mfma16(wa[k], fr[i % 3], acc[f.r[k]]);
@@ -1484,10 +1499,12 @@ int g_wgrad_tr = 1;
 // 3x3x3 weight gradients use k_wgrad27 (every tap in one workgroup); 0 = the
 // tap-group kernels above, for comparison.
 int g_wgrad27 = 1;
-// k_conv_fwd16 loads its brick straight into LDS (0: through registers) and
-// runs g_fwd16_wgs persistent workgroups per CU.
-int g_fwd16_dma = 1;
-int g_fwd16_wgs = 3;
+// k_conv_fwd16: 0 = brick and weights loaded straight into LDS, the weights
+// then held in registers (two workgroups per CU); 1 = the weights read from
+// LDS for every MFMA (g_fwd16_wgs workgroups per CU); 2 = brick and weights
+// staged through registers (two per CU).
+int g_fwd16_mode = 0;
+int g_fwd16_wgs = 2;
 // k_wgrad27 loads its tiles straight into LDS where its blocking allows more
 // than one workgroup per CU (0: staged through registers everywhere).
 int g_wgrad27_dma = 1;
@@ -1714,7 +1731,11 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
     // persistent workgroups: g_fwd16_wgs (of eight waves) per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)g_fwd16_wgs * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    if (g_fwd16_dma)
+    if (g_fwd16_mode == 0)
+      hipLaunchKernelGGL((k_conv_fwd16<true, 4, true>), dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2,
+                         xs2, ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu,
+                         accumulate);
+    else if (g_fwd16_mode == 1)
       hipLaunchKernelGGL(k_conv_fwd16<true>, dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2, xs2,
                          ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu,
                          accumulate);
@@ -1779,9 +1800,8 @@ extern "C" int abx_conv3d_set_kernel(int flags) {
   g_fwd16 = (flags & 1) ? 1 : 0;
   g_wgrad_tr = (flags & 2) ? 1 : 0;
   g_wgrad27 = (flags & 4) ? 0 : 1;
-  g_fwd16_dma = (flags & 8) ? 0 : 1;
-  g_fwd16_wgs = g_fwd16_dma ? 3 : 2;
-  if (flags & 16) g_fwd16_wgs = 2;
+  g_fwd16_mode = (flags & 8) ? 2 : ((flags & 64) ? 1 : 0);
+  g_fwd16_wgs = g_fwd16_mode == 1 && !(flags & 16) ? 3 : 2;
   g_wgrad27_dma = (flags & 32) ? 0 : 1;
   return 0;
 }

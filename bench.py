@@ -281,8 +281,9 @@ def bench_unet(a, dev):
 
 def bench_conv(a, dev):
     """The surrogate's 16-channel 3x3x3 convolutions (batch 8, 96 x 48 x 64):
-    forward, the 16-channel kernel (its brick loaded straight into LDS, or
-    through registers) against the general one, and the weight
+    forward, the 16-channel kernel (its weights held in registers, read
+    from LDS, or its brick and weights staged through registers) against
+    the general one, and the weight
     gradient: every tap per workgroup (k_wgrad27, the default) against the
     tap-group kernels with transposing LDS reads and with gathering; each
     timed with HIP events over --steps calls."""
@@ -303,9 +304,9 @@ def bench_conv(a, dev):
         us = {}
         try:
             # flags: bit 0 k_conv_fwd16 (else the general kernel), bit 1 (default) transposed weight gradient,
-            # bit 3 the brick through registers (else straight into LDS, three workgroups per CU), bit 4 two
-            # workgroups per CU
-            for k in (2, 3, 11, 19):
+            # bit 3 brick and weights through registers (else straight into LDS), bit 6 the weights read from
+            # LDS for every MFMA (else held in registers; three workgroups per CU, two with bit 4)
+            for k in (2, 3, 11, 67, 83):
                 L.abx_conv3d_set_kernel(k)
                 for _ in range(3):
                     assert L.abx_conv3d_forward(s, x.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), None, co, 3, 1, 0,
@@ -346,7 +347,8 @@ def bench_conv(a, dev):
             L.abx_conv3d_set_kernel(3)
         fl = 2 * n * d * h * w * ci * co * 27
         rows.append(dict(ci=ci, co=co, gflop=round(fl / 1e9, 2), fwd16_us=round(us[3], 1), general_us=round(us[2], 1),
-                         fwd16_regs_us=round(us[11], 1), fwd16_lds_2wg_us=round(us[19], 1),
+                         fwd16_staged_us=round(us[11], 1), fwd16_lds_weights_us=round(us[67], 1),
+                         fwd16_lds_weights_2wg_us=round(us[83], 1),
                          fwd16_tflops=round(fl / us[3] / 1e6, 1), general_tflops=round(fl / us[2] / 1e6, 1),
                          wgrad27_us=round(wg[3], 1), wgrad27_regs_us=round(wg[35], 1), wgrad_tr_us=round(wg[7], 1),
                          wgrad_gather_us=round(wg[5], 1),
@@ -357,8 +359,8 @@ def bench_conv(a, dev):
         "value": r0["fwd16_tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": calls, "warmup": 3,
         "ms_per_step": round(r0["fwd16_us"] / 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic (random volumes)",
-        "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16, its brick straight "
-                               "into LDS or through registers, against k_conv_fwd) "
+        "config": {"workload": "the surrogate's 16-channel convolutions: forward (k_conv_fwd16 with its weights in "
+                               "registers, read from LDS, or staged through registers, against k_conv_fwd) "
                                "and weight gradient (k_wgrad27 against k_conv_wgrad_tr and k_conv_wgrad)",
                    "shapes": rows},
         "roofline": {"bound": "mfma", "achieved": r0["fwd16_tflops"], "peak": 2500.0, "unit": "TFLOP/s",

@@ -467,13 +467,14 @@ int abx_conv3d_forward_parts(void* stream, const uint16_t* x, int xs, int ci1, c
  * of gathering 16-bit values; bit 2: 3x3x3 weight gradients use those
  * tap-group kernels instead of the every-tap kernel (two-part inputs always
  * take the every-tap kernel); bit 3: the 16-channel forward kernel stages its
- * input brick and weights through registers (two workgroups per CU) instead
- * of loading them straight into LDS (three per CU); bit 4: two workgroups per
- * CU on the direct path too; bit 5: the every-tap weight gradient stages its
+ * input brick and weights through registers instead of loading them straight
+ * into LDS; bit 6: it reads the weights from LDS for every MFMA (three
+ * workgroups per CU; bit 4: two) instead of holding them in registers; bit
+ * 5: the every-tap weight gradient stages its
  * tiles through registers everywhere (else it loads them straight into LDS
  * where its channel blocking then fits more workgroups per CU, which may
- * group its partial sums in more rows).  Bits 3 and 4 change timing only, not
- * a bit of the output.  Default 3.  The 16-channel and general forward
+ * group its partial sums in more rows).  Bits 3, 4 and 6 change timing only,
+ * not a bit of the output.  Default 3.  The 16-channel and general forward
  * kernels, and the two tap-group weight gradients, compute the same sums; the
  * every-tap weight gradient sums its partials in another grouping. */
 int abx_conv3d_set_kernel(int flags);

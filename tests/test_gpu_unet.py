@@ -152,9 +152,10 @@ def test_conv_forward16_matches_general_kernel(co, ci):
                          ids=lambda c: "x".join(map(str, c)))
 def test_conv_forward16_brick_paths_equal(case):
     """k_conv_fwd16 loads its input brick and weights straight into LDS
-    (global_load_lds, the default, three workgroups per CU) or through
-    registers (flag bit 3, two per CU); flag bit 4 keeps the direct loads at
-    two workgroups per CU. The LDS contents and the MFMA order are the same,
+    (global_load_lds) and holds the weights in registers (the default), or
+    reads them from LDS for every MFMA (flag bit 6; three workgroups per CU,
+    two with bit 4), or stages brick and weights through registers (flag bit
+    3). The LDS contents and the MFMA order are the same,
     so the outputs are equal bit for bit: with and without accumulation and
     ReLU, over several 16-channel input chunks, on grids that are not
     multiples of the 16 x 4 x 8 tile (the zero halo comes from a zero source
@@ -170,7 +171,7 @@ def test_conv_forward16_brick_paths_equal(case):
     lib = abx.load()
     outs = []
     try:
-        for flags in (3, 11, 19):
+        for flags in (3, 11, 67, 83):
             ck(lib.abx_conv3d_set_kernel(flags))
             y = y0.clone()
             ck(lib.abx_conv3d_forward(stream(), xt.data_ptr(), ci, n, d, h, w, ci, wp.data_ptr(), bt.data_ptr(), co,
@@ -181,6 +182,7 @@ def test_conv_forward16_brick_paths_equal(case):
         ck(lib.abx_conv3d_set_kernel(3))
     assert torch.equal(outs[0], outs[1])
     assert torch.equal(outs[0], outs[2])
+    assert torch.equal(outs[0], outs[3])
     ref = R.direct_conv3d(f32(xt), f32(wp).reshape(co, 27, ci), bt.cpu().numpy(), 3, relu=False)
     if acc:
         ref = ref + f32(y0)
