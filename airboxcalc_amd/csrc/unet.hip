@@ -229,9 +229,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, int
 // Workgroups are persistent over tiles.  With 16 input channels the weights
 // are loaded once per workgroup (else once per 16-channel chunk, between the
 // brick's two barriers); each thread owns fixed slots of the brick
-// (decoded once).  The bias is read once per workgroup, not per tile, so the
-// epilogue issues no load that it has to wait for (unless it accumulates or
-// is gated).
+// (decoded once).  The biases are staged in LDS once per workgroup and
+// read there by each epilogue, so it issues no global load that it has to
+// wait for (unless it accumulates or is gated); its addresses are a scalar
+// base per tile plus one 32-bit lane offset.
 constexpr int F16X = 16, F16Y = 4, F16Z = 8;
 constexpr int F16BX = F16X + 2, F16BY = F16Y + 2, F16BZ = F16Z + 2, F16BV = F16BX * F16BY * F16BZ;
 constexpr int F16SP = 16;  // u16 per brick voxel: 16 channels, unpadded (measured faster than 24, 32, 40)
@@ -310,6 +311,13 @@ __device__ __forceinline__ F16Tile f16_tile(int t, int D, int H, int W) {
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// max(v, 0) on the bits: one integer max (the float max would first quiet
+// NaNs); -0 and negative NaNs give +0, positive NaNs pass.
+__device__ __forceinline__ float relu_bits(float v) {
+  const int b = __builtin_bit_cast(int, v);
+  return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
 
 // Two floats to packed bf16, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
@@ -424,6 +432,9 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
     }
   };
 
+  // the tile's biases, read in each epilogue (after the tile's barriers)
+  __shared__ __attribute__((aligned(16))) float sbias[16];
+  if (threadIdx.x < 16) sbias[threadIdx.x] = bias ? bias[co0 + threadIdx.x] : 0.f;
   // with WREG the weights still arrive in LDS and are read into registers once per chunk
   u32x4 wr[WREG ? 14 : 1];
   auto read_weights = [&]() {
@@ -442,9 +453,14 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
   }
   // C element i of lane l: output channel co0 + 4 lg + i, voxel lr of row r
   const int co = co0 + 4 * lg;
-  float bv[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) bv[i] = bias ? bias[co + i] : 0.f;
+  // output: the lane's offset from a tile's first voxel (its z-slice, voxel
+  // lr, channels), below 2^30 (checked by the host)
+  const bool o2 = (int)blockIdx.y >= ncot1;
+  u16* const yb = o2 ? y2 : y;
+  const int ysb = o2 ? ys2 : ys;
+  const uint32_t yoffb =
+      2u * (uint32_t)((wave * H * W + lr) * ysb + (o2 ? ((int)blockIdx.y - ncot1) * 16 : co0) + 4 * lg);
+  const uint32_t yrowb = 2u * (uint32_t)(W * ysb);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     f32x4 acc[F16Y];
 #pragma unroll
@@ -504,16 +520,21 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
       });
     }
 
+    // a scalar base per tile (its first voxel) and the lane's 32-bit offset
     const F16Tile tl = f16_tile(t, D, H, W);
-    const int gx = tl.x0 + lr, gz = tl.z0 + wave;
-    const bool o2 = (int)blockIdx.y >= ncot1;
-    u16* yb = o2 ? y2 : y;
-    const int ysb = o2 ? ys2 : ys, cob = (o2 ? ((int)blockIdx.y - ncot1) * 16 : co0) + 4 * lg;
+    const int64_t v0 = (((int64_t)tl.n * D + tl.z0) * H + tl.y0) * W + tl.x0;
+    char* const ytile = (char*)(yb + v0 * ysb);
+    const bool lane_ok = tl.x0 + lr < W && tl.z0 + wave < D;
+    // the row offsets derive from yo here: hoisted out of the tile loop they
+    // held four 64-bit registers
+    uint32_t yo = yoffb;
+    asm volatile("" : "+v"(yo));
+    const f32x4 bv = *(const f32x4*)(sbias + 4 * lg);
 #pragma unroll
     for (int r = 0; r < F16Y; ++r) {
-      const int gy = tl.y0 + r;
-      if (gx >= W || gy >= H || gz >= D) continue;
-      u16* yp = yb + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * ysb + cob;
+      if (tl.y0 + r >= H) break;
+      if (!lane_ok) continue;
+      u16* yp = (u16*)(ytile + (yo + (uint32_t)r * yrowb));
       float v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) v[i] = acc[r][i] + bv[i];
@@ -526,11 +547,12 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
       }
       if (relu)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = v[i] > 0.f ? v[i] : 0.f;
+        for (int i = 0; i < 4; ++i) v[i] = relu_bits(v[i]);
       u32x2 o;
       o.x = pack_bf16(v[0], v[1]);
       o.y = pack_bf16(v[2], v[3]);
-      if (gate) o = gate4(o, *(const u32x2*)(gate + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * gs + co));
+      if (gate)
+        o = gate4(o, *(const u32x2*)(gate + v0 * gs + (uint32_t)(((wave * H + r) * W + lr) * gs + co)));
       *(u32x2*)yp = o;
     }
   }
@@ -1710,8 +1732,12 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
     return -1;
   }
   const bool parts = ci1 < ci || co1 < co;
-  if (parts && (ksize != 3 || !g_fwd16 || gate)) {
-    set_errorf("%s: channel parts need the 16-channel 3x3x3 kernel and no gate", fn);
+  // k_conv_fwd16 addresses its outputs by 32-bit offsets within 8 planes
+  const bool f16_fits =
+      (int64_t)F16Z * h * w * std::max({ys, parts ? ys2 : ys, gate ? gs : 0}) < ((int64_t)1 << 30);
+  if (parts && (ksize != 3 || !g_fwd16 || gate || !f16_fits)) {
+    set_errorf("%s: channel parts need the 16-channel 3x3x3 kernel, no gate and 8 planes of output below 2^30 "
+               "elements", fn);
     return -1;
   }
   if (ci1 == ci) {
@@ -1726,7 +1752,7 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
   const int nct = co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1);
   const dim3 grid((unsigned)tiles_of(n, d, h, w), (unsigned)(co / (16 * nct)));
   hipStream_t s = (hipStream_t)stream;
-  if (ksize == 3 && g_fwd16) {
+  if (ksize == 3 && g_fwd16 && f16_fits) {
     const int64_t tiles = (int64_t)n * ((d + F16Z - 1) / F16Z) * ((h + F16Y - 1) / F16Y) * ((w + F16X - 1) / F16X);
     // persistent workgroups: g_fwd16_wgs (of eight waves) per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)g_fwd16_wgs * device_cus() / (co / 16));
