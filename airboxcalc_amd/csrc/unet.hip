@@ -362,33 +362,52 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
   const int bx1 = lbase + (hi ? F16SP : 0), bxy = lbase + (hi ? F16BX * F16SP : 0),
             bxz = lbase + (hi ? F16BY * F16BX * F16SP : 0);
 
-  // brick slots: piece e = thread + 512 m is voxel e / 2 (x fastest), channels 8 (e % 2) .. + 8
-  int sb[F16SLOTS];  // bx | by << 5 | bz << 8 | g << 12, or -1
+  // brick slots: piece e = thread + 512 m is voxel e / 2 (x fastest),
+  // channels 8 (e % 2) .. + 8; two slot codes per register, bx | by << 5 |
+  // bz << 8 | g << 12 in bits 16 (m % 2) .., bit 13 set past the brick
+  uint32_t sb2[(F16SLOTS + 1) / 2] = {};
 #pragma unroll
   for (int m = 0; m < F16SLOTS; ++m) {
     const int e = threadIdx.x + 512 * m, vb = e >> 1;
     const int bx = vb % F16BX, r = vb / F16BX, by = r % F16BY, bz = r / F16BY;
-    sb[m] = e < 2 * F16BV ? bx | by << 5 | bz << 8 | (e & 1) << 12 : -1;
+    const uint32_t code = e < 2 * F16BV ? (uint32_t)(bx | by << 5 | bz << 8 | (e & 1) << 12) : 0x2000u;
+    sb2[m >> 1] |= code << (16 * (m & 1));
   }
+  // slot m's code, or -1 past the brick
+  auto slot_code = [](uint32_t w, int m) -> int {
+    const uint32_t c = (w >> (16 * (m & 1))) & 0x3fffu;
+    return (c & 0x2000u) ? -1 : (int)c;
+  };
+  int sb[F16SLOTS];  // the staged path's codes, decoded once
+#pragma unroll
+  for (int m = 0; m < F16SLOTS; ++m) sb[m] = DMA ? 0 : slot_code(sb2[m >> 1], m);
+  // a buffer resource based at the brick's origin voxel (z0 - 1, y0 - 1,
+  // x0 - 1; it may lie before the tensor, only in-grid pieces are read):
+  // each piece is a 32-bit byte offset from it (below 2^31, checked by the
+  // host), and pieces outside the grid take offset 2^31, beyond the
+  // resource's range, which the hardware reads as zeros
   auto fetch_lds = [&](int t, int c) {
     const F16Tile tl = f16_tile(t, D, H, W);
     const bool p2 = c >= nch1;
     const u16* xb = p2 ? x2 : x;
     const int xsb = p2 ? xs2 : xs, cb = p2 ? c - nch1 : c;
+    const char* org =
+        (const char*)xb + ((((int64_t)tl.n * D + tl.z0 - 1) * H + tl.y0 - 1) * W + tl.x0 - 1) * xsb * 2 + 32 * cb;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)org, (short)0, 0x7fffffff, 0x00020000);
 #pragma unroll
     for (int m = 0; m < F16SLOTS; ++m) {
       if (512 * m + 64 * wave >= 2 * F16BV) continue;  // the wave's pieces are all past the brick
-      const void* src = g_zero16;
       // decode the slot here: hoisted out of the tile loop, its 15 fields
       // were spilled at six waves per SIMD
-      int sv = sb[m];
-      asm volatile("" : "+v"(sv));
-      if (sv >= 0) {
-        const int gx = tl.x0 + (sv & 31) - 1, gy = tl.y0 + ((sv >> 5) & 7) - 1, gz = tl.z0 + ((sv >> 8) & 15) - 1;
-        if ((unsigned)gx < (unsigned)W && (unsigned)gy < (unsigned)H && (unsigned)gz < (unsigned)D)
-          src = xb + ((((int64_t)tl.n * D + gz) * H + gy) * W + gx) * xsb + 16 * cb + 8 * (sv >> 12);
-      }
-      load_lds16(src, sx + (512 * m + 64 * wave) * 8);
+      uint32_t sw2 = sb2[m >> 1];
+      asm volatile("" : "+v"(sw2));
+      const int sv = slot_code(sw2, m);
+      const int bx = sv & 31, by = (sv >> 5) & 7, bz = (sv >> 8) & 15;
+      const bool in = sv >= 0 && (unsigned)(tl.x0 + bx - 1) < (unsigned)W &&
+                      (unsigned)(tl.y0 + by - 1) < (unsigned)H && (unsigned)(tl.z0 + bz - 1) < (unsigned)D;
+      const uint32_t off = in ? (uint32_t)(((bz * H + by) * W + bx) * xsb * 2 + 16 * (sv >> 12)) : 0x80000000u;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(sx + (512 * m + 64 * wave) * 8),
+                                               16, off, 0, 0, 0);
     }
   };
   auto load_weights_lds = [&](int c) {
@@ -1732,12 +1751,14 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
     return -1;
   }
   const bool parts = ci1 < ci || co1 < co;
-  // k_conv_fwd16 addresses its outputs by 32-bit offsets within 8 planes
+  // k_conv_fwd16 addresses its outputs by 32-bit offsets within 8 planes and
+  // its brick by 32-bit byte offsets within 10
   const bool f16_fits =
-      (int64_t)F16Z * h * w * std::max({ys, parts ? ys2 : ys, gate ? gs : 0}) < ((int64_t)1 << 30);
+      (int64_t)F16Z * h * w * std::max({ys, parts ? ys2 : ys, gate ? gs : 0}) < ((int64_t)1 << 30) &&
+      (int64_t)(F16Z + 2) * h * w * std::max(xs, ci1 < ci ? xs2 : xs) * 2 < ((int64_t)1 << 31);
   if (parts && (ksize != 3 || !g_fwd16 || gate || !f16_fits)) {
-    set_errorf("%s: channel parts need the 16-channel 3x3x3 kernel, no gate and 8 planes of output below 2^30 "
-               "elements", fn);
+    set_errorf("%s: channel parts need the 16-channel 3x3x3 kernel, no gate, 8 planes of output below 2^30 "
+               "elements and 10 planes of input below 2^31 bytes", fn);
     return -1;
   }
   if (ci1 == ci) {
