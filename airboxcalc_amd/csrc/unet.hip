@@ -988,17 +988,47 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
                                                   min(max(gx, 0), W - 1)) * (p2 ? xs2 : xs) + (p2 ? gc - ci1 : gc));
     }
   };
+  // dZ tile through a buffer resource based at the tile's first voxel, the
+  // brick through one based at its origin voxel when the input block lies in
+  // one tensor: 32-bit byte offsets (below 2^31, checked by the host), and
+  // pieces outside the grid at offset 2^31, read as zeros by the range check
+  const bool x_one = ci1 >= ci0 + CI || ci1 <= ci0;
   auto fetch_lds = [&](int t) {
     const Tile tl = w27_tile(t, D, H, W, TZW);
+    const __amdgpu_buffer_rsrc_t rz = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(dz + vox(tl, D, H, W, tl.z0, tl.y0, tl.x0) * dzs + co0), (short)0, 0x7fffffff, 0x00020000);
 #pragma unroll
     for (int m = 0; m < ZSL; ++m) {
       const int e = (int)threadIdx.x + 512 * m;
       const int v = e / (NCT * 2), q = e - v * (NCT * 2);
-      const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
-      const void* src = g_zero16;
-      if (gx < W && gy < H && gz < D) src = dz + vox(tl, D, H, W, gz, gy, gx) * dzs + co0 + 8 * q;
-      load_lds16(src, sz + (512 * m + 64 * wave) * 8);
+      const int vx = v & 15, vy = (v >> 4) & 3, vz = v >> 6;
+      const bool in = tl.x0 + vx < W && tl.y0 + vy < H && tl.z0 + vz < D;
+      const uint32_t off = in ? (uint32_t)(((vz * H + vy) * W + vx) * dzs * 2 + 16 * q) : 0x80000000u;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rz, (__attribute__((address_space(3))) void*)(sz + (512 * m + 64 * wave) * 8),
+                                               16, off, 0, 0, 0);
     }
+    if (x_one) {
+      const bool p2 = ci0 >= ci1;
+      const u16* xb = p2 ? x2 : x;
+      const int xsb = p2 ? xs2 : xs, cb = p2 ? ci0 - ci1 : ci0;
+      const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)(xb + ((((int64_t)tl.n * D + tl.z0 - 1) * H + tl.y0 - 1) * W + tl.x0 - 1) * xsb + cb), (short)0,
+          0x7fffffff, 0x00020000);
+#pragma unroll
+      for (int m = 0; m < XSL; ++m) {
+        if (512 * m + 64 * wave >= XP) continue;  // the wave's pieces are all past the brick
+        int sv = xsl[m];
+        asm volatile("" : "+v"(sv));
+        const int bx = sv & 31, by = (sv >> 5) & 7, bz = (sv >> 8) & 15, q = sv >> 12;
+        const bool in = (int)threadIdx.x + 512 * m < XP && (unsigned)(tl.x0 + bx - 1) < (unsigned)W &&
+                        (unsigned)(tl.y0 + by - 1) < (unsigned)H && (unsigned)(tl.z0 + bz - 1) < (unsigned)D;
+        const uint32_t off = in ? (uint32_t)(((bz * H + by) * W + bx) * xsb * 2 + 16 * q) : 0x80000000u;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rx, (__attribute__((address_space(3))) void*)(sx + (512 * m + 64 * wave) * 8), 16, off, 0, 0, 0);
+      }
+      return;
+    }
+    // an input block across the split of a two-part input: global addresses, the tensor chosen per piece
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
       if (512 * m + 64 * wave >= XP) continue;  // the wave's pieces are all past the brick
@@ -1610,7 +1640,7 @@ constexpr int w27_wgs(int nct, int nci, int tz) {
 
 // Channel blocking nci (1-3 tiles of 16 input channels) with the given
 // output blocking nct.
-W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, int nci) {
+W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, int nci, bool dma) {
   W27Shape s;
   s.nct = nct;
   s.nci = nci;
@@ -1625,7 +1655,7 @@ W27Shape wgrad27_shape_nci(int n, int d, int h, int w, int ci, int co, int nct, 
   // one round of resident workgroups (two per CU for the blockings within
   // 128 registers, one for the wider ones; more rounds only add partial
   // sums to reduce), at most about 16 M partial sums
-  const int wgs = g_wgrad27_dma ? w27_wgs(s.nct, s.nci, s.tz) : 0;
+  const int wgs = dma ? w27_wgs(s.nct, s.nci, s.tz) : 0;
   const int res = wgs > 0 ? wgs : (s.nct * s.nci <= 2 ? 2 : 1);
   const int64_t want = (res * (int64_t)device_cus() + s.blocks_y - 1) / s.blocks_y;
   const int64_t by_size = std::max<int64_t>(1, ((int64_t)16 << 20) / ((int64_t)co * 27 * ci + co));
@@ -1637,10 +1667,11 @@ int w27_nct(int co) { return co % 64 == 0 ? 4 : (co % 32 == 0 ? 2 : 1); }
 
 // The widest input block (48, 32 or 16 channels) that divides ci; blocks may
 // straddle the split of a two-part input.
-W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co) {
+// dma: the direct-load variant may run (its workgroups per CU set the rows)
+W27Shape wgrad27_shape(int n, int d, int h, int w, int ci, int co, bool dma) {
   const int nct = w27_nct(co);
   const int nci = (ci % 48 == 0 && nct <= 2) ? 3 : (ci % 32 == 0 ? 2 : 1);
-  return wgrad27_shape_nci(n, d, h, w, ci, co, nct, nci);
+  return wgrad27_shape_nci(n, d, h, w, ci, co, nct, nci, dma);
 }
 
 // cx: the input channels that x itself holds (ci, or the first part's)
@@ -1860,18 +1891,19 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
   const WgShape s = wgrad_shape(n, d, h, w, ci, co, T);
   int64_t rows = s.S;
   // enough for either 3x3x3 kernel, whichever abx_conv3d_set_kernel selects
-  if (T == 27) rows = std::max<int64_t>(rows, wgrad27_shape(n, d, h, w, ci, co).S);
+  // the direct-load variant's rows, at least those of the staged one
+  if (T == 27) rows = std::max<int64_t>(rows, wgrad27_shape(n, d, h, w, ci, co, g_wgrad27_dma).S);
   return rows * ((int64_t)co * T * ci + co) * 4;
 }
 
 namespace {
 template <int A, int B, int Z>
-void launch_w27(dim3 g, hipStream_t s, const u16* x, int xs, const u16* x2, int xs2, int ci1, const u16* dz, int dzs,
-                float* part, float* partb, int d, int h, int w, int ci, int co, int nt) {
+void launch_w27(dim3 g, hipStream_t s, bool dma, const u16* x, int xs, const u16* x2, int xs2, int ci1, const u16* dz,
+                int dzs, float* part, float* partb, int d, int h, int w, int ci, int co, int nt) {
   constexpr int wgs = w27_wgs(A, B, Z);
   static_assert(w27_lds_bytes<A, B, Z>() * wgs <= 160 * 1024, "LDS of the resident workgroups");
   if constexpr (wgs > 0)
-    if (g_wgrad27_dma) {
+    if (dma) {
       hipLaunchKernelGGL((k_wgrad27<A, B, Z, wgs>), g, dim3(512), 0, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d,
                          h, w, ci, co, nt);
       return;
@@ -1904,7 +1936,11 @@ int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1,
   // for the tap-group kernel on the 96x48x64 batch of 8)
   const bool odd_wide = w27_nct(co) == 1 && co > 16;
   const bool use27 = T == 27 && ((g_wgrad27 && !odd_wide) || ci1 < ci);
-  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co) : W27Shape{1, 1, 4, 1, 1};
+  // the direct-load variant addresses its tiles by 32-bit byte offsets: 10
+  // planes of input and 8 of dZ below 2^31 bytes
+  const bool dma27 = g_wgrad27_dma && (int64_t)10 * h * w * std::max(xs, ci1 < ci ? xs2 : xs) * 2 < ((int64_t)1 << 31) &&
+                     (int64_t)8 * h * w * dzs * 2 < ((int64_t)1 << 31);
+  const W27Shape s27 = use27 ? wgrad27_shape(n, d, h, w, ci, co, dma27) : W27Shape{1, 1, 4, 1, 1};
   const int nrows = use27 ? s27.S : sh.S;
   const int64_t nw = (int64_t)co * T * ci;
   if (ws_bytes < (int64_t)nrows * (nw + co) * 4) {
@@ -1919,7 +1955,7 @@ int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1,
     const int nt27 = (int)((int64_t)n * ((d + s27.tz - 1) / s27.tz) * ((h + W27Y - 1) / W27Y) * ((w + W27X - 1) / W27X));
     const dim3 g27((unsigned)s27.S, (unsigned)s27.blocks_y);
 #define ABX_W27(A, B, Z)                                                                                         \
-  launch_w27<A, B, Z>(g27, s, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w, ci, co, nt27)
+  launch_w27<A, B, Z>(g27, s, dma27, x, xs, x2, xs2, ci1, dz, dzs, part, partb, d, h, w, ci, co, nt27)
     if (s27.tz == 8) ABX_W27(1, 1, 8);
     else if (s27.nct == 4 && s27.nci == 2) ABX_W27(4, 2, 4);
     else if (s27.nct == 4) ABX_W27(4, 1, 4);
