@@ -346,8 +346,10 @@ __device__ __forceinline__ void load_lds16(const void* src, void* lds) {
 // before the barrier that ends this chunk's MFMAs.  F16DMA_WAVES: waves per
 // SIMD the register allocation must allow.
 // WREG: the 14 weight fragments are held in registers (56 of them), not read
-// from LDS for every MFMA.
-template <bool DMA, int F16DMA_WAVES = 6, bool WREG = false>
+// from LDS for every MFMA.  GST (with DMA, gated calls): the tile's gate
+// values arrive in LDS with the brick of its last input chunk, so the
+// epilogue does not wait on a global load per row.
+template <bool DMA, int F16DMA_WAVES = 6, bool WREG = false, bool GST = false>
 __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
                                                     int nch1, const u16* __restrict__ wt,
                                                     const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2,
@@ -356,6 +358,9 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
   // whole slots of 64 pieces: the last wave of the last slot writes past the brick
   __shared__ __attribute__((aligned(16))) u16 sx[(2 * F16BV + 63) / 64 * 64 * 8];
   __shared__ u32x4 sw[14 * 64];
+  // GST: the gate values of the output tile, voxel v = (z 4 + y) 16 + x at 32 v bytes
+  __shared__ __attribute__((aligned(16))) u16 sgate[GST ? F16X * F16Y * F16Z * 16 : 8];
+  static_assert(!GST || DMA, "gate staging rides on the direct loads");
   const int co0 = blockIdx.y * 16, nch = Ci / 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4, hi = lg >> 1;
   const int lbase = (wave * F16BY * F16BX + lr) * F16SP + 8 * (lg & 1);
@@ -408,6 +413,26 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
       const uint32_t off = in ? (uint32_t)(((bz * H + by) * W + bx) * xsb * 2 + 16 * (sv >> 12)) : 0x80000000u;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(sx + (512 * m + 64 * wave) * 8),
                                                16, off, 0, 0, 0);
+    }
+  };
+  // the gate tile (two 16-byte pieces per voxel) through a buffer resource
+  // based at the tile's first voxel, out-of-grid voxels at offset 2^31
+  auto fetch_gate = [&](int t) {
+    const F16Tile tl = f16_tile(t, D, H, W);
+    const __amdgpu_buffer_rsrc_t rg = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(gate + ((((int64_t)tl.n * D + tl.z0) * H + tl.y0) * W + tl.x0) * gs + co0), (short)0, 0x7fffffff,
+        0x00020000);
+    // derived here: hoisted out of the tile loop, the offsets spilled
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int e = tid + 512 * m, v = e >> 1;
+      const int vx = v & 15, vy = (v >> 4) & 3, vz = v >> 6;
+      const bool in = tl.x0 + vx < W && tl.y0 + vy < H && tl.z0 + vz < D;
+      const uint32_t off = in ? (uint32_t)(((vz * H + vy) * W + vx) * gs * 2 + 16 * (e & 1)) : 0x80000000u;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rg, (__attribute__((address_space(3))) void*)(sgate + (512 * m + 64 * wave) * 8), 16, off, 0, 0, 0);
     }
   };
   auto load_weights_lds = [&](int c) {
@@ -489,6 +514,8 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
         __syncthreads();  // the previous chunk's MFMAs are done with the brick and the weights
         fetch_lds(t, c);
         if (nch > 1) load_weights_lds(c);
+        if constexpr (GST)
+          if (c == nch - 1) fetch_gate(t);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       } else {
         fetch(t, c);
@@ -570,7 +597,11 @@ __global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(cons
       u32x2 o;
       o.x = pack_bf16(v[0], v[1]);
       o.y = pack_bf16(v[2], v[3]);
-      if (gate)
+      // gated calls of the register-weight variant take GST, so its other
+      // instantiation carries no global gate path
+      if constexpr (GST)
+        o = gate4(o, *(const u32x2*)(sgate + ((wave * F16Y + r) * F16X + lr) * 16 + 4 * lg));
+      else if (!(DMA && WREG) && gate)
         o = gate4(o, *(const u32x2*)(gate + v0 * gs + (uint32_t)(((wave * H + r) * W + lr) * gs + co)));
       *(u32x2*)yp = o;
     }
@@ -1809,7 +1840,11 @@ int conv_forward(const char* fn, void* stream, const uint16_t* x, int xs, int ci
     // persistent workgroups: g_fwd16_wgs (of eight waves) per CU across the co tiles
     const int64_t per_co = std::max<int64_t>(1, (int64_t)g_fwd16_wgs * device_cus() / (co / 16));
     const unsigned gx = (unsigned)std::min<int64_t>(tiles, per_co);
-    if (g_fwd16_mode == 0)
+    if (g_fwd16_mode == 0 && gate)
+      hipLaunchKernelGGL((k_conv_fwd16<true, 4, true, true>), dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs,
+                         x2, xs2, ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles,
+                         relu, accumulate);
+    else if (g_fwd16_mode == 0)
       hipLaunchKernelGGL((k_conv_fwd16<true, 4, true>), dim3(gx, (unsigned)(co / 16)), dim3(512), 0, s, x, xs, x2,
                          xs2, ci1 / 16, wpack, bias, y, ys, y2, ys2, co1 / 16, gate, gs, d, h, w, ci, (int)tiles, relu,
                          accumulate);
