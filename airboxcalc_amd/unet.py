@@ -19,6 +19,12 @@ backward are written out by hand.  A skip concatenation (decoder input =
 the decoder's first convolution, its input gradient and its weight gradient
 read or write as one input (abx_conv3d_*_parts): no copy is made, and the
 pooling and upsampling kernels touch whole rows instead of channel slices.
+
+The weight gradients (and their partial-sum reductions) run on a second
+stream, each after an event that marks the main stream's work so far (its dZ
+and input are then written), while the main stream carries on with the input
+gradients; the main stream waits for them before Adam.  Only the weight
+gradients use the workspace, so they stay in order on their own stream.
 """
 from __future__ import annotations
 
@@ -142,11 +148,46 @@ class UNet:
         self.repack()
         self.shape = None
         self.ws = None
+        # weight gradients on a second stream (default priority), joined before Adam
+        self.overlap = True
+        self._side = None
+        self._evs: List = []
+        self._nev = 0
 
     # -- parameters ---------------------------------------------------------
     @property
     def stream(self) -> int:
         return self.torch.cuda.current_stream(self.dev).cuda_stream
+
+    def _event(self):
+        """A reusable event (recording it again after its wait was enqueued
+        does not change that wait)."""
+        if self._nev == len(self._evs):
+            self._evs.append(self.torch.cuda.Event())
+        ev = self._evs[self._nev]
+        self._nev += 1
+        return ev
+
+    def _wstream(self) -> int:
+        """The stream of the next weight gradient: the side stream, made to
+        wait for everything enqueued on the main stream so far, or the main
+        stream itself without overlap."""
+        if not self.overlap:
+            return self.stream
+        torch = self.torch
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.dev)
+        ev = self._event()
+        ev.record(torch.cuda.current_stream(self.dev))
+        self._side.wait_event(ev)
+        return self._side.cuda_stream
+
+    def _join(self) -> None:
+        """The main stream waits for the weight gradients enqueued so far."""
+        if self._side is not None:
+            ev = self._event()
+            ev.record(self._side)
+            self.torch.cuda.current_stream(self.dev).wait_event(ev)
 
     def wptr(self, l: Layer, t=None) -> int:
         t = self.w if t is None else t
@@ -251,7 +292,7 @@ class UNet:
         l = self.by[name]
         n = self.shape[0]
         d, h, w = self.dims[self._level(l)]
-        _ck(abx.load().abx_conv3d_wgrad_parts(self.stream, x1.p, x1.s, x1.c, x2.p, x2.s, dz.p, dz.s, n, d, h, w,
+        _ck(abx.load().abx_conv3d_wgrad_parts(self._wstream(), x1.p, x1.s, x1.c, x2.p, x2.s, dz.p, dz.s, n, d, h, w,
                                               l.ci, l.co, self.ws.data_ptr(), self.ws_bytes, self.wptr(l, self.g),
                                               self.bptr(l, self.g), 0))
 
@@ -276,7 +317,7 @@ class UNet:
         l = self.by[name]
         n = self.shape[0]
         d, h, w = self.dims[self._level(l)]
-        _ck(abx.load().abx_conv3d_wgrad(self.stream, x.p, x.s, dz.p, dz.s, n, d, h, w, l.ci, l.co, l.ksize,
+        _ck(abx.load().abx_conv3d_wgrad(self._wstream(), x.p, x.s, dz.p, dz.s, n, d, h, w, l.ci, l.co, l.ksize,
                                         self.ws.data_ptr(), self.ws_bytes, self.wptr(l, self.g),
                                         self.bptr(l, self.g), 0))
 
@@ -348,6 +389,7 @@ class UNet:
         a = lambda k: T(A[k])
         g = lambda k: T(G[k])
         dO = T(dout)
+        self._nev = 0
         self._wgrad("head", a("d1b"), dO)
         self._dgrad("head", dO, g("d1b"), gate=a("d1b"))
         # level 1 decoder
@@ -393,6 +435,7 @@ class UNet:
         self._wgrad("enc1b", a("a1"), de1)
         self._dgrad("enc1b", de1, g("a1"), gate=a("a1"))
         self._wgrad("enc1a", T(self.x_in), g("a1"))
+        self._join()
 
     def adam(self, lr: float = 1e-3, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:
         self.step_no += 1

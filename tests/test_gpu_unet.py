@@ -769,6 +769,31 @@ def test_unet_close_to_plain_fp32_network():
         assert cos > 0.98, (l.name, cos)
 
 
+def test_unet_side_stream_weight_gradients_equal_one_stream():
+    """The weight gradients run on a second stream (each after an event on
+    the main stream, joined before Adam) by default: three training steps
+    give the same parameters bit for bit as with every kernel on the main
+    stream."""
+    from airboxcalc_amd import unet as U
+    say("unet side-stream weight gradients")
+    rng = np.random.default_rng(9)
+    n, d, h, w = 2, 16, 16, 24
+    X = np.zeros((n, d, h, w, U.IN_CH), np.float32)
+    X[..., :5] = rng.standard_normal((n, d, h, w, 5))
+    x, y = bf(X), bf(0.5 * rng.standard_normal((n, d, h, w, U.OUT_CH)))
+    mask = torch.ones((n, d, h, w), dtype=torch.uint8, device=_dev())
+    cw = torch.ones(U.OUT_CH, dtype=torch.float32, device=_dev())
+    ws = []
+    for overlap in (True, False):
+        net = U.UNet(seed=3)
+        net.overlap = overlap
+        for _ in range(3):
+            net.train_step(x, y, mask, cw, lr=1e-3, count=n * d * h * w)
+        torch.cuda.synchronize()
+        ws.append(net.w.cpu())
+    assert torch.equal(ws[0].view(torch.int32), ws[1].view(torch.int32))
+
+
 def test_unet_training_reduces_loss():
     from airboxcalc_amd import unet as U
     say("unet training smoke")
