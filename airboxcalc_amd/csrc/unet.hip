@@ -743,9 +743,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const u16* __restrict__ x, i
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
 
-__device__ __forceinline__ u32x4 frag_tr(const u16* p, int pitch) {
+// hi_rows: the rows of the second read (elements 4-7) follow the first's by
+// this many rows.
+__device__ __forceinline__ u32x4 frag_tr(const u16* p, int pitch, int hi_rows = 4) {
   const s16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(void*)p);
-  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(void*)(p + 4 * pitch));
+  const s16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(void*)(p + hi_rows * pitch));
   const u32x2 l = __builtin_bit_cast(u32x2, lo), h = __builtin_bit_cast(u32x2, hi);
   return (u32x4){l.x, l.y, h.x, h.y};
 }
@@ -958,6 +960,13 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
   constexpr int ZP = W27V * NCT * 2, XP = W27BV * NCI * 2;  // 16-byte pieces of the dZ tile and the brick
   constexpr int ZSL = (ZP + 511) / 512, XSL = (XP + 511) / 512;
   static_assert(ZP % 512 == 0, "dZ pieces fill whole slots");
+  // the lane groups interleaved by 4 voxels where both tiles' voxel rows
+  // (32 NCT and 32 NCI bytes) then give the two groups of a half-wave
+  // disjoint LDS banks: 32 bytes (groups 128 bytes apart) and 96 bytes
+  // (384 apart: rows on banks {0-7, 24-31, 48-55, 8-15} and {32-39, 56-63,
+  // 16-23, 40-47}, for every tap shift); at 64 bytes every voxel offset lands
+  // on the same banks
+  constexpr bool K4 = NCT == 1 && (NCI == 1 || NCI == 3);
   __shared__ __attribute__((aligned(16))) u16 sz[W27V * PZ];
   // whole slots of 64 pieces: with direct loads the last wave of the last slot writes past the brick
   __shared__ __attribute__((aligned(16))) u16 sx[(XP + 63) / 64 * 64 * 8];
@@ -1116,16 +1125,20 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
       if (t + (int)gridDim.x < ntiles) fetch(t + (int)gridDim.x);
 #pragma unroll 1
     for (int s = 0; s < W27V / 32; ++s) {
-      // k = voxels 32 s + 8 lg + e (e = 0..7), consecutive in x
-      const int vb0 = 32 * s + 8 * lg;
+      // k step s = the 32 voxels of x rows 2 s and 2 s + 1.  Lane group lg
+      // takes 8 of them: x 8 (lg & 1) .. + 7 of row 2 s + (lg >> 1) (elements
+      // 0-3 and 4-7 from rows 4 apart), or with K4 x 4 (lg & 1) .. + 3 and
+      // x 8 + 4 (lg & 1) .. + 3 (rows 8 apart), so that the two groups of a
+      // half-wave read 4 voxels apart instead of 8
+      const int vb0 = K4 ? 32 * s + 16 * (lg >> 1) + 4 * (lg & 1) : 32 * s + 8 * lg;
       const int tx0 = vb0 & 15, ty = (vb0 >> 4) & 3, tz = vb0 >> 6;
       u32x4 av[NCT];
 #pragma unroll
-      for (int c = 0; c < NCT; ++c) av[c] = frag_tr(sz + (vb0 + lrow) * PZ + c * 16 + lcol, PZ);
+      for (int c = 0; c < NCT; ++c) av[c] = frag_tr(sz + (vb0 + lrow) * PZ + c * 16 + lcol, PZ, K4 ? 8 : 4);
       const u16* lbase = sx + ((tz * W27BY + ty) * W27BX + tx0 + lrow) * PX + lcol;
 #pragma unroll
       for (int u = 0; u < MAXI; ++u) {
-        const u32x4 b = frag_tr(lbase + boff[u], PX);
+        const u32x4 b = frag_tr(lbase + boff[u], PX, K4 ? 8 : 4);
         if (u + 1 < MAXI || last_item)
 #pragma unroll
           for (int c = 0; c < NCT; ++c) acc[u][c] = mfma16(av[c], b, acc[u][c]);
