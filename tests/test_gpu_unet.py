@@ -227,13 +227,17 @@ def test_wgrad_transposed_reads_equal_gathered(case):
 
 
 @pytest.mark.parametrize("case", [(2, 5, 9, 37, 16, 16, 3, 3, 0), (1, 4, 6, 20, 32, 48, 3, 0, 1),
-                                  (1, 3, 5, 18, 16, 64, 3, 3, 1), (2, 5, 4, 17, 16, 16, 1, 3, 0)],
+                                  (1, 3, 5, 18, 16, 64, 3, 3, 1), (2, 5, 4, 17, 16, 16, 1, 3, 0),
+                                  (1, 9, 7, 21, 48, 32, 3, 3, 0), (1, 6, 5, 19, 32, 16, 3, 67, 0)],
                          ids=lambda c: "x".join(map(str, c)))
 def test_conv_forward_gated_equals_forward_then_relu_backward(case):
     """abx_conv3d_forward_gated (the ReLU backward folded into the store)
     equals abx_conv3d_forward followed by abx_relu_backward bit for bit, on
     the 16-channel-tile kernel and the general one, with and without
-    accumulation, the gate a channel slice of a wider tensor."""
+    accumulation, the gate a channel slice of a wider tensor; the 16-channel
+    kernel stages the gate values in LDS with the brick of the last input
+    chunk (one, two or three chunks here), or with flag bit 6 reads them from
+    global memory."""
     n, d, h, w, ci, co, k, flags, acc = case
     say("conv forward gated", case)
     rng = np.random.default_rng(11)
