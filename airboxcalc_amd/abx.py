@@ -226,6 +226,10 @@ def load() -> C.CDLL:
         "abx_upsample2_backward_gated": (i32, [vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, vp, i32]),
         "abx_relu_backward": (i32, [vp, vp, i32, vp, i32, i64, i32]),
         "abx_mse_partials": (i64, [i64]),
+        "abx_head_loss_partials": (i64, [i64]),
+        "abx_head_loss_workspace": (i64, [i64]),
+        "abx_head_loss_backward": (i32, [vp, vp, i32, i32, vp, vp, i32, vp, i32, vp, i64, vp, C.c_float, vp, vp, i32,
+                                         vp, i64, vp, vp]),
         "abx_mse_loss": (i32, [vp, vp, i32, vp, i32, vp, i64, i32, vp, C.c_float, vp, vp, i32]),
         "abx_adam": (i32, [vp, vp, vp, vp, vp, i64, C.c_float, C.c_float, C.c_float, C.c_float, i32]),
     }

@@ -1587,6 +1587,112 @@ __global__ __launch_bounds__(256) void k_mse(const u16* __restrict__ pred, int p
   if (threadIdx.x == 0) loss[blockIdx.x] = red[0] * scale;
 }
 
+// The surrogate's head and loss in one pass, for training: the head is a
+// 1x1x1 convolution of the 16-channel decoder output x (itself a ReLU output),
+// of which c <= 4 outputs enter the loss.  Per voxel: out_j = bf16(sum_k
+// W[j][k] x_k + b_j) (the head's output as stored), the masked weighted
+// squared error and its gradient d_j = bf16(2 scale cw_j (out_j - target_j))
+// as in k_mse_v, dx_k = bf16(sum_j W[j][k] d_j), zero where x_k <= 0 (the
+// ReLU's backward), and the head's weight and bias gradients d_j x_k and d_j
+// summed per thread.  A workgroup writes its loss sum times scale to
+// lossp[block] and its gradient sums, reduced across lanes and waves in a
+// fixed order, to part row block: [4][16] weight sums, then 4 bias sums.
+// Workgroups stride over the voxels.
+constexpr int HEAD_CI = 16, HEAD_C = 4, HEAD_NP = HEAD_C * HEAD_CI + HEAD_C;
+__global__ __launch_bounds__(256) void k_head_loss(const u16* __restrict__ x, int xs, const u16* __restrict__ wp,
+                                                   const float* __restrict__ bias, int c, const u16* __restrict__ tgt,
+                                                   int ts, const uint8_t* __restrict__ mask, int64_t nvox,
+                                                   const float* __restrict__ cw, float scale, float* __restrict__ lossp,
+                                                   u16* __restrict__ dx, int dxs, float* __restrict__ part) {
+  __shared__ float red[4][HEAD_NP + 1];
+  // the weights are the same for every lane: scalar registers (as vector
+  // registers beside the 64 gradient sums they left two waves per SIMD)
+  auto uni = [](float f) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(f))); };
+  float W[HEAD_C][HEAD_CI], b[HEAD_C], cwv[HEAD_C];
+#pragma unroll
+  for (int j = 0; j < HEAD_C; ++j) {
+#pragma unroll
+    for (int k = 0; k < HEAD_CI; ++k) W[j][k] = uni(j < c ? bf2f(wp[j * HEAD_CI + k]) : 0.f);
+    b[j] = uni(j < c && bias ? bias[j] : 0.f);
+    cwv[j] = uni(j < c ? cw[j] : 0.f);
+  }
+  float gw[HEAD_C][HEAD_CI], gb[HEAD_C], l = 0.f;
+#pragma unroll
+  for (int j = 0; j < HEAD_C; ++j) {
+    gb[j] = 0.f;
+#pragma unroll
+    for (int k = 0; k < HEAD_CI; ++k) gw[j][k] = 0.f;
+  }
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (int64_t)gridDim.x * 256) {
+    const u16x8 a0 = *(const u16x8*)(x + v * xs), a1 = *(const u16x8*)(x + v * xs + 8);
+    float xf[HEAD_CI];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      xf[k] = bf2f(a0[k]);
+      xf[8 + k] = bf2f(a1[k]);
+    }
+    float d[HEAD_C] = {0.f, 0.f, 0.f, 0.f};
+    if (mask[v]) {
+#pragma unroll
+      for (int j = 0; j < HEAD_C; ++j) {
+        if (j >= c) continue;
+        float o = 0.f;
+#pragma unroll
+        for (int k = 0; k < HEAD_CI; ++k) o = fmaf(W[j][k], xf[k], o);
+        const float e = bf2f(f2bf(o + b[j])) - bf2f(tgt[v * ts + j]);
+        l += cwv[j] * e * e;
+        d[j] = bf2f(f2bf(2.f * scale * cwv[j] * e));
+      }
+    }
+    u16x8 o0, o1;
+#pragma unroll
+    for (int k = 0; k < HEAD_CI; ++k) {
+      float g = 0.f;
+#pragma unroll
+      for (int j = 0; j < HEAD_C; ++j) g = fmaf(W[j][k], d[j], g);
+      const u16 gv = xf[k] > 0.f ? f2bf(g) : (u16)0;
+      if (k < 8) o0[k] = gv;
+      else o1[k - 8] = gv;
+    }
+    *(u16x8*)(dx + v * dxs) = o0;
+    *(u16x8*)(dx + v * dxs + 8) = o1;
+#pragma unroll
+    for (int j = 0; j < HEAD_C; ++j) {
+      gb[j] += d[j];
+#pragma unroll
+      for (int k = 0; k < HEAD_CI; ++k) gw[j][k] = fmaf(d[j], xf[k], gw[j][k]);
+    }
+  }
+  // lanes, then the four waves, in a fixed order
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto wave_sum = [&](float t) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    return t;
+  };
+#pragma unroll
+  for (int j = 0; j < HEAD_C; ++j) {
+#pragma unroll
+    for (int k = 0; k < HEAD_CI; ++k) {
+      const float t = wave_sum(gw[j][k]);
+      if (lane == 0) red[wave][j * HEAD_CI + k] = t;
+    }
+    const float t = wave_sum(gb[j]);
+    if (lane == 0) red[wave][HEAD_C * HEAD_CI + j] = t;
+  }
+  {
+    const float t = wave_sum(l);
+    if (lane == 0) red[wave][HEAD_NP] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x <= HEAD_NP) {
+    const int i = threadIdx.x;
+    const float t = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    if (i < HEAD_NP) part[(int64_t)blockIdx.x * HEAD_NP + i] = t;
+    else lossp[blockIdx.x] = t * scale;
+  }
+}
+
 // Adam (Kingma & Ba 2015) on FP32 master weights.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        int64_t n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
@@ -1945,6 +2051,25 @@ extern "C" int64_t abx_conv3d_wgrad_workspace(int n, int d, int h, int w, int ci
 }
 
 namespace {
+// Sums nrows partial rows (stride floats apart) element by element in a fixed
+// order: chunks of 32 rows in place, then the chunk sums; elements below n1
+// go to out, the others to out2 (element i - n1), added to them if accumulate.
+int reduce_rows(hipStream_t s, float* part, int nrows, int64_t nred, int64_t stride, int64_t n1, float* out,
+                float* out2, int accumulate) {
+  constexpr int kChunk = 32;
+  int rows = nrows;
+  if (nrows > kChunk) {
+    rows = (nrows + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nred), (unsigned)rows), dim3(256), 0, s, part, nrows, nred, stride,
+                       kChunk);
+    stride *= kChunk;
+  }
+  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nred, 64)), dim3(256), 0, s, part, rows, nred, stride, n1, out,
+                     out2, accumulate);
+  ABX_HIP(hipGetLastError());
+  return 0;
+}
+
 template <int A, int B, int Z>
 void launch_w27(dim3 g, hipStream_t s, bool dma, const u16* x, int xs, const u16* x2, int xs2, int ci1, const u16* dz,
                 int dzs, float* part, float* partb, int d, int h, int w, int ci, int co, int nt) {
@@ -2043,22 +2168,8 @@ int conv_wgrad(const char* fn, void* stream, const uint16_t* x, int xs, int ci1,
 #undef ABX_WG
 #undef ABX_LAUNCH
   ABX_HIP(hipGetLastError());
-  // weights and bias in one reduction (rows of nw + co partials): chunks of
-  // kChunk rows summed in place, then the chunk sums
-  const int64_t nred = nw + (db ? co : 0);
-  constexpr int kChunk = 32;
-  int rows = nrows;
-  int64_t stride = nw + co;
-  if (nrows > kChunk) {
-    rows = (nrows + kChunk - 1) / kChunk;
-    hipLaunchKernelGGL(k_sum_chunks, dim3(blocks_for(nred), (unsigned)rows), dim3(256), 0, s, part, nrows, nred, stride,
-                       kChunk);
-    stride *= kChunk;
-  }
-  hipLaunchKernelGGL(k_reduce_parts, dim3(blocks_for(nred, 64)), dim3(256), 0, s, part, rows, nred, stride, nw, dw,
-                     db, accumulate);
-  ABX_HIP(hipGetLastError());
-  return 0;
+  // weights and bias in one reduction (rows of nw + co partials)
+  return reduce_rows(s, part, nrows, nw + (db ? co : 0), nw + co, nw, dw, db, accumulate);
 }
 }  // namespace
 
@@ -2211,6 +2322,43 @@ extern "C" int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint
 }
 
 extern "C" int64_t abx_mse_partials(int64_t nvox) { return nvox < 1 ? -1 : (int64_t)blocks_for(nvox); }
+
+namespace abx {
+namespace {
+// k_head_loss's grid: up to 1024 workgroups striding over the voxels
+int64_t head_blocks(int64_t nvox) { return nvox < 1 ? -1 : std::min<int64_t>(blocks_for(nvox), 1024); }
+}  // namespace
+}  // namespace abx
+
+extern "C" int64_t abx_head_loss_partials(int64_t nvox) { return head_blocks(nvox); }
+
+extern "C" int64_t abx_head_loss_workspace(int64_t nvox) {
+  return nvox < 1 ? -1 : head_blocks(nvox) * HEAD_NP * (int64_t)sizeof(float);
+}
+
+extern "C" int abx_head_loss_backward(void* stream, const uint16_t* x, int xs, int ci, const uint16_t* wpack,
+                                      const float* bias, int c, const uint16_t* target, int ts, const uint8_t* mask,
+                                      int64_t nvox, const float* chan_w, float scale, float* loss_parts, uint16_t* dx,
+                                      int dxs, float* workspace, int64_t ws_bytes, float* dw, float* db) {
+  if (!x || !wpack || !target || !mask || !chan_w || !loss_parts || !dx || !workspace || !dw || !db || nvox < 1 ||
+      ci != HEAD_CI || c < 1 || c > HEAD_C || xs < ci || xs % 8 || dxs < ci || dxs % 8 || ts < c ||
+      !aligned16(x) || !aligned16(dx)) {
+    set_error("abx_head_loss_backward: bad argument (16 input channels, 1-4 outputs, strides multiples of 8, "
+              "16-byte aligned x and dx)");
+    return -1;
+  }
+  if (ws_bytes < abx_head_loss_workspace(nvox)) {
+    set_errorf("abx_head_loss_backward: workspace of %lld bytes is smaller than the %lld needed", (long long)ws_bytes,
+               (long long)abx_head_loss_workspace(nvox));
+    return -1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t g = head_blocks(nvox);
+  hipLaunchKernelGGL(k_head_loss, dim3((unsigned)g), dim3(256), 0, s, x, xs, wpack, bias, c, target, ts, mask, nvox,
+                     chan_w, scale, loss_parts, dx, dxs, workspace);
+  ABX_HIP(hipGetLastError());
+  return reduce_rows(s, workspace, (int)g, HEAD_NP, HEAD_NP, HEAD_C * HEAD_CI, dw, db, 0);
+}
 
 extern "C" int abx_mse_loss(void* stream, const uint16_t* pred, int ps, const uint16_t* target, int ts,
                             const uint8_t* mask, int64_t nvox, int c, const float* chan_w, float scale, float* loss_parts,

@@ -20,6 +20,9 @@ the decoder's first convolution, its input gradient and its weight gradient
 read or write as one input (abx_conv3d_*_parts): no copy is made, and the
 pooling and upsampling kernels touch whole rows instead of channel slices.
 
+In training, the head, the loss and the head's backward are one kernel
+(abx_head_loss_backward), between the decoder's forward and its backward.
+
 The weight gradients (and their partial-sum reductions) run on a second
 stream, each after an event that marks the main stream's work so far (its dZ
 and input are then written), while the main stream carries on with the input
@@ -150,6 +153,8 @@ class UNet:
         self.ws = None
         # weight gradients on a second stream (default priority), joined before Adam
         self.overlap = True
+        # train_step: head, loss and head backward in one kernel
+        self.fused_head = True
         self._side = None
         self._evs: List = []
         self._nev = 0
@@ -347,9 +352,11 @@ class UNet:
                                                     gate.s))
 
     # -- network ----------------------------------------------------------------
-    def forward(self, x) -> "object":
+    def forward(self, x, head: bool = True) -> "object":
         """x: (n, d, h, w, IN_CH) bf16 on the device.  Returns the head output
-        (n, d, h, w, HEAD_CH) bf16 (channels >= OUT_CH are zero)."""
+        (n, d, h, w, HEAD_CH) bf16 (channels >= OUT_CH are zero); with
+        head=False the head is left to head_loss_backward (out is not
+        written)."""
         n, d, h, w, c = x.shape
         assert c == IN_CH and x.dtype == self.torch.bfloat16 and x.is_contiguous()
         self._alloc(n, d, h, w)
@@ -377,21 +384,24 @@ class UNet:
         self._up(T(A["d2b"]), T(A["up1"]), 1)
         self._conv_parts("dec1a", T(A["up1"]), e1, T(A["d1a"]))
         self._conv("dec1b", T(A["d1a"]), T(A["d1b"]))
-        self._conv("head", T(A["d1b"]), T(A["out"]))
+        if head:
+            self._conv("head", T(A["d1b"]), T(A["out"]))
         self.x_in = x
         return A["out"]
 
     def backward(self, dout) -> None:
         """Gradients of every parameter into self.g (overwritten) from the
-        gradient of the head output (n, d, h, w, HEAD_CH) bf16."""
+        gradient of the head output (n, d, h, w, HEAD_CH) bf16, or with dout
+        None from the head's gradients that head_loss_backward wrote."""
         A, G = self.A, self.G
         w1, w2, w3, w4 = WIDTHS
         a = lambda k: T(A[k])
         g = lambda k: T(G[k])
-        dO = T(dout)
         self._nev = 0
-        self._wgrad("head", a("d1b"), dO)
-        self._dgrad("head", dO, g("d1b"), gate=a("d1b"))
+        if dout is not None:
+            dO = T(dout)
+            self._wgrad("head", a("d1b"), dO)
+            self._dgrad("head", dO, g("d1b"), gate=a("d1b"))
         # level 1 decoder
         self._wgrad("dec1b", a("d1a"), g("d1b"))
         self._dgrad("dec1b", g("d1b"), g("d1a"), gate=a("d1a"))
@@ -464,9 +474,38 @@ class UNet:
                              self._dout.data_ptr(), HEAD_CH))
         return self._lp.sum(), self._dout
 
+    def head_loss_backward(self, target, mask, chan_w, count: Optional[int] = None) -> "object":
+        """After forward(x, head=False): the head, the loss of loss_and_grad
+        and the head's input and weight gradients in one kernel
+        (abx_head_loss_backward).  Returns the loss as a device scalar."""
+        torch = self.torch
+        l = self.by["head"]
+        d1b = T(self.A["d1b"])
+        nvox = self.A["d1b"].numel() // d1b.s
+        lib = abx.load()
+        nparts = lib.abx_head_loss_partials(nvox)
+        if not hasattr(self, "_hlp") or self._hlp.numel() != nparts:
+            self._hlp = torch.empty(nparts, dtype=torch.float32, device=self.dev)
+            self._hws_bytes = lib.abx_head_loss_workspace(nvox)
+            self._hws = torch.empty(self._hws_bytes // 4, dtype=torch.float32, device=self.dev)
+        cnt = float(mask.sum().item()) if count is None else float(count)
+        scale = 1.0 / max(cnt * OUT_CH, 1.0)
+        gx = T(self.G["d1b"])
+        _ck(lib.abx_head_loss_backward(self.stream, d1b.p, d1b.s, l.ci, self.packs["head"][0].data_ptr(),
+                                       self.bptr(l), OUT_CH, target.data_ptr(), target.shape[-1], mask.data_ptr(),
+                                       nvox, chan_w.data_ptr(), scale, self._hlp.data_ptr(), gx.p, gx.s,
+                                       self._hws.data_ptr(), self._hws_bytes, self.wptr(l, self.g),
+                                       self.bptr(l, self.g)))
+        return self._hlp.sum()
+
     def train_step(self, x, target, mask, chan_w, lr: float = 1e-3, count: Optional[int] = None) -> "object":
-        self.forward(x)
-        loss, dout = self.loss_and_grad(target, mask, chan_w, count)
-        self.backward(dout)
+        if self.fused_head:
+            self.forward(x, head=False)
+            loss = self.head_loss_backward(target, mask, chan_w, count)
+            self.backward(None)
+        else:
+            self.forward(x)
+            loss, dout = self.loss_and_grad(target, mask, chan_w, count)
+            self.backward(dout)
         self.adam(lr)
         return loss

@@ -525,6 +525,27 @@ int abx_upsample2_backward_gated(void* stream, const uint16_t* dy, int dys, int 
 /* dy *= (y > 0) in place over nvox voxels of c channels. */
 int abx_relu_backward(void* stream, uint16_t* dy, int dys, const uint16_t* y, int ys, int64_t nvox, int c);
 
+/* Training's head, loss and head backward in one pass (no reference
+ * counterpart: the surrogate network's output layer).  x: the 16-channel
+ * ReLU output that feeds the head, a 1x1x1 convolution (wpack: its forward
+ * pack, bf16 [co >= c][16]; bias may be NULL) of which the first c (1-4)
+ * outputs enter the loss.  Per voxel: out_j = bf16(sum_k W[j][k] x_k + b_j),
+ * the loss and its gradient d_j = bf16(dloss / dout_j) as abx_mse_loss
+ * computes them from out, and dx_k = bf16(sum_j W[j][k] d_j), zero where
+ * x_k <= 0 (the ReLU's backward); dw[j][k] = sum over voxels of d_j x_k and
+ * db[j] = sum of d_j for j < 4 (rows from c on get zeros; rows 4.. of a wider
+ * head are not written).  loss_parts: abx_head_loss_partials(nvox) floats
+ * whose sum is the loss.  Deterministic: per-workgroup partial sums in the
+ * workspace (abx_head_loss_workspace(nvox) bytes), reduced in a fixed
+ * order.  Numerically the head convolution, abx_mse_loss and the head's
+ * gated input and weight gradients, up to the order of the FP32 sums. */
+int64_t abx_head_loss_partials(int64_t nvox);
+int64_t abx_head_loss_workspace(int64_t nvox);
+int abx_head_loss_backward(void* stream, const uint16_t* x, int xs, int ci, const uint16_t* wpack, const float* bias,
+                           int c, const uint16_t* target, int ts, const uint8_t* mask, int64_t nvox,
+                           const float* chan_w, float scale, float* loss_parts, uint16_t* dx, int dxs,
+                           float* workspace, int64_t ws_bytes, float* dw, float* db);
+
 /* Number of per-workgroup loss partial sums abx_mse_loss writes. */
 int64_t abx_mse_partials(int64_t nvox);
 

@@ -773,6 +773,53 @@ def test_unet_close_to_plain_fp32_network():
         assert cos > 0.98, (l.name, cos)
 
 
+def test_unet_fused_head_loss_equals_separate_kernels():
+    """train_step's head, loss and head backward in one kernel
+    (abx_head_loss_backward) against the head convolution, abx_mse_loss and
+    the head's gated input and weight gradients as separate kernels, on a
+    mask that leaves 30 % of the volume out and unequal channel weights: the
+    loss within 1e-4 (relative), the decoder output's gradient within the
+    bf16 bar, the head's weight and bias gradients within 1e-3 of their range
+    and all the parameters' gradients within 1e-2 (relative L2: the head's
+    FP32 sums run in another order, so an output may round to the next bf16
+    value); and the same bits on a second fused pass."""
+    from airboxcalc_amd import unet as U
+    say("unet fused head and loss")
+    rng = np.random.default_rng(13)
+    n, d, h, w = 2, 16, 16, 24
+    X = np.zeros((n, d, h, w, U.IN_CH), np.float32)
+    X[..., :5] = rng.standard_normal((n, d, h, w, 5))
+    x, y = bf(X), bf(0.5 * rng.standard_normal((n, d, h, w, U.OUT_CH)))
+    mask = torch.as_tensor((rng.random((n, d, h, w)) < 0.7).astype(np.uint8)).to(_dev())
+    cw = torch.tensor([1.0, 0.5, 2.0, 1.5], dtype=torch.float32, device=_dev())
+    cnt = int(mask.sum().item())
+    net = U.UNet(seed=4)
+    net.forward(x)
+    loss_s, dout = net.loss_and_grad(y, mask, cw, count=cnt)
+    net.backward(dout)
+    torch.cuda.synchronize()
+    ls, g_s, gx_s = float(loss_s.item()), net.g.clone(), net.G["d1b"].clone()
+    runs = []
+    for _ in range(2):
+        net.forward(x, head=False)
+        loss_f = net.head_loss_backward(y, mask, cw, count=cnt)
+        net.backward(None)
+        torch.cuda.synchronize()
+        runs.append((float(loss_f.item()), net.g.clone(), net.G["d1b"].clone()))
+    lf, g_f, gx_f = runs[0]
+    assert runs[1][0] == lf
+    assert torch.equal(runs[1][1].view(torch.int32), g_f.view(torch.int32))
+    assert torch.equal(runs[1][2].view(torch.int16), gx_f.view(torch.int16))
+    assert abs(lf - ls) <= 1e-4 * abs(ls), (lf, ls)
+    close_bf16(f32(gx_f), f32(gx_s), "gradient of the decoder output")
+    hl = net.by["head"]
+    gs_np, gf_np = g_s.cpu().numpy().astype(np.float64), g_f.cpu().numpy().astype(np.float64)
+    for off, cnt_ in ((hl.w_off, hl.nw), (hl.b_off, hl.co)):
+        a, b = gf_np[off:off + cnt_], gs_np[off:off + cnt_]
+        assert np.abs(a - b).max() <= 1e-3 * max(np.abs(b).max(), 1e-30), "head gradient"
+    assert np.linalg.norm(gf_np - gs_np) <= 1e-2 * np.linalg.norm(gs_np)
+
+
 def test_unet_side_stream_weight_gradients_equal_one_stream():
     """The weight gradients run on a second stream (each after an event on
     the main stream, joined before Adam) by default: three training steps
