@@ -1592,21 +1592,27 @@ __global__ __launch_bounds__(256) void k_mse(const u16* __restrict__ pred, int p
 // of which c <= 4 outputs enter the loss.  Per voxel: out_j = bf16(sum_k
 // W[j][k] x_k + b_j) (the head's output as stored), the masked weighted
 // squared error and its gradient d_j = bf16(2 scale cw_j (out_j - target_j))
-// as in k_mse_v, dx_k = bf16(sum_j W[j][k] d_j), zero where x_k <= 0 (the
-// ReLU's backward), and the head's weight and bias gradients d_j x_k and d_j
-// summed per thread.  A workgroup writes its loss sum times scale to
-// lossp[block] and its gradient sums, reduced across lanes and waves in a
-// fixed order, to part row block: [4][16] weight sums, then 4 bias sums.
-// Workgroups stride over the voxels.
+// as in k_mse_v, and dx_k = bf16(sum_j W[j][k] d_j), zero where x_k <= 0 (the
+// ReLU's backward).  The head's weight gradient sum_v d_j x_k is an MFMA
+// product per 64 voxels: each wave stages its voxels' x rows and d rows
+// (bf16, voxel-major) in its own 4 KB of LDS and builds the fragments with
+// transposing reads as k_wgrad27 does (dW rows j = the A rows, only 0-3
+// non-zero).  A workgroup writes its loss sum times scale to lossp[block] and
+// its sums, reduced across its four waves in a fixed order, to part row
+// block: [4][16] weight sums, then 4 bias sums.  Workgroups stride over the
+// voxels, each lane loading its next voxel before it works on this one.
 constexpr int HEAD_CI = 16, HEAD_C = 4, HEAD_NP = HEAD_C * HEAD_CI + HEAD_C;
 __global__ __launch_bounds__(256) void k_head_loss(const u16* __restrict__ x, int xs, const u16* __restrict__ wp,
                                                    const float* __restrict__ bias, int c, const u16* __restrict__ tgt,
                                                    int ts, const uint8_t* __restrict__ mask, int64_t nvox,
                                                    const float* __restrict__ cw, float scale, float* __restrict__ lossp,
                                                    u16* __restrict__ dx, int dxs, float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) u16 sx[4][64 * 16];
+  __shared__ __attribute__((aligned(16))) u16 sd[4][64 * 16];
   __shared__ float red[4][HEAD_NP + 1];
-  // the weights are the same for every lane: scalar registers (as vector
-  // registers beside the 64 gradient sums they left two waves per SIMD)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 15, lg = lane >> 4;
+  const int lrow = lr >> 2, lcol = 4 * (lr & 3);
+  // the weights are the same for every lane: scalar registers
   auto uni = [](float f) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(f))); };
   float W[HEAD_C][HEAD_CI], b[HEAD_C], cwv[HEAD_C];
 #pragma unroll
@@ -1616,67 +1622,111 @@ __global__ __launch_bounds__(256) void k_head_loss(const u16* __restrict__ x, in
     b[j] = uni(j < c && bias ? bias[j] : 0.f);
     cwv[j] = uni(j < c ? cw[j] : 0.f);
   }
-  float gw[HEAD_C][HEAD_CI], gb[HEAD_C], l = 0.f;
+  u16* const mx = sx[wave];
+  u16* const md = sd[wave];
+  // d rows: channels 4-15 stay zero (the loop writes 0-3)
+  *(u32x4*)(md + lane * 16) = (u32x4){0u, 0u, 0u, 0u};
+  *(u32x4*)(md + lane * 16 + 8) = (u32x4){0u, 0u, 0u, 0u};
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  float gb[HEAD_C] = {0.f, 0.f, 0.f, 0.f}, l = 0.f;
+  const int64_t step = (int64_t)gridDim.x * 256;
+  int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t v_end = ((nvox + 255) / 256) * 256;  // whole waves run the loop, lanes past nvox hold zeros
+  auto load = [&](int64_t vv, u32x4& a0, u32x4& a1, uint32_t& m, u32x2& tg) {
+    a0 = a1 = (u32x4){0u, 0u, 0u, 0u};
+    m = 0;
+    tg = (u32x2){0u, 0u};
+    if (vv < nvox) {
+      a0 = *(const u32x4*)(x + vv * xs);
+      a1 = *(const u32x4*)(x + vv * xs + 8);
+      m = mask[vv];
+      if (m) {
+        u16 t4[4] = {0, 0, 0, 0};
 #pragma unroll
-  for (int j = 0; j < HEAD_C; ++j) {
-    gb[j] = 0.f;
-#pragma unroll
-    for (int k = 0; k < HEAD_CI; ++k) gw[j][k] = 0.f;
-  }
-  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nvox; v += (int64_t)gridDim.x * 256) {
-    const u16x8 a0 = *(const u16x8*)(x + v * xs), a1 = *(const u16x8*)(x + v * xs + 8);
+        for (int j = 0; j < HEAD_C; ++j)
+          if (j < c) t4[j] = tgt[vv * ts + j];
+        tg.x = t4[0] | (uint32_t)t4[1] << 16;
+        tg.y = t4[2] | (uint32_t)t4[3] << 16;
+      }
+    }
+  };
+  u32x4 a0, a1;
+  uint32_t m;
+  u32x2 tg;
+  load(v, a0, a1, m, tg);
+  for (; v < v_end; v += step) {
+    u32x4 n0, n1;
+    uint32_t nm;
+    u32x2 ntg;
+    load(v + step, n0, n1, nm, ntg);
     float xf[HEAD_CI];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      xf[k] = bf2f(a0[k]);
-      xf[8 + k] = bf2f(a1[k]);
+    for (int k = 0; k < 4; ++k) {
+      xf[2 * k] = __uint_as_float(a0[k] << 16);
+      xf[2 * k + 1] = __uint_as_float(a0[k] & 0xffff0000u);
+      xf[8 + 2 * k] = __uint_as_float(a1[k] << 16);
+      xf[8 + 2 * k + 1] = __uint_as_float(a1[k] & 0xffff0000u);
     }
     float d[HEAD_C] = {0.f, 0.f, 0.f, 0.f};
-    if (mask[v]) {
+    if (m) {
+      const float t[HEAD_C] = {__uint_as_float(tg.x << 16), __uint_as_float(tg.x & 0xffff0000u),
+                               __uint_as_float(tg.y << 16), __uint_as_float(tg.y & 0xffff0000u)};
 #pragma unroll
       for (int j = 0; j < HEAD_C; ++j) {
         if (j >= c) continue;
         float o = 0.f;
 #pragma unroll
         for (int k = 0; k < HEAD_CI; ++k) o = fmaf(W[j][k], xf[k], o);
-        const float e = bf2f(f2bf(o + b[j])) - bf2f(tgt[v * ts + j]);
+        const float e = bf2f(f2bf(o + b[j])) - t[j];
         l += cwv[j] * e * e;
         d[j] = bf2f(f2bf(2.f * scale * cwv[j] * e));
       }
     }
-    u16x8 o0, o1;
+    if (v < nvox) {
+      u32x4 o0, o1;
 #pragma unroll
-    for (int k = 0; k < HEAD_CI; ++k) {
-      float g = 0.f;
+      for (int k = 0; k < HEAD_CI; k += 2) {
+        float g0 = 0.f, g1 = 0.f;
 #pragma unroll
-      for (int j = 0; j < HEAD_C; ++j) g = fmaf(W[j][k], d[j], g);
-      const u16 gv = xf[k] > 0.f ? f2bf(g) : (u16)0;
-      if (k < 8) o0[k] = gv;
-      else o1[k - 8] = gv;
+        for (int j = 0; j < HEAD_C; ++j) {
+          g0 = fmaf(W[j][k], d[j], g0);
+          g1 = fmaf(W[j][k + 1], d[j], g1);
+        }
+        const uint32_t lo = xf[k] > 0.f ? f2bf(g0) : 0u, hi = xf[k + 1] > 0.f ? f2bf(g1) : 0u;
+        if (k < 8) o0[k / 2] = lo | hi << 16;
+        else o1[(k - 8) / 2] = lo | hi << 16;
+      }
+      *(u32x4*)(dx + v * dxs) = o0;
+      *(u32x4*)(dx + v * dxs + 8) = o1;
     }
-    *(u16x8*)(dx + v * dxs) = o0;
-    *(u16x8*)(dx + v * dxs + 8) = o1;
 #pragma unroll
-    for (int j = 0; j < HEAD_C; ++j) {
-      gb[j] += d[j];
+    for (int j = 0; j < HEAD_C; ++j) gb[j] += d[j];
+    // the wave's 64 voxels as rows of its LDS: x as loaded, d as bf16
+    *(u32x4*)(mx + lane * 16) = a0;
+    *(u32x4*)(mx + lane * 16 + 8) = a1;
+    *(u32x2*)(md + lane * 16) = (u32x2){(uint32_t)f2bf(d[0]) | (uint32_t)f2bf(d[1]) << 16,
+                                        (uint32_t)f2bf(d[2]) | (uint32_t)f2bf(d[3]) << 16};
 #pragma unroll
-      for (int k = 0; k < HEAD_CI; ++k) gw[j][k] = fmaf(d[j], xf[k], gw[j][k]);
+    for (int ks = 0; ks < 2; ++ks) {
+      const int vb = 32 * ks + 8 * lg;
+      acc = mfma16(frag_tr(md + (vb + lrow) * 16 + lcol, 16), frag_tr(mx + (vb + lrow) * 16 + lcol, 16), acc);
     }
+    a0 = n0;
+    a1 = n1;
+    m = nm;
+    tg = ntg;
   }
-  // lanes, then the four waves, in a fixed order
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // C element i of lane (lr, lg): dW[4 lg + i][lr]; rows 0-3 are in lanes 0-15
   auto wave_sum = [&](float t) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
     return t;
   };
+  if (lg == 0)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][i * HEAD_CI + lr] = acc[i];
 #pragma unroll
   for (int j = 0; j < HEAD_C; ++j) {
-#pragma unroll
-    for (int k = 0; k < HEAD_CI; ++k) {
-      const float t = wave_sum(gw[j][k]);
-      if (lane == 0) red[wave][j * HEAD_CI + k] = t;
-    }
     const float t = wave_sum(gb[j]);
     if (lane == 0) red[wave][HEAD_C * HEAD_CI + j] = t;
   }
@@ -2325,8 +2375,8 @@ extern "C" int64_t abx_mse_partials(int64_t nvox) { return nvox < 1 ? -1 : (int6
 
 namespace abx {
 namespace {
-// k_head_loss's grid: up to 1024 workgroups striding over the voxels
-int64_t head_blocks(int64_t nvox) { return nvox < 1 ? -1 : std::min<int64_t>(blocks_for(nvox), 1024); }
+// k_head_loss's grid: up to 1536 workgroups striding over the voxels (six per CU)
+int64_t head_blocks(int64_t nvox) { return nvox < 1 ? -1 : std::min<int64_t>(blocks_for(nvox), 1536); }
 }  // namespace
 }  // namespace abx
 
