@@ -287,60 +287,92 @@ __global__ void k_field3d(EvalDev e, int kB, double u0, double q, double rhoRef,
   loss[c] = (float)((pin - (r / 3 + 0.5 * (ux * ux + uy * uy + uz * uz))) / q);
 }
 
-// ---- streamlines (engine.js:721-758), one thread per seed ---------------
-__device__ double js_round(double x) {
-  const double r = floor(x);
-  return (x - r >= 0.5) ? r + 1 : r;
-}
+// ---- streamlines (engine.js:721-758), 8 lanes per seed ------------------
 
+// Trilinear velocity at (x, y, z) from the fluid corners (engine.js:724-737),
+// by the 8 lanes of a group, lane g taking corner (a, b, c) = (g >> 2, g >> 1
+// & 1, g & 1): the corners' row lookups, fluid tests and loads run at once,
+// and every lane then adds the corners' products in the reference's order (a,
+// then b, then c), so all lanes hold the same sums, those of one thread
+// walking the corners.  round_ok: the cell of the rounded position (JS
+// Math.round per axis: floor(x) + 1 where x - floor(x) >= 0.5) lies in the
+// grid and is fluid, the loop's first test (engine.js:741); that cell is the
+// corner whose offsets are the fractions >= 0.5, whose lane has made the same
+// test.
 template <typename S>
-__device__ double sl_vel(const EvalDev& e, double x, double y, double z, double out[3]) {
+__device__ double sl_vel8(const EvalDev& e, int g, double x, double y, double z, double out[3], bool* round_ok) {
   const double i0 = floor(x), j0 = floor(y), k0 = floor(z), fx = x - i0, fy = y - j0, fz = z - k0;
+  const int a = g >> 2, b = (g >> 1) & 1, c = g & 1;
+  const double i = i0 + a, j = j0 + b, k = k0 + c;
+  bool ok = !(i < 0 || j < 0 || k < 0 || i >= e.nx || j >= e.ny || k >= e.nz);
+  double w = 0, p0 = 0, p1 = 0, p2 = 0;
+  if (ok) {
+    w = (a ? fx : 1 - fx) * (b ? fy : 1 - fy) * (c ? fz : 1 - fz);
+    const int64_t d = dpos(e, (int64_t)i, (int64_t)j, (int64_t)k);
+    ok = efluid(e, d);
+    if (ok) {
+      p0 = w * U<S>(e, 0, d);
+      p1 = w * U<S>(e, 1, d);
+      p2 = w * U<S>(e, 2, d);
+    }
+  }
+  *round_ok = __shfl((int)ok, (fx >= 0.5 ? 4 : 0) | (fy >= 0.5 ? 2 : 0) | (fz >= 0.5 ? 1 : 0), 8) != 0;
   out[0] = out[1] = out[2] = 0;
   double ws = 0;
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int c = 0; c < 2; ++c) {
-        const double i = i0 + a, j = j0 + b, k = k0 + c;
-        if (i < 0 || j < 0 || k < 0 || i >= e.nx || j >= e.ny || k >= e.nz) continue;
-        const double w = (a ? fx : 1 - fx) * (b ? fy : 1 - fy) * (c ? fz : 1 - fz);
-        const int64_t d = dpos(e, (int64_t)i, (int64_t)j, (int64_t)k);
-        if (!efluid(e, d)) continue;
-        out[0] += w * U<S>(e, 0, d);
-        out[1] += w * U<S>(e, 1, d);
-        out[2] += w * U<S>(e, 2, d);
-        ws += w;
-      }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const bool okq = __shfl((int)ok, q, 8) != 0;
+    const double wq = __shfl(w, q, 8), a0 = __shfl(p0, q, 8), a1 = __shfl(p1, q, 8), a2 = __shfl(p2, q, 8);
+    if (okq) {
+      out[0] += a0;
+      out[1] += a1;
+      out[2] += a2;
+      ws += wq;
+    }
+  }
   return ws;
 }
 
+// One streamline per group of 8 lanes (engine.js:721-758); every lane of a
+// group follows the line, lane 0 writes its points.
 template <typename S>
-__global__ void k_streamlines(EvalDev e, const double* seeds, int nlines, double u0, double top, int maxp,
-                              double* P, double* V, int* np_out) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nlines) return;
-  double x = seeds[3 * l], y = seeds[3 * l + 1], z = seeds[3 * l + 2];
+__global__ __launch_bounds__(64) void k_streamlines(EvalDev e, const double* seeds, int nlines, double u0, double top,
+                                                    int maxp, double* P, double* V, int* np_out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = t >> 3, g = t & 7;
+  // whole groups only: the shuffles read all 8 lanes of the group
+  const bool live = l < nlines;
+  double x = 0, y = 0, z = 0;
+  if (live) {
+    x = seeds[3 * l];
+    y = seeds[3 * l + 1];
+    z = seeds[3 * l + 2];
+  }
   double* Pl = P + (int64_t)l * 3 * maxp;
   double* Vl = V + (int64_t)l * maxp;
   int np = 0;
   const double ds = 0.35;
   double v1[3], v2[3];
-  for (int st = 0; st < 6000; ++st) {
-    const double ri = js_round(x), rj = js_round(y), rk = js_round(z);
-    if (ri < 0 || rj < 0 || rk < 0 || ri >= e.nx || rj >= e.ny || rk >= e.nz) break;
-    if (!efluid(e, dpos(e, (int64_t)ri, (int64_t)rj, (int64_t)rk))) break;
-    if (sl_vel<S>(e, x, y, z, v1) < 0.3) break;
+  bool rok;
+  for (int st = 0; live && st < 6000; ++st) {
+    // the rounded cell's test (out of the grid or not fluid: stop) comes from
+    // the corner lanes of the interpolation at the same point
+    const double ws1 = sl_vel8<S>(e, g, x, y, z, v1, &rok);
+    if (!rok) break;
+    if (ws1 < 0.3) break;
     const double m1 = js_hypot3(v1[0], v1[1], v1[2]);
     if (m1 < 0.01 * u0) break;
     if (st % 3 == 0 && np < maxp) {
-      Pl[3 * np] = x;
-      Pl[3 * np + 1] = y;
-      Pl[3 * np + 2] = z;
-      Vl[np] = m1 / u0;
+      if (g == 0) {
+        Pl[3 * np] = x;
+        Pl[3 * np + 1] = y;
+        Pl[3 * np + 2] = z;
+        Vl[np] = m1 / u0;
+      }
       np++;
     }
     const double xm = x + 0.5 * ds * v1[0] / m1, ym = y + 0.5 * ds * v1[1] / m1, zm = z + 0.5 * ds * v1[2] / m1;
-    if (sl_vel<S>(e, xm, ym, zm, v2) < 0.3) break;
+    if (sl_vel8<S>(e, g, xm, ym, zm, v2, &rok) < 0.3) break;
     const double m2 = js_hypot3(v2[0], v2[1], v2[2]);
     if (m2 < 0.01 * u0) break;
     x += ds * v2[0] / m2;
@@ -348,16 +380,18 @@ __global__ void k_streamlines(EvalDev e, const double* seeds, int nlines, double
     z += ds * v2[2] / m2;
     if (y < top + 0.5) {
       if (np < maxp) {
-        Pl[3 * np] = x;
-        Pl[3 * np + 1] = y;
-        Pl[3 * np + 2] = z;
-        Vl[np] = m2 / u0;
+        if (g == 0) {
+          Pl[3 * np] = x;
+          Pl[3 * np + 1] = y;
+          Pl[3 * np + 2] = z;
+          Vl[np] = m2 / u0;
+        }
         np++;
       }
       break;
     }
   }
-  np_out[l] = np;
+  if (live && g == 0) np_out[l] = np;
 }
 
 inline unsigned nbk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -619,11 +653,11 @@ extern "C" int abx_streamlines(abx_sim* s, const abx_eval_geom* g, int b, double
     return -1;
   ABX_HIP(hipMemcpyAsync(dseeds.p, sel.data(), 8 * sel.size(), hipMemcpyHostToDevice, st));
   if (v.ssz == 8)
-    hipLaunchKernelGGL(k_streamlines<double>, dim3(nbk(nl, 64)), dim3(64), 0, st, e, dseeds.as<double>(), nl, v.u0,
-                       (double)g->top, maxp, dP.as<double>(), dV.as<double>(), dn.as<int>());
+    hipLaunchKernelGGL(k_streamlines<double>, dim3(nbk((int64_t)nl * 8, 64)), dim3(64), 0, st, e, dseeds.as<double>(),
+                       nl, v.u0, (double)g->top, maxp, dP.as<double>(), dV.as<double>(), dn.as<int>());
   else
-    hipLaunchKernelGGL(k_streamlines<float>, dim3(nbk(nl, 64)), dim3(64), 0, st, e, dseeds.as<double>(), nl, v.u0,
-                       (double)g->top, maxp, dP.as<double>(), dV.as<double>(), dn.as<int>());
+    hipLaunchKernelGGL(k_streamlines<float>, dim3(nbk((int64_t)nl * 8, 64)), dim3(64), 0, st, e, dseeds.as<double>(),
+                       nl, v.u0, (double)g->top, maxp, dP.as<double>(), dV.as<double>(), dn.as<int>());
   ABX_HIP(hipGetLastError());
   std::vector<double> P((size_t)3 * maxp * nl), V((size_t)maxp * nl);
   std::vector<int> np((size_t)nl);

@@ -279,6 +279,81 @@ def bench_unet(a, dev):
     print(json.dumps(out), flush=True)
 
 
+def bench_results(a, dev):
+    """The result stage of a solve on the 1 mm airbox (the default workload):
+    the probe (gamma and mass ratio, engine.js:324-339, 452-462), evaluate
+    with and without field3d (engine.js:632-719) and the streamlines
+    (engine.js:721-758), each timed per call with its host copies (the
+    reference's results are host arrays too), after --steps steps (default
+    600, the last third averaged).  Beside them, the oracle's C restatement
+    of the same functions on the same averaged field, one host core, timed
+    once each."""
+    from airboxcalc_amd import abx
+    spec, tris, p = workload(a.dx, a.seed)
+    run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
+    g = abx.Geometry(dev, tris, p, max_cells=0)
+    sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
+    steps = a.steps if a.steps != 6000 else 600
+    avg = steps // 3
+    sim.steps(steps - avg)
+    sim.steps(avg, True)
+    sim.sync()
+    ev = g.eval_view()
+
+    def per_call(fn, n):
+        fn()
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            out = fn()
+        sim.sync()
+        return (time.perf_counter() - t0) * 1e3 / n, out
+
+    probe_ms, _ = per_call(lambda: sim.probe(steps), 50)
+    eval_ms, res = per_call(lambda: sim.evaluate(ev, divisor=avg, field3d=False), 10)
+    eval3d_ms, res3 = per_call(lambda: sim.evaluate(ev, divisor=avg, field3d=True), 10)
+    lines_ms, L = per_call(lambda: sim.streamlines(ev, divisor=avg), 10)
+    acc = sim.read_acc(avg)
+    out = {
+        "metric": "result stage ms per call (evaluate + field3d on the averaged field)",
+        "value": round(eval3d_ms, 3), "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f32": "f32", "f32_f64": "f32 storage, f64 arithmetic", "f64": "f64"}[a.precision],
+        "data": "synthetic (procedural airbox STL, seeded)",
+        "config": {"workload": f"airbox 300x150x200 mm, dx={a.dx:g} mm, seed {a.seed}, results after {steps} steps "
+                               f"averaging {avg}", "grid": [g.nx, g.ny, g.nz], "fluid_cells": sim.NF},
+        "probe_ms": round(probe_ms, 4), "evaluate_ms": round(eval_ms, 3), "evaluate_field3d_ms": round(eval3d_ms, 3),
+        "streamlines_ms": round(lines_ms, 3), "streamline_points": int(L["spd"].size),
+        "gamma": res["metrics"]["gamma"], "zeta": res["metrics"]["zeta"],
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        from oracle import oracle as O
+        og = O.prepare(tris, p, max_cells=0)
+        S = O.setup_sim(og, run["K"], run["aniso"], run["u"], run["tau"])
+        f = S.f0()
+        t0 = time.perf_counter()
+        O.probe(S, f, steps)
+        t_probe = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oe = O.evaluate(og, S, run["u"], acc)
+        t_eval = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.field3d(og, S, run["u"], acc, oe["fluid"], oe["rhoRef"])
+        t_f3 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.streamlines(og, S, run["u"], acc, oe["fluid"])
+        t_lines = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round((t_eval + t_f3) * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "probe_ms": round(t_probe * 1e3, 2), "evaluate_ms": round(t_eval * 1e3, 1),
+            "field3d_ms": round(t_f3 * 1e3, 1), "streamlines_ms": round(t_lines * 1e3, 1),
+            "same_gamma": abs(oe["metrics"]["gamma"] - res["metrics"]["gamma"]) <= 1e-6,
+            "sample": "oracle/engine_oracle.c probe, evaluate, field3d and streamlines once each on the GPU's "
+                      "averaged field, one thread", "host": host_cpu()}
+    print(json.dumps(out), flush=True)
+
+
 def bench_conv(a, dev):
     """The surrogate's 16-channel 3x3x3 convolutions (batch 8, 96 x 48 x 64):
     forward, the 16-channel kernel (its weights held in registers, read
@@ -389,11 +464,12 @@ def main():
     ap.add_argument("--spinup", type=int, default=3000,
                     help="untimed steps from rest before the warmup, so the timed probes read a developed flow")
     ap.add_argument("--workload", default="roofline",
-                    choices=["roofline", "batch", "fp64", "unet", "conv", "warmstart"],
+                    choices=["roofline", "batch", "fp64", "unet", "conv", "warmstart", "results"],
                     help="roofline: one dx=1 mm airbox (default); batch: 64 seeded uneven variants in one batch; "
                          "fp64: the FP64 error report of the roofline airbox at dx=2 mm; "
                          "unet: surrogate training steps; conv: the surrogate's 16-channel convolutions; "
-                         "warmstart: time to steady, warm against cold, on held-out boxes at dx=2 mm")
+                         "warmstart: time to steady, warm against cold, on held-out boxes at dx=2 mm; "
+                         "results: probe, evaluate, field3d and streamlines of the roofline airbox")
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the library's float4 copy kernel (HBM ceiling and counter calibration)")
@@ -421,6 +497,8 @@ def main():
         return bench_conv(a, dev)
     if a.workload == "warmstart":
         return bench_warmstart(a, dev)
+    if a.workload == "results":
+        return bench_results(a, dev)
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
     t = time.perf_counter()
