@@ -238,10 +238,6 @@ constexpr int F16BX = F16X + 2, F16BY = F16Y + 2, F16BZ = F16Z + 2, F16BV = F16B
 constexpr int F16SP = 16;  // u16 per brick voxel: 16 channels, unpadded (measured faster than 24, 32, 40)
 constexpr int F16SLOTS = (2 * F16BV + 511) / 512;  // 16-byte brick pieces per thread
 
-__host__ __device__ constexpr int f16_tapoff(int tap) {
-  return tap < 27 ? ((tap / 9) * F16BY + (tap / 3) % 3) * F16BX + tap % 3 : 0;
-}
-
 // Taps of pair s (27: none); tap = (dz * 3 + dy) * 3 + dx.
 __host__ __device__ constexpr int f16_tapA(int s) {
   return s < 9 ? s * 3 : (s < 12 ? (s - 9) * 3 + 2 : (s == 12 ? 20 : 26));
@@ -350,11 +346,10 @@ __device__ __forceinline__ void load_lds16(const void* src, void* lds) {
 // values arrive in LDS with the brick of its last input chunk, so the
 // epilogue does not wait on a global load per row.
 template <bool DMA, int F16DMA_WAVES = 6, bool WREG = false, bool GST = false>
-__global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(const u16* __restrict__ x, int xs, const u16* x2, int xs2,
-                                                    int nch1, const u16* __restrict__ wt,
-                                                    const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2,
-                                                    int ncot1, const u16* __restrict__ gate, int gs, int D, int H,
-                                                    int W, int Ci, int ntiles, int relu, int accum) {
+__global__ __launch_bounds__(512, DMA ? F16DMA_WAVES : 1) void k_conv_fwd16(
+    const u16* __restrict__ x, int xs, const u16* x2, int xs2, int nch1, const u16* __restrict__ wt,
+    const float* __restrict__ bias, u16* y, int ys, u16* y2, int ys2, int ncot1, const u16* __restrict__ gate, int gs,
+    int D, int H, int W, int Ci, int ntiles, int relu, int accum) {
   // whole slots of 64 pieces: the last wave of the last slot writes past the brick
   __shared__ __attribute__((aligned(16))) u16 sx[(2 * F16BV + 63) / 64 * 64 * 8];
   __shared__ u32x4 sw[14 * 64];
