@@ -288,11 +288,29 @@ def bench_results(a, dev):
     600, the last third averaged).  Beside them, the oracle's C restatement
     of the same functions on the same averaged field, one host core, timed
     once each."""
+    import dataclasses
     from airboxcalc_amd import abx
+    from airboxcalc_amd import geometry as G
     spec, tris, p = workload(a.dx, a.seed)
     run = dict(K=3.0, aniso=10.0, u=0.06, tau=0.51)
+    # set-up stages: the STL reader on the same airbox subdivided to about 200 k triangles, prepare of both
+    # meshes, the simulation handle
+    big = G.airbox_triangles(dataclasses.replace(spec, subdiv=51))
+    stl = G.stl_binary_bytes(big)
+    t0 = time.perf_counter()
+    parsed = abx.parse_stl(stl)
+    parse_ms = (time.perf_counter() - t0) * 1e3
+    assert parsed.size == big.size
+    t0 = time.perf_counter()
+    abx.Geometry(dev, parsed, p, max_cells=0)
+    prep_big_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
     g = abx.Geometry(dev, tris, p, max_cells=0)
+    prep_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
     sim = abx.Sim(dev, g.grid(), precision=a.precision, **run)
+    sim.sync()
+    create_ms = (time.perf_counter() - t0) * 1e3
     steps = a.steps if a.steps != 6000 else 600
     avg = steps // 3
     sim.steps(steps - avg)
@@ -322,6 +340,9 @@ def bench_results(a, dev):
         "data": "synthetic (procedural airbox STL, seeded)",
         "config": {"workload": f"airbox 300x150x200 mm, dx={a.dx:g} mm, seed {a.seed}, results after {steps} steps "
                                f"averaging {avg}", "grid": [g.nx, g.ny, g.nz], "fluid_cells": sim.NF},
+        "setup": {"stl_triangles": int(big.size // 9), "stl_bytes": len(stl), "parse_stl_ms": round(parse_ms, 2),
+                  "prepare_ms": round(prep_ms, 1), "prepare_200k_triangles_ms": round(prep_big_ms, 1),
+                  "sim_create_ms": round(create_ms, 1)},
         "probe_ms": round(probe_ms, 4), "evaluate_ms": round(eval_ms, 3), "evaluate_field3d_ms": round(eval3d_ms, 3),
         "streamlines_ms": round(lines_ms, 3), "streamline_points": int(L["spd"].size),
         "gamma": res["metrics"]["gamma"], "zeta": res["metrics"]["zeta"],
