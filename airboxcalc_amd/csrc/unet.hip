@@ -955,13 +955,19 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
   constexpr int ZP = W27V * NCT * 2, XP = W27BV * NCI * 2;  // 16-byte pieces of the dZ tile and the brick
   constexpr int ZSL = (ZP + 511) / 512, XSL = (XP + 511) / 512;
   static_assert(ZP % 512 == 0, "dZ pieces fill whole slots");
-  // the lane groups interleaved by 4 voxels where both tiles' voxel rows
-  // (32 NCT and 32 NCI bytes) then give the two groups of a half-wave
-  // disjoint LDS banks: 32 bytes (groups 128 bytes apart) and 96 bytes
-  // (384 apart: rows on banks {0-7, 24-31, 48-55, 8-15} and {32-39, 56-63,
-  // 16-23, 40-47}, for every tap shift); at 64 bytes every voxel offset lands
-  // on the same banks
-  constexpr bool K4 = NCT == 1 && (NCI == 1 || NCI == 3);
+  // the lane groups interleaved by 4 voxels, which gives the two groups of a
+  // half-wave disjoint LDS banks on voxel rows of 32 bytes (groups 128 bytes
+  // apart) and 96 bytes (384 apart: rows on banks {0-7, 24-31, 48-55, 8-15}
+  // and {32-39, 56-63, 16-23, 40-47}, for every tap shift).  On 64-byte rows
+  // (two channel tiles) every voxel offset lands on the same banks, so there
+  // the two 16-channel halves of a voxel are exchanged where bit 2 of its x
+  // (in the tile or the brick row) is set (SWZ): rows 4 apart, one in each
+  // group, then read opposite halves.  128-byte rows (four output tiles) keep
+  // the plain layout.
+  // (the swizzle relies on the interleave: elements 4-7 come from rows 8 on,
+  // whose bit 2 of x is that of elements 0-3)
+  constexpr bool K4 = NCT <= 2;
+  constexpr bool SWZ_Z = NCT == 2, SWZ_X = NCI == 2 && K4;
   __shared__ __attribute__((aligned(16))) u16 sz[W27V * PZ];
   // whole slots of 64 pieces: with direct loads the last wave of the last slot writes past the brick
   __shared__ __attribute__((aligned(16))) u16 sx[(XP + 63) / 64 * 64 * 8];
@@ -977,12 +983,15 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
   const u32x4 ones = {0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u};
 
   // this wave's items: brick offset of the tap plus the input tile's column
+  // (with SWZ_X the input tile's column in the lane's brick row, whose x is
+  // 4 (lg & 1) + lane row + the tap's x shift)
   int boff[MAXI];
 #pragma unroll
   for (int u = 0; u < MAXI; ++u) {
     const int q = wave + 8 * u, qq = q < NI ? q : 0;
     const int tap = qq / NCI, it = qq % NCI;
-    boff[u] = (((tap / 9) * W27BY + (tap / 3) % 3) * W27BX + tap % 3) * PX + it * 16;
+    const int itc = SWZ_X ? it ^ (((4 * (lg & 1) + (lr >> 2) + tap % 3) >> 2) & 1) : it;
+    boff[u] = (((tap / 9) * W27BY + (tap / 3) % 3) * W27BX + tap % 3) * PX + itc * 16;
   }
   // staging slots of this thread: dZ piece threadIdx + 512 m is voxel
   // e / (2 NCT), channels 8 (e % (2 NCT)) .. + 8; brick piece likewise with
@@ -993,7 +1002,9 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
     const int e = min((int)threadIdx.x + 512 * m, XP - 1);
     const int vb = e / (NCI * 2), q = e - vb * (NCI * 2);
     const int bx = vb % W27BX, row = vb / W27BX, by = row % W27BY, bz = row / W27BY;
-    xsl[m] = bx | by << 5 | bz << 8 | q << 12;
+    // the piece the slot holds: with SWZ_X the other half's where bit 2 of bx is set
+    const int qg = SWZ_X ? q ^ (((bx >> 2) & 1) << 1) : q;
+    xsl[m] = bx | by << 5 | bz << 8 | qg << 12;
   }
   const bool xlast = (int)threadIdx.x + 512 * (XSL - 1) < XP;
   u32x4 zv[ZSL], xv[XSL];
@@ -1006,11 +1017,11 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
 #pragma unroll
     for (int m = 0; m < ZSL; ++m) {
       const int e = (int)threadIdx.x + 512 * m;
-      const int v = e / (NCT * 2), q = e - v * (NCT * 2);
+      const int v = e / (NCT * 2), q = e - v * (NCT * 2), qg = SWZ_Z ? q ^ ((((v & 15) >> 2) & 1) << 1) : q;
       const int gx = tl.x0 + (v & 15), gy = tl.y0 + ((v >> 4) & 3), gz = tl.z0 + (v >> 6);
       zok |= (uint32_t)(gx < W && gy < H && gz < D) << m;
       zv[m] = *(const u32x4*)(dz + vox(tl, D, H, W, min(gz, D - 1), min(gy, H - 1), min(gx, W - 1)) * dzs + co0 +
-                              8 * q);
+                              8 * qg);
     }
 #pragma unroll
     for (int m = 0; m < XSL; ++m) {
@@ -1037,8 +1048,9 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
       const int e = (int)threadIdx.x + 512 * m;
       const int v = e / (NCT * 2), q = e - v * (NCT * 2);
       const int vx = v & 15, vy = (v >> 4) & 3, vz = v >> 6;
+      const int qg = SWZ_Z ? q ^ (((vx >> 2) & 1) << 1) : q;
       const bool in = tl.x0 + vx < W && tl.y0 + vy < H && tl.z0 + vz < D;
-      const uint32_t off = in ? (uint32_t)(((vz * H + vy) * W + vx) * dzs * 2 + 16 * q) : 0x80000000u;
+      const uint32_t off = in ? (uint32_t)(((vz * H + vy) * W + vx) * dzs * 2 + 16 * qg) : 0x80000000u;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rz, (__attribute__((address_space(3))) void*)(sz + (512 * m + 64 * wave) * 8),
                                                16, off, 0, 0, 0);
     }
@@ -1111,8 +1123,8 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
 #pragma unroll
       for (int m = 0; m < XSL; ++m) {
         if (m == XSL - 1 && !xlast) break;
-        const int bx = xsl[m] & 31, by = (xsl[m] >> 5) & 7, bz = (xsl[m] >> 8) & 15, q = xsl[m] >> 12;
-        *(u32x4*)(sx + ((bz * W27BY + by) * W27BX + bx) * PX + 8 * q) = ((xok >> m) & 1u) ? xv[m] : zero4;
+        // piece e sits at LDS byte 16 e (its slot code names the global piece it holds)
+        *(u32x4*)(sx + 8 * ((int)threadIdx.x + 512 * m)) = ((xok >> m) & 1u) ? xv[m] : zero4;
       }
     }
     __syncthreads();
@@ -1129,7 +1141,8 @@ __global__ __launch_bounds__(512, WGS > 0 ? 2 * WGS : 1) void k_wgrad27(
       const int tx0 = vb0 & 15, ty = (vb0 >> 4) & 3, tz = vb0 >> 6;
       u32x4 av[NCT];
 #pragma unroll
-      for (int c = 0; c < NCT; ++c) av[c] = frag_tr(sz + (vb0 + lrow) * PZ + c * 16 + lcol, PZ, K4 ? 8 : 4);
+      for (int c = 0; c < NCT; ++c)
+        av[c] = frag_tr(sz + (vb0 + lrow) * PZ + (SWZ_Z ? c ^ (lg & 1) : c) * 16 + lcol, PZ, K4 ? 8 : 4);
       const u16* lbase = sx + ((tz * W27BY + ty) * W27BX + tx0 + lrow) * PX + lcol;
 #pragma unroll
       for (int u = 0; u < MAXI; ++u) {
